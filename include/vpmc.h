@@ -1,0 +1,246 @@
+/*
+ * vpmc.h -- C ABI of the MI355X (gfx950) vortex-particle Monte Carlo hot path.
+ *
+ * Every pointer argument is a device pointer unless stated otherwise; sizes
+ * are element counts.  `stream` is a hipStream_t passed as void* (NULL is the
+ * null stream).  Functions return 0 (VPMC_OK) or a vpmc_status code and never
+ * throw.  Nothing here allocates: functions that need scratch memory take a
+ * caller buffer sized by the matching *_workspace_bytes query.  No function
+ * synchronises the stream unless its comment says so.
+ *
+ * Each entry point names the reference interface it replaces (file:line of
+ * GargantuaOmni/VortexParticleMC_cuda).  float2 data are interleaved xy float
+ * pairs; "points" are float4 (x, y, w, 0) records.
+ */
+#ifndef VPMC_H
+#define VPMC_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum vpmc_status {
+    VPMC_OK = 0,
+    VPMC_ERR_ARG = 1,           /* bad size, NULL pointer, bad mode           */
+    VPMC_ERR_HIP = 2,           /* a HIP call failed (vpmc_last_hip_error)    */
+    VPMC_ERR_WORKSPACE = 3,     /* workspace smaller than the query result    */
+    VPMC_ERR_GRID = 4,          /* cell grid cannot cover the 2 h_w cutoff
+                                   without self-overlap (the reference's
+                                   rsh < 3 case, include/vorticity_query.cuh:36) */
+    VPMC_ERR_CG_BREAKDOWN = 5,  /* p.Ap <= 0 in CG (indefinite Gram matrix)   */
+    VPMC_ERR_NOT_CONVERGED = 6  /* CG hit max_iter                            */
+};
+
+const char *vpmc_status_string(int status);
+int vpmc_last_hip_error(void);          /* hipError_t of the last VPMC_ERR_HIP */
+int vpmc_version(void);                 /* major*10000 + minor*100 + patch     */
+
+/* ---------------------------------------------------------------------------
+ * Spatial hash on the reference's grid (rsh = ResolutionX / int(hwr) / 2,
+ * src/vortex_particle_mc.cu:125).  Cell key ix*rshy+iy with
+ * ix = clamp(floor(x * (rshx/Lx)), 0, rshx-1) in FP32 (NaN maps to 0).
+ */
+size_t vpmc_hash_workspace_bytes(int n, int cells);
+
+/* replaces FillCells_cuda (src/cuda/hash.cu:42-69, include/spatial_hash.hpp:60-67):
+   vp_cell[i] = key(pos[sub[i]]), cell_num = histogram, cell_acc = exclusive
+   scan.  sub may be NULL (identity). */
+int vpmc_fill_cells(int rshx, int rshy, float Lx, float Ly, int n, const int *sub,
+                    const float *pos_xy, int *cell_num, int *cell_acc, int *vp_cell,
+                    void *workspace, size_t workspace_bytes, void *stream);
+
+/* replaces CountingSort_cuda (src/cuda/hash.cu:71-91, include/spatial_hash.hpp:69-73).
+   Stable: vp_map lists, per cell, local indices in ascending order -- the
+   order of CountingSort_cpu (src/cpu/hash.cpp:36-47); cell_num ends at 0. */
+int vpmc_counting_sort(int n, int cells, int *cell_num, const int *cell_acc,
+                       const int *vp_cell, int *vp_map, void *workspace,
+                       size_t workspace_bytes, void *stream);
+
+/* ---------------------------------------------------------------------------
+ * Sign subsets and the |w| CDF in FP64.
+ * replaces Simulation::build_subsets_cuda (src/cuda/build_particle_sets.cu:11-89):
+ * sub_pos/sub_neg list indices with w > 0 / w < 0 ascending; cdf_* is the
+ * normalised inclusive prefix of |w| over the subset (FP64), counts[0..1] and
+ * totals[0..1] (device) receive the subset sizes and sum |w|.
+ */
+size_t vpmc_subsets_workspace_bytes(int n);
+int vpmc_build_subsets(int n, const float *omega, int *sub_pos, int *sub_neg,
+                       double *cdf_pos, double *cdf_neg, int *counts, double *totals,
+                       void *workspace, size_t workspace_bytes, void *stream);
+
+/* Guide table for CDF inversion (no reference counterpart): guide[k] =
+   upper_bound(cdf, count, k / 2^m_log2) for k in [0, 2^m_log2].  With it the
+   sampler returns exactly std::upper_bound's index (sample_vorticity_idx_cdf,
+   src/cpu/monte_carlo_sampling.cpp:23-33) after a short in-bucket search.
+   count_dev is a device int (e.g. counts from vpmc_build_subsets). */
+int vpmc_build_guide(const double *cdf, const int *count_dev, int m_log2, int *guide,
+                     void *stream);
+
+/* replaces sample_vorticity_idx_cdf (src/cpu/monte_carlo_sampling.cpp:23-33,
+   include/monte_carlo_bs.cuh:105-115) for a batch of draws u in [0,1):
+   idx_out[i] = upper_bound(cdf, u[i]) clamped to count-1 (-1 if count is 0). */
+int vpmc_sample_idx(int nu, const float *u, const double *cdf, const int *count_dev,
+                    const int *guide, int m_log2, int *idx_out, void *stream);
+
+/* ---------------------------------------------------------------------------
+ * Vorticity query on the reference's hashed view.
+ */
+typedef struct vpmc_view {            /* include/vortex_particle_mc.hpp:77-94 */
+    const float *pos_xy;              /* global positions (float2)             */
+    const float *w;                   /* global strengths                      */
+    int          N;
+    const int   *vp_sort;             /* local indices grouped by cell         */
+    const int   *cell_acc;            /* rshx*rshy exclusive offsets           */
+    int          rshx, rshy;
+    float        Lx, Ly;
+    int          cnt_view;            /* end of the last cell                  */
+    const int   *sub;                 /* local -> global                       */
+} vpmc_view;
+
+/* replaces queryVorticityAbs (include/vorticity_query.cuh:15-79) with the
+   semantics of evalVorticityAbs_cpu (src/cpu/evaluate_vorticity.cpp:8-75):
+   out[i] = sum over the 3x3 cells of (signed or |w|) W(|d|), d folded across
+   the periodic box, cut at h_w; sign filters w > 0 / w < 0 / none (0). */
+int vpmc_query_view(const vpmc_view *view, int npts, const float *pts_xy, int sign,
+                    int use_signed, int exclude_center, float h_w, int periodic, float *out,
+                    void *stream);
+
+/* ---------------------------------------------------------------------------
+ * Fine cell grid ("field") used by the estimator and the RBF matvec: the
+ * particles of one set, sorted by cell (stable, ascending local index), as
+ * float4 (x, y, weight, 0) records with cell_start offsets.  The grid is
+ * chosen so that cells are <= h_w / cells_per_hw wide; queries visit, per
+ * cell column, only the rows within h_w of the point.
+ */
+typedef struct vpmc_field {
+    int          nx, ny;
+    float        Lx, Ly;
+    int          n;
+    const float *pts;                 /* n float4 records, cell order          */
+    const int   *cell_start;          /* nx*ny + 1                             */
+    const int   *order;               /* sorted position -> local index        */
+} vpmc_field;
+
+int vpmc_field_grid(float h_w, float Lx, float Ly, int cells_per_hw, int *nx, int *ny);
+size_t vpmc_field_workspace_bytes(int n, int nx, int ny);
+/* weight = w[sub[j]] (abs_w: |w|); sub NULL = identity; set_xy (optional)
+   receives pos[sub[j]] in local order. */
+int vpmc_field_build(int nx, int ny, float Lx, float Ly, int n, const int *sub,
+                     const float *pos_xy, const float *w, int abs_w, float *pts,
+                     int *cell_start, int *order, float *set_xy, void *workspace,
+                     size_t workspace_bytes, void *stream);
+/* periodic query of the field's weighted kernel sum at npts points */
+int vpmc_field_query(const vpmc_field *f, float h_w, int npts, const float *pts_xy,
+                     float *out, void *stream);
+
+/* ---------------------------------------------------------------------------
+ * Monte Carlo Biot-Savart estimator.
+ *   mode VPMC_MODE_REFERENCE: the formulas of monte_carlo_bs_cpu
+ *     (src/cpu/monte_carlo_bs.cpp:11-128, aux = 0): CDF index + periodic
+ *     image shift, area-uniform disk of radius sqrt(Lx^2+Ly^2), mis_contrib
+ *     with cum = 9 * Omega.  Reproduces the reference on injected uniforms.
+ *   mode VPMC_MODE_CONSISTENT: the same two samplers with the densities
+ *     matched (blob jitter drawn from W, disk radius uniform so p2 = alpha/r)
+ *     on the min-image cell, with the doubly periodic kernel; unbiased for
+ *     the blob field's periodic velocity.
+ * Uniform source: Philox4x32-10 keyed by (seed; k, owner, step,
+ * sim/sign/branch) or, when `uniforms` is non-NULL, injected draws in the
+ * order monte_carlo_bs_cpu consumes them.
+ */
+enum { VPMC_MODE_REFERENCE = 0, VPMC_MODE_CONSISTENT = 1 };
+
+typedef struct vpmc_sign_set {
+    int           n;                  /* particles in the set                  */
+    const int    *count_dev;          /* device copy of n (NULL: use n)        */
+    const double *cdf;                /* FP64 normalised prefix of |w|         */
+    const double *total_dev;          /* device sum |w| (FP64)                 */
+    const int    *guide;              /* vpmc_build_guide table                */
+    int           guide_log2;
+    const float  *set_xy;             /* positions of the set in local order   */
+    vpmc_field    field;              /* the set on the fine grid, weight |w|  */
+} vpmc_sign_set;
+
+typedef struct vpmc_mc_desc {
+    int          mode;
+    int          N1, N2;
+    float        h_w, Lx, Ly;
+    int          periodic;
+    uint64_t     seed;
+    uint32_t     step, sim;
+    const float *uniforms;            /* replay stream or NULL                 */
+    int          aux_draws;           /* replay stride: 10 draws per branch-2
+                                         sample (main disk first)              */
+    int          n_owner;
+    const float *owner_xy;            /* destinations (float2)                 */
+    vpmc_sign_set set[2];             /* 0: w > 0, 1: w < 0                    */
+    float       *vel_sign;            /* out: 2*n_owner float2, sign-major     */
+    unsigned long long *eval_count;   /* optional: += kernel evaluations within
+                                         h_w and candidates visited ([0],[1])  */
+} vpmc_mc_desc;
+
+/* replaces monte_carlo_bs_kernel_safe (src/cuda/monte_carlo_bs.cu:144-271)
+   for both signs: vel_sign[s*n_owner + i] = estimate for owner i, sign s. */
+int vpmc_mc_estimate(const vpmc_mc_desc *d, void *stream);
+
+/* replaces the combine + Euler + wrap of step_cuda
+   (src/cuda/simulation_step.cu:63-67,77-89) and step_cpu
+   (src/cpu/simulation_step.cpp:41-71): vel = v+ - v-, pos += vel*dt, wrap. */
+int vpmc_advect(int n, float *pos_xy, const float *vel_sign, float *vel_xy, float dt,
+                float Lx, float Ly, int periodic, void *stream);
+
+/* ---------------------------------------------------------------------------
+ * One plain time step (replaces Simulation::step_cuda,
+ * src/cuda/simulation_step.cu:9-90, with aux = 0): rebuild both sign fields
+ * from the current positions, estimate, advect.  Subset, CDF and guide arrays
+ * are inputs (w is constant in the inviscid step).  Field buffers are
+ * caller-owned in `d->set[s].field` (cast away const) and `set_xy`.
+ */
+size_t vpmc_step_workspace_bytes(int n_total, int n_pos, int n_neg, int nx, int ny);
+int vpmc_step(vpmc_mc_desc *d, int n_total, float *pos_xy, const float *omega,
+              const int *sub_pos, const int *sub_neg, float *vel_xy, float dt,
+              void *workspace, size_t workspace_bytes, void *stream);
+
+/* ---------------------------------------------------------------------------
+ * RBF strength initialisation.
+ * replaces Ax_cuda / matvec_kernel (src/cuda/cg_rbf_init.cu:11-50):
+ *   y_i = sum_j x_j W(|x_i - x_j|) over all particles (diagonal included).
+ * Order "cell" means x and y are in the field's sorted order.
+ */
+size_t vpmc_rbf_workspace_bytes(int n, int nx, int ny);
+int vpmc_rbf_matvec(const vpmc_field *f_all, float h_w, const float *x_sorted,
+                    float *y_sorted, void *stream);
+int vpmc_rbf_matvec_f64(const vpmc_field *f_all, float h_w, const double *x_sorted,
+                        double *y_sorted, void *stream);
+
+/* replaces conjugate_gradient_rbf (src/cuda/cg_rbf_init.cu:53-116): solves
+   A w = b from w0 = b / W(0).  FP32 matvec, FP64 dot products; with
+   refine != 0 an outer FP64 residual refresh (iterative refinement) drives
+   ||b - A w|| / ||b|| (FP64 matvec) below tol.  Synchronises the stream each
+   iteration (host loop).  iters/rel_res (host pointers) report the total
+   inner iterations and the final FP64 relative residual. */
+int vpmc_cg_rbf(int n, const float *pos_xy, const float *b, float *w_out, float h_w,
+                float Lx, float Ly, int cells_per_hw, double tol, int max_iter, int refine,
+                int *iters, double *rel_res, void *workspace, size_t workspace_bytes,
+                void *stream);
+size_t vpmc_cg_workspace_bytes(int n, int nx, int ny);
+
+/* ---------------------------------------------------------------------------
+ * FP64 direct sums (error oracles on the device).
+ * replaces biot_savart_naive_kernel (src/cuda/naive_biot_savart.cu:15-40) in
+ * FP64: u(q) = sum_j g_j perp(d)/(2 pi (|d|^2+eps^2)), d = minimg(q - x_j);
+ * skip_self skips j == probe index. */
+int vpmc_direct_sum_eps(int nprobe, const double *probe_xy, int skip_self, int n,
+                        const double *pos_xy, const double *g, double L, int periodic,
+                        double eps, double *u_out, void *stream);
+/* no reference counterpart: exact periodic velocity of the blob field
+   sum_j g_j W(|y - x_j|) (consistent-mode expectation). */
+int vpmc_direct_sum_blob(int nprobe, const double *probe_xy, int n, const double *pos_xy,
+                         const double *g, double h_w, double L, double *u_out, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,188 @@
+// field.hip -- fine cell grid: stable counting sort of one particle set into
+// cell order, stored as float4 (x, y, weight, 0) records, and its query.
+//
+// The sort is the same four passes as the reference-grid hash (hash.hip):
+// keys + atomic histogram, exclusive scan into cell_start, atomic countdown
+// scatter, and a within-cell rank fix-up that makes the order ascending in
+// local index (deterministic, independent of atomic timing).
+#include "field_query.hpp"
+
+namespace vpmc {
+
+__global__ void k_field_keys(int n, const int *__restrict__ sub, const float2 *__restrict__ pos,
+                             int nx, int ny, float inv_cx, float inv_cy, int *__restrict__ key,
+                             int *__restrict__ count, float2 *__restrict__ set_xy)
+{
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    int I = sub ? sub[j] : j;
+    float2 p = pos[I];
+    int ix = min(max((int)floorf(p.x * inv_cx), 0), nx - 1);
+    int iy = min(max((int)floorf(p.y * inv_cy), 0), ny - 1);
+    int c = ix * ny + iy;
+    key[j] = c;
+    atomicAdd(count + c, 1);
+    if (set_xy) set_xy[j] = p;
+}
+
+__global__ void k_field_scatter(int n, const int *__restrict__ key, int *__restrict__ count,
+                                const int *__restrict__ cell_start, int *__restrict__ tmp)
+{
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    int c = key[j];
+    int slot = atomicSub(count + c, 1) - 1;
+    tmp[cell_start[c] + slot] = j;
+}
+
+__global__ void k_field_fixup(int n, const int *__restrict__ key, const int *__restrict__ cell_start,
+                              const int *__restrict__ tmp, const int *__restrict__ sub,
+                              const float2 *__restrict__ pos, const float *__restrict__ w,
+                              int abs_w, int *__restrict__ order, float4 *__restrict__ pts)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    int j = tmp[t];
+    int c = key[j];
+    int beg = cell_start[c], end = cell_start[c + 1];
+    int rank = 0;
+    for (int k = beg; k < end; ++k) rank += (tmp[k] < j) ? 1 : 0;
+    int dst = beg + rank;
+    int I = sub ? sub[j] : j;
+    float2 p = pos[I];
+    float wv = w ? w[I] : 0.f;
+    if (abs_w) wv = fabsf(wv);
+    if (order) order[dst] = j;
+    pts[dst] = make_float4(p.x, p.y, wv, 0.f);
+}
+
+int field_geom(const vpmc_field *f, float h_w, FieldGeom *g)
+{
+    if (!f || f->nx <= 0 || f->ny <= 0 || !(h_w > 0.f) || !(f->Lx > 0.f) || !(f->Ly > 0.f))
+        return VPMC_ERR_ARG;
+    g->nx = f->nx;
+    g->ny = f->ny;
+    g->Lx = f->Lx;
+    g->Ly = f->Ly;
+    g->inv_cx = (float)f->nx / f->Lx;
+    g->inv_cy = (float)f->ny / f->Ly;
+    g->cw = f->Lx / (float)f->nx;
+    g->ch = f->Ly / (float)f->ny;
+    // margin covers FP32 rounding of keys and coordinates (a few ulp of L)
+    float Lm = f->Lx > f->Ly ? f->Lx : f->Ly;
+    g->rs = h_w * (1.f + 1e-5f) + 5e-7f * Lm;
+    g->rs2 = g->rs * g->rs;
+    int rx = (int)ceilf(g->rs / g->cw), ry = (int)ceilf(g->rs / g->ch);
+    g->reach = rx;
+    if (2 * rx + 1 > f->nx || 2 * ry + 1 > f->ny) return VPMC_ERR_GRID;
+    return VPMC_OK;
+}
+
+int field_build_impl(int nx, int ny, float Lx, float Ly, int n, const int *n_dev,
+                     const int *sub, const float *pos_xy, const float *w, int abs_w,
+                     float *pts, int *cell_start, int *order, float *set_xy, void *ws,
+                     size_t ws_bytes, hipStream_t s)
+{
+    (void)n_dev;
+    if (nx <= 0 || ny <= 0 || n < 0 || !cell_start || (n > 0 && (!pos_xy || !pts)))
+        return VPMC_ERR_ARG;
+    const int cells = nx * ny;
+    Carve cv(ws, ws_bytes);
+    void *scan_ws = cv.take<char>(scan_workspace_bytes(cells, sizeof(int)));
+    int *key = cv.take<int>((size_t)(n > 0 ? n : 1));
+    int *tmp = cv.take<int>((size_t)(n > 0 ? n : 1));
+    int *count = cv.take<int>((size_t)cells);
+    if (!scan_ws || !key || !tmp || !count) return VPMC_ERR_WORKSPACE;
+    VPMC_TRY(fill_i32(count, 0, (size_t)cells, s));
+    float inv_cx = (float)nx / Lx, inv_cy = (float)ny / Ly;
+    unsigned gb = (unsigned)((n + 255) / 256);
+    if (n > 0) {
+        hipLaunchKernelGGL(k_field_keys, dim3(gb), dim3(256), 0, s, n, sub,
+                           (const float2 *)pos_xy, nx, ny, inv_cx, inv_cy, key, count,
+                           (float2 *)set_xy);
+        VPMC_CHECK_LAUNCH();
+    }
+    VPMC_TRY(scan_i32(count, cell_start, cells, false, cell_start + cells, scan_ws,
+                      scan_workspace_bytes(cells, sizeof(int)), s));
+    if (n > 0) {
+        hipLaunchKernelGGL(k_field_scatter, dim3(gb), dim3(256), 0, s, n, (const int *)key, count,
+                           (const int *)cell_start, tmp);
+        hipLaunchKernelGGL(k_field_fixup, dim3(gb), dim3(256), 0, s, n, (const int *)key,
+                           (const int *)cell_start, (const int *)tmp, sub,
+                           (const float2 *)pos_xy, w, abs_w, order, (float4 *)pts);
+        VPMC_CHECK_LAUNCH();
+    }
+    return VPMC_OK;
+}
+
+__global__ void k_field_query(FieldD f, float inv_hw, float sig2, int npts,
+                              const float2 *__restrict__ q, float *__restrict__ out)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= npts) return;
+    float2 p = q[i];
+    // wrap into the box as the reference query does (vorticity_query.cuh:20-22)
+    p.x = p.x - floorf(p.x / f.g.Lx) * f.g.Lx;
+    p.y = p.y - floorf(p.y / f.g.Ly) * f.g.Ly;
+    EvalCount ec;
+    out[i] = sig2 * field_sum<false>(f, p.x, p.y, inv_hw, ec);
+}
+
+}  // namespace vpmc
+
+using namespace vpmc;
+
+extern "C" {
+
+int vpmc_field_grid(float h_w, float Lx, float Ly, int cells_per_hw, int *nx, int *ny)
+{
+    if (!(h_w > 0.f) || !(Lx > 0.f) || !(Ly > 0.f) || cells_per_hw <= 0 || !nx || !ny)
+        return VPMC_ERR_ARG;
+    double c = (double)h_w / cells_per_hw;
+    int a = (int)floor((double)Lx / c * (1.0 + 1e-6));
+    int b = (int)floor((double)Ly / c * (1.0 + 1e-6));
+    if (a < 1) a = 1;
+    if (b < 1) b = 1;
+    if (a > 32768) a = 32768;
+    if (b > 32768) b = 32768;
+    *nx = a;
+    *ny = b;
+    vpmc_field f{a, b, Lx, Ly, 0, nullptr, nullptr, nullptr};
+    FieldGeom g;
+    return field_geom(&f, h_w, &g);
+}
+
+size_t vpmc_field_workspace_bytes(int n, int nx, int ny)
+{
+    size_t cells = (size_t)nx * ny;
+    size_t m = (size_t)(n > 0 ? n : 1);
+    return scan_workspace_bytes((int)cells, sizeof(int)) + 2 * align256(m * 4) +
+           align256(cells * 4);
+}
+
+int vpmc_field_build(int nx, int ny, float Lx, float Ly, int n, const int *sub,
+                     const float *pos_xy, const float *w, int abs_w, float *pts,
+                     int *cell_start, int *order, float *set_xy, void *workspace,
+                     size_t workspace_bytes, void *stream)
+{
+    return field_build_impl(nx, ny, Lx, Ly, n, nullptr, sub, pos_xy, w, abs_w, pts, cell_start,
+                            order, set_xy, workspace, workspace_bytes, as_stream(stream));
+}
+
+int vpmc_field_query(const vpmc_field *f, float h_w, int npts, const float *pts_xy, float *out,
+                     void *stream)
+{
+    FieldGeom g;
+    VPMC_TRY(field_geom(f, h_w, &g));
+    if (npts < 0) return VPMC_ERR_ARG;
+    if (npts == 0) return VPMC_OK;
+    if (!pts_xy || !out || !f->cell_start || (f->n > 0 && !f->pts)) return VPMC_ERR_ARG;
+    FieldD d{(const float4 *)f->pts, f->cell_start, g};
+    hipLaunchKernelGGL(k_field_query, dim3((npts + 255) / 256), dim3(256), 0, as_stream(stream),
+                       d, 1.f / h_w, 40.f / (7.f * PI_F * h_w * h_w), npts,
+                       (const float2 *)pts_xy, out);
+    VPMC_CHECK_LAUNCH();
+    return VPMC_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,133 @@
+// vpmc_internal.hpp -- shared device helpers and host plumbing for libvpmc.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "vpmc.h"
+
+namespace vpmc {
+
+constexpr float PI_F = 3.14159265358979323846f;
+constexpr double PI_D = 3.14159265358979323846;
+constexpr int WAVE = 64;
+
+// ---- status ---------------------------------------------------------------
+void set_hip_error(hipError_t e);
+#define VPMC_CHECK_LAUNCH()                                                    \
+    do {                                                                       \
+        hipError_t e_ = hipGetLastError();                                     \
+        if (e_ != hipSuccess) { ::vpmc::set_hip_error(e_); return VPMC_ERR_HIP; } \
+    } while (0)
+#define VPMC_TRY(x)                                                            \
+    do {                                                                       \
+        int s_ = (x);                                                          \
+        if (s_ != VPMC_OK) return s_;                                          \
+    } while (0)
+
+inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }
+inline size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
+
+// bump allocator over a caller workspace
+struct Carve {
+    char *base;
+    size_t cap, used = 0;
+    Carve(void *p, size_t c) : base(static_cast<char *>(p)), cap(c) {}
+    template <class T> T *take(size_t count) {
+        size_t b = align256(count * sizeof(T));
+        if (used + b > cap) return nullptr;
+        T *p = reinterpret_cast<T *>(base + used);
+        used += b;
+        return p;
+    }
+};
+
+// ---- scans (scan.hip) -------------------------------------------------------
+constexpr int SCAN_THREADS = 256;
+constexpr int SCAN_ITEMS = 8;
+constexpr int SCAN_CHUNK = SCAN_THREADS * SCAN_ITEMS;
+inline int scan_blocks(int n) { return (n + SCAN_CHUNK - 1) / SCAN_CHUNK; }
+size_t scan_workspace_bytes(int n, size_t elem);
+// out[i] = sum_{j<i} in[j] (exclusive) or sum_{j<=i} (inclusive);
+// total (optional, device) receives the grand total.  In-place allowed.
+int scan_i32(const int *in, int *out, int n, bool inclusive, int *total, void *ws,
+             size_t ws_bytes, hipStream_t s);
+int scan_f64(const double *in, double *out, int n, bool inclusive, double *total, void *ws,
+             size_t ws_bytes, hipStream_t s);
+int fill_i32(int *p, int v, size_t n, hipStream_t s);
+int fill_f32(float *p, float v, size_t n, hipStream_t s);
+
+// ---- field (field.hip) -------------------------------------------------------
+struct FieldGeom {           // device-side copy of vpmc_field's geometry
+    int nx, ny;
+    float Lx, Ly;
+    float inv_cx, inv_cy;    // nx / Lx, ny / Ly
+    float cw, ch;            // Lx / nx, Ly / ny
+    int reach;               // columns each side
+    float rs, rs2;           // search radius (h_w plus rounding margin)
+};
+int field_geom(const vpmc_field *f, float h_w, FieldGeom *g);
+int field_build_impl(int nx, int ny, float Lx, float Ly, int n, const int *n_dev,
+                     const int *sub, const float *pos_xy, const float *w, int abs_w,
+                     float *pts, int *cell_start, int *order, float *set_xy, void *ws,
+                     size_t ws_bytes, hipStream_t s);
+
+// ---- device math --------------------------------------------------------------
+__device__ __forceinline__ float u01(uint32_t x) { return (float)(x >> 8) * 0x1p-24f; }
+
+// Philox4x32-10 (Salmon et al., SC'11)
+__device__ __forceinline__ uint4 philox10(uint4 c, uint2 k)
+{
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        uint32_t hi0 = __umulhi(0xD2511F53u, c.x), lo0 = 0xD2511F53u * c.x;
+        uint32_t hi1 = __umulhi(0xCD9E8D57u, c.z), lo1 = 0xCD9E8D57u * c.z;
+        c = make_uint4(hi1 ^ c.y ^ k.x, lo1, hi0 ^ c.w ^ k.y, lo0);
+        k.x += 0x9E3779B9u;
+        k.y += 0xBB67AE85u;
+    }
+    return c;
+}
+
+// include/utilities.hpp:7-17, branch-free: W = sig2 (2 t^3 - 8 u^3) with
+// t = max(1-q, 0), u = max(1/2-q, 0) (equal to 1 - 6q^2 + 6q^3 for q <= 1/2).
+__device__ __forceinline__ float spline_sig2(float h_w)
+{
+    return 40.f / (7.f * PI_F * h_w * h_w);
+}
+
+__device__ __forceinline__ double spline_d(double r, double h_w)
+{
+    const double sig2 = 40.0 / (7.0 * PI_D * h_w * h_w);
+    double q = r / h_w;
+    if (q <= 0.5) return sig2 * (6.0 * (q * q * q - q * q) + 1.0);
+    if (q < 1.0) return sig2 * 2.0 * (1.0 - q) * (1.0 - q) * (1.0 - q);
+    return 0.0;
+}
+
+// std::upper_bound(cdf, cdf+n, u) through a guide table (guide[k] =
+// upper_bound(cdf, k / 2^glog2)); j == n -> n-1 as sample_vorticity_idx_cdf
+// does (src/cpu/monte_carlo_sampling.cpp:23-33).  u in [0,1).
+__device__ __forceinline__ int guided_upper_bound(float u, const double *__restrict__ cdf,
+                                                  const int *__restrict__ guide, int glog2, int n)
+{
+    int kb = (int)(u * (float)(1u << glog2));
+    int lo = guide[kb], hi = guide[kb + 1];
+    double ud = (double)u;
+    while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        if (ud < cdf[mid]) hi = mid; else lo = mid + 1;
+    }
+    return lo >= n ? n - 1 : lo;
+}
+
+// wave-level sums (deterministic butterfly)
+template <class T> __device__ __forceinline__ T wave_sum(T v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, WAVE);
+    return v;
+}
+
+}  // namespace vpmc
