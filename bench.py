@@ -1,0 +1,245 @@
+"""Benchmark of the plain GPU path (one MI355X).
+
+Workload (default): Taylor-Green vorticity 4 pi sin(2 pi x) sin(2 pi y) on the
+periodic unit box, Nx = Ny = 200 lattice (40,000 particles), hwr = 16
+(h_w = 0.08), N1 = N2 = 128, dt = 1e-3, consistent estimator, Philox seed 0,
+strengths from CG-RBF (FP64 MINRES, tol 1e-6).  A step is one
+pass of the hot path: rebuild both sign fields from the current positions,
+estimate both signs, advect.  Inputs are resident in HBM before timing.
+
+Prints one JSON line: particle-steps/s over K timed steps, the estimator's
+roofline (algorithmic FLOPs per launch / its event-timed duration), the RMS
+velocity error against the FP64 direct sum, and the reference's CPU path
+(oracle/_ref, compiled from its sources) timed on a bounded sample of the same
+workload on this host.
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+FLOP_PER_EVAL = 15      # d 2, r^2 3, sqrt 1, q 1, spline 5, weight+accumulate 3
+FLOP_PER_SAMPLE = 30    # MIS contribution ~20, sampling ~10
+PEAK_FP32 = 157.3e12    # MI355X FP32 (vector) peak, FLOP/s
+PEAK_HBM = 8.0e12
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def setup(Nx, hwr, N1, N2, device, seed=0, cells_per_hw=8):
+    import torch
+
+    import vortexparticlemc_cuda_amd as v
+    from vortexparticlemc_cuda_amd import ops
+    from vortexparticlemc_cuda_amd.sim import SimParam, Simulation, lattice_positions
+
+    P = SimParam(dt=1e-3, resX=Nx, hwr=hwr)
+    P.N1, P.N2 = N1, N2
+    pos = lattice_positions(Nx)
+    b = (4 * np.pi * np.sin(2 * np.pi * pos[:, 0].astype(np.float64)) *
+         np.sin(2 * np.pi * pos[:, 1].astype(np.float64))).astype(np.float32)
+    sim = Simulation(P, device=device, cells_per_hw=cells_per_hw, mode=v.MODE_CONSISTENT,
+                     seed=seed)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    it, rr, st = sim.init_strengths_cg(pos, b, tol=1e-6, refine=True)
+    torch.cuda.synchronize()
+    t_cg = time.perf_counter() - t0
+    return sim, dict(cg_iters=it, cg_rel_res=rr, cg_status=st, cg_seconds=t_cg)
+
+
+def run_steps(sim, K, events=None):
+    """K plain steps; if events is a list, records estimator start/stop events"""
+    import torch
+
+    from vortexparticlemc_cuda_amd import ops
+    from vortexparticlemc_cuda_amd._capi import check, lib
+
+    s = torch.cuda.current_stream()
+    P = sim.P
+    for _ in range(K):
+        d = sim.desc(P.N1, P.N2)
+        ws = sim._ws_for(d)   # one buffer serves the field builds and the estimator
+        for k, key in enumerate(("sub_pos", "sub_neg")):
+            ops.field_build(sim.fields[k], sim.pos, sim.omega, sub=sim.sub[key], abs_w=True,
+                            ws=ws)
+        if events is not None:
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record(s)
+        sim.launch_estimate(d)
+        if events is not None:
+            e1.record(s)
+            events.append((e0, e1))
+        ops.advect(sim.pos, sim.vel_sign[:2 * sim.N], float(P.dt), float(P.Lx), float(P.Ly))
+        sim.step_index += 1
+
+
+def count_evals(sim):
+    import torch
+
+    from vortexparticlemc_cuda_amd import ops
+    from vortexparticlemc_cuda_amd._capi import check, lib
+
+    sim.eval_count.zero_()
+    sim.rebuild_fields()
+    d = sim.desc(sim.P.N1, sim.P.N2, count=True)
+    sim.launch_estimate(d)
+    torch.cuda.synchronize()
+    c = sim.eval_count.cpu().numpy()
+    return int(c[0]), int(c[1])
+
+
+def velocity_error(sim):
+    """RMS |u_MC - u_oracle| / RMS |u_oracle| at the current positions; oracle =
+    FP64 direct sum of the blob field with the periodic kernel (device)"""
+    import torch
+
+    from vortexparticlemc_cuda_amd import ops
+
+    u = sim.estimate(sim.P.N1, sim.P.N2).double()
+    pos64 = sim.pos.double()
+    ud = ops.direct_sum_blob(pos64, pos64, sim.omega.double(), float(sim.P.h_w))
+    x = pos64[:, 0]
+    y = pos64[:, 1]
+    ut = torch.stack([torch.sin(2 * np.pi * x) * torch.cos(2 * np.pi * y),
+                      -torch.cos(2 * np.pi * x) * torch.sin(2 * np.pi * y)], 1)
+    rms = lambda a: float(torch.sqrt((a * a).sum(1).mean()))  # noqa: E731
+    return dict(rms_err_vs_direct=rms(u - ud) / rms(ud), direct_vs_tg=rms(ud - ut) / rms(ut),
+                rms_err_vs_tg=rms(u - ut) / rms(ut))
+
+
+def cpu_baseline(Nx, hwr, N1, N2, target_s=12.0):
+    """the reference's monte_carlo_bs_cpu (its compiled src/cpu sources,
+    oracle/_ref) over a bounded subset of owners of the same workload,
+    single-threaded like its step_cpu; aux = 0 (the work the GPU path does)"""
+    import oracle as O
+    R = O.ref_lib()
+    kind = "reference"
+    if R is None:
+        return None
+    x, y = O.lattice(Nx)
+    x = x.astype(np.float32)
+    y = y.astype(np.float32)
+    w = (O.tg_vorticity(x, y) / O.tg_lambda11(Nx, hwr)).astype(np.float32)
+    rsh = O.rsh_of(Nx, hwr)
+    hw = np.float32(hwr / Nx)
+    n = x.size
+    probe = 40
+    vel = np.zeros(2 * probe, np.float32)
+    t0 = time.perf_counter()
+    R.ref_mc_velocities(n, x, y, w, rsh, rsh, 1.0, 1.0, hw, 1, N1, N2, 0, 5489, 0, probe, vel)
+    t1 = time.perf_counter() - t0
+    m = int(max(probe, min(n, target_s / max(t1 / probe, 1e-9))))
+    vel = np.zeros(2 * m, np.float32)
+    t0 = time.perf_counter()
+    R.ref_mc_velocities(n, x, y, w, rsh, rsh, 1.0, 1.0, hw, 1, N1, N2, 0, 5489, 0, m, vel)
+    t = time.perf_counter() - t0
+    return {"value": m / t, "unit": "particle-steps/s", "cores": 1, "kind": kind,
+            "sample": "reference monte_carlo_bs_cpu (src/cpu, compiled from source) for %d of "
+                      "%d owners, both signs, N1=N2=%d, aux=0, then scaled per particle; "
+                      "%.1f s single-threaded" % (m, n, N1, t)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--Nx", type=int, default=200)
+    ap.add_argument("--hwr", type=int, default=16)
+    ap.add_argument("--N1", type=int, default=128)
+    ap.add_argument("--N2", type=int, default=128)
+    ap.add_argument("--cells-per-hw", type=int, default=8)
+    ap.add_argument("--algo", choices=["binned", "direct"], default="binned")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-error", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU; this build has no multi-GPU path "
+              "(--gpus %d requested)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    device = torch.device("cuda:0")
+    log("setup Nx=%d hwr=%d N1=%d N2=%d" % (args.Nx, args.hwr, args.N1, args.N2))
+    sim, init = setup(args.Nx, args.hwr, args.N1, args.N2, device,
+                      cells_per_hw=args.cells_per_hw)
+    sim.algo = 0 if args.algo == "binned" else 1
+    log("cg init", init)
+    err = None
+    if not args.no_error:
+        err = velocity_error(sim)
+        log("error", err)
+    run_steps(sim, args.warmup)
+    torch.cuda.synchronize()
+    log("warmup done")
+    events = []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    run_steps(sim, args.steps, events)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    est_ms = [a.elapsed_time(b) for a, b in events]
+    est_avg_s = float(np.mean(est_ms)) / 1e3
+    log("timed %d steps in %.3f s, estimator avg %.3f ms" % (args.steps, elapsed, est_avg_s * 1e3))
+    n_in, n_vis = count_evals(sim)
+    N = sim.N
+    S = N * 2 * (args.N1 + args.N2)
+    flop = FLOP_PER_EVAL * n_in + FLOP_PER_SAMPLE * S
+    achieved = flop / est_avg_s
+    traffic = None
+    tp = os.path.join(ROOT, "profiles", "traffic_mc_estimate_tg200.json")
+    if os.path.exists(tp):
+        try:
+            traffic = json.load(open(tp)).get("hbm_bytes_per_launch")
+        except Exception:
+            traffic = None
+    cpu = None if args.no_cpu else cpu_baseline(args.Nx, args.hwr, args.N1, args.N2)
+    value = N * args.steps / elapsed
+    line = {
+        "metric": "particle-steps/s at fixed RMS vel. error vs FP64 direct sum; batched vs plain",
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 (FP64 CDF/Omega)",
+        "data": "synthetic",
+        "config": {"workload": "plain GPU path: Taylor-Green on periodic [0,1]^2, "
+                               "Nx=Ny=%d lattice (%d particles), hwr=%d, N1=N2=%d, dt=1e-3, "
+                               "consistent estimator, Philox seed 0, CG-RBF strengths"
+                               % (args.Nx, N, args.hwr, args.N1),
+                   "particles": N, "N1": args.N1, "N2": args.N2, "hwr": args.hwr,
+                   "cells_per_hw": args.cells_per_hw, "estimator_algo": args.algo,
+                   "parallelism": "single GPU"},
+        "roofline": {"kernel": "k_mc_estimate", "bound": "valu", "achieved": achieved,
+                     "peak": PEAK_FP32, "unit": "FLOP/s", "frac": achieved / PEAK_FP32,
+                     "traffic": traffic,
+                     "flop_per_launch": flop, "kernel_evals_per_launch": n_in,
+                     "candidates_per_launch": n_vis, "samples_per_launch": S,
+                     "avg_launch_ms": est_avg_s * 1e3},
+        "kernel_evals_per_s": n_in / est_avg_s,
+        "estimator_fraction_of_step": est_avg_s / (elapsed / args.steps),
+        "error": err,
+        "cg_init": init,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

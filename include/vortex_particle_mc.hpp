@@ -1,0 +1,202 @@
+// vortex_particle_mc.hpp -- simulation parameters, particle buffers, views and
+// the Simulation driver of the GPU path.
+//
+// Replaces include/vortex_particle_mc.hpp:16-201.  Names, fields and the
+// SimParam construction semantics are the reference's; the implementation
+// (src/cxx_api.cpp in the package's csrc/) calls the HIP kernels through the C
+// ABI of vpmc.h.  The CPU entry points of the reference (step_cpu,
+// build_subsets_cpu, eval_vorticity_cpu) belong to its CPU backend and are not
+// part of this library; step_cpu / build_subsets_cpu throw std::logic_error.
+#pragma once
+
+#include <cassert>
+#include <cmath>
+#include <cstdint>
+#include <random>
+#include <vector>
+
+#include "device_vector_alias.hpp"
+#include "spatial_hash.hpp"
+#include "spatial_hasher.hpp"
+
+struct ParticleSet {
+    int N_max{0};
+    int N_cur{0};
+
+    dvec<float2> pos;
+    dvec<float2> pos_left, pos_right, pos_top, pos_bottom;
+    dvec<float2> pos_temp;
+    dvec<float2> vel;
+
+    dvec<float> omega;
+    dvec<float> omega_field;
+    dvec<float> jac;
+
+    dvec<float4> F;
+
+    void resize(int cap)
+    {
+        N_max = cap;
+        N_cur = cap;
+        pos.resize(cap);
+        vel.resize(cap);
+        pos_left.resize(cap);
+        pos_right.resize(cap);
+        pos_top.resize(cap);
+        pos_bottom.resize(cap);
+        pos_temp.resize(cap);
+        omega.resize(cap);
+        omega_field.resize(cap);
+        jac.resize(cap);
+        F.resize(cap);
+    }
+    void set_n_current(const int N_cur_) { N_cur = N_cur_; }
+
+    // device pointers
+    float2 *d_p() { return raw_ptr(pos); }
+    float2 *d_pt() { return raw_ptr(pos_temp); }
+    float2 *d_v() { return raw_ptr(vel); }
+    float4 *d_F() { return raw_ptr(F); }
+    float *d_omega() { return raw_ptr(omega); }
+    float *d_omega_field() { return raw_ptr(omega_field); }
+    float *d_jac() { return raw_ptr(jac); }
+    const float2 *d_p() const { return raw_ptr(pos); }
+    const float2 *d_pt() const { return raw_ptr(pos_temp); }
+    const float2 *d_v() const { return raw_ptr(vel); }
+    const float4 *d_F() const { return raw_ptr(F); }
+    const float *d_omega() const { return raw_ptr(omega); }
+    const float *d_omega_field() const { return raw_ptr(omega_field); }
+    const float *d_jac() const { return raw_ptr(jac); }
+};
+
+struct VorticityView {
+    const float2 *pos;
+    const float4 *F;
+    const float *w;
+    int N;
+
+    const int *vp_sort;   // local id
+    const int *cell_acc;
+    int rshx{}, rshy{};
+    float Lx, Ly;
+    float InvLx, InvLy;
+    int cnt_view;
+
+    const int *sub;       // local id -> global id
+};
+
+struct SimParam {
+    float dt = 0.001f;
+    float Lx = 1.f;
+    float Ly = 1.f;
+    int ResolutionX = 200;
+    float hwr = 8.f;
+    int N_max = 6000;
+    int N1 = 1000;
+    int N2 = 1000;
+
+    int ResolutionY = 0;
+    float h = 0.f;
+    float h_w = 0.f;
+    int length_iCDF = 10;
+
+    float dx = 0.f;
+
+    bool periodic = true;
+
+    // the constructor's defaults win over the member initialisers
+    explicit SimParam(float dt_ = 0.01f, float Lx_ = 1.f, float Ly_ = 1.f, int resX = 200,
+                      float hwr_ = 16.f)
+        : dt(dt_), Lx(Lx_), Ly(Ly_), ResolutionX(resX), hwr(hwr_)
+    {
+        assert(ResolutionX > 0 && "ResolutionX must be positive");
+        h = Lx / float(ResolutionX);
+        ResolutionY = int(std::round(Ly / h));
+        if (ResolutionY == 0) ResolutionY = 1;
+        h_w = h * hwr;
+        dx = 0.5f * h_w;
+    }
+};
+
+// Estimator selection of the GPU step (no reference counterpart: the reference
+// has one estimator per backend).  Reference reproduces monte_carlo_bs_cpu's
+// formulas; Consistent is the unbiased estimator of the periodic velocity.
+enum class EstimatorMode : int { Reference = 0, Consistent = 1 };
+
+class Simulation {
+public:
+    ParticleSet particles_;
+    SimParam P_;
+
+    explicit Simulation(SimParam p);
+    ~Simulation();
+    Simulation(const Simulation &) = delete;
+    Simulation &operator=(const Simulation &) = delete;
+
+    // src/vortex_particle_mc.cu:77-120: num_of_p uniform random positions
+    // (std::mt19937(42)), strengths fitted by CG to the Taylor-Green field
+    void init(int num_of_p);
+    // positions and target vorticity given by the caller (e.g. a lattice);
+    // strengths fitted to b (FP64 MINRES to relative residual tol)
+    void init_with(const std::vector<float2> &pos, const std::vector<float> &b,
+                   float tol = 1e-6f, int max_iter = 1000);
+    // positions and strengths given directly
+    void set_particles(const std::vector<float2> &pos, const std::vector<float> &omega);
+
+    void do_spatial_hashing();
+    void build_global_hash();
+
+    void build_subsets_cpu();   // throws: CPU backend not provided
+    void build_subsets_cuda();
+    void prepare_rhs_b();
+
+    std::mt19937 rng;
+    void step_cpu(bool periodic);   // throws: CPU backend not provided
+    void step_cuda(bool periodic);
+
+    void compute_velocity_naive(bool periodic, float eps_core);
+    void step_cuda_naive(bool periodic);
+
+    void test_vorticity_grid(int res);
+
+    void ensure_icdf_kernel(int L);
+
+    VorticityView makePosView(const float *w_ptr = nullptr) const;
+    VorticityView makeNegView(const float *w_ptr = nullptr) const;
+    VorticityView makePosViewNaive(const float *w_ptr = nullptr) const;
+    VorticityView makeNegViewNaive(const float *w_ptr = nullptr) const;
+    VorticityView makeAllView(const float *w_ptr = nullptr) const;
+
+    // ---- build extensions (no reference counterpart) ----
+    EstimatorMode mode = EstimatorMode::Consistent;
+    uint64_t seed = 0;        // Philox key
+    uint32_t sim_id = 0;      // ensemble member (Philox counter)
+    uint32_t step_index = 0;  // advanced by step_cuda
+    int cells_per_hw = 8;     // fine grid of the estimator
+    hipStream_t stream = nullptr;
+    int cg_iterations = 0;
+    double cg_rel_residual = 0.0;
+    // velocity estimate at the current positions without advancing
+    void estimate_velocity(bool periodic);
+    int subset_count(int sign) const { return sign > 0 ? cnt_pos_ : cnt_neg_; }
+
+    struct Impl;
+
+private:
+    SpatialHash hash_pos_, hash_neg_, hash_all_;
+
+    dvec<int> sub_all_;
+    dvec<int> sub_pos_, sub_neg_;
+    dvec<double> cdf_pos_, cdf_neg_;   // FP64 (the reference keeps FP32)
+    int cnt_pos_{}, cnt_neg_{};
+    double cum_pos_{}, cum_neg_{};
+
+    dvec<float> cdf_kernel, icdf_kernel;
+    dvec<float> icdf_kernel_;
+    int icdf_L_ = 0;
+
+    VorticityView pos_view;
+    VorticityView neg_view;
+
+    Impl *impl_ = nullptr;
+};

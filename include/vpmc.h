@@ -179,11 +179,24 @@ typedef struct vpmc_mc_desc {
     float       *vel_sign;            /* out: 2*n_owner float2, sign-major     */
     unsigned long long *eval_count;   /* optional: += kernel evaluations within
                                          h_w and candidates visited ([0],[1])  */
+    int          algo;                /* VPMC_ALGO_BINNED (default) or _DIRECT */
 } vpmc_mc_desc;
 
+enum { VPMC_ALGO_BINNED = 0, VPMC_ALGO_DIRECT = 1 };
+
 /* replaces monte_carlo_bs_kernel_safe (src/cuda/monte_carlo_bs.cu:144-271)
-   for both signs: vel_sign[s*n_owner + i] = estimate for owner i, sign s. */
+   for both signs: vel_sign[s*n_owner + i] = estimate for owner i, sign s.
+   Direct algorithm: one workgroup per (owner, sign), no workspace. */
 int vpmc_mc_estimate(const vpmc_mc_desc *d, void *stream);
+
+/* Same estimate, binned algorithm (periodic boxes): samples are generated,
+   counting-sorted by fine cell and evaluated cell by cell, then reduced per
+   owner in sample order.  Needs vpmc_mc_binned_workspace_bytes(d) bytes
+   (about 40 bytes per sample).  Results equal the direct algorithm's up to
+   FP32 summation order. */
+size_t vpmc_mc_binned_workspace_bytes(const vpmc_mc_desc *d);
+int vpmc_mc_estimate_binned(const vpmc_mc_desc *d, void *workspace, size_t workspace_bytes,
+                            void *stream);
 
 /* replaces the combine + Euler + wrap of step_cuda
    (src/cuda/simulation_step.cu:63-67,77-89) and step_cpu
@@ -198,7 +211,7 @@ int vpmc_advect(int n, float *pos_xy, const float *vel_sign, float *vel_xy, floa
  * are inputs (w is constant in the inviscid step).  Field buffers are
  * caller-owned in `d->set[s].field` (cast away const) and `set_xy`.
  */
-size_t vpmc_step_workspace_bytes(int n_total, int n_pos, int n_neg, int nx, int ny);
+size_t vpmc_step_workspace_bytes(const vpmc_mc_desc *d);
 int vpmc_step(vpmc_mc_desc *d, int n_total, float *pos_xy, const float *omega,
               const int *sub_pos, const int *sub_neg, float *vel_xy, float dt,
               void *workspace, size_t workspace_bytes, void *stream);
@@ -214,13 +227,23 @@ int vpmc_rbf_matvec(const vpmc_field *f_all, float h_w, const float *x_sorted,
                     float *y_sorted, void *stream);
 int vpmc_rbf_matvec_f64(const vpmc_field *f_all, float h_w, const double *x_sorted,
                         double *y_sorted, void *stream);
+/* the same operator in particle order (Ax_cuda's contract): builds the
+   all-particle field from pos_xy in the workspace (vpmc_cg_workspace_bytes)
+   and applies it to x. */
+int vpmc_rbf_matvec_particles(int n, const float *pos_xy, const float *x, float *y, float h_w,
+                              float Lx, float Ly, int cells_per_hw, void *workspace,
+                              size_t workspace_bytes, void *stream);
 
 /* replaces conjugate_gradient_rbf (src/cuda/cg_rbf_init.cu:53-116): solves
-   A w = b from w0 = b / W(0).  FP32 matvec, FP64 dot products; with
-   refine != 0 an outer FP64 residual refresh (iterative refinement) drives
-   ||b - A w|| / ||b|| (FP64 matvec) below tol.  Synchronises the stream each
-   iteration (host loop).  iters/rel_res (host pointers) report the total
-   inner iterations and the final FP64 relative residual. */
+   A w = b.  refine == 0: the reference's FP32 CG from w0 = b / W(0) (FP32
+   matvec, FP64 dot products), VPMC_ERR_CG_BREAKDOWN when p.Ap <= 0 (the Gram
+   matrix is indefinite).  refine != 0: MINRES (symmetric indefinite) on FP64
+   vectors with the FP64 matvec from w0 = 0, until ||b - A w|| / ||b|| < tol,
+   else VPMC_ERR_NOT_CONVERGED after max_iter.  The strengths are returned in
+   FP32, so tol below ~1e-7 amplifies the rounding noise of b in the near-null
+   modes of A.  Synchronises the stream each iteration (host loop).
+   iters/rel_res (host pointers) report the iterations and the final FP64
+   relative residual of the FP64 iterate. */
 int vpmc_cg_rbf(int n, const float *pos_xy, const float *b, float *w_out, float h_w,
                 float Lx, float Ly, int cells_per_hw, double tol, int max_iter, int refine,
                 int *iters, double *rel_res, void *workspace, size_t workspace_bytes,

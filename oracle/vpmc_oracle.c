@@ -173,7 +173,14 @@ float orc_eval_vorticity(const orc_view *v, float qx, float qy, int sign, int us
 #define ORC_VISIT(I)                                                                \
     do {                                                                            \
         float dx = v->px[I] - px, dy = v->py[I] - py;                               \
-        if (periodic) {                                                             \
+        if (periodic && !v->vp_sort) {                                              \
+            /* naive view: rshx = 0, the fold below would divide by zero    \
+               (src/cpu/evaluate_vorticity.cpp:30-37); plain minimum image */ \
+            if (dx > 0.5f * v->Lx) dx -= v->Lx;                                     \
+            if (dx < -0.5f * v->Lx) dx += v->Lx;                                    \
+            if (dy > 0.5f * v->Ly) dy -= v->Ly;                                     \
+            if (dy < -0.5f * v->Ly) dy += v->Ly;                                    \
+        } else if (periodic) {                                                      \
             const float cellx = v->Lx / v->rshx, celly = v->Ly / v->rshy;           \
             if (dx > v->Lx - 2 * cellx) dx -= v->Lx;                                \
             if (dx < -v->Lx + 2 * cellx) dx += v->Lx;                               \

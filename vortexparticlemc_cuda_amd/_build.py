@@ -45,16 +45,33 @@ def _compile(src):
 def build(jobs=None, verbose=False):
     os.makedirs(BUILD, exist_ok=True)
     os.makedirs(os.path.dirname(LIB), exist_ok=True)
-    srcs = sorted(glob.glob(os.path.join(CSRC, "*.hip")))
+    srcs = sorted(glob.glob(os.path.join(CSRC, "*.hip")) + glob.glob(os.path.join(CSRC, "*.cpp")))
     jobs = jobs or min(8, os.cpu_count() or 4)
     with cf.ThreadPoolExecutor(jobs) as ex:
         objs = list(ex.map(_compile, srcs))
     tmp = LIB + ".tmp"
     _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs)
     os.replace(tmp, LIB)
+    build_cxx_check()
     if verbose:
         print("built", LIB)
     return LIB
+
+
+CXX_CHECK_SRC = os.path.join(ROOT, "tests", "cxx", "cxx_api_check.cpp")
+CXX_CHECK_BIN = os.path.join(ROOT, "tests", "cxx", "bin", "vpmc_cxx_check")
+
+
+def build_cxx_check():
+    """the C++ API driver used by the GPU tests (links libvpmc.so)"""
+    if not os.path.exists(CXX_CHECK_SRC):
+        return None
+    os.makedirs(os.path.dirname(CXX_CHECK_BIN), exist_ok=True)
+    libdir = os.path.dirname(LIB)
+    _run([HIPCC, "--offload-arch=" + ARCH, "-O2", "-std=c++17", "-I", INC, CXX_CHECK_SRC,
+          "-L", libdir, "-lvpmc", "-Wl,-rpath," + libdir, "-Wl,-rpath,$ORIGIN/../../../"
+          "vortexparticlemc_cuda_amd/lib", "-o", CXX_CHECK_BIN])
+    return CXX_CHECK_BIN
 
 
 if __name__ == "__main__":

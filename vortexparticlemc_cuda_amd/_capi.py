@@ -13,6 +13,8 @@ LIB_PATH = os.path.join(_HERE, "lib", "libvpmc.so")
 VPMC_OK = 0
 MODE_REFERENCE = 0
 MODE_CONSISTENT = 1
+ALGO_BINNED = 0
+ALGO_DIRECT = 1
 
 vp = C.c_void_p
 ci = C.c_int
@@ -52,7 +54,7 @@ class McDesc(C.Structure):
                 ("periodic", ci), ("seed", C.c_uint64), ("step", C.c_uint32),
                 ("sim", C.c_uint32), ("uniforms", vp), ("aux_draws", ci), ("n_owner", ci),
                 ("owner_xy", vp), ("set", SignSet * 2), ("vel_sign", vp),
-                ("eval_count", vp)]
+                ("eval_count", vp), ("algo", ci)]
 
 
 _lib = None
@@ -87,14 +89,18 @@ def lib():
                                    vp]
     L.vpmc_field_query.argtypes = [C.POINTER(Field), cf, ci, vp, vp, vp]
     L.vpmc_mc_estimate.argtypes = [C.POINTER(McDesc), vp]
+    L.vpmc_mc_binned_workspace_bytes.restype = csz
+    L.vpmc_mc_binned_workspace_bytes.argtypes = [C.POINTER(McDesc)]
+    L.vpmc_mc_estimate_binned.argtypes = [C.POINTER(McDesc), vp, csz, vp]
     L.vpmc_advect.argtypes = [ci, vp, vp, vp, cf, cf, cf, ci, vp]
     L.vpmc_step_workspace_bytes.restype = csz
-    L.vpmc_step_workspace_bytes.argtypes = [ci, ci, ci, ci, ci]
+    L.vpmc_step_workspace_bytes.argtypes = [C.POINTER(McDesc)]
     L.vpmc_step.argtypes = [C.POINTER(McDesc), ci, vp, vp, vp, vp, vp, cf, vp, csz, vp]
     L.vpmc_rbf_workspace_bytes.restype = csz
     L.vpmc_rbf_workspace_bytes.argtypes = [ci, ci, ci]
     L.vpmc_rbf_matvec.argtypes = [C.POINTER(Field), cf, vp, vp, vp]
     L.vpmc_rbf_matvec_f64.argtypes = [C.POINTER(Field), cf, vp, vp, vp]
+    L.vpmc_rbf_matvec_particles.argtypes = [ci, vp, vp, vp, cf, cf, cf, ci, vp, csz, vp]
     L.vpmc_cg_workspace_bytes.restype = csz
     L.vpmc_cg_workspace_bytes.argtypes = [ci, ci, ci]
     L.vpmc_cg_rbf.argtypes = [ci, vp, vp, vp, cf, cf, cf, ci, cd, ci, ci, C.POINTER(ci),
@@ -117,7 +123,8 @@ EXPORTS = [
     "vpmc_hash_workspace_bytes", "vpmc_fill_cells", "vpmc_counting_sort",
     "vpmc_subsets_workspace_bytes", "vpmc_build_subsets", "vpmc_build_guide", "vpmc_sample_idx",
     "vpmc_query_view", "vpmc_field_grid", "vpmc_field_workspace_bytes", "vpmc_field_build",
-    "vpmc_field_query", "vpmc_mc_estimate", "vpmc_advect", "vpmc_step_workspace_bytes",
-    "vpmc_step", "vpmc_rbf_workspace_bytes", "vpmc_rbf_matvec", "vpmc_rbf_matvec_f64",
+    "vpmc_field_query", "vpmc_mc_estimate", "vpmc_mc_binned_workspace_bytes",
+    "vpmc_mc_estimate_binned", "vpmc_advect", "vpmc_step_workspace_bytes",
+    "vpmc_step", "vpmc_rbf_workspace_bytes", "vpmc_rbf_matvec", "vpmc_rbf_matvec_f64", "vpmc_rbf_matvec_particles",
     "vpmc_cg_rbf", "vpmc_cg_workspace_bytes", "vpmc_direct_sum_eps", "vpmc_direct_sum_blob",
 ]

@@ -4,9 +4,12 @@
 // src/cuda/cg_rbf_init.cu:11-31) is applied matrix-free on the fine-grid
 // field.  CG runs on vectors kept in the field's cell order (A is permutation
 // invariant), so the weights of a cell's neighbours are read contiguously.
-// Dot products accumulate in FP64 with a fixed reduction tree; with
-// refinement, an outer loop recomputes the residual with an FP64 matvec and
-// solves for the correction in FP32 (mixed-precision iterative refinement).
+// Dot products accumulate in FP64 with a fixed reduction tree.  Without
+// refinement the solve is the reference's FP32 CG; with it, MINRES runs on
+// FP64 vectors and an FP64 matvec from zero.  (FP32 corrections on an FP64
+// residual stall near 1e-7: the residual of a rounded iterate spreads over
+// every lattice mode, and the indefinite Gram matrix has eigenvalues on both
+// sides of zero, so each correction would need thousands of iterations.)
 #include <math.h>
 
 #include <vector>
@@ -114,18 +117,10 @@ __global__ void k_xpby(int n, const float *__restrict__ r, float beta, float *__
     if (i < n) p[i] = r[i] + beta * p[i];
 }
 __global__ void k_res64(int n, const double *__restrict__ b, const double *__restrict__ Aw,
-                        double *__restrict__ r64, float *__restrict__ r32)
+                        double *__restrict__ r64)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    double v = b[i] - Aw[i];
-    r64[i] = v;
-    r32[i] = (float)v;
-}
-__global__ void k_add_corr(int n, const float *__restrict__ d, double *__restrict__ w64)
-{
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) w64[i] += (double)d[i];
+    if (i < n) r64[i] = b[i] - Aw[i];
 }
 __global__ void k_f2d(int n, const float *__restrict__ x, double *__restrict__ y)
 {
@@ -136,11 +131,15 @@ __global__ void k_f2d(int n, const float *__restrict__ x, double *__restrict__ y
 struct CgBufs {
     float *pts;
     int *cell_start, *order;
-    float *bs, *w, *r, *p, *Ap, *d;
-    double *b64, *w64, *r64, *Aw64, *part, *scal;
+    float *bs, *w, *r, *p, *Ap;          // FP32 CG
+    double *b64, *w64, *r64, *Aw64;      // FP64 right-hand side, iterate, residual
+    double *m[7];                        // FP64 MINRES vectors
+    double *part, *scal;
     void *fws;
     size_t fws_bytes;
 };
+
+constexpr int CG_F32_VECS = 5, CG_F64_VECS = 4 + 7;
 
 static int carve_cg(int n, int nx, int ny, void *ws, size_t bytes, CgBufs *B)
 {
@@ -149,23 +148,17 @@ static int carve_cg(int n, int nx, int ny, void *ws, size_t bytes, CgBufs *B)
     B->pts = cv.take<float>(4 * m);
     B->cell_start = cv.take<int>((size_t)nx * ny + 1);
     B->order = cv.take<int>(m);
-    B->bs = cv.take<float>(m);
-    B->w = cv.take<float>(m);
-    B->r = cv.take<float>(m);
-    B->p = cv.take<float>(m);
-    B->Ap = cv.take<float>(m);
-    B->d = cv.take<float>(m);
-    B->b64 = cv.take<double>(m);
-    B->w64 = cv.take<double>(m);
-    B->r64 = cv.take<double>(m);
-    B->Aw64 = cv.take<double>(m);
+    float **f32[CG_F32_VECS] = {&B->bs, &B->w, &B->r, &B->p, &B->Ap};
+    double **f64[CG_F64_VECS] = {&B->b64, &B->w64, &B->r64, &B->Aw64, &B->m[0], &B->m[1],
+                                 &B->m[2], &B->m[3], &B->m[4], &B->m[5], &B->m[6]};
+    bool ok = B->pts && B->cell_start && B->order;
+    for (float **q : f32) ok = ok && (*q = cv.take<float>(m)) != nullptr;
+    for (double **q : f64) ok = ok && (*q = cv.take<double>(m)) != nullptr;
     B->part = cv.take<double>(DOT_BLOCKS);
     B->scal = cv.take<double>(4);
     B->fws_bytes = vpmc_field_workspace_bytes(n, nx, ny);
     B->fws = cv.take<char>(B->fws_bytes);
-    if (!B->pts || !B->cell_start || !B->order || !B->bs || !B->w || !B->r || !B->p || !B->Ap ||
-        !B->d || !B->b64 || !B->w64 || !B->r64 || !B->Aw64 || !B->part || !B->scal || !B->fws)
-        return VPMC_ERR_WORKSPACE;
+    if (!ok || !B->part || !B->scal || !B->fws) return VPMC_ERR_WORKSPACE;
     return VPMC_OK;
 }
 
@@ -222,6 +215,100 @@ static int cg_fp32(const FieldD &f, float inv_hw, float sig2, float diag, int n,
     return VPMC_OK;
 }
 
+template <class T>
+__global__ void k_scale(int n, const T *__restrict__ x, T a, T *__restrict__ y)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) y[i] = x[i] * a;
+}
+template <class T>
+__global__ void k_axpy(int n, T a, const T *__restrict__ x, T *__restrict__ y)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) y[i] += a * x[i];
+}
+template <class T>
+__global__ void k_minres_dir(int n, const T *__restrict__ v, const T *__restrict__ w1,
+                             const T *__restrict__ w2, T oldeps, T delta, T inv_gamma, T phi,
+                             T *__restrict__ w, T *__restrict__ x)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    T wn = (v[i] - oldeps * w1[i] - delta * w2[i]) * inv_gamma;
+    w[i] = wn;
+    x[i] += phi * wn;
+}
+
+// MINRES (Paige & Saunders, SIAM J. Numer. Anal. 12, 1975) on A x = rhs from
+// x = 0, FP64 vectors and FP64 matvec: the Gram matrix is symmetric but
+// indefinite, where CG can meet p.Ap <= 0.  Stops when the residual estimate
+// phibar < tol * ||rhs|| or after max_iter iterations.
+static int minres_f64(const FieldD &f, float h_w, int n, const double *rhs, double *x,
+                      CgBufs &B, double tol, int max_iter, hipStream_t s, int *it)
+{
+    unsigned gb = (unsigned)((n + 255) / 256);
+    *it = 0;
+    double *r1 = B.m[0], *r2 = B.m[1], *y = B.m[2], *v = B.m[3], *w = B.m[4], *w1 = B.m[5],
+           *w2 = B.m[6];
+    hipLaunchKernelGGL(k_scale<double>, dim3(gb), dim3(256), 0, s, n, rhs, 0.0, x);
+    hipLaunchKernelGGL(k_scale<double>, dim3(gb), dim3(256), 0, s, n, rhs, 0.0, w);
+    hipLaunchKernelGGL(k_scale<double>, dim3(gb), dim3(256), 0, s, n, rhs, 0.0, w1);
+    hipLaunchKernelGGL(k_scale<double>, dim3(gb), dim3(256), 0, s, n, rhs, 0.0, w2);
+    hipLaunchKernelGGL(k_scale<double>, dim3(gb), dim3(256), 0, s, n, rhs, 1.0, r1);
+    hipLaunchKernelGGL(k_scale<double>, dim3(gb), dim3(256), 0, s, n, rhs, 1.0, r2);
+    VPMC_CHECK_LAUNCH();
+    double b2;
+    VPMC_TRY(dot_host(rhs, rhs, n, B, s, &b2));
+    const double beta1 = sqrt(b2);
+    if (beta1 == 0.0) return VPMC_OK;
+    double oldb = 0.0, beta = beta1, dbar = 0.0, epsln = 0.0, phibar = beta1, cs = -1.0, sn = 0.0;
+    // r1, r2: the last two Lanczos residuals; y: free buffer the matvec fills
+    for (int itn = 1; itn <= max_iter; ++itn) {
+        hipLaunchKernelGGL(k_scale<double>, dim3(gb), dim3(256), 0, s, n, (const double *)r2,
+                           1.0 / beta, v);
+        hipLaunchKernelGGL(k_rbf_matvec_f64, dim3(gb), dim3(256), 0, s, f, h_w, n,
+                           (const double *)v, y);
+        if (itn >= 2)
+            hipLaunchKernelGGL(k_axpy<double>, dim3(gb), dim3(256), 0, s, n, -beta / oldb,
+                               (const double *)r1, y);
+        VPMC_CHECK_LAUNCH();
+        double alfa;
+        VPMC_TRY(dot_host(v, y, n, B, s, &alfa));
+        hipLaunchKernelGGL(k_axpy<double>, dim3(gb), dim3(256), 0, s, n, -alfa / beta,
+                           (const double *)r2, y);
+        VPMC_CHECK_LAUNCH();
+        double *t = r1;
+        r1 = r2;
+        r2 = y;
+        y = t;
+        oldb = beta;
+        double bb;
+        VPMC_TRY(dot_host(r2, r2, n, B, s, &bb));
+        beta = sqrt(bb);
+        double oldeps = epsln;
+        double delta = cs * dbar + sn * alfa;
+        double gbar = sn * dbar - cs * alfa;
+        epsln = sn * beta;
+        dbar = -cs * beta;
+        double gamma = fmax(hypot(gbar, beta), 1e-300);
+        cs = gbar / gamma;
+        sn = beta / gamma;
+        double phi = cs * phibar;
+        phibar = sn * phibar;
+        t = w1;
+        w1 = w2;
+        w2 = w;
+        w = t;
+        hipLaunchKernelGGL(k_minres_dir<double>, dim3(gb), dim3(256), 0, s, n,
+                           (const double *)v, (const double *)w1, (const double *)w2, oldeps,
+                           delta, 1.0 / gamma, phi, w, x);
+        VPMC_CHECK_LAUNCH();
+        *it = itn;
+        if (phibar < tol * beta1 || beta == 0.0) break;
+    }
+    return VPMC_OK;
+}
+
 }  // namespace vpmc
 
 using namespace vpmc;
@@ -233,8 +320,9 @@ size_t vpmc_rbf_workspace_bytes(int n, int nx, int ny) { return vpmc_cg_workspac
 size_t vpmc_cg_workspace_bytes(int n, int nx, int ny)
 {
     size_t m = (size_t)(n > 0 ? n : 1);
-    return align256(16 * m) + align256(((size_t)nx * ny + 1) * 4) + 7 * align256(4 * m) +
-           4 * align256(8 * m) + align256(8 * DOT_BLOCKS) + align256(32) +
+    return align256(16 * m) + align256(((size_t)nx * ny + 1) * 4) + align256(4 * m) +
+           CG_F32_VECS * align256(4 * m) + CG_F64_VECS * align256(8 * m) +
+           align256(8 * DOT_BLOCKS) + align256(32) +
            vpmc_field_workspace_bytes(n, nx, ny) + 4096;
 }
 
@@ -260,6 +348,47 @@ int vpmc_rbf_matvec_f64(const vpmc_field *f, float h_w, const double *x, double 
     FieldD d{(const float4 *)f->pts, f->cell_start, g};
     hipLaunchKernelGGL(k_rbf_matvec_f64, dim3((f->n + 255) / 256), dim3(256), 0, as_stream(stream),
                        d, h_w, f->n, x, y);
+    VPMC_CHECK_LAUNCH();
+    return VPMC_OK;
+}
+
+__global__ void k_gather_sorted(int n, const int *__restrict__ order, const float *__restrict__ x,
+                                float *__restrict__ xs)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) xs[t] = x[order[t]];
+}
+__global__ void k_scatter_sorted(int n, const int *__restrict__ order,
+                                 const float *__restrict__ ys, float *__restrict__ y)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) y[order[t]] = ys[t];
+}
+
+int vpmc_rbf_matvec_particles(int n, const float *pos_xy, const float *x, float *y, float h_w,
+                              float Lx, float Ly, int cells_per_hw, void *workspace,
+                              size_t workspace_bytes, void *stream)
+{
+    if (n < 0 || !(h_w > 0.f)) return VPMC_ERR_ARG;
+    if (n == 0) return VPMC_OK;
+    if (!pos_xy || !x || !y) return VPMC_ERR_ARG;
+    hipStream_t s = as_stream(stream);
+    int nx, ny;
+    VPMC_TRY(vpmc_field_grid(h_w, Lx, Ly, cells_per_hw, &nx, &ny));
+    CgBufs B;
+    VPMC_TRY(carve_cg(n, nx, ny, workspace, workspace_bytes, &B));
+    VPMC_TRY(field_build_impl(nx, ny, Lx, Ly, n, nullptr, nullptr, pos_xy, nullptr, 0, B.pts,
+                              B.cell_start, B.order, nullptr, B.fws, B.fws_bytes, s));
+    vpmc_field fh{nx, ny, Lx, Ly, n, B.pts, B.cell_start, B.order};
+    FieldGeom g;
+    VPMC_TRY(field_geom(&fh, h_w, &g));
+    FieldD f{(const float4 *)B.pts, B.cell_start, g};
+    unsigned gb = (unsigned)((n + 255) / 256);
+    hipLaunchKernelGGL(k_gather_sorted, dim3(gb), dim3(256), 0, s, n, (const int *)B.order, x, B.p);
+    hipLaunchKernelGGL(k_rbf_matvec, dim3(gb), dim3(256), 0, s, f, 1.f / h_w,
+                       40.f / (7.f * PI_F * h_w * h_w), n, (const float *)B.p, B.Ap);
+    hipLaunchKernelGGL(k_scatter_sorted, dim3(gb), dim3(256), 0, s, n, (const int *)B.order,
+                       (const float *)B.Ap, y);
     VPMC_CHECK_LAUNCH();
     return VPMC_OK;
 }
@@ -304,34 +433,14 @@ int vpmc_cg_rbf(int n, const float *pos_xy, const float *b, float *w_out, float 
         hipLaunchKernelGGL(k_f2d, dim3(gb), dim3(256), 0, s, n, (const float *)B.w, B.w64);
         VPMC_CHECK_LAUNCH();
     } else {
-        // w64 = 0; outer: r = b - A64 w64 ; solve A d = r in FP32 ; w64 += d
-        hipLaunchKernelGGL(k_scale_f, dim3(gb), dim3(256), 0, s, n, (const float *)B.bs, 0.f, B.d);
-        hipLaunchKernelGGL(k_f2d, dim3(gb), dim3(256), 0, s, n, (const float *)B.d, B.w64);
-        VPMC_CHECK_LAUNCH();
-        for (int outer = 0; outer < 30; ++outer) {
-            hipLaunchKernelGGL(k_rbf_matvec_f64, dim3(gb), dim3(256), 0, s, f, h_w, n,
-                               (const double *)B.w64, B.Aw64);
-            hipLaunchKernelGGL(k_res64, dim3(gb), dim3(256), 0, s, n, (const double *)B.b64,
-                               (const double *)B.Aw64, B.r64, B.bs);
-            VPMC_CHECK_LAUNCH();
-            double r2;
-            VPMC_TRY(dot_host(B.r64, B.r64, n, B, s, &r2));
-            rr = bn > 0 ? sqrt(r2) / bn : 0.0;
-            if (rr < tol || total_it >= max_iter) break;
-            int it;
-            VPMC_TRY(cg_fp32(f, inv_hw, sig2, diag, n, B.bs, sqrt(r2), B.d, B, 1e-4,
-                             max_iter - total_it, s, &it, &broke));
-            total_it += it;
-            hipLaunchKernelGGL(k_add_corr, dim3(gb), dim3(256), 0, s, n, (const float *)B.d, B.w64);
-            VPMC_CHECK_LAUNCH();
-            if (broke) break;
-        }
+        VPMC_TRY(minres_f64(f, h_w, n, (const double *)B.b64, B.w64, B, tol, max_iter, s,
+                            &total_it));
     }
-    // true FP64 residual of the returned strengths
+    // FP64 residual of the FP64 iterate (the strengths are returned rounded to FP32)
     hipLaunchKernelGGL(k_rbf_matvec_f64, dim3(gb), dim3(256), 0, s, f, h_w, n,
                        (const double *)B.w64, B.Aw64);
     hipLaunchKernelGGL(k_res64, dim3(gb), dim3(256), 0, s, n, (const double *)B.b64,
-                       (const double *)B.Aw64, B.r64, B.d);
+                       (const double *)B.Aw64, B.r64);
     hipLaunchKernelGGL(k_permute_out, dim3(gb), dim3(256), 0, s, n, (const int *)B.order,
                        (const double *)B.w64, w_out);
     VPMC_CHECK_LAUNCH();
@@ -342,9 +451,10 @@ int vpmc_cg_rbf(int n, const float *pos_xy, const float *b, float *w_out, float 
     if (rel_res) *rel_res = rr;
     if (broke) return VPMC_ERR_CG_BREAKDOWN;
     if (!(rr < tol) && tol > 0.0) {
-        // the FP32 recursion may stop on its own residual before the FP64
-        // residual of the cast result meets tol
-        if (total_it >= max_iter) return VPMC_ERR_NOT_CONVERGED;
+        // with refinement the FP64 residual is the criterion; without it the
+        // FP32 recursion stops on its own residual, which the FP64 residual of
+        // the returned strengths may exceed slightly -- only max_iter counts
+        if (refine || total_it >= max_iter) return VPMC_ERR_NOT_CONVERGED;
     }
     return VPMC_OK;
 }

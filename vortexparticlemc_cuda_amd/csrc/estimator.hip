@@ -12,136 +12,11 @@
 
 #include <mutex>
 
-#include "field_query.hpp"
+#include "mc_common.hpp"
 
 namespace vpmc {
 
 constexpr int MC_THREADS = 256;
-constexpr int KPER_TERMS_F32 = 16;
-
-struct SetD {
-    int n;
-    const int *count_dev;
-    const double *cdf;
-    const double *total_dev;
-    const int *guide;
-    int glog2;
-    const float2 *set_xy;
-    FieldD field;
-};
-
-struct McD {
-    int N1, N2;
-    float h_w, Lx, Ly, inv_hw, sig2;
-    float w1, w2, inv_tot;
-    float R_ref, alpha_ref;   // reference disk: sqrt(Lx^2 + Ly^2)
-    float Rc, alpha_c;        // consistent disk: circumradius of the min-image cell
-    float inv_L, inv_2piL;
-    uint32_t key0, key1, step, sim;
-    const float *U;
-    int b1, b2;
-    int n_owner;
-    const float2 *owner;
-    SetD set[2];
-    float2 *vel_sign;
-    unsigned long long *eval_count;
-    int nterms;
-    float kc[KPER_TERMS_F32];
-};
-
-// Inverse of the cubic spline's radial mass fraction (FP64 bracketed
-// Newton); the oracle runs the same iteration.  Inner piece solves
-// q^2/2 - 1.5 q^4 + 1.2 q^5 = 7u/80 on [0, 1/2]; outer piece, in t = 1-q,
-// t^4/2 - 0.4 t^5 = 7(1-u)/80 on [0, 1/2].
-__device__ double mass_inverse(double u)
-{
-    if (u <= 0.0) return 0.0;
-    if (u >= 1.0) return 1.0;
-    const double u_half = 11.0 / 14.0;
-    if (u <= u_half) {
-        double a = 7.0 * u / 80.0, lo = 0.0, hi = 0.5;
-        double x = sqrt(2.0 * a);
-        if (x > hi) x = hi;
-        for (int it = 0; it < 40; ++it) {
-            double x2 = x * x;
-            double f = 0.5 * x2 - 1.5 * x2 * x2 + 1.2 * x2 * x2 * x - a;
-            if (f > 0.0) hi = x; else lo = x;
-            double d = x - 6.0 * x2 * x + 6.0 * x2 * x2;
-            double xn = (d > 0.0) ? x - f / d : 0.5 * (lo + hi);
-            if (!(xn >= lo && xn <= hi)) xn = 0.5 * (lo + hi);
-            if (xn == x) break;
-            x = xn;
-        }
-        return x;
-    }
-    double b = 7.0 * (1.0 - u) / 80.0, lo = 0.0, hi = 0.5;
-    double t = sqrt(sqrt(2.0 * b));
-    if (t > hi) t = hi;
-    for (int it = 0; it < 40; ++it) {
-        double t2 = t * t;
-        double f = 0.5 * t2 * t2 - 0.4 * t2 * t2 * t - b;
-        if (f > 0.0) hi = t; else lo = t;
-        double d = 2.0 * t2 * t - 2.0 * t2 * t2;
-        double tn = (d > 0.0) ? t - f / d : 0.5 * (lo + hi);
-        if (!(tn >= lo && tn <= hi)) tn = 0.5 * (lo + hi);
-        if (tn == t) break;
-        t = tn;
-    }
-    return 1.0 - t;
-}
-
-__device__ __forceinline__ void draws(const McD &d, int owner, int si, int branch, int k,
-                                      float u[3])
-{
-    if (d.U) {
-        size_t per = (size_t)d.N1 * d.b1 + (size_t)d.N2 * d.b2;
-        const float *base = d.U + ((size_t)owner * 2 + si) * per;
-        if (branch == 1) {
-            const float *q = base + (size_t)k * d.b1;
-            u[0] = q[0];
-            u[1] = d.b1 > 1 ? q[1] : 0.f;
-            u[2] = d.b1 > 2 ? q[2] : 0.f;
-        } else {
-            const float *q = base + (size_t)d.N1 * d.b1 + (size_t)k * d.b2;
-            u[0] = q[0];
-            u[1] = q[1];
-            u[2] = 0.f;
-        }
-    } else {
-        uint4 c = make_uint4((uint32_t)k, (uint32_t)owner, d.step,
-                             (d.sim << 8) | ((uint32_t)si << 7) | ((uint32_t)branch << 4));
-        uint4 x = philox10(c, make_uint2(d.key0, d.key1));
-        u[0] = u01(x.x);
-        u[1] = u01(x.y);
-        u[2] = u01(x.z);
-    }
-}
-
-__device__ __forceinline__ int sample_idx(float u, const SetD &S, int n)
-{
-    return guided_upper_bound(u, S.cdf, S.guide, S.glog2, n);
-}
-
-// smooth part of the doubly periodic kernel on the square lattice L(Z+iZ):
-// u - i v = (i / (2 pi L)) (sum_m c_m s^(4m-1) + pi conj(s)),  s = r / L
-__device__ __forceinline__ void kper_smooth(const McD &d, float rx, float ry, float &kx,
-                                            float &ky)
-{
-    float sr = rx * d.inv_L, si = ry * d.inv_L;
-    float s2r = sr * sr - si * si, s2i = 2.f * sr * si;
-    float wr = s2r * s2r - s2i * s2i, wi = 2.f * s2r * s2i;
-    float hr = d.kc[d.nterms - 1], hi = 0.f;
-    for (int m = d.nterms - 2; m >= 0; --m) {
-        float t = hr * wr - hi * wi + d.kc[m];
-        hi = hr * wi + hi * wr;
-        hr = t;
-    }
-    float s3r = s2r * sr - s2i * si, s3i = s2r * si + s2i * sr;
-    float Ar = hr * s3r - hi * s3i + PI_F * sr;
-    float Ai = hr * s3i + hi * s3r - PI_F * si;
-    kx = -Ai * d.inv_2piL;
-    ky = -Ar * d.inv_2piL;
-}
 
 template <bool COUNT, int MODE, bool PER>
 __global__ void __launch_bounds__(MC_THREADS) k_mc_estimate(McD d)
@@ -158,97 +33,10 @@ __global__ void __launch_bounds__(MC_THREADS) k_mc_estimate(McD d)
         if (MODE == VPMC_MODE_REFERENCE && PER) cum *= 9.f;
         const int ns = d.N1 + d.N2;
         for (int k = threadIdx.x; k < ns; k += MC_THREADS) {
-            const bool b1 = k < d.N1;
-            float u[3];
-            draws(d, owner, si, b1 ? 1 : 2, b1 ? k : k - d.N1, u);
-            float spx, spy, rx = 0.f, ry = 0.f;
-            bool live = true;
-            if (MODE == VPMC_MODE_REFERENCE) {
-                if (b1) {
-                    int j = sample_idx(u[0], S, n_s);
-                    float2 xj = S.set_xy[j];
-                    spx = xj.x;
-                    spy = xj.y;
-                    if (PER) {   // include/monte_carlo_bs.hpp:92-102
-                        float ddx = 0.f, ddy = 0.f;
-                        if (u[1] < 1.f / 3.f) ddx = -1.f; else if (u[1] > 2.f / 3.f) ddx = 1.f;
-                        if (u[2] < 1.f / 3.f) ddy = -1.f; else if (u[2] > 2.f / 3.f) ddy = 1.f;
-                        spx = __fadd_rn(spx, __fmul_rn(ddx, d.Lx));
-                        spy = __fadd_rn(spy, __fmul_rn(ddy, d.Ly));
-                    }
-                } else {   // src/cpu/monte_carlo_sampling.cpp:35-45
-                    float phi = __fmul_rn(2.f * PI_F, u[0]);
-                    float r = __fmul_rn(d.R_ref, sqrtf(u[1]));
-                    float s, c;
-                    sincosf(phi, &s, &c);
-                    spx = __fadd_rn(dst.x, __fmul_rn(r, c));
-                    spy = __fadd_rn(dst.y, __fmul_rn(r, s));
-                }
-            } else {
-                if (b1) {
-                    int j = sample_idx(u[0], S, n_s);
-                    float2 xj = S.set_xy[j];
-                    float rho = (float)mass_inverse((double)u[1]) * d.h_w;
-                    float s, c;
-                    sincosf(2.f * PI_F * u[2], &s, &c);
-                    spx = xj.x + rho * c;
-                    spy = xj.y + rho * s;
-                    rx = dst.x - spx;
-                    ry = dst.y - spy;
-                    if (PER) {
-                        if (rx > 0.5f * d.Lx) rx -= d.Lx; else if (rx < -0.5f * d.Lx) rx += d.Lx;
-                        if (ry > 0.5f * d.Ly) ry -= d.Ly; else if (ry < -0.5f * d.Ly) ry += d.Ly;
-                    }
-                } else {
-                    float rho = d.Rc * u[1];
-                    float s, c;
-                    sincosf(2.f * PI_F * u[0], &s, &c);
-                    float cx = rho * c, cy = rho * s;
-                    spx = dst.x + cx;
-                    spy = dst.y + cy;
-                    rx = -cx;
-                    ry = -cy;
-                    if (PER && (fabsf(rx) > 0.5f * d.Lx || fabsf(ry) > 0.5f * d.Ly)) live = false;
-                }
-            }
-            if (!live) continue;
-            float px = spx, py = spy;
-            if (PER) {
-                px = px - floorf(px / d.Lx) * d.Lx;
-                py = py - floorf(py / d.Ly) * d.Ly;
-            } else if (px < 0.f || px > d.Lx || py < 0.f || py > d.Ly) {
-                continue;
-            }
-            float om = d.sig2 * field_sum<COUNT, PER>(S.field, px, py, d.inv_hw, ec);
-            if (!(om > 0.f)) continue;
-            if (MODE == VPMC_MODE_REFERENCE) {
-                // mis_contrib, src/cpu/monte_carlo_bs.cpp:11-26
-                float ax = dst.x - spx, ay = dst.y - spy;
-                float r2 = ax * ax + ay * ay;
-                float rn = fmaxf(sqrtf(r2), 1e-6f);
-                float inv_r2 = 1.f / (r2 + 1e-8f);
-                float kx = -ay * inv_r2 / (2.f * PI_F);
-                float ky = ax * inv_r2 / (2.f * PI_F);
-                float denom = d.w1 * om / cum + d.w2 * d.alpha_ref / rn;
-                float g = om / denom;
-                sx += kx * g;
-                sy += ky * g;
-            } else {
-                float rn = sqrtf(rx * rx + ry * ry);
-                if (!(rn > 0.f)) continue;
-                // omega / (w1 p1 + w2 p2) with p1 = omega/Omega, p2 = alpha/r
-                float g = om * rn / (d.w1 * om * rn / Om + d.w2 * d.alpha_c);
-                float f = g / (2.f * PI_F * rn * rn);
-                float kx = -ry * f, ky = rx * f;
-                if (PER) {
-                    float ksx, ksy;
-                    kper_smooth(d, rx, ry, ksx, ksy);
-                    kx += ksx * g;
-                    ky += ksy * g;
-                }
-                sx += kx;
-                sy += ky;
-            }
+            float qx, qy, rx, ry;
+            if (!gen_sample<MODE, PER>(d, S, n_s, owner, si, k, dst, qx, qy, rx, ry)) continue;
+            float om = d.sig2 * field_sum<COUNT, PER>(S.field, qx, qy, d.inv_hw, ec);
+            mis_contrib<MODE, PER>(d, om, rx, ry, Om, cum, sx, sy);
         }
     }
     // fixed-order workgroup reduction
@@ -350,7 +138,7 @@ static const float *kper_coeffs_f32()
     return kc;
 }
 
-static int make_mcd(const vpmc_mc_desc *in, McD *d)
+int make_mcd(const vpmc_mc_desc *in, McD *d)
 {
     if (!in || in->N1 < 0 || in->N2 < 0 || in->N1 + in->N2 <= 0 || in->n_owner < 0 ||
         !(in->h_w > 0.f) || !(in->Lx > 0.f) || !(in->Ly > 0.f))
@@ -472,11 +260,16 @@ int vpmc_advect(int n, float *pos_xy, const float *vel_sign, float *vel_xy, floa
     return VPMC_OK;
 }
 
-size_t vpmc_step_workspace_bytes(int n_total, int n_pos, int n_neg, int nx, int ny)
+size_t vpmc_step_workspace_bytes(const vpmc_mc_desc *d)
 {
-    int m = n_pos > n_neg ? n_pos : n_neg;
-    (void)n_total;
-    return vpmc_field_workspace_bytes(m, nx, ny);
+    if (!d) return 0;
+    int m = d->set[0].n > d->set[1].n ? d->set[0].n : d->set[1].n;
+    size_t b = vpmc_field_workspace_bytes(m, d->set[0].field.nx, d->set[0].field.ny);
+    if (d->algo == VPMC_ALGO_BINNED) {   // field build and estimate use it in turn
+        size_t e = vpmc_mc_binned_workspace_bytes(d);
+        if (e > b) b = e;
+    }
+    return b;
 }
 
 int vpmc_step(vpmc_mc_desc *d, int n_total, float *pos_xy, const float *omega,
@@ -494,7 +287,10 @@ int vpmc_step(vpmc_mc_desc *d, int n_total, float *pos_xy, const float *omega,
                                   const_cast<int *>(S.field.order),
                                   const_cast<float *>(S.set_xy), workspace, workspace_bytes, s));
     }
-    VPMC_TRY(mc_estimate_impl(d, s));
+    if (d->algo == VPMC_ALGO_BINNED)
+        VPMC_TRY(vpmc_mc_estimate_binned(d, workspace, workspace_bytes, stream));
+    else
+        VPMC_TRY(mc_estimate_impl(d, s));
     return vpmc_advect(n_total, pos_xy, d->vel_sign, vel_xy, dt, d->Lx, d->Ly, d->periodic,
                        stream);
 }

@@ -121,6 +121,10 @@ __global__ void k_field_query(FieldD f, float inv_hw, float sig2, int npts,
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= npts) return;
     float2 p = q[i];
+    if (!(isfinite(p.x) && isfinite(p.y))) {
+        out[i] = 0.f;
+        return;
+    }
     // wrap into the box as the reference query does (vorticity_query.cuh:20-22)
     p.x = p.x - floorf(p.x / f.g.Lx) * f.g.Lx;
     p.y = p.y - floorf(p.y / f.g.Ly) * f.g.Ly;

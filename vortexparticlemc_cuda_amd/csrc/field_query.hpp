@@ -7,7 +7,7 @@
 // pieces by the periodic wrap; each piece is streamed as float4 records.
 //
 // The cubic spline of include/utilities.hpp:7-17 is evaluated branch-free:
-//   W(q) = sig2 * (2 t^3 - 8 u^3),  t = sat(1 - q),  u = sat(1/2 - q),
+//   W(q) = sig2 * (2 t^3 - 8 u^3),  t = max(1 - q, 0),  u = max(1/2 - q, 0),
 // which equals sig2 (1 - 6 q^2 + 6 q^3) for q <= 1/2, 2 sig2 (1-q)^3 for
 // 1/2 < q < 1 and 0 beyond; the two cubes are accumulated separately.
 #pragma once
@@ -35,10 +35,10 @@ __device__ __forceinline__ void seg_accum(const float4 *__restrict__ pts, int be
         float dx = c.x - qx, dy = c.y - qy;
         float r2 = fmaf(dx, dx, dy * dy);
         float q = __builtin_amdgcn_sqrtf(r2) * inv_hw;
-        float t = __saturatef(1.f - q);
-        float u = __saturatef(0.5f - q);
-        a1 = fmaf(c.w * t, t * t, a1);
-        a2 = fmaf(c.w * u, u * u, a2);
+        float t = fmaxf(1.f - q, 0.f);   // q >= 0: the upper clamp never binds
+        float u = fmaxf(0.5f - q, 0.f);
+        a1 = fmaf(c.z * t, t * t, a1);   // record = (x, y, |w|, 0)
+        a2 = fmaf(c.z * u, u * u, a2);
         if (COUNT) {
             ec.in_support += (q < 1.f) ? 1u : 0u;
             ec.visited += 1u;
@@ -170,8 +170,8 @@ __device__ __forceinline__ float field_sum_wts(const FieldD &f, const float *__r
                 float dx = c.x - qx, dy = c.y - sy[p];
                 float r2 = fmaf(dx, dx, dy * dy);
                 float q = __builtin_amdgcn_sqrtf(r2) * inv_hw;
-                float t = __saturatef(1.f - q);
-                float u = __saturatef(0.5f - q);
+                float t = fmaxf(1.f - q, 0.f);   // q >= 0: the upper clamp never binds
+                float u = fmaxf(0.5f - q, 0.f);
                 float wv = wts[k];
                 a1 = fmaf(wv * t, t * t, a1);
                 a2 = fmaf(wv * u, u * u, a2);

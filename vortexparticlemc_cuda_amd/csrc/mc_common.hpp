@@ -1,0 +1,245 @@
+// mc_common.hpp -- sample generation and MIS contribution shared by the
+// direct and the binned estimator (device code), and the kernel-argument
+// block built from vpmc_mc_desc.
+#pragma once
+
+#include "field_query.hpp"
+
+namespace vpmc {
+
+constexpr int KPER_TERMS_F32 = 16;
+constexpr int MAX_COL_EXT = 129;   // columns of the binned neighbour stencil
+
+struct SetD {
+    int n;
+    const int *count_dev;
+    const double *cdf;
+    const double *total_dev;
+    const int *guide;
+    int glog2;
+    const float2 *set_xy;
+    FieldD field;
+};
+
+struct McD {
+    int N1, N2;
+    float h_w, Lx, Ly, inv_hw, sig2;
+    float w1, w2, inv_tot;
+    float R_ref, alpha_ref;   // reference disk: sqrt(Lx^2 + Ly^2)
+    float Rc, alpha_c;        // consistent disk: circumradius of the min-image cell
+    float inv_L, inv_2piL;
+    uint32_t key0, key1, step, sim;
+    const float *U;
+    int b1, b2;
+    int n_owner;
+    const float2 *owner;
+    SetD set[2];
+    float2 *vel_sign;
+    unsigned long long *eval_count;
+    int nterms;
+    float kc[KPER_TERMS_F32];
+};
+
+int make_mcd(const vpmc_mc_desc *in, McD *d);
+
+// Inverse of the cubic spline's radial mass fraction (FP64 bracketed
+// Newton); the oracle runs the same iteration.  Inner piece solves
+// q^2/2 - 1.5 q^4 + 1.2 q^5 = 7u/80 on [0, 1/2]; outer piece, in t = 1-q,
+// t^4/2 - 0.4 t^5 = 7(1-u)/80 on [0, 1/2].
+__device__ inline double mass_inverse(double u)
+{
+    if (u <= 0.0) return 0.0;
+    if (u >= 1.0) return 1.0;
+    const double u_half = 11.0 / 14.0;
+    if (u <= u_half) {
+        double a = 7.0 * u / 80.0, lo = 0.0, hi = 0.5;
+        double x = sqrt(2.0 * a);
+        if (x > hi) x = hi;
+        for (int it = 0; it < 40; ++it) {
+            double x2 = x * x;
+            double f = 0.5 * x2 - 1.5 * x2 * x2 + 1.2 * x2 * x2 * x - a;
+            if (f > 0.0) hi = x; else lo = x;
+            double d = x - 6.0 * x2 * x + 6.0 * x2 * x2;
+            double xn = (d > 0.0) ? x - f / d : 0.5 * (lo + hi);
+            if (!(xn >= lo && xn <= hi)) xn = 0.5 * (lo + hi);
+            if (xn == x) break;
+            x = xn;
+        }
+        return x;
+    }
+    double b = 7.0 * (1.0 - u) / 80.0, lo = 0.0, hi = 0.5;
+    double t = sqrt(sqrt(2.0 * b));
+    if (t > hi) t = hi;
+    for (int it = 0; it < 40; ++it) {
+        double t2 = t * t;
+        double f = 0.5 * t2 * t2 - 0.4 * t2 * t2 * t - b;
+        if (f > 0.0) hi = t; else lo = t;
+        double d = 2.0 * t2 * t - 2.0 * t2 * t2;
+        double tn = (d > 0.0) ? t - f / d : 0.5 * (lo + hi);
+        if (!(tn >= lo && tn <= hi)) tn = 0.5 * (lo + hi);
+        if (tn == t) break;
+        t = tn;
+    }
+    return 1.0 - t;
+}
+
+__device__ __forceinline__ void draws(const McD &d, int owner, int si, int branch, int k,
+                                      float u[3])
+{
+    if (d.U) {
+        size_t per = (size_t)d.N1 * d.b1 + (size_t)d.N2 * d.b2;
+        const float *base = d.U + ((size_t)owner * 2 + si) * per;
+        if (branch == 1) {
+            const float *q = base + (size_t)k * d.b1;
+            u[0] = q[0];
+            u[1] = d.b1 > 1 ? q[1] : 0.f;
+            u[2] = d.b1 > 2 ? q[2] : 0.f;
+        } else {
+            const float *q = base + (size_t)d.N1 * d.b1 + (size_t)k * d.b2;
+            u[0] = q[0];
+            u[1] = q[1];
+            u[2] = 0.f;
+        }
+    } else {
+        uint4 c = make_uint4((uint32_t)k, (uint32_t)owner, d.step,
+                             (d.sim << 8) | ((uint32_t)si << 7) | ((uint32_t)branch << 4));
+        uint4 x = philox10(c, make_uint2(d.key0, d.key1));
+        u[0] = u01(x.x);
+        u[1] = u01(x.y);
+        u[2] = u01(x.z);
+    }
+}
+
+// smooth part of the doubly periodic kernel on the square lattice L(Z+iZ):
+// u - i v = (i / (2 pi L)) (sum_m c_m s^(4m-1) + pi conj(s)),  s = r / L
+__device__ __forceinline__ void kper_smooth(const McD &d, float rx, float ry, float &kx,
+                                            float &ky)
+{
+    float sr = rx * d.inv_L, si = ry * d.inv_L;
+    float s2r = sr * sr - si * si, s2i = 2.f * sr * si;
+    float wr = s2r * s2r - s2i * s2i, wi = 2.f * s2r * s2i;
+    float hr = d.kc[d.nterms - 1], hi = 0.f;
+    for (int m = d.nterms - 2; m >= 0; --m) {
+        float t = hr * wr - hi * wi + d.kc[m];
+        hi = hr * wi + hi * wr;
+        hr = t;
+    }
+    float s3r = s2r * sr - s2i * si, s3i = s2r * si + s2i * sr;
+    float Ar = hr * s3r - hi * s3i + PI_F * sr;
+    float Ai = hr * s3i + hi * s3r - PI_F * si;
+    kx = -Ai * d.inv_2piL;
+    ky = -Ar * d.inv_2piL;
+}
+
+// One sample of owner `owner`, sign set si, index k in [0, N1+N2):
+// returns false for a sample that contributes nothing without a query (a
+// branch-2 point outside the min-image cell, or a non-finite point).
+// (qx, qy): the query point inside the box; (rx, ry): the displacement the
+// MIS contribution uses (reference: dst - sp unwrapped; consistent: min image).
+template <int MODE, bool PER>
+__device__ __forceinline__ bool gen_sample(const McD &d, const SetD &S, int n_s, int owner,
+                                           int si, int k, float2 dst, float &qx, float &qy,
+                                           float &rx, float &ry)
+{
+    const bool b1 = k < d.N1;
+    float u[3];
+    draws(d, owner, si, b1 ? 1 : 2, b1 ? k : k - d.N1, u);
+    float spx, spy;
+    if (MODE == VPMC_MODE_REFERENCE) {
+        if (b1) {   // src/cpu/monte_carlo_bs.cpp:44-58
+            int j = guided_upper_bound(u[0], S.cdf, S.guide, S.glog2, n_s);
+            float2 xj = S.set_xy[j];
+            spx = xj.x;
+            spy = xj.y;
+            if (PER) {   // include/monte_carlo_bs.hpp:92-102
+                float ddx = 0.f, ddy = 0.f;
+                if (u[1] < 1.f / 3.f) ddx = -1.f; else if (u[1] > 2.f / 3.f) ddx = 1.f;
+                if (u[2] < 1.f / 3.f) ddy = -1.f; else if (u[2] > 2.f / 3.f) ddy = 1.f;
+                spx = __fadd_rn(spx, __fmul_rn(ddx, d.Lx));
+                spy = __fadd_rn(spy, __fmul_rn(ddy, d.Ly));
+            }
+        } else {   // src/cpu/monte_carlo_sampling.cpp:35-45
+            float phi = __fmul_rn(2.f * PI_F, u[0]);
+            float r = __fmul_rn(d.R_ref, sqrtf(u[1]));
+            float s, c;
+            sincosf(phi, &s, &c);
+            spx = __fadd_rn(dst.x, __fmul_rn(r, c));
+            spy = __fadd_rn(dst.y, __fmul_rn(r, s));
+        }
+        rx = dst.x - spx;
+        ry = dst.y - spy;
+    } else {
+        if (b1) {   // particle from |w|/Omega, jitter from W (exact radial inverse)
+            int j = guided_upper_bound(u[0], S.cdf, S.guide, S.glog2, n_s);
+            float2 xj = S.set_xy[j];
+            float rho = (float)mass_inverse((double)u[1]) * d.h_w;
+            float s, c;
+            sincosf(2.f * PI_F * u[2], &s, &c);
+            spx = xj.x + rho * c;
+            spy = xj.y + rho * s;
+            rx = dst.x - spx;
+            ry = dst.y - spy;
+            if (PER) {
+                if (rx > 0.5f * d.Lx) rx -= d.Lx; else if (rx < -0.5f * d.Lx) rx += d.Lx;
+                if (ry > 0.5f * d.Ly) ry -= d.Ly; else if (ry < -0.5f * d.Ly) ry += d.Ly;
+            }
+        } else {   // disk radius uniform in [0, Rc): density alpha / r
+            float rho = d.Rc * u[1];
+            float s, c;
+            sincosf(2.f * PI_F * u[0], &s, &c);
+            float cx = rho * c, cy = rho * s;
+            spx = dst.x + cx;
+            spy = dst.y + cy;
+            rx = -cx;
+            ry = -cy;
+            if (PER && (fabsf(rx) > 0.5f * d.Lx || fabsf(ry) > 0.5f * d.Ly)) return false;
+        }
+    }
+    if (!(isfinite(spx) && isfinite(spy))) return false;
+    if (PER) {
+        qx = spx - floorf(spx / d.Lx) * d.Lx;
+        qy = spy - floorf(spy / d.Ly) * d.Ly;
+    } else {
+        if (spx < 0.f || spx > d.Lx || spy < 0.f || spy > d.Ly) return false;
+        qx = spx;
+        qy = spy;
+    }
+    return true;
+}
+
+// MIS contribution of one sample with vorticity om at displacement r
+template <int MODE, bool PER>
+__device__ __forceinline__ void mis_contrib(const McD &d, float om, float rx, float ry, float Om,
+                                            float cum, float &sx, float &sy)
+{
+    if (!(om > 0.f)) return;
+    if (MODE == VPMC_MODE_REFERENCE) {
+        // mis_contrib, src/cpu/monte_carlo_bs.cpp:11-26
+        float r2 = rx * rx + ry * ry;
+        float rn = fmaxf(sqrtf(r2), 1e-6f);
+        float inv_r2 = 1.f / (r2 + 1e-8f);
+        float kx = -ry * inv_r2 / (2.f * PI_F);
+        float ky = rx * inv_r2 / (2.f * PI_F);
+        float denom = d.w1 * om / cum + d.w2 * d.alpha_ref / rn;
+        float g = om / denom;
+        sx += kx * g;
+        sy += ky * g;
+    } else {
+        float rn = sqrtf(rx * rx + ry * ry);
+        if (!(rn > 0.f)) return;
+        // omega / (w1 p1 + w2 p2) with p1 = omega/Omega, p2 = alpha/r
+        float g = om * rn / (d.w1 * om * rn / Om + d.w2 * d.alpha_c);
+        float f = g / (2.f * PI_F * rn * rn);
+        float kx = -ry * f, ky = rx * f;
+        if (PER) {
+            float ksx, ksy;
+            kper_smooth(d, rx, ry, ksx, ksy);
+            kx += ksx * g;
+            ky += ksy * g;
+        }
+        sx += kx;
+        sy += ky;
+    }
+}
+
+}  // namespace vpmc

@@ -41,6 +41,10 @@ __global__ void k_query_view(ViewD v, int npts, const float2 *__restrict__ q, in
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= npts) return;
     float2 p = q[i];
+    if (!(isfinite(p.x) && isfinite(p.y))) {
+        out[i] = 0.f;
+        return;
+    }
     if (periodic) {
         p.x = p.x - floorf(p.x / v.Lx) * v.Lx;
         p.y = p.y - floorf(p.y / v.Ly) * v.Ly;
@@ -54,7 +58,14 @@ __global__ void k_query_view(ViewD v, int npts, const float2 *__restrict__ q, in
     auto visit = [&](int I) {
         float2 xj = v.pos[I];
         float dx = xj.x - p.x, dy = xj.y - p.y;
-        if (periodic) {
+        if (periodic && !v.vp_sort) {
+            // all-particle view: no cell grid to fold against (the CPU path
+            // divides by rshx = 0 here); plain minimum image
+            if (dx > 0.5f * v.Lx) dx -= v.Lx;
+            if (dx < -0.5f * v.Lx) dx += v.Lx;
+            if (dy > 0.5f * v.Ly) dy -= v.Ly;
+            if (dy < -0.5f * v.Ly) dy += v.Ly;
+        } else if (periodic) {
             if (dx > v.Lx - 2 * cellx) dx -= v.Lx;
             if (dx < -v.Lx + 2 * cellx) dx += v.Lx;
             if (dy > v.Ly - 2 * celly) dy -= v.Ly;

@@ -108,11 +108,17 @@ __device__ __forceinline__ double spline_d(double r, double h_w)
 
 // std::upper_bound(cdf, cdf+n, u) through a guide table (guide[k] =
 // upper_bound(cdf, k / 2^glog2)); j == n -> n-1 as sample_vorticity_idx_cdf
-// does (src/cpu/monte_carlo_sampling.cpp:23-33).  u in [0,1).
+// does (src/cpu/monte_carlo_sampling.cpp:23-33).  Any float is accepted:
+// injected draws may equal 1 (std::uniform_real_distribution<float> can return
+// 1.0f) and the bucket index is clamped to the table, so u >= 1 gives n-1 and
+// u < 0 gives 0 exactly as upper_bound does.
 __device__ __forceinline__ int guided_upper_bound(float u, const double *__restrict__ cdf,
                                                   const int *__restrict__ guide, int glog2, int n)
 {
-    int kb = (int)(u * (float)(1u << glog2));
+    const int M = 1 << glog2;
+    int kb;
+    if (u >= 0.f && u < 1.f) kb = min((int)(u * (float)M), M - 1);
+    else kb = (u < 0.f) ? 0 : M - 1;   // u >= 1 or NaN: no cdf value exceeds it
     int lo = guide[kb], hi = guide[kb + 1];
     double ud = (double)u;
     while (lo < hi) {

@@ -180,7 +180,7 @@ def advect(pos, vel_sign, dt, Lx=1.0, Ly=1.0, periodic=True):
 # ---------------------------------------------------------------------------
 # RBF / CG
 
-def cg_rbf(pos, b, h_w, Lx=1.0, Ly=1.0, tol=1e-8, max_iter=1000, refine=True, cells_per_hw=8):
+def cg_rbf(pos, b, h_w, Lx=1.0, Ly=1.0, tol=1e-6, max_iter=1000, refine=True, cells_per_hw=8):
     pos = _xy(pos)
     n = pos.shape[0]
     nx, ny = field_grid(h_w, Lx, Ly, cells_per_hw)

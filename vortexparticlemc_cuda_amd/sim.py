@@ -11,7 +11,8 @@ import numpy as np
 import torch
 
 from . import ops
-from ._capi import McDesc, SignSet, check, lib, MODE_CONSISTENT, MODE_REFERENCE
+from ._capi import (ALGO_BINNED, ALGO_DIRECT, MODE_CONSISTENT, MODE_REFERENCE, McDesc, SignSet,
+                    check, lib)
 
 f32 = np.float32
 
@@ -55,8 +56,11 @@ class Simulation:
     """One periodic simulation on the device (the reference's Simulation,
     include/vortex_particle_mc.hpp:136-192, for its GPU path)."""
 
-    def __init__(self, P, device="cuda", cells_per_hw=8, mode=MODE_CONSISTENT, seed=0, sim=0):
+    def __init__(self, P, device="cuda", cells_per_hw=8, mode=MODE_CONSISTENT, seed=0, sim=0,
+                 algo=ALGO_BINNED):
         self.P = P
+        self.algo = int(algo)
+        self.ws = None
         self.device = torch.device(device)
         self.cells_per_hw = int(cells_per_hw)
         self.mode = int(mode)
@@ -75,9 +79,11 @@ class Simulation:
         self.build_subsets()
         self._alloc()
 
-    def init_strengths_cg(self, pos, b, tol=1e-8, max_iter=1000, refine=True):
+    def init_strengths_cg(self, pos, b, tol=1e-6, max_iter=1000, refine=True):
         """Simulation::init + conjugate_gradient_rbf (src/vortex_particle_mc.cu:77-120,
-        src/cuda/cg_rbf_init.cu:53-116): strengths w solving A w = b."""
+        src/cuda/cg_rbf_init.cu:53-116): strengths w solving A w = b.  With refine
+        the solve is FP64 MINRES; strengths are FP32, so tol below ~1e-7 only
+        amplifies the rounding noise of b in the Gram matrix's near-null modes."""
         pos_t = torch.as_tensor(np.asarray(pos, np.float32), device=self.device).contiguous()
         b_t = torch.as_tensor(np.asarray(b, np.float32), device=self.device).contiguous()
         w, it, rr, st = ops.cg_rbf(pos_t, b_t, float(self.P.h_w), float(self.P.Lx),
@@ -106,8 +112,6 @@ class Simulation:
         self.vel_sign = torch.zeros((2 * max(self.N, 1), 2), dtype=torch.float32,
                                     device=self.device)
         self.vel = torch.zeros((max(self.N, 1), 2), dtype=torch.float32, device=self.device)
-        self.ws = ops.workspace(lib().vpmc_step_workspace_bytes(self.N, self.cnt[0], self.cnt[1],
-                                                                self.nx, self.ny), self.device)
         self.eval_count = torch.zeros(2, dtype=torch.int64, device=self.device)
 
     def desc(self, N1, N2, uniforms=None, aux_draws=0, count=False):
@@ -137,7 +141,23 @@ class Simulation:
         d.set[0], d.set[1] = sets[0], sets[1]
         d.vel_sign = ops._p(self.vel_sign)
         d.eval_count = ops._p(self.eval_count) if count else None
+        d.algo = self.algo
         return d
+
+    def _ws_for(self, d):
+        need = lib().vpmc_step_workspace_bytes(C.byref(d))
+        if self.ws is None or self.ws.numel() < need:
+            self.ws = ops.workspace(need, self.device)
+        return self.ws
+
+    def launch_estimate(self, d):
+        """the estimator kernels for descriptor d (fields already built)"""
+        if self.algo == ALGO_BINNED:
+            ws = self._ws_for(d)
+            check(lib().vpmc_mc_estimate_binned(C.byref(d), ops._p(ws), ws.numel(),
+                                                ops._stream()), "vpmc_mc_estimate_binned")
+        else:
+            check(lib().vpmc_mc_estimate(C.byref(d), ops._stream()), "vpmc_mc_estimate")
 
     def rebuild_fields(self):
         """do_spatial_hashing on the fine grid for both sign sets"""
@@ -151,7 +171,7 @@ class Simulation:
         """velocity estimate at the current positions (no advection)"""
         self.rebuild_fields()
         d = self.desc(N1, N2, uniforms, aux_draws, count)
-        check(lib().vpmc_mc_estimate(C.byref(d), ops._stream()), "vpmc_mc_estimate")
+        self.launch_estimate(d)
         vs = self.vel_sign[:2 * self.N].view(2, self.N, 2)
         return vs[0] - vs[1]
 
@@ -161,6 +181,7 @@ class Simulation:
         N2 = self.P.N2 if N2 is None else N2
         dt = float(self.P.dt if dt is None else dt)
         d = self.desc(N1, N2, uniforms, aux_draws)
+        self._ws_for(d)
         check(lib().vpmc_step(C.byref(d), self.N, ops._p(self.pos), ops._p(self.omega),
                               ops._p(self.sub["sub_pos"]), ops._p(self.sub["sub_neg"]),
                               ops._p(self.vel), dt, ops._p(self.ws), self.ws.numel(),
