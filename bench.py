@@ -57,31 +57,60 @@ def setup(Nx, hwr, N1, N2, device, seed=0, cells_per_hw=8):
     return sim, dict(cg_iters=it, cg_rel_res=rr, cg_status=st, cg_seconds=t_cg)
 
 
+def _event_pair(s):
+    """two timing events, recorded once so that their hipEvent_t exists"""
+    import torch
+    e = (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+    e[0].record(s)
+    e[1].record(s)
+    return e
+
+
 def run_steps(sim, K, events=None):
-    """K plain steps; if events is a list, records estimator start/stop events"""
+    """K plain steps.  If events is a dict of lists, it receives per step the
+    event pairs around the whole estimator ("est", torch's stream) and around
+    its evaluation kernel ("eval", recorded by the library on the launch
+    stream)."""
     import torch
 
     from vortexparticlemc_cuda_amd import ops
-    from vortexparticlemc_cuda_amd._capi import check, lib
 
     s = torch.cuda.current_stream()
     P = sim.P
-    for _ in range(K):
+    pairs = []
+    if events is not None:
+        pairs = [(_event_pair(s), _event_pair(s)) for _ in range(K)]
+    for i in range(K):
         d = sim.desc(P.N1, P.N2)
         ws = sim._ws_for(d)   # one buffer serves the field builds and the estimator
         for k, key in enumerate(("sub_pos", "sub_neg")):
             ops.field_build(sim.fields[k], sim.pos, sim.omega, sub=sim.sub[key], abs_w=True,
                             ws=ws)
         if events is not None:
-            e0 = torch.cuda.Event(enable_timing=True)
-            e1 = torch.cuda.Event(enable_timing=True)
-            e0.record(s)
+            est, ev = pairs[i]
+            d.eval_events[0] = ev[0].cuda_event
+            d.eval_events[1] = ev[1].cuda_event
+            est[0].record(s)
         sim.launch_estimate(d)
         if events is not None:
-            e1.record(s)
-            events.append((e0, e1))
+            est[1].record(s)
+            events["est"].append(est)
+            events["eval"].append(ev)
         ops.advect(sim.pos, sim.vel_sign[:2 * sim.N], float(P.dt), float(P.Lx), float(P.Ly))
         sim.step_index += 1
+
+
+def copy_probe(nbytes, reps=3):
+    """a plain device copy of nbytes (reads nbytes, writes nbytes), for
+    calibrating HBM byte counters"""
+    import torch
+    n = nbytes // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda:0")
+    b = torch.empty_like(a)
+    for _ in range(reps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    log("copy probe: %d x %d bytes" % (reps, n * 4))
 
 
 def count_evals(sim):
@@ -116,6 +145,22 @@ def velocity_error(sim):
     rms = lambda a: float(torch.sqrt((a * a).sum(1).mean()))  # noqa: E731
     return dict(rms_err_vs_direct=rms(u - ud) / rms(ud), direct_vs_tg=rms(ud - ut) / rms(ut),
                 rms_err_vs_tg=rms(u - ut) / rms(ut))
+
+
+def read_traffic(kernel, args):
+    """HBM bytes per launch of `kernel` at this workload, from the committed
+    counter summary (FETCH_SIZE + WRITE_SIZE, copy-calibrated); None if absent"""
+    path = os.path.join(ROOT, "profiles", "traffic_%s_tg%d.txt" % (kernel, args.Nx))
+    if not os.path.exists(path):
+        return None
+    want = "N1=%d N2=%d hwr=%d cells_per_hw=%d" % (args.N1, args.N2, args.hwr, args.cells_per_hw)
+    ok = False
+    for line in open(path):
+        if line.startswith("workload:"):
+            ok = want in line
+        if ok and line.startswith("hbm_bytes_per_launch:"):
+            return float(line.split(":", 1)[1])
+    return None
 
 
 def cpu_baseline(Nx, hwr, N1, N2, target_s=12.0):
@@ -163,6 +208,8 @@ def main():
     ap.add_argument("--algo", choices=["binned", "direct"], default="binned")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-error", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="also run a plain copy of MIB MiB (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU; this build has no multi-GPU path "
@@ -183,27 +230,27 @@ def main():
     run_steps(sim, args.warmup)
     torch.cuda.synchronize()
     log("warmup done")
-    events = []
+    events = {"est": [], "eval": []}
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     run_steps(sim, args.steps, events)
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
-    est_ms = [a.elapsed_time(b) for a, b in events]
-    est_avg_s = float(np.mean(est_ms)) / 1e3
-    log("timed %d steps in %.3f s, estimator avg %.3f ms" % (args.steps, elapsed, est_avg_s * 1e3))
+    est_avg_s = float(np.mean([a.elapsed_time(b) for a, b in events["est"]])) / 1e3
+    eval_avg_s = float(np.mean([a.elapsed_time(b) for a, b in events["eval"]])) / 1e3
+    log("timed %d steps in %.3f s, estimator avg %.3f ms, evaluation kernel avg %.3f ms"
+        % (args.steps, elapsed, est_avg_s * 1e3, eval_avg_s * 1e3))
     n_in, n_vis = count_evals(sim)
     N = sim.N
     S = N * 2 * (args.N1 + args.N2)
-    flop = FLOP_PER_EVAL * n_in + FLOP_PER_SAMPLE * S
-    achieved = flop / est_avg_s
-    traffic = None
-    tp = os.path.join(ROOT, "profiles", "traffic_mc_estimate_tg200.json")
-    if os.path.exists(tp):
-        try:
-            traffic = json.load(open(tp)).get("hbm_bytes_per_launch")
-        except Exception:
-            traffic = None
+    eval_kernel = "k_bin_eval" if args.algo == "binned" else "k_mc_estimate"
+    # the evaluation kernel's algorithmic work: 15 FLOP per in-support pair
+    # (for the direct kernel, which also samples, 30 FLOP per sample on top)
+    flop = FLOP_PER_EVAL * n_in + (FLOP_PER_SAMPLE * S if args.algo == "direct" else 0)
+    achieved = flop / eval_avg_s
+    traffic = read_traffic(eval_kernel, args)
+    if args.copy_probe:
+        copy_probe(args.copy_probe << 20)
     cpu = None if args.no_cpu else cpu_baseline(args.Nx, args.hwr, args.N1, args.N2)
     value = N * args.steps / elapsed
     line = {
@@ -226,13 +273,16 @@ def main():
                    "particles": N, "N1": args.N1, "N2": args.N2, "hwr": args.hwr,
                    "cells_per_hw": args.cells_per_hw, "estimator_algo": args.algo,
                    "parallelism": "single GPU"},
-        "roofline": {"kernel": "k_mc_estimate", "bound": "valu", "achieved": achieved,
+        "roofline": {"kernel": eval_kernel, "bound": "valu", "achieved": achieved,
                      "peak": PEAK_FP32, "unit": "FLOP/s", "frac": achieved / PEAK_FP32,
                      "traffic": traffic,
                      "flop_per_launch": flop, "kernel_evals_per_launch": n_in,
                      "candidates_per_launch": n_vis, "samples_per_launch": S,
-                     "avg_launch_ms": est_avg_s * 1e3},
-        "kernel_evals_per_s": n_in / est_avg_s,
+                     "avg_launch_ms": eval_avg_s * 1e3,
+                     "unpacked_fp32_issue_peak": PEAK_FP32 / 2,
+                     "frac_of_unpacked": achieved / (PEAK_FP32 / 2)},
+        "kernel_evals_per_s": n_in / eval_avg_s,
+        "estimator_ms": est_avg_s * 1e3,
         "estimator_fraction_of_step": est_avg_s / (elapsed / args.steps),
         "error": err,
         "cg_init": init,

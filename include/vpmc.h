@@ -180,6 +180,11 @@ typedef struct vpmc_mc_desc {
     unsigned long long *eval_count;   /* optional: += kernel evaluations within
                                          h_w and candidates visited ([0],[1])  */
     int          algo;                /* VPMC_ALGO_BINNED (default) or _DIRECT */
+    void        *eval_events[2];      /* optional hipEvent_t pair, recorded on the
+                                         stream right before and after the
+                                         evaluation kernel (k_bin_eval, or
+                                         k_mc_estimate for the direct algorithm);
+                                         NULL: not recorded                    */
 } vpmc_mc_desc;
 
 enum { VPMC_ALGO_BINNED = 0, VPMC_ALGO_DIRECT = 1 };

@@ -54,7 +54,7 @@ class McDesc(C.Structure):
                 ("periodic", ci), ("seed", C.c_uint64), ("step", C.c_uint32),
                 ("sim", C.c_uint32), ("uniforms", vp), ("aux_draws", ci), ("n_owner", ci),
                 ("owner_xy", vp), ("set", SignSet * 2), ("vel_sign", vp),
-                ("eval_count", vp), ("algo", ci)]
+                ("eval_count", vp), ("algo", ci), ("eval_events", vp * 2)]
 
 
 _lib = None

@@ -341,6 +341,7 @@ int mc_binned_impl(const vpmc_mc_desc *d, void *ws, size_t ws_bytes, hipStream_t
     VPMC_CHECK_LAUNCH();
     const float4 *p0 = m.set[0].field.pts, *p1 = m.set[1].field.pts;
     const int *c0 = m.set[0].field.cell_start, *c1 = m.set[1].field.cell_start;
+    VPMC_TRY(record_event(d->eval_events[0], s));
     if (d->eval_count)
         hipLaunchKernelGGL((k_bin_eval<2, true>), dim3(T), dim3(BIN_THREADS), 0, s, g, m.inv_hw,
                            m.sig2, m.Lx, m.Ly, p0, c0, p1, c1, (const int *)B.start,
@@ -351,6 +352,7 @@ int mc_binned_impl(const vpmc_mc_desc *d, void *ws, size_t ws_bytes, hipStream_t
                            (const float2 *)B.sq, (const uint32_t *)B.ssid, B.om,
                            (unsigned long long *)nullptr);
     VPMC_CHECK_LAUNCH();
+    VPMC_TRY(record_event(d->eval_events[1], s));
     launch_mode(d->mode, d->periodic, m, g, B, true, s);
     VPMC_CHECK_LAUNCH();
     return VPMC_OK;

@@ -230,9 +230,11 @@ int mc_estimate_impl(const vpmc_mc_desc *desc, hipStream_t s)
     McD m;
     VPMC_TRY(make_mcd(desc, &m));
     if (m.n_owner == 0) return VPMC_OK;
+    VPMC_TRY(record_event(desc->eval_events[0], s));
     if (desc->eval_count) launch_mc<true>(m, desc->mode, desc->periodic, s);
     else launch_mc<false>(m, desc->mode, desc->periodic, s);
     VPMC_CHECK_LAUNCH();
+    VPMC_TRY(record_event(desc->eval_events[1], s));
     return VPMC_OK;
 }
 

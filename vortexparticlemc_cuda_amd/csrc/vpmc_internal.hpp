@@ -29,6 +29,15 @@ void set_hip_error(hipError_t e);
 inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }
 inline size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
 
+// records a caller's timing event (hipEvent_t) on s; no-op for NULL
+inline int record_event(void *ev, hipStream_t s)
+{
+    if (!ev) return VPMC_OK;
+    hipError_t e = hipEventRecord(reinterpret_cast<hipEvent_t>(ev), s);
+    if (e != hipSuccess) { set_hip_error(e); return VPMC_ERR_HIP; }
+    return VPMC_OK;
+}
+
 // bump allocator over a caller workspace
 struct Carve {
     char *base;
