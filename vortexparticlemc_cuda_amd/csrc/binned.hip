@@ -8,12 +8,19 @@
 //   gen     one thread per sample: Philox (or injected) draws -> query point,
 //           MIS displacement, fine-cell key; atomic per-cell count (slot).
 //   scatter samples into cell order (exclusive scan of the counts).
-//   eval    one workgroup per cell: every wave takes 64*SPL samples of the
-//           cell and walks the cell's neighbour stencil (the cells within
-//           h_w of the cell rectangle, one contiguous run of the sorted
-//           particle array per column).  Candidate records are read at
-//           wave-uniform addresses (scalar loads), the per-sample work is
-//           pure VALU.  omega is written back to the sample's id.
+//   eval    one workgroup per cell ("tile").  The workgroup stages the
+//           cell's candidates in LDS once: the neighbour stencil (per column,
+//           the rows within h_w of the cell rectangle: contiguous runs of
+//           the sorted particle array) minus every particle farther than the
+//           search radius from the rectangle, which no sample of the cell can
+//           reach (exact: such terms are zero).  Records are stored relative
+//           to the cell centre o as (-2c'x, -2c'y, |c'|^2, w), c' = c - o, so
+//           that with p' = p - o (|p'| <= half a cell diagonal)
+//             r^2 = |p'|^2 + |c'|^2 - 2 c'.p'        (3 VALU)
+//           and the spline W/sig2 = 2 (t^3 - v^3), t = clamp(1 - q),
+//           v = clamp(4^(1/3) (t - 1/2)), accumulates in 8 more VALU per
+//           (sample, candidate) pair.  Every wave takes 64*SPL samples of the
+//           cell at a time; omega goes back to the sample's id.
 //   final   one wave per (owner, sign): MIS contributions of its N1+N2
 //           samples in sample order, butterfly sum -> deterministic result.
 // Sample ids are owner-major: sid = (owner*2 + sign)*(N1+N2) + k.
@@ -26,6 +33,8 @@ constexpr int BIN_THREADS = 256;
 struct BinGeom {
     int ncell, nx, ny;
     float inv_cx, inv_cy;
+    float cw, ch, rs2;        // cell size, squared search radius (field geometry)
+    int ntile, per_xcd;       // 2 * ncell tiles; tiles per XCD in the launch order
     int cols;                 // stencil half-width (columns)
     int ext[MAX_COL_EXT];     // rows each side for column offset ox (index ox+cols), -1: none
     uint32_t NS;              // N1 + N2
@@ -87,46 +96,169 @@ __global__ void __launch_bounds__(BIN_THREADS)
     ssid[dst] = (uint32_t)sid;
 }
 
-// one contiguous run [b, e) of candidates seen through a uniform image shift
-template <int SPL, bool COUNT>
-__device__ __forceinline__ void bin_run(const float4 *__restrict__ pts, int b, int e,
-                                        const float (&qx)[SPL], const float (&qy)[SPL],
-                                        float sx, float sy, float inv_hw, float (&a1)[SPL],
-                                        float (&a2)[SPL], EvalCount &ec,
-                                        const bool (&live)[SPL])
+constexpr int EVAL_CAP = 1024;                  // staged candidates per pass (16 KB)
+constexpr int MAX_PIECES = 3 * MAX_COL_EXT;     // <= 3 runs per stencil column
+constexpr int BIN_WAVES = BIN_THREADS / WAVE;
+constexpr float CBRT4 = 1.5874010519681994f;    // 4^(1/3)
+
+// exclusive block-wide prefix of v (one value per thread); *total gets the sum
+__device__ __forceinline__ int block_excl_scan(int v, int *wsum, int *total)
 {
-    float px[SPL], py[SPL];
+    const int lane = threadIdx.x & (WAVE - 1), wid = threadIdx.x / WAVE;
+    int x = v;
 #pragma unroll
-    for (int s = 0; s < SPL; ++s) {
-        px[s] = qx[s] - sx;
-        py[s] = qy[s] - sy;
+    for (int o = 1; o < WAVE; o <<= 1) {
+        int y = __shfl_up(x, o, WAVE);
+        if (lane >= o) x += y;
     }
-    // q >= 0, so 1-q <= 1 and 1/2-q <= 1/2: a max with 0 is the whole clamp
+    if (lane == WAVE - 1) wsum[wid] = x;
+    __syncthreads();
+    int base = 0, tot = 0;
+#pragma unroll
+    for (int w = 0; w < BIN_WAVES; ++w) {
+        base += (w < wid) ? wsum[w] : 0;
+        tot += wsum[w];
+    }
+    __syncthreads();   // wsum may be reused by the caller
+    *total = tot;
+    return base + x - v;
+}
+
+// The tile's stencil as runs ("pieces") of the sorted particle array, in the
+// order column -cols..cols and, per column, [rows wrapped below] [rows
+// wrapped above] [rows in range]; p_off is the running candidate count.
+struct TileLds {
+    float4 cand[EVAL_CAP];
+    int p_beg[MAX_PIECES];
+    int p_off[MAX_PIECES + 1];
+    float p_sx[MAX_PIECES], p_sy[MAX_PIECES];
+    int wsum[BIN_WAVES];
+    int n_stage;
+};
+
+__device__ __forceinline__ int tile_pieces(const BinGeom &g, const int *__restrict__ cell_start,
+                                           int cx, int cy, float Lx, float Ly, TileLds &L)
+{
+    const int ncol = 2 * g.cols + 1;   // <= MAX_COL_EXT <= BIN_THREADS
+    const int j = threadIdx.x;
+    int len[3] = {0, 0, 0}, pb[3] = {0, 0, 0};
+    float sx = 0.f, sy[3] = {-Ly, Ly, 0.f};
+    if (j < ncol) {
+        const int e = g.ext[j];
+        if (e >= 0) {
+            int colw = cx + j - g.cols;
+            if (colw < 0) { colw += g.nx; sx = -Lx; }
+            else if (colw >= g.nx) { colw -= g.nx; sx = Lx; }
+            const int *csc = cell_start + (size_t)colw * g.ny;
+            int r0 = cy - e, r1 = cy + e;
+            if (r0 < 0) {   // rows ny+r0 .. ny-1: particle image y - Ly
+                pb[0] = csc[g.ny + r0];
+                len[0] = csc[g.ny] - pb[0];
+                r0 = 0;
+            }
+            if (r1 >= g.ny) {   // rows 0 .. r1-ny: particle image y + Ly
+                pb[1] = csc[0];
+                len[1] = csc[r1 - g.ny + 1] - pb[1];
+                r1 = g.ny - 1;
+            }
+            pb[2] = csc[r0];
+            len[2] = csc[r1 + 1] - pb[2];
+        }
+    }
+    int total;
+    int off = block_excl_scan(len[0] + len[1] + len[2], L.wsum, &total);
+    if (j < ncol) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            L.p_beg[3 * j + k] = pb[k];
+            L.p_off[3 * j + k] = off;
+            L.p_sx[3 * j + k] = sx;
+            L.p_sy[3 * j + k] = sy[k];
+            off += len[k];
+        }
+    }
+    if (j == 0) L.p_off[3 * ncol] = total;
+    __syncthreads();
+    return total;
+}
+
+// stages the candidates with raw index in [r0, r1) (r1 - r0 <= EVAL_CAP) that
+// can reach the cell: cand[0 .. n) in raw order, padded with zero weights to
+// a multiple of 4; returns n (padded).  Starts and ends with a barrier.
+__device__ __forceinline__ int tile_stage(const BinGeom &g, const float4 *__restrict__ pts,
+                                          int npieces, int r0, int r1, float ox, float oy,
+                                          TileLds &L)
+{
+    const int lane = threadIdx.x & (WAVE - 1), wid = threadIdx.x / WAVE;
+    const float hx = 0.5f * g.cw, hy = 0.5f * g.ch;
+    int n = 0;
+    __syncthreads();   // the previous pass's readers are done with cand
+    for (int i0 = r0; i0 < r1; i0 += BIN_THREADS) {
+        const int i = i0 + threadIdx.x;
+        bool keep = false;
+        float4 rec = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (i < r1) {
+            int lo = 0, hi = npieces;   // last piece with p_off <= i
+            while (hi - lo > 1) {
+                int mid = (lo + hi) >> 1;
+                if (L.p_off[mid] <= i) lo = mid; else hi = mid;
+            }
+            const float4 c = pts[L.p_beg[lo] + (i - L.p_off[lo])];
+            const float x = (c.x + L.p_sx[lo]) - ox, y = (c.y + L.p_sy[lo]) - oy;
+            const float gx = fmaxf(fabsf(x) - hx, 0.f), gy = fmaxf(fabsf(y) - hy, 0.f);
+            keep = fmaf(gx, gx, gy * gy) <= g.rs2;
+            rec = make_float4(-2.f * x, -2.f * y, fmaf(x, x, y * y), c.z);
+        }
+        const unsigned long long m = __ballot(keep);
+        const int pre = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                                  __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+        if (lane == 0) L.wsum[wid] = __popcll(m);
+        __syncthreads();
+        int base = n, tot = 0;
+#pragma unroll
+        for (int w = 0; w < BIN_WAVES; ++w) {
+            base += (w < wid) ? L.wsum[w] : 0;
+            tot += L.wsum[w];
+        }
+        if (keep) L.cand[base + pre] = rec;
+        n += tot;
+        __syncthreads();   // wsum is rewritten by the next round
+    }
+    const int npad = (n + 3) & ~3;
+    if ((int)threadIdx.x < npad - n) L.cand[n + threadIdx.x] = make_float4(0.f, 0.f, 0.f, 0.f);
+    __syncthreads();
+    return npad;
+}
+
+template <int SPL, bool COUNT>
+__device__ __forceinline__ void tile_eval(const float4 *__restrict__ cand, int n,
+                                          const float (&px)[SPL], const float (&py)[SPL],
+                                          const float (&pp)[SPL], float inv_hw,
+                                          float (&acc)[SPL], EvalCount &ec,
+                                          const bool (&live)[SPL])
+{
     auto one = [&](const float4 &c) {
 #pragma unroll
         for (int s = 0; s < SPL; ++s) {
-            float dx = c.x - px[s], dy = c.y - py[s];
-            float r2 = fmaf(dx, dx, dy * dy);
-            float qq = __builtin_amdgcn_sqrtf(r2) * inv_hw;
-            float t = fmaxf(1.f - qq, 0.f);
-            float u = fmaxf(0.5f - qq, 0.f);
-            a1[s] = fmaf(c.z * t, t * t, a1[s]);   // record = (x, y, |w|, 0)
-            a2[s] = fmaf(c.z * u, u * u, a2[s]);
-            if (COUNT && live[s]) {
-                ec.in_support += (qq < 1.f) ? 1u : 0u;
+            float r2 = fmaf(c.x, px[s], fmaf(c.y, py[s], c.z + pp[s]));
+            float q = __builtin_amdgcn_sqrtf(fabsf(r2));   // |.|: rounding below 0
+            float t = __builtin_amdgcn_fmed3f(fmaf(-inv_hw, q, 1.f), 0.f, 1.f);
+            float v = __builtin_amdgcn_fmed3f(fmaf(t, CBRT4, -0.5f * CBRT4), 0.f, 1.f);
+            float t2 = t * t, v3 = v * v * v;
+            acc[s] = fmaf(c.w, fmaf(t2, t, -v3), acc[s]);
+            if (COUNT && live[s] && c.w > 0.f) {   // padding records have w = 0
+                ec.in_support += (t > 0.f) ? 1u : 0u;
                 ec.visited += 1u;
             }
         }
     };
-    int k = b;
-    for (; k + 4 <= e; k += 4) {   // four wave-uniform records per batch of loads
-        const float4 c0 = pts[k], c1 = pts[k + 1], c2 = pts[k + 2], c3 = pts[k + 3];
+    for (int k = 0; k < n; k += 4) {   // n is a multiple of 4 (zero-weight padding)
+        const float4 c0 = cand[k], c1 = cand[k + 1], c2 = cand[k + 2], c3 = cand[k + 3];
         one(c0);
         one(c1);
         one(c2);
         one(c3);
     }
-    for (; k < e; ++k) one(pts[k]);
 }
 
 template <int SPL, bool COUNT>
@@ -138,62 +270,53 @@ __global__ void __launch_bounds__(BIN_THREADS)
                const uint32_t *__restrict__ ssid, float *__restrict__ om,
                unsigned long long *__restrict__ eval_count)
 {
-    const int tile = blockIdx.x;
+    // workgroups are dealt round-robin to the 8 XCDs: give each XCD a
+    // contiguous range of tiles so that neighbouring cells share its L2
+    const int tile = (int)(blockIdx.x & 7) * g.per_xcd + (int)(blockIdx.x >> 3);
+    if (tile >= g.ntile) return;
     const int beg = start[tile], end = start[tile + 1];
+    if (beg >= end) return;   // uniform over the workgroup
+    __shared__ TileLds L;
     EvalCount ec;
-    if (beg < end) {
-        const int si = tile >= g.ncell ? 1 : 0;
-        const int c = tile - si * g.ncell;
-        const int cx = c / g.ny, cy = c - (c / g.ny) * g.ny;
-        const float4 *__restrict__ pts = si ? pts1 : pts0;
-        const int *__restrict__ cell_start = si ? cs1 : cs0;
-        const int lane = threadIdx.x & (WAVE - 1), wid = threadIdx.x / WAVE;
-        constexpr int CHUNK = WAVE * SPL;
-        for (int base = beg + wid * CHUNK; base < end; base += (BIN_THREADS / WAVE) * CHUNK) {
-            float qx[SPL], qy[SPL], a1[SPL], a2[SPL];
-            bool live[SPL];
+    const int si = tile >= g.ncell ? 1 : 0;
+    const int c = tile - si * g.ncell;
+    const int cx = c / g.ny, cy = c - cx * g.ny;
+    const float ox = ((float)cx + 0.5f) * g.cw, oy = ((float)cy + 0.5f) * g.ch;
+    const float4 *__restrict__ pts = si ? pts1 : pts0;
+    const int total = tile_pieces(g, si ? cs1 : cs0, cx, cy, Lx, Ly, L);
+    const int npieces = 3 * (2 * g.cols + 1);
+    const int npass = (total + EVAL_CAP - 1) / EVAL_CAP;
+    int n = 0;
+    if (npass == 1) n = tile_stage(g, pts, npieces, 0, total, ox, oy, L);
+    const int lane = threadIdx.x & (WAVE - 1), wid = threadIdx.x / WAVE;
+    for (int bb = beg; bb < end; bb += BIN_THREADS * SPL) {   // uniform trip count
+        float px[SPL], py[SPL], pp[SPL], acc[SPL];
+        bool live[SPL];
 #pragma unroll
-            for (int s = 0; s < SPL; ++s) {
-                int idx = base + s * WAVE + lane;
-                live[s] = idx < end;   // tail lanes repeat a valid point, results dropped
-                float2 p = live[s] ? sq[idx] : sq[beg];
-                qx[s] = p.x;
-                qy[s] = p.y;
-                a1[s] = 0.f;
-                a2[s] = 0.f;
-            }
-            for (int ox = -g.cols; ox <= g.cols; ++ox) {
-                const int e = g.ext[ox + g.cols];
-                if (e < 0) continue;
-                int colw = cx + ox;
-                float sx = 0.f;
-                if (colw < 0) { colw += g.nx; sx = -Lx; }
-                else if (colw >= g.nx) { colw -= g.nx; sx = Lx; }
-                const int *csc = cell_start + (size_t)colw * g.ny;
-                int r0 = cy - e, r1 = cy + e;
-                if (r0 < 0) {   // rows ny+r0 .. ny-1, particle image y - Ly
-                    bin_run<SPL, COUNT>(pts, csc[g.ny + r0], csc[g.ny], qx, qy, sx, -Ly, inv_hw,
-                                        a1, a2, ec, live);
-                    r0 = 0;
-                }
-                if (r1 >= g.ny) {   // rows 0 .. r1-ny, particle image y + Ly
-                    bin_run<SPL, COUNT>(pts, csc[0], csc[r1 - g.ny + 1], qx, qy, sx, Ly, inv_hw,
-                                        a1, a2, ec, live);
-                    r1 = g.ny - 1;
-                }
-                bin_run<SPL, COUNT>(pts, csc[r0], csc[r1 + 1], qx, qy, sx, 0.f, inv_hw, a1, a2,
-                                    ec, live);
-            }
+        for (int s = 0; s < SPL; ++s) {
+            const int idx = bb + (wid * SPL + s) * WAVE + lane;
+            live[s] = idx < end;   // tail lanes take the cell centre, results dropped
+            const float2 p = live[s] ? sq[idx] : make_float2(ox, oy);
+            px[s] = p.x - ox;
+            py[s] = p.y - oy;
+            pp[s] = fmaf(px[s], px[s], py[s] * py[s]);
+            acc[s] = 0.f;
+        }
+        for (int pass = 0; pass < npass; ++pass) {
+            if (npass > 1)
+                n = tile_stage(g, pts, npieces, pass * EVAL_CAP,
+                               min(total, (pass + 1) * EVAL_CAP), ox, oy, L);
+            tile_eval<SPL, COUNT>(L.cand, n, px, py, pp, inv_hw, acc, ec, live);
+        }
 #pragma unroll
-            for (int s = 0; s < SPL; ++s) {
-                int idx = base + s * WAVE + lane;
-                if (live[s]) om[ssid[idx]] = sig2 * (2.f * a1[s] - 8.f * a2[s]);
-            }
+        for (int s = 0; s < SPL; ++s) {
+            const int idx = bb + (wid * SPL + s) * WAVE + lane;
+            if (live[s]) om[ssid[idx]] = 2.f * sig2 * acc[s];
         }
     }
     if (COUNT) {
         unsigned a = wave_sum(ec.in_support), b = wave_sum(ec.visited);
-        if ((threadIdx.x & (WAVE - 1)) == 0) {
+        if (lane == 0) {
             atomicAdd(eval_count, (unsigned long long)a);
             atomicAdd(eval_count + 1, (unsigned long long)b);
         }
@@ -236,6 +359,11 @@ static int bin_geom(const McD &m, const vpmc_mc_desc *in, BinGeom *g)
     g->ncell = f.nx * f.ny;
     g->inv_cx = f.inv_cx;
     g->inv_cy = f.inv_cy;
+    g->cw = f.cw;
+    g->ch = f.ch;
+    g->rs2 = f.rs2;
+    g->ntile = 2 * g->ncell;
+    g->per_xcd = (g->ntile + 7) / 8;
     g->cols = 1 + (int)floorf(f.rs / f.cw);
     if (2 * g->cols + 1 > MAX_COL_EXT || 2 * g->cols + 1 > f.nx) return VPMC_ERR_GRID;
     int emax = 0;
@@ -342,12 +470,13 @@ int mc_binned_impl(const vpmc_mc_desc *d, void *ws, size_t ws_bytes, hipStream_t
     const float4 *p0 = m.set[0].field.pts, *p1 = m.set[1].field.pts;
     const int *c0 = m.set[0].field.cell_start, *c1 = m.set[1].field.cell_start;
     VPMC_TRY(record_event(d->eval_events[0], s));
+    const dim3 eval_grid((unsigned)(8 * g.per_xcd));
     if (d->eval_count)
-        hipLaunchKernelGGL((k_bin_eval<2, true>), dim3(T), dim3(BIN_THREADS), 0, s, g, m.inv_hw,
+        hipLaunchKernelGGL((k_bin_eval<2, true>), eval_grid, dim3(BIN_THREADS), 0, s, g, m.inv_hw,
                            m.sig2, m.Lx, m.Ly, p0, c0, p1, c1, (const int *)B.start,
                            (const float2 *)B.sq, (const uint32_t *)B.ssid, B.om, d->eval_count);
     else
-        hipLaunchKernelGGL((k_bin_eval<2, false>), dim3(T), dim3(BIN_THREADS), 0, s, g, m.inv_hw,
+        hipLaunchKernelGGL((k_bin_eval<2, false>), eval_grid, dim3(BIN_THREADS), 0, s, g, m.inv_hw,
                            m.sig2, m.Lx, m.Ly, p0, c0, p1, c1, (const int *)B.start,
                            (const float2 *)B.sq, (const uint32_t *)B.ssid, B.om,
                            (unsigned long long *)nullptr);
