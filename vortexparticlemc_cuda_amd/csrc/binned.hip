@@ -6,8 +6,10 @@
 // step is split in four passes so that the 64 lanes of a wave share one
 // neighbourhood:
 //   gen     one thread per sample: Philox (or injected) draws -> query point,
-//           MIS displacement, fine-cell key; atomic per-cell count (slot).
-//   scatter samples into cell order (exclusive scan of the counts).
+//           MIS displacement, fine-cell ("tile") key.
+//   sort    stable radix sort of the keys (sort.hip: no global atomics, whose
+//           throughput would bound this step), ending in cell-sorted 16-byte
+//           sample records and the tile start offsets.
 //   eval    one workgroup per cell ("tile").  The workgroup stages the
 //           cell's candidates in LDS once: the neighbour stencil (per column,
 //           the rows within h_w of the cell rectangle: contiguous runs of
@@ -42,21 +44,20 @@ struct BinGeom {
 };
 
 struct BinBufs {
-    int *key, *slot;
+    uint32_t *key;            // tile key per sample (ntile: no query)
     float2 *q, *r;
     float *om;
-    float2 *sq;
-    uint32_t *ssid;
-    int *count, *start;
-    void *scan_ws;
-    size_t scan_bytes;
+    float4 *srec;             // cell-sorted samples: (x, y, sample id bits, 0)
+    uint32_t *keys_sorted;
+    int *start;               // ntile + 1 tile offsets into srec
+    void *sort_ws;
+    size_t sort_bytes;
 };
 
 template <int MODE, bool PER>
 __global__ void __launch_bounds__(BIN_THREADS)
-    k_bin_gen(McD d, BinGeom g, int *__restrict__ key, int *__restrict__ slot,
-              float2 *__restrict__ q, float2 *__restrict__ r, float *__restrict__ om,
-              int *__restrict__ count)
+    k_bin_gen(McD d, BinGeom g, uint32_t *__restrict__ key, float2 *__restrict__ q,
+              float2 *__restrict__ r, float *__restrict__ om)
 {
     size_t sid = (size_t)blockIdx.x * BIN_THREADS + threadIdx.x;
     if (sid >= g.S) return;
@@ -70,30 +71,14 @@ __global__ void __launch_bounds__(BIN_THREADS)
     bool live = n_s > 0 &&
                 gen_sample<MODE, PER>(d, S, n_s, owner, si, k, d.owner[owner], qx, qy, rx, ry);
     r[sid] = make_float2(rx, ry);
+    q[sid] = make_float2(qx, qy);
     if (!live) {
-        key[sid] = -1;
+        key[sid] = (uint32_t)g.ntile;   // sorts after every tile
         return;
     }
-    q[sid] = make_float2(qx, qy);
     int cx = min(max((int)(qx * g.inv_cx), 0), g.nx - 1);
     int cy = min(max((int)(qy * g.inv_cy), 0), g.ny - 1);
-    int kk = si * g.ncell + cx * g.ny + cy;
-    key[sid] = kk;
-    slot[sid] = atomicAdd(count + kk, 1);
-}
-
-__global__ void __launch_bounds__(BIN_THREADS)
-    k_bin_scatter(BinGeom g, const int *__restrict__ key, const int *__restrict__ slot,
-                  const float2 *__restrict__ q, const int *__restrict__ start,
-                  float2 *__restrict__ sq, uint32_t *__restrict__ ssid)
-{
-    size_t sid = (size_t)blockIdx.x * BIN_THREADS + threadIdx.x;
-    if (sid >= g.S) return;
-    int kk = key[sid];
-    if (kk < 0) return;
-    int dst = start[kk] + slot[sid];
-    sq[dst] = q[sid];
-    ssid[dst] = (uint32_t)sid;
+    key[sid] = (uint32_t)(si * g.ncell + cx * g.ny + cy);
 }
 
 constexpr int EVAL_CAP = 1024;                  // staged candidates per pass (16 KB)
@@ -266,9 +251,8 @@ __global__ void __launch_bounds__(BIN_THREADS)
     k_bin_eval(BinGeom g, float inv_hw, float sig2, float Lx, float Ly,
                const float4 *__restrict__ pts0, const int *__restrict__ cs0,
                const float4 *__restrict__ pts1, const int *__restrict__ cs1,
-               const int *__restrict__ start, const float2 *__restrict__ sq,
-               const uint32_t *__restrict__ ssid, float *__restrict__ om,
-               unsigned long long *__restrict__ eval_count)
+               const int *__restrict__ start, const float4 *__restrict__ srec,
+               float *__restrict__ om, unsigned long long *__restrict__ eval_count)
 {
     // workgroups are dealt round-robin to the 8 XCDs: give each XCD a
     // contiguous range of tiles so that neighbouring cells share its L2
@@ -291,12 +275,14 @@ __global__ void __launch_bounds__(BIN_THREADS)
     const int lane = threadIdx.x & (WAVE - 1), wid = threadIdx.x / WAVE;
     for (int bb = beg; bb < end; bb += BIN_THREADS * SPL) {   // uniform trip count
         float px[SPL], py[SPL], pp[SPL], acc[SPL];
+        uint32_t sid[SPL];
         bool live[SPL];
 #pragma unroll
         for (int s = 0; s < SPL; ++s) {
             const int idx = bb + (wid * SPL + s) * WAVE + lane;
             live[s] = idx < end;   // tail lanes take the cell centre, results dropped
-            const float2 p = live[s] ? sq[idx] : make_float2(ox, oy);
+            const float4 p = live[s] ? srec[idx] : make_float4(ox, oy, 0.f, 0.f);
+            sid[s] = __float_as_uint(p.z);
             px[s] = p.x - ox;
             py[s] = p.y - oy;
             pp[s] = fmaf(px[s], px[s], py[s] * py[s]);
@@ -309,10 +295,8 @@ __global__ void __launch_bounds__(BIN_THREADS)
             tile_eval<SPL, COUNT>(L.cand, n, px, py, pp, inv_hw, acc, ec, live);
         }
 #pragma unroll
-        for (int s = 0; s < SPL; ++s) {
-            const int idx = bb + (wid * SPL + s) * WAVE + lane;
-            if (live[s]) om[ssid[idx]] = 2.f * sig2 * acc[s];
-        }
+        for (int s = 0; s < SPL; ++s)
+            if (live[s]) om[sid[s]] = 2.f * sig2 * acc[s];
     }
     if (COUNT) {
         unsigned a = wave_sum(ec.in_support), b = wave_sum(ec.visited);
@@ -384,29 +368,26 @@ static int bin_geom(const McD &m, const vpmc_mc_desc *in, BinGeom *g)
 
 static size_t bin_ws_bytes(const BinGeom &g)
 {
-    size_t S = g.S > 0 ? g.S : 1, T = (size_t)2 * g.ncell;
-    return 2 * align256(S * 4) + 2 * align256(S * 8) + align256(S * 4) + align256(S * 8) +
-           align256(S * 4) + align256(T * 4) + align256((T + 1) * 4) +
-           scan_workspace_bytes((int)T, sizeof(int)) + 4096;
+    size_t S = g.S > 0 ? g.S : 1, T = (size_t)g.ntile;
+    return align256(S * 4) + 2 * align256(S * 8) + align256(S * 4) + align256(S * 16) +
+           align256(S * 4) + align256((T + 1) * 4) + radix_workspace_bytes((int)S) + 4096;
 }
 
 static int carve_bin(const BinGeom &g, void *ws, size_t bytes, BinBufs *B)
 {
     Carve cv(ws, bytes);
-    size_t S = g.S > 0 ? g.S : 1, T = (size_t)2 * g.ncell;
-    B->key = cv.take<int>(S);
-    B->slot = cv.take<int>(S);
+    size_t S = g.S > 0 ? g.S : 1, T = (size_t)g.ntile;
+    B->key = cv.take<uint32_t>(S);
     B->q = cv.take<float2>(S);
     B->r = cv.take<float2>(S);
     B->om = cv.take<float>(S);
-    B->sq = cv.take<float2>(S);
-    B->ssid = cv.take<uint32_t>(S);
-    B->count = cv.take<int>(T);
+    B->srec = cv.take<float4>(S);
+    B->keys_sorted = cv.take<uint32_t>(S);
     B->start = cv.take<int>(T + 1);
-    B->scan_bytes = scan_workspace_bytes((int)T, sizeof(int));
-    B->scan_ws = cv.take<char>(B->scan_bytes);
-    if (!B->key || !B->slot || !B->q || !B->r || !B->om || !B->sq || !B->ssid || !B->count ||
-        !B->start || !B->scan_ws)
+    B->sort_bytes = radix_workspace_bytes((int)S);
+    B->sort_ws = cv.take<char>(B->sort_bytes);
+    if (!B->key || !B->q || !B->r || !B->om || !B->srec || !B->keys_sorted || !B->start ||
+        !B->sort_ws)
         return VPMC_ERR_WORKSPACE;
     return VPMC_OK;
 }
@@ -418,7 +399,7 @@ static void launch_gen_final(const McD &m, const BinGeom &g, const BinBufs &B, b
     if (!final) {
         unsigned gb = (unsigned)((g.S + BIN_THREADS - 1) / BIN_THREADS);
         hipLaunchKernelGGL((k_bin_gen<MODE, PER>), dim3(gb), dim3(BIN_THREADS), 0, s, m, g, B.key,
-                           B.slot, B.q, B.r, B.om, B.count);
+                           B.q, B.r, B.om);
     } else {
         unsigned gb = (unsigned)((2 * (size_t)m.n_owner + 3) / 4);
         hipLaunchKernelGGL((k_bin_final<MODE, PER>), dim3(gb), dim3(BIN_THREADS), 0, s, m, g,
@@ -458,15 +439,12 @@ int mc_binned_impl(const vpmc_mc_desc *d, void *ws, size_t ws_bytes, hipStream_t
     VPMC_TRY(bin_geom(m, d, &g));
     BinBufs B;
     VPMC_TRY(carve_bin(g, ws, ws_bytes, &B));
-    const int T = 2 * g.ncell;
-    VPMC_TRY(fill_i32(B.count, 0, (size_t)T, s));
     launch_mode(d->mode, d->periodic, m, g, B, false, s);
     VPMC_CHECK_LAUNCH();
-    VPMC_TRY(scan_i32(B.count, B.start, T, false, B.start + T, B.scan_ws, B.scan_bytes, s));
-    unsigned gb = (unsigned)((g.S + BIN_THREADS - 1) / BIN_THREADS);
-    hipLaunchKernelGGL(k_bin_scatter, dim3(gb), dim3(BIN_THREADS), 0, s, g, (const int *)B.key,
-                       (const int *)B.slot, (const float2 *)B.q, (const int *)B.start, B.sq, B.ssid);
-    VPMC_CHECK_LAUNCH();
+    int key_bits = 1;
+    while (key_bits < 32 && ((uint32_t)g.ntile >> key_bits) != 0u) ++key_bits;
+    VPMC_TRY(radix_sort_to_records(B.key, (int)g.S, key_bits, (const float2 *)B.q, B.srec,
+                                   B.keys_sorted, g.ntile, B.start, B.sort_ws, B.sort_bytes, s));
     const float4 *p0 = m.set[0].field.pts, *p1 = m.set[1].field.pts;
     const int *c0 = m.set[0].field.cell_start, *c1 = m.set[1].field.cell_start;
     VPMC_TRY(record_event(d->eval_events[0], s));
@@ -474,12 +452,11 @@ int mc_binned_impl(const vpmc_mc_desc *d, void *ws, size_t ws_bytes, hipStream_t
     if (d->eval_count)
         hipLaunchKernelGGL((k_bin_eval<2, true>), eval_grid, dim3(BIN_THREADS), 0, s, g, m.inv_hw,
                            m.sig2, m.Lx, m.Ly, p0, c0, p1, c1, (const int *)B.start,
-                           (const float2 *)B.sq, (const uint32_t *)B.ssid, B.om, d->eval_count);
+                           (const float4 *)B.srec, B.om, d->eval_count);
     else
         hipLaunchKernelGGL((k_bin_eval<2, false>), eval_grid, dim3(BIN_THREADS), 0, s, g, m.inv_hw,
                            m.sig2, m.Lx, m.Ly, p0, c0, p1, c1, (const int *)B.start,
-                           (const float2 *)B.sq, (const uint32_t *)B.ssid, B.om,
-                           (unsigned long long *)nullptr);
+                           (const float4 *)B.srec, B.om, (unsigned long long *)nullptr);
     VPMC_CHECK_LAUNCH();
     VPMC_TRY(record_event(d->eval_events[1], s));
     launch_mode(d->mode, d->periodic, m, g, B, true, s);

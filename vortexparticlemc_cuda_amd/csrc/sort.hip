@@ -1,0 +1,240 @@
+// sort.hip -- stable LSD radix sort of (key, value) pairs without global atomics.
+//
+// Random-key binning with one global atomic per element is bound by the
+// device's atomic throughput (about 2e10/s on MI355X, 1 ms for 2e7 keys).
+// Here every pass over an 8-bit digit is:
+//   hist     one block per RS_TILE contiguous keys: digit histogram in LDS,
+//            written digit-major (hist[d * nblk + b]);
+//   scan     exclusive prefix of hist: the output offset of (digit, block);
+//   scatter  the block re-reads its keys in order; the rank of a key among
+//            equal digits of its block comes from wave ballots (eight
+//            ballots give the lanes sharing a digit) and per-wave counts in
+//            LDS, so the pass is stable and the result is the unique stable
+//            order.
+// The 16-byte sample records move with the keys in every pass.
+#include "vpmc_internal.hpp"
+
+namespace vpmc {
+
+constexpr int RS_THREADS = 256;
+constexpr int RS_ITEMS = 16;
+constexpr int RS_TILE = RS_THREADS * RS_ITEMS;   // keys per block
+constexpr int RS_BINS = 256;                     // 8-bit digits
+constexpr int RS_WAVES = RS_THREADS / WAVE;
+static_assert(RS_THREADS == RS_BINS, "one thread per digit");
+
+__global__ void __launch_bounds__(RS_THREADS)
+    k_rs_hist(const uint32_t *__restrict__ keys, int n, int shift, int nblk, int *__restrict__ hist)
+{
+    __shared__ int h[RS_BINS];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    const int base = blockIdx.x * RS_TILE;
+    for (int r = 0; r < RS_ITEMS; ++r) {
+        const int i = base + r * RS_THREADS + threadIdx.x;
+        if (i < n) atomicAdd(&h[(keys[i] >> shift) & (RS_BINS - 1)], 1);
+    }
+    __syncthreads();
+    hist[(size_t)threadIdx.x * nblk + blockIdx.x] = h[threadIdx.x];
+}
+
+// exclusive prefix over the block of one value per thread; *total = sum
+__device__ __forceinline__ int rs_block_excl_scan(int v, int *wsum, int *total)
+{
+    const int lane = threadIdx.x & (WAVE - 1), wid = threadIdx.x / WAVE;
+    int x = v;
+#pragma unroll
+    for (int o = 1; o < WAVE; o <<= 1) {
+        int y = __shfl_up(x, o, WAVE);
+        if (lane >= o) x += y;
+    }
+    if (lane == WAVE - 1) wsum[wid] = x;
+    __syncthreads();
+    int base = 0, tot = 0;
+#pragma unroll
+    for (int w = 0; w < RS_WAVES; ++w) {
+        base += (w < wid) ? wsum[w] : 0;
+        tot += wsum[w];
+    }
+    *total = tot;
+    return base + x - v;
+}
+
+// One block = RS_TILE consecutive input positions; wave w owns the
+// contiguous quarter [w * RS_TILE/4, (w+1) * RS_TILE/4) and ranks its keys in
+// element order against a wave-private digit histogram in LDS (no barrier
+// inside the wave's loop: one wave's LDS operations complete in order).  One
+// barrier later, per-digit offsets across waves give every key its stable
+// position inside the block; keys and local input indices are placed in LDS
+// in digit order and written out in that order, so that a digit's elements
+// of this block go to one contiguous range starting at offs[d * nblk + b].
+// The 16-byte records travel with the keys: each is read from the block's
+// own input slice.  recs_in == NULL (first pass): the record of input
+// position i is built as (q[i].x, q[i].y, i bits, 0).
+constexpr int RS_WITEMS = RS_TILE / RS_WAVES / WAVE;   // keys per lane
+__global__ void __launch_bounds__(RS_THREADS)
+    k_rs_scatter(const uint32_t *__restrict__ keys_in, const float4 *__restrict__ recs_in,
+                 const float2 *__restrict__ q, int n, int shift, int nblk,
+                 const int *__restrict__ offs, uint32_t *__restrict__ keys_out,
+                 float4 *__restrict__ recs_out)
+{
+    __shared__ uint32_t lk[RS_TILE];
+    __shared__ uint16_t li[RS_TILE];        // local input index (< RS_TILE)
+    __shared__ int wh[RS_WAVES][RS_BINS];   // per-wave digit counts, then offsets
+    __shared__ int gofs[RS_BINS];           // global position - local position
+    __shared__ int wsum[RS_WAVES];
+    const int t = threadIdx.x, lane = t & (WAVE - 1), wid = t / WAVE;
+    const int base = blockIdx.x * RS_TILE;
+    const int cnt = min(RS_TILE, n - base);
+#pragma unroll
+    for (int w = 0; w < RS_WAVES; ++w) wh[w][t] = 0;
+    __syncthreads();
+    const unsigned long long lt = (1ull << lane) - 1ull;
+    uint32_t kr[RS_WITEMS];
+    int rk[RS_WITEMS];
+#pragma unroll
+    for (int r = 0; r < RS_WITEMS; ++r) {
+        const int e = (wid * RS_WITEMS + r) * WAVE + lane;
+        const bool valid = e < cnt;
+        kr[r] = valid ? keys_in[base + e] : 0u;
+        const int d = (int)((kr[r] >> shift) & (RS_BINS - 1));
+        unsigned long long peers = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+            const bool bit = (d >> b) & 1;
+            const unsigned long long m = __ballot(bit);
+            peers &= bit ? m : ~m;
+        }
+        const int before = __popcll(peers & lt);
+        const int prior = valid ? wh[wid][d] : 0;
+        __builtin_amdgcn_wave_barrier();   // every lane's read precedes the leader's write
+        rk[r] = prior + before;
+        if (valid && before == 0) wh[wid][d] = prior + __popcll(peers);
+        __builtin_amdgcn_wave_barrier();
+    }
+    __syncthreads();
+    // digit t: exclusive offsets of the waves inside the block
+    int c[RS_WAVES], sum = 0;
+#pragma unroll
+    for (int w = 0; w < RS_WAVES; ++w) {
+        c[w] = wh[w][t];
+        sum += c[w];
+    }
+    int tot;
+    const int ls = rs_block_excl_scan(sum, wsum, &tot);
+    gofs[t] = offs[(size_t)t * nblk + blockIdx.x] - ls;
+    int acc = ls;
+#pragma unroll
+    for (int w = 0; w < RS_WAVES; ++w) {
+        wh[w][t] = acc;   // column t: read above by this thread only
+        acc += c[w];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < RS_WITEMS; ++r) {
+        const int e = (wid * RS_WITEMS + r) * WAVE + lane;
+        if (e < cnt) {
+            const int pos = wh[wid][(kr[r] >> shift) & (RS_BINS - 1)] + rk[r];
+            lk[pos] = kr[r];
+            li[pos] = (uint16_t)e;
+        }
+    }
+    __syncthreads();
+    // all RS_ITEMS record loads in flight before the stores
+    uint32_t ko[RS_ITEMS];
+    int po[RS_ITEMS];
+    float4 rv[RS_ITEMS];
+#pragma unroll
+    for (int j = 0; j < RS_ITEMS; ++j) {
+        const int i = j * RS_THREADS + t;
+        po[j] = -1;
+        if (i < cnt) {
+            ko[j] = lk[i];
+            const int src = base + li[i];
+            po[j] = gofs[(ko[j] >> shift) & (RS_BINS - 1)] + i;
+            if (recs_in) {
+                rv[j] = recs_in[src];
+            } else {
+                const float2 p = q[src];
+                rv[j] = make_float4(p.x, p.y, __uint_as_float((uint32_t)src), 0.f);
+            }
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < RS_ITEMS; ++j) {
+        if (po[j] >= 0) {
+            keys_out[po[j]] = ko[j];
+            recs_out[po[j]] = rv[j];
+        }
+    }
+}
+
+// start[k] = first position with key >= k, for k in [0, nkeys]; keys sorted,
+// every key in [0, nkeys]
+__global__ void k_rs_bounds(const uint32_t *__restrict__ keys, int n, int nkeys,
+                            int *__restrict__ start)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i > n) return;
+    const int prev = i == 0 ? -1 : (int)keys[i - 1];
+    const int cur = i == n ? nkeys : (int)keys[i];
+    for (int k = prev + 1; k <= cur; ++k) start[k] = i;
+}
+
+static int rs_nblk(int n) { return (n + RS_TILE - 1) / RS_TILE; }
+
+size_t radix_workspace_bytes(int n)
+{
+    const size_t m = (size_t)(n > 0 ? n : 1);
+    const size_t h = (size_t)RS_BINS * rs_nblk(n > 0 ? n : 1);
+    return align256(m * 4) + align256(m * 16) + 2 * align256(h * 4) +
+           scan_workspace_bytes((int)h, sizeof(int)) + 1024;
+}
+
+// Sorts keys[0..n) stably (key_bits significant bits) and moves with them the
+// records (q[i].x, q[i].y, i bits, 0) of the input positions i: rec and
+// keys_sorted receive the sorted records and keys, start[0..nkeys] the first
+// position with key >= k (every key must lie in [0, nkeys]).  keys is left
+// unchanged.  Pass p writes the final buffers when P-1-p is even, the
+// workspace copies otherwise, so the last pass always ends in the outputs.
+int radix_sort_to_records(uint32_t *keys, int n, int key_bits, const float2 *q, float4 *rec,
+                          uint32_t *keys_sorted, int nkeys, int *start, void *ws,
+                          size_t ws_bytes, hipStream_t s)
+{
+    if (n < 0 || key_bits < 1 || key_bits > 32) return VPMC_ERR_ARG;
+    if (n == 0) return fill_i32(start, 0, (size_t)nkeys + 1, s);
+    const int nblk = rs_nblk(n);
+    const size_t h = (size_t)RS_BINS * nblk;
+    if (h >= ((size_t)1 << 31)) return VPMC_ERR_ARG;
+    Carve cv(ws, ws_bytes);
+    uint32_t *kb = cv.take<uint32_t>((size_t)n);
+    float4 *rb = cv.take<float4>((size_t)n);
+    int *hist = cv.take<int>(h);
+    int *offs = cv.take<int>(h);
+    const size_t sb = scan_workspace_bytes((int)h, sizeof(int));
+    void *sws = cv.take<char>(sb);
+    if (!kb || !rb || !hist || !offs || !sws) return VPMC_ERR_WORKSPACE;
+    const int passes = (key_bits + 7) / 8;
+    const uint32_t *kin = keys;
+    const float4 *rin = nullptr;
+    for (int p = 0; p < passes; ++p) {
+        const bool to_out = ((passes - 1 - p) & 1) == 0;
+        uint32_t *kout = to_out ? keys_sorted : kb;
+        float4 *rout = to_out ? rec : rb;
+        hipLaunchKernelGGL(k_rs_hist, dim3(nblk), dim3(RS_THREADS), 0, s, kin, n, 8 * p, nblk,
+                           hist);
+        VPMC_CHECK_LAUNCH();
+        VPMC_TRY(scan_i32(hist, offs, (int)h, false, nullptr, sws, sb, s));
+        hipLaunchKernelGGL(k_rs_scatter, dim3(nblk), dim3(RS_THREADS), 0, s, kin, rin, q, n,
+                           8 * p, nblk, (const int *)offs, kout, rout);
+        VPMC_CHECK_LAUNCH();
+        kin = kout;
+        rin = rout;
+    }
+    hipLaunchKernelGGL(k_rs_bounds, dim3((n + 1 + 255) / 256), dim3(256), 0, s,
+                       (const uint32_t *)keys_sorted, n, nkeys, start);
+    VPMC_CHECK_LAUNCH();
+    return VPMC_OK;
+}
+
+}  // namespace vpmc

@@ -65,6 +65,11 @@ int scan_i32(const int *in, int *out, int n, bool inclusive, int *total, void *w
 int scan_f64(const double *in, double *out, int n, bool inclusive, double *total, void *ws,
              size_t ws_bytes, hipStream_t s);
 int fill_i32(int *p, int v, size_t n, hipStream_t s);
+// stable LSD radix sort of tile keys (sort.hip)
+size_t radix_workspace_bytes(int n);
+int radix_sort_to_records(uint32_t *keys, int n, int key_bits, const float2 *q, float4 *rec,
+                          uint32_t *keys_sorted, int nkeys, int *start, void *ws,
+                          size_t ws_bytes, hipStream_t s);
 int fill_f32(float *p, float v, size_t n, hipStream_t s);
 
 // ---- field (field.hip) -------------------------------------------------------
