@@ -185,6 +185,10 @@ typedef struct vpmc_mc_desc {
                                          evaluation kernel (k_bin_eval, or
                                          k_mc_estimate for the direct algorithm);
                                          NULL: not recorded                    */
+    int          spl;                 /* binned evaluation: samples per lane (1,
+                                         2 or 4; 0 means 2).  A tuning knob only:
+                                         every sample sums its candidates in the
+                                         same order, so results are identical */
 } vpmc_mc_desc;
 
 enum { VPMC_ALGO_BINNED = 0, VPMC_ALGO_DIRECT = 1 };
@@ -220,6 +224,55 @@ size_t vpmc_step_workspace_bytes(const vpmc_mc_desc *d);
 int vpmc_step(vpmc_mc_desc *d, int n_total, float *pos_xy, const float *omega,
               const int *sub_pos, const int *sub_neg, float *vel_xy, float dt,
               void *workspace, size_t workspace_bytes, void *stream);
+
+/* ---------------------------------------------------------------------------
+ * Ensemble of independent periodic simulations (no reference counterpart: the
+ * reference runs one Simulation per process).  All simulations advance in
+ * one set of launches per step: one field sort, one sample generation, one
+ * sample sort, one evaluation, one reduction and one advection for the whole
+ * ensemble.  Simulation s is the particle range [h_offset[s], h_offset[s+1])
+ * of pos_xy / omega, with its own h_w and box side (Lx = Ly = h_L[s]).  Its
+ * trajectory equals vpmc_step's for that simulation alone with sim = s, the
+ * same seed, cells_per_hw and spl, bit for bit.  The workspace holds the
+ * ensemble's state (subsets, CDFs, per-simulation tables, the step counter):
+ * keep it and the descriptor unchanged between init and the steps.
+ */
+typedef struct vpmc_ensemble_desc {
+    int          n_sim;
+    const int   *h_offset;            /* host: n_sim + 1 particle offsets      */
+    const float *h_h_w;               /* host: per-simulation h_w               */
+    const float *h_L;                 /* host: per-simulation box side          */
+    int          N1, N2;
+    int          mode;                /* VPMC_MODE_*                            */
+    int          cells_per_hw;        /* fine grid (vpmc_field_grid)            */
+    int          spl;                 /* samples per lane of the evaluation     */
+    uint64_t     seed;
+    float        dt;
+    float       *pos_xy;              /* device: all particles (float2)         */
+    const float *omega;               /* device: strengths                      */
+    float       *vel_xy;              /* device out (float2), or NULL           */
+    void        *eval_events[2];      /* optional hipEvent_t pair around the
+                                         estimator stages                       */
+} vpmc_ensemble_desc;
+
+size_t vpmc_ensemble_workspace_bytes(const vpmc_ensemble_desc *e);
+/* subsets, FP64 CDFs, guides and tables; step counter 0.  Synchronises. */
+int vpmc_ensemble_init(const vpmc_ensemble_desc *e, void *workspace, size_t workspace_bytes,
+                       void *stream);
+/* one step of every simulation; no host synchronisation, capturable */
+int vpmc_ensemble_step(const vpmc_ensemble_desc *e, void *workspace, size_t workspace_bytes,
+                       void *stream);
+/* sets the step counter (the Philox step of the next step) */
+int vpmc_ensemble_set_step(const vpmc_ensemble_desc *e, void *workspace, size_t workspace_bytes,
+                           uint32_t step, void *stream);
+/* captures `steps` consecutive vpmc_ensemble_step calls on `stream` (not the
+   null stream) into an executable HIP graph; replaying it advances the
+   ensemble by `steps` steps (the step counter lives in the workspace). */
+int vpmc_ensemble_graph_create(const vpmc_ensemble_desc *e, void *workspace,
+                               size_t workspace_bytes, int steps, void *stream,
+                               void **graph_exec);
+int vpmc_graph_launch(void *graph_exec, void *stream);
+int vpmc_graph_destroy(void *graph_exec);
 
 /* ---------------------------------------------------------------------------
  * RBF strength initialisation.

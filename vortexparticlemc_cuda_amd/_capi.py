@@ -54,7 +54,14 @@ class McDesc(C.Structure):
                 ("periodic", ci), ("seed", C.c_uint64), ("step", C.c_uint32),
                 ("sim", C.c_uint32), ("uniforms", vp), ("aux_draws", ci), ("n_owner", ci),
                 ("owner_xy", vp), ("set", SignSet * 2), ("vel_sign", vp),
-                ("eval_count", vp), ("algo", ci), ("eval_events", vp * 2)]
+                ("eval_count", vp), ("algo", ci), ("eval_events", vp * 2), ("spl", ci)]
+
+
+class EnsembleDesc(C.Structure):
+    _fields_ = [("n_sim", ci), ("h_offset", vp), ("h_h_w", vp), ("h_L", vp), ("N1", ci),
+                ("N2", ci), ("mode", ci), ("cells_per_hw", ci), ("spl", ci),
+                ("seed", C.c_uint64), ("dt", cf), ("pos_xy", vp), ("omega", vp),
+                ("vel_xy", vp), ("eval_events", vp * 2)]
 
 
 _lib = None
@@ -107,6 +114,15 @@ def lib():
                               C.POINTER(cd), vp, csz, vp]
     L.vpmc_direct_sum_eps.argtypes = [ci, vp, ci, ci, vp, vp, cd, ci, cd, vp, vp]
     L.vpmc_direct_sum_blob.argtypes = [ci, vp, ci, vp, vp, cd, cd, vp, vp]
+    ep = C.POINTER(EnsembleDesc)
+    L.vpmc_ensemble_workspace_bytes.restype = csz
+    L.vpmc_ensemble_workspace_bytes.argtypes = [ep]
+    L.vpmc_ensemble_init.argtypes = [ep, vp, csz, vp]
+    L.vpmc_ensemble_step.argtypes = [ep, vp, csz, vp]
+    L.vpmc_ensemble_set_step.argtypes = [ep, vp, csz, C.c_uint32, vp]
+    L.vpmc_ensemble_graph_create.argtypes = [ep, vp, csz, ci, vp, C.POINTER(vp)]
+    L.vpmc_graph_launch.argtypes = [vp, vp]
+    L.vpmc_graph_destroy.argtypes = [vp]
     _lib = L
     return L
 
@@ -127,4 +143,7 @@ EXPORTS = [
     "vpmc_mc_estimate_binned", "vpmc_advect", "vpmc_step_workspace_bytes",
     "vpmc_step", "vpmc_rbf_workspace_bytes", "vpmc_rbf_matvec", "vpmc_rbf_matvec_f64", "vpmc_rbf_matvec_particles",
     "vpmc_cg_rbf", "vpmc_cg_workspace_bytes", "vpmc_direct_sum_eps", "vpmc_direct_sum_blob",
+    "vpmc_ensemble_workspace_bytes", "vpmc_ensemble_init", "vpmc_ensemble_step",
+    "vpmc_ensemble_set_step", "vpmc_ensemble_graph_create", "vpmc_graph_launch",
+    "vpmc_graph_destroy",
 ]

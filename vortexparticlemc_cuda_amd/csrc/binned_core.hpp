@@ -1,0 +1,295 @@
+// binned_core.hpp -- device building blocks of the binned estimator, shared
+// by the single-simulation entry points (binned.hip) and the ensemble
+// (ensemble.hip) so that both compute bit-identical results per simulation.
+// The algorithm is described in binned.hip.
+#pragma once
+#include "mc_common.hpp"
+
+namespace vpmc {
+
+constexpr int BIN_THREADS = 256;
+
+struct BinGeom {
+    int ncell, nx, ny;
+    float inv_cx, inv_cy;
+    float cw, ch, rs2;        // cell size, squared search radius (field geometry)
+    int ntile, per_xcd;       // 2 * ncell tiles; tiles per XCD in the launch order
+    int cols;                 // stencil half-width (columns)
+    int ext[MAX_COL_EXT];     // rows each side for column offset ox (index ox+cols), -1: none
+    uint32_t NS;              // N1 + N2
+    size_t S;                 // samples
+};
+
+// grid of the tiles and stencil for descriptor geometry (host)
+int bin_geom(const McD &m, int N1, int N2, int n_owner, BinGeom *g);
+
+// One sample: draws, query point, MIS displacement and tile key.  Writes
+// key[out] (tile_base + local tile, or dead_key without a query), q[out],
+// r[out] and om[out] = 0.
+template <int MODE, bool PER>
+__device__ __forceinline__ void bin_gen_one(const McD &d, const BinGeom &g, size_t sid,
+                                            size_t out, uint32_t tile_base, uint32_t dead_key,
+                                            uint32_t *__restrict__ key, float2 *__restrict__ q,
+                                            float2 *__restrict__ r, float *__restrict__ om)
+{
+    const uint32_t pair = (uint32_t)(sid / g.NS);
+    const int k = (int)(sid - (size_t)pair * g.NS);
+    const int owner = (int)(pair >> 1), si = (int)(pair & 1);
+    const SetD &S = d.set[si];
+    const int n_s = S.count_dev ? *S.count_dev : S.n;
+    om[out] = 0.f;
+    float qx = 0.f, qy = 0.f, rx = 0.f, ry = 0.f;
+    bool live = n_s > 0 &&
+                gen_sample<MODE, PER>(d, S, n_s, owner, si, k, d.owner[owner], qx, qy, rx, ry);
+    r[out] = make_float2(rx, ry);
+    q[out] = make_float2(qx, qy);
+    if (!live) {
+        key[out] = dead_key;
+        return;
+    }
+    int cx = min(max((int)(qx * g.inv_cx), 0), g.nx - 1);
+    int cy = min(max((int)(qy * g.inv_cy), 0), g.ny - 1);
+    key[out] = tile_base + (uint32_t)(si * g.ncell + cx * g.ny + cy);
+}
+
+constexpr int EVAL_CAP = 1024;                  // staged candidates per pass (16 KB)
+constexpr int MAX_PIECES = 3 * MAX_COL_EXT;     // <= 3 runs per stencil column
+constexpr int BIN_WAVES = BIN_THREADS / WAVE;
+constexpr float CBRT4 = 1.5874010519681994f;    // 4^(1/3)
+
+// exclusive block-wide prefix of v (one value per thread); *total gets the sum
+__device__ __forceinline__ int block_excl_scan(int v, int *wsum, int *total)
+{
+    const int lane = threadIdx.x & (WAVE - 1), wid = threadIdx.x / WAVE;
+    int x = v;
+#pragma unroll
+    for (int o = 1; o < WAVE; o <<= 1) {
+        int y = __shfl_up(x, o, WAVE);
+        if (lane >= o) x += y;
+    }
+    if (lane == WAVE - 1) wsum[wid] = x;
+    __syncthreads();
+    int base = 0, tot = 0;
+#pragma unroll
+    for (int w = 0; w < BIN_WAVES; ++w) {
+        base += (w < wid) ? wsum[w] : 0;
+        tot += wsum[w];
+    }
+    __syncthreads();   // wsum may be reused by the caller
+    *total = tot;
+    return base + x - v;
+}
+
+// The tile's stencil as runs ("pieces") of the sorted particle array, in the
+// order column -cols..cols and, per column, [rows wrapped below] [rows
+// wrapped above] [rows in range]; p_off is the running candidate count.
+struct TileLds {
+    float4 cand[EVAL_CAP];
+    int p_beg[MAX_PIECES];
+    int p_off[MAX_PIECES + 1];
+    float p_sx[MAX_PIECES], p_sy[MAX_PIECES];
+    int wsum[BIN_WAVES];
+    int n_stage;
+};
+
+__device__ __forceinline__ int tile_pieces(const BinGeom &g, const int *__restrict__ cell_start,
+                                           int cx, int cy, float Lx, float Ly, TileLds &L)
+{
+    const int ncol = 2 * g.cols + 1;   // <= MAX_COL_EXT <= BIN_THREADS
+    const int j = threadIdx.x;
+    int len[3] = {0, 0, 0}, pb[3] = {0, 0, 0};
+    float sx = 0.f, sy[3] = {-Ly, Ly, 0.f};
+    if (j < ncol) {
+        const int e = g.ext[j];
+        if (e >= 0) {
+            int colw = cx + j - g.cols;
+            if (colw < 0) { colw += g.nx; sx = -Lx; }
+            else if (colw >= g.nx) { colw -= g.nx; sx = Lx; }
+            const int *csc = cell_start + (size_t)colw * g.ny;
+            int r0 = cy - e, r1 = cy + e;
+            if (r0 < 0) {   // rows ny+r0 .. ny-1: particle image y - Ly
+                pb[0] = csc[g.ny + r0];
+                len[0] = csc[g.ny] - pb[0];
+                r0 = 0;
+            }
+            if (r1 >= g.ny) {   // rows 0 .. r1-ny: particle image y + Ly
+                pb[1] = csc[0];
+                len[1] = csc[r1 - g.ny + 1] - pb[1];
+                r1 = g.ny - 1;
+            }
+            pb[2] = csc[r0];
+            len[2] = csc[r1 + 1] - pb[2];
+        }
+    }
+    int total;
+    int off = block_excl_scan(len[0] + len[1] + len[2], L.wsum, &total);
+    if (j < ncol) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            L.p_beg[3 * j + k] = pb[k];
+            L.p_off[3 * j + k] = off;
+            L.p_sx[3 * j + k] = sx;
+            L.p_sy[3 * j + k] = sy[k];
+            off += len[k];
+        }
+    }
+    if (j == 0) L.p_off[3 * ncol] = total;
+    __syncthreads();
+    return total;
+}
+
+// stages the candidates with raw index in [r0, r1) (r1 - r0 <= EVAL_CAP) that
+// can reach the cell: cand[0 .. n) in raw order, padded with zero weights to
+// a multiple of 4; returns n (padded).  Starts and ends with a barrier.
+__device__ __forceinline__ int tile_stage(const BinGeom &g, const float4 *__restrict__ pts,
+                                          int npieces, int r0, int r1, float ox, float oy,
+                                          TileLds &L)
+{
+    const int lane = threadIdx.x & (WAVE - 1), wid = threadIdx.x / WAVE;
+    const float hx = 0.5f * g.cw, hy = 0.5f * g.ch;
+    int n = 0;
+    __syncthreads();   // the previous pass's readers are done with cand
+    for (int i0 = r0; i0 < r1; i0 += BIN_THREADS) {
+        const int i = i0 + threadIdx.x;
+        bool keep = false;
+        float4 rec = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (i < r1) {
+            int lo = 0, hi = npieces;   // last piece with p_off <= i
+            while (hi - lo > 1) {
+                int mid = (lo + hi) >> 1;
+                if (L.p_off[mid] <= i) lo = mid; else hi = mid;
+            }
+            const float4 c = pts[L.p_beg[lo] + (i - L.p_off[lo])];
+            const float x = (c.x + L.p_sx[lo]) - ox, y = (c.y + L.p_sy[lo]) - oy;
+            const float gx = fmaxf(fabsf(x) - hx, 0.f), gy = fmaxf(fabsf(y) - hy, 0.f);
+            keep = fmaf(gx, gx, gy * gy) <= g.rs2;
+            rec = make_float4(-2.f * x, -2.f * y, fmaf(x, x, y * y), c.z);
+        }
+        const unsigned long long m = __ballot(keep);
+        const int pre = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                                  __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+        if (lane == 0) L.wsum[wid] = __popcll(m);
+        __syncthreads();
+        int base = n, tot = 0;
+#pragma unroll
+        for (int w = 0; w < BIN_WAVES; ++w) {
+            base += (w < wid) ? L.wsum[w] : 0;
+            tot += L.wsum[w];
+        }
+        if (keep) L.cand[base + pre] = rec;
+        n += tot;
+        __syncthreads();   // wsum is rewritten by the next round
+    }
+    const int npad = (n + 3) & ~3;
+    if ((int)threadIdx.x < npad - n) L.cand[n + threadIdx.x] = make_float4(0.f, 0.f, 0.f, 0.f);
+    __syncthreads();
+    return npad;
+}
+
+template <int SPL, bool COUNT>
+__device__ __forceinline__ void tile_eval(const float4 *__restrict__ cand, int n,
+                                          const float (&px)[SPL], const float (&py)[SPL],
+                                          const float (&pp)[SPL], float inv_hw,
+                                          float (&acc)[SPL], EvalCount &ec,
+                                          const bool (&live)[SPL])
+{
+    auto one = [&](const float4 &c) {
+#pragma unroll
+        for (int s = 0; s < SPL; ++s) {
+            float r2 = fmaf(c.x, px[s], fmaf(c.y, py[s], c.z + pp[s]));
+            float q = __builtin_amdgcn_sqrtf(fabsf(r2));   // |.|: rounding below 0
+            float t = __builtin_amdgcn_fmed3f(fmaf(-inv_hw, q, 1.f), 0.f, 1.f);
+            float v = __builtin_amdgcn_fmed3f(fmaf(t, CBRT4, -0.5f * CBRT4), 0.f, 1.f);
+            float t2 = t * t, v3 = v * v * v;
+            acc[s] = fmaf(c.w, fmaf(t2, t, -v3), acc[s]);
+            if (COUNT && live[s] && c.w > 0.f) {   // padding records have w = 0
+                ec.in_support += (t > 0.f) ? 1u : 0u;
+                ec.visited += 1u;
+            }
+        }
+    };
+    for (int k = 0; k < n; k += 4) {   // n is a multiple of 4 (zero-weight padding)
+        const float4 c0 = cand[k], c1 = cand[k + 1], c2 = cand[k + 2], c3 = cand[k + 3];
+        one(c0);
+        one(c1);
+        one(c2);
+        one(c3);
+    }
+}
+
+// The samples srec[beg, end) of local tile `tile` (sign tile / ncell) of
+// the simulation described by (m, g): stage, evaluate, write omega to
+// om[sample id].  Called by the whole workgroup.
+template <int SPL, bool COUNT>
+__device__ __forceinline__ void bin_eval_tile(const McD &m, const BinGeom &g, int tile, int beg,
+                                              int end, const float4 *__restrict__ srec,
+                                              float *__restrict__ om, TileLds &L,
+                                              EvalCount &ec)
+{
+    const int si = tile >= g.ncell ? 1 : 0;
+    const int c = tile - si * g.ncell;
+    const int cx = c / g.ny, cy = c - cx * g.ny;
+    const float ox = ((float)cx + 0.5f) * g.cw, oy = ((float)cy + 0.5f) * g.ch;
+    const float4 *__restrict__ pts = m.set[si].field.pts;
+    const int total = tile_pieces(g, m.set[si].field.cell_start, cx, cy, m.Lx, m.Ly, L);
+    const int npieces = 3 * (2 * g.cols + 1);
+    const int npass = (total + EVAL_CAP - 1) / EVAL_CAP;
+    int n = 0;
+    if (npass == 1) n = tile_stage(g, pts, npieces, 0, total, ox, oy, L);
+    const int lane = threadIdx.x & (WAVE - 1), wid = threadIdx.x / WAVE;
+    for (int bb = beg; bb < end; bb += BIN_THREADS * SPL) {   // uniform trip count
+        float px[SPL], py[SPL], pp[SPL], acc[SPL];
+        uint32_t sid[SPL];
+        bool live[SPL];
+#pragma unroll
+        for (int s = 0; s < SPL; ++s) {
+            const int idx = bb + (wid * SPL + s) * WAVE + lane;
+            live[s] = idx < end;   // tail lanes take the cell centre, results dropped
+            const float4 p = live[s] ? srec[idx] : make_float4(ox, oy, 0.f, 0.f);
+            sid[s] = __float_as_uint(p.z);
+            px[s] = p.x - ox;
+            py[s] = p.y - oy;
+            pp[s] = fmaf(px[s], px[s], py[s] * py[s]);
+            acc[s] = 0.f;
+        }
+        for (int pass = 0; pass < npass; ++pass) {
+            if (npass > 1)
+                n = tile_stage(g, pts, npieces, pass * EVAL_CAP,
+                               min(total, (pass + 1) * EVAL_CAP), ox, oy, L);
+            tile_eval<SPL, COUNT>(L.cand, n, px, py, pp, m.inv_hw, acc, ec, live);
+        }
+#pragma unroll
+        for (int s = 0; s < SPL; ++s)
+            if (live[s]) om[sid[s]] = 2.f * m.sig2 * acc[s];
+    }
+}
+
+// One wave: the estimate of (owner, sign) = pair from its N1+N2 samples at
+// r[base..], om[base..] in sample order (deterministic butterfly sum).
+template <int MODE, bool PER>
+__device__ __forceinline__ void bin_final_pair(const McD &d, const BinGeom &g, int pair,
+                                               const float2 *__restrict__ r,
+                                               const float *__restrict__ om)
+{
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int owner = pair >> 1, si = pair & 1;
+    const SetD &S = d.set[si];
+    float sx = 0.f, sy = 0.f;
+    const int n_s = S.count_dev ? *S.count_dev : S.n;
+    if (n_s > 0) {
+        const float Om = (float)(*S.total_dev);
+        float cum = Om;
+        if (MODE == VPMC_MODE_REFERENCE && PER) cum *= 9.f;
+        const size_t base = (size_t)pair * g.NS;
+        for (uint32_t k = lane; k < g.NS; k += WAVE) {
+            float2 rr = r[base + k];
+            mis_contrib<MODE, PER>(d, om[base + k], rr.x, rr.y, Om, cum, sx, sy);
+        }
+    }
+    sx = wave_sum(sx);
+    sy = wave_sum(sy);
+    if (lane == 0)
+        d.vel_sign[(size_t)si * d.n_owner + owner] = make_float2(sx * d.inv_tot, sy * d.inv_tot);
+}
+
+}  // namespace vpmc

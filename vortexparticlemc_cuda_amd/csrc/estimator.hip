@@ -78,17 +78,7 @@ __global__ void k_advect(int n, float2 *__restrict__ pos, const float2 *__restri
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    float2 a = vs[i], b = vs[(size_t)n + i];
-    float2 v = make_float2(__fsub_rn(a.x, b.x), __fsub_rn(a.y, b.y));
-    if (vel) vel[i] = v;
-    float2 p = pos[i];
-    p.x = __fadd_rn(p.x, __fmul_rn(v.x, dt));
-    p.y = __fadd_rn(p.y, __fmul_rn(v.y, dt));
-    if (per) {   // src/cpu/simulation_step.cpp:57-59
-        p.x = __fsub_rn(p.x, __fmul_rn(floorf(p.x / Lx), Lx));
-        p.y = __fsub_rn(p.y, __fmul_rn(floorf(p.y / Ly), Ly));
-    }
-    pos[i] = p;
+    pos[i] = advect_one(vs[i], vs[(size_t)n + i], pos[i], vel ? vel + i : nullptr, dt, Lx, Ly, per);
 }
 
 // c_m = G_{4m}(Z+iZ): G_4 from the Eisenstein q-series, the rest from the

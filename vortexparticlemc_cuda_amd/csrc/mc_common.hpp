@@ -236,4 +236,19 @@ __device__ __forceinline__ void mis_contrib(const McD &d, float om, float rx, fl
     }
 }
 
+// u = v+ - v-, x += u dt, periodic wrap (src/cpu/simulation_step.cpp:46-71)
+__device__ __forceinline__ float2 advect_one(float2 a, float2 b, float2 p, float2 *vel_out,
+                                             float dt, float Lx, float Ly, int per)
+{
+    float2 v = make_float2(__fsub_rn(a.x, b.x), __fsub_rn(a.y, b.y));
+    if (vel_out) *vel_out = v;
+    p.x = __fadd_rn(p.x, __fmul_rn(v.x, dt));
+    p.y = __fadd_rn(p.y, __fmul_rn(v.y, dt));
+    if (per) {   // src/cpu/simulation_step.cpp:57-59
+        p.x = __fsub_rn(p.x, __fmul_rn(floorf(p.x / Lx), Lx));
+        p.y = __fsub_rn(p.y, __fmul_rn(floorf(p.y / Ly), Ly));
+    }
+    return p;
+}
+
 }  // namespace vpmc

@@ -191,15 +191,16 @@ size_t radix_workspace_bytes(int n)
            scan_workspace_bytes((int)h, sizeof(int)) + 1024;
 }
 
-// Sorts keys[0..n) stably (key_bits significant bits) and moves with them the
-// records (q[i].x, q[i].y, i bits, 0) of the input positions i: rec and
-// keys_sorted receive the sorted records and keys, start[0..nkeys] the first
-// position with key >= k (every key must lie in [0, nkeys]).  keys is left
-// unchanged.  Pass p writes the final buffers when P-1-p is even, the
-// workspace copies otherwise, so the last pass always ends in the outputs.
-int radix_sort_to_records(uint32_t *keys, int n, int key_bits, const float2 *q, float4 *rec,
-                          uint32_t *keys_sorted, int nkeys, int *start, void *ws,
-                          size_t ws_bytes, hipStream_t s)
+// Sorts keys[0..n) stably (key_bits significant bits) and moves with them
+// the 16-byte records of the input positions i: recs0[i] if recs0 is given,
+// else (q[i].x, q[i].y, i bits, 0).  rec and keys_sorted receive the sorted
+// records and keys, start[0..nkeys] the first position with key >= k (every
+// key must lie in [0, nkeys]).  keys and recs0 are left unchanged.  Pass p
+// writes the final buffers when P-1-p is even, the workspace copies
+// otherwise, so the last pass always ends in the outputs.
+int radix_sort_to_records(const uint32_t *keys, int n, int key_bits, const float2 *q,
+                          const float4 *recs0, float4 *rec, uint32_t *keys_sorted, int nkeys,
+                          int *start, void *ws, size_t ws_bytes, hipStream_t s)
 {
     if (n < 0 || key_bits < 1 || key_bits > 32) return VPMC_ERR_ARG;
     if (n == 0) return fill_i32(start, 0, (size_t)nkeys + 1, s);
@@ -216,7 +217,7 @@ int radix_sort_to_records(uint32_t *keys, int n, int key_bits, const float2 *q, 
     if (!kb || !rb || !hist || !offs || !sws) return VPMC_ERR_WORKSPACE;
     const int passes = (key_bits + 7) / 8;
     const uint32_t *kin = keys;
-    const float4 *rin = nullptr;
+    const float4 *rin = recs0;
     for (int p = 0; p < passes; ++p) {
         const bool to_out = ((passes - 1 - p) & 1) == 0;
         uint32_t *kout = to_out ? keys_sorted : kb;

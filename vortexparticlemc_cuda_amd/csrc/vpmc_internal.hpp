@@ -67,9 +67,15 @@ int scan_f64(const double *in, double *out, int n, bool inclusive, double *total
 int fill_i32(int *p, int v, size_t n, hipStream_t s);
 // stable LSD radix sort of tile keys (sort.hip)
 size_t radix_workspace_bytes(int n);
-int radix_sort_to_records(uint32_t *keys, int n, int key_bits, const float2 *q, float4 *rec,
-                          uint32_t *keys_sorted, int nkeys, int *start, void *ws,
-                          size_t ws_bytes, hipStream_t s);
+int radix_sort_to_records(const uint32_t *keys, int n, int key_bits, const float2 *q,
+                          const float4 *recs0, float4 *rec, uint32_t *keys_sorted, int nkeys,
+                          int *start, void *ws, size_t ws_bytes, hipStream_t s);
+inline int key_bits_for(uint32_t max_key)
+{
+    int b = 1;
+    while (b < 32 && (max_key >> b) != 0u) ++b;
+    return b;
+}
 int fill_f32(float *p, float v, size_t n, hipStream_t s);
 
 // ---- field (field.hip) -------------------------------------------------------

@@ -1,0 +1,124 @@
+"""Ensemble of independent periodic simulations advanced together.
+
+The reference runs one Simulation per process; this class drives the
+library's ensemble entry points (vpmc_ensemble_*): every stage of a step is
+one launch for all simulations, and a sequence of steps can be captured into
+a HIP graph and replayed.  Simulation s evolves exactly as a plain
+Simulation(sim=s) with the same seed, cells_per_hw and spl would.
+"""
+import ctypes as C
+
+import numpy as np
+import torch
+
+from . import ops
+from ._capi import MODE_CONSISTENT, EnsembleDesc, check, lib
+
+
+class Ensemble:
+    def __init__(self, positions, strengths, h_w, L=1.0, N1=64, N2=64, dt=5e-4,
+                 mode=MODE_CONSISTENT, seed=0, cells_per_hw=8, spl=0, device="cuda"):
+        """positions[s]: (N_s, 2) array, strengths[s]: (N_s,), h_w: per-simulation
+        kernel radius (scalar or sequence), L: box side (scalar or sequence)"""
+        self.device = torch.device(device)
+        ns = len(positions)
+        assert ns == len(strengths) and ns > 0
+        sizes = [int(np.asarray(p).shape[0]) for p in positions]
+        self.offsets = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int32)
+        self.h_w = np.broadcast_to(np.asarray(h_w, np.float32), (ns,)).copy()
+        self.L = np.broadcast_to(np.asarray(L, np.float32), (ns,)).copy()
+        pos = np.concatenate([np.asarray(p, np.float32).reshape(-1, 2) for p in positions])
+        om = np.concatenate([np.asarray(w, np.float32).ravel() for w in strengths])
+        self.pos = torch.as_tensor(pos, device=self.device).contiguous()
+        self.omega = torch.as_tensor(om, device=self.device).contiguous()
+        self.vel = torch.zeros_like(self.pos)
+        self.n_sim = ns
+        self.N = int(self.offsets[-1])
+        d = EnsembleDesc()
+        d.n_sim = ns
+        d.h_offset = self.offsets.ctypes.data
+        d.h_h_w = self.h_w.ctypes.data
+        d.h_L = self.L.ctypes.data
+        d.N1, d.N2 = int(N1), int(N2)
+        d.mode = int(mode)
+        d.cells_per_hw = int(cells_per_hw)
+        d.spl = int(spl)
+        d.seed = int(seed)
+        d.dt = float(dt)
+        d.pos_xy = ops._p(self.pos)
+        d.omega = ops._p(self.omega)
+        d.vel_xy = ops._p(self.vel)
+        self.desc = d
+        need = lib().vpmc_ensemble_workspace_bytes(C.byref(d))
+        if need == 0:
+            raise ValueError("invalid ensemble description (sizes, h_w, grid)")
+        self.ws = ops.workspace(need, self.device)
+        self.step_index = 0
+        self._graphs = {}
+        check(lib().vpmc_ensemble_init(C.byref(d), ops._p(self.ws), self.ws.numel(),
+                                       ops._stream()), "vpmc_ensemble_init")
+
+    @property
+    def workspace_bytes(self):
+        return self.ws.numel()
+
+    def set_step(self, step):
+        check(lib().vpmc_ensemble_set_step(C.byref(self.desc), ops._p(self.ws), self.ws.numel(),
+                                           int(step), ops._stream()), "vpmc_ensemble_set_step")
+        self.step_index = int(step)
+
+    def step(self, k=1, events=None):
+        """k eager steps; events: optional (start, stop) torch.cuda.Event pair
+        recorded by the library around the estimator stages of the last step"""
+        for i in range(k):
+            if events is not None and i == k - 1:
+                self.desc.eval_events[0] = events[0].cuda_event
+                self.desc.eval_events[1] = events[1].cuda_event
+            check(lib().vpmc_ensemble_step(C.byref(self.desc), ops._p(self.ws), self.ws.numel(),
+                                           ops._stream()), "vpmc_ensemble_step")
+            self.step_index += 1
+        self.desc.eval_events[0] = None
+        self.desc.eval_events[1] = None
+
+    def capture(self, steps):
+        """a HIP graph of `steps` steps on the current (non-default) stream"""
+        if steps in self._graphs:
+            return self._graphs[steps]
+        s = torch.cuda.current_stream()
+        if s.cuda_stream == 0:
+            raise RuntimeError("graph capture needs a non-default stream")
+        h = C.c_void_p()
+        check(lib().vpmc_ensemble_graph_create(C.byref(self.desc), ops._p(self.ws),
+                                               self.ws.numel(), int(steps), C.c_void_p(s.cuda_stream),
+                                               C.byref(h)), "vpmc_ensemble_graph_create")
+        self._graphs[steps] = h
+        return h
+
+    def replay(self, steps):
+        """advance by `steps` steps through the captured graph"""
+        h = self.capture(steps)
+        check(lib().vpmc_graph_launch(h, ops._stream()), "vpmc_graph_launch")
+        self.step_index += steps
+
+    def close(self):
+        for h in self._graphs.values():
+            lib().vpmc_graph_destroy(h)
+        self._graphs = {}
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def positions(self, s=None):
+        p = self.pos.cpu().numpy()
+        if s is None:
+            return p
+        return p[self.offsets[s]:self.offsets[s + 1]]
+
+    def velocities(self, s=None):
+        v = self.vel.cpu().numpy()
+        if s is None:
+            return v
+        return v[self.offsets[s]:self.offsets[s + 1]]
