@@ -66,6 +66,39 @@ def _event_pair(s):
     return e
 
 
+PLAIN_TUNE = tuple((c, s, t) for c in (8, 12, 16) for s in (1, 2, 4) for t in (64, 128, 256))
+
+
+def tune_plain(sim, steps=3):
+    """(cells_per_hw, spl, eval_threads) with the shortest plain step on this
+    workload; every candidate computes identical velocities (the knobs change
+    only the launch shape and the grid), timed with HIP events"""
+    import torch
+
+    from vortexparticlemc_cuda_amd.sim import Simulation
+    pos = sim.pos.cpu().numpy()
+    om = sim.omega.cpu().numpy()
+    times = {}
+    for cph, spl, th in PLAIN_TUNE:
+        try:
+            t = Simulation(sim.P, device=sim.device, cells_per_hw=cph, mode=sim.mode,
+                           seed=sim.seed, spl=spl, eval_threads=th)
+            t.set_particles(pos, om)
+            run_steps(t, 1)
+            a = torch.cuda.Event(enable_timing=True)
+            b = torch.cuda.Event(enable_timing=True)
+            a.record()
+            run_steps(t, steps)
+            b.record()
+            torch.cuda.synchronize()
+        except Exception:   # stencil does not fit this workgroup / grid
+            continue
+        times["%d,%d,%d" % (cph, spl, th)] = a.elapsed_time(b) / steps
+        del t
+    best = min(times, key=times.get)
+    return tuple(int(v) for v in best.split(",")), times
+
+
 def run_steps(sim, K, events=None):
     """K plain steps.  If events is a dict of lists, it receives per step the
     event pairs around the whole estimator ("est", torch's stream) and around
@@ -147,6 +180,113 @@ def velocity_error(sim):
                 rms_err_vs_tg=rms(u - ut) / rms(ut))
 
 
+def ensemble_bench(args, device):
+    """The batched ensemble (64 periodic simulations, Nx drawn from
+    {64, 96, 128, 192, 256}, 8 Gaussian vortices each, hwr = 8, N1 = N2 = 64,
+    dt = 5e-4): particle-steps/s of the batched path (one launch per stage,
+    eager and replayed from a HIP graph) and of the plain per-simulation path
+    on the same ensemble.  Both compute the same estimator (bit-identical per
+    simulation), so their RMS velocity errors are equal by construction."""
+    import torch
+
+    from vortexparticlemc_cuda_amd import ops
+    from vortexparticlemc_cuda_amd.ensemble import Ensemble, autotune
+    from vortexparticlemc_cuda_amd.sim import SimParam, Simulation
+    from vortexparticlemc_cuda_amd.workloads import gaussian_ensemble
+
+    data = gaussian_ensemble(args.ens_sims)
+    hwr, N1, N2, dt = 8, args.ens_N, args.ens_N, 5e-4
+    Ps, pos, w = [], [], []
+    t0 = time.perf_counter()
+    for nx, p, om, _ in data:
+        P = SimParam(dt=dt, resX=nx, hwr=hwr)
+        P.N1, P.N2 = N1, N2
+        h = 1.0 / nx
+        b = (om * 1.0).astype(np.float32)
+        # the reference's tolerance (src/cuda/cg_rbf_init.cu:53): tighter fits
+        # of these fields amplify the Gram matrix's near-null modes into
+        # alternating strengths, which inflate the estimator's variance
+        wt, it, rr, st = ops.cg_rbf(torch.as_tensor(p, device=device), torch.as_tensor(b, device=device),
+                                    float(P.h_w), tol=5e-3, max_iter=300, refine=True)
+        Ps.append(P)
+        pos.append(p)
+        w.append(wt.cpu().numpy())
+    t_cg = time.perf_counter() - t0
+    hw = [float(P.h_w) for P in Ps]
+    N = sum(p.shape[0] for p in pos)
+    log("ensemble: %d sims, %d particles, CG init %.1f s" % (len(pos), N, t_cg))
+    cph, spl, th, tune_ms = autotune(pos, w, hw, N1=N1, N2=N2, device=device)
+    log("autotune: cells_per_hw=%d spl=%d eval_threads=%d" % (cph, spl, th))
+    ens = Ensemble(pos, w, hw, N1=N1, N2=N2, dt=dt, seed=0, cells_per_hw=cph, spl=spl,
+                   eval_threads=th, device=device)
+    ens.step(1)
+    v0 = ens.vel.clone()
+    p0 = [torch.as_tensor(p, device=device) for p in pos]
+    # RMS error vs the FP64 blob direct sum on the four smallest simulations
+    small = sorted(range(len(pos)), key=lambda s: pos[s].shape[0])[:4]
+    num = den = 0.0
+    for s in small:
+        pd = p0[s].double()
+        ud = ops.direct_sum_blob(pd, pd, torch.as_tensor(w[s], device=device).double(), hw[s])
+        um = v0[int(ens.offsets[s]):int(ens.offsets[s + 1])].double()
+        num += float(((um - ud) ** 2).sum())
+        den += float((ud ** 2).sum())
+    err = (num / den) ** 0.5
+    K = args.ens_steps
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ens.step(K)
+    torch.cuda.synchronize()
+    t_eager = time.perf_counter() - t0
+    side = torch.cuda.Stream()
+    with torch.cuda.stream(side):
+        ens.capture(K)
+        ens.replay(K)   # first replay uploads the graph
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        ens.replay(K)
+        torch.cuda.synchronize()
+        t_graph = time.perf_counter() - t0
+    ens.close()
+    batched = N * K / min(t_eager, t_graph)
+    del ens
+    torch.cuda.empty_cache()
+    # plain path: one Simulation per member, stepped one after the other
+    sims = []
+    for s in range(len(pos)):
+        sim = Simulation(Ps[s], device=device, cells_per_hw=cph, seed=0, sim=s, spl=spl,
+                         eval_threads=th)
+        sim.set_particles(pos[s], w[s])
+        sims.append(sim)
+    for sim in sims:
+        sim.step()
+    Kp = max(1, args.ens_plain_steps)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(Kp):
+        for sim in sims:
+            sim.step()
+    torch.cuda.synchronize()
+    t_plain = time.perf_counter() - t0
+    plain = N * Kp / t_plain
+    del sims
+    torch.cuda.empty_cache()
+    log("batched %.3g particle-steps/s (eager %.1f ms/step, graph %.1f ms/step), plain %.3g"
+        % (batched, t_eager / K * 1e3, t_graph / K * 1e3, plain))
+    return {"workload": "64 periodic simulations, Nx from {64,96,128,192,256} (seed 20261203), "
+                        "8 Gaussian vortices each, hwr=8, N1=N2=%d, dt=5e-4, consistent "
+                        "estimator, CG-RBF strengths" % N1,
+            "simulations": len(pos), "particles": N,
+            "batched_particle_steps_per_s": batched,
+            "batched_ms_per_step_eager": t_eager / K * 1e3,
+            "batched_ms_per_step_graph": t_graph / K * 1e3,
+            "plain_particle_steps_per_s": plain, "plain_ms_per_step": t_plain / Kp * 1e3,
+            "batched_over_plain": batched / plain, "steps_timed": K, "plain_steps_timed": Kp,
+            "rms_err_vs_direct": err, "cells_per_hw": cph, "spl": spl, "eval_threads": th,
+            "autotune_ms": tune_ms,
+            "cg_init_s": t_cg}
+
+
 def read_traffic(kernel, args):
     """HBM bytes per launch of `kernel` at this workload, from the committed
     counter summary (FETCH_SIZE + WRITE_SIZE, copy-calibrated); None if absent"""
@@ -208,8 +348,15 @@ def main():
     ap.add_argument("--algo", choices=["binned", "direct"], default="binned")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-error", action="store_true")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="skip the launch-shape tuning of the plain step")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="also run a plain copy of MIB MiB (HBM counter calibration)")
+    ap.add_argument("--ens-sims", type=int, default=64,
+                    help="simulations in the batched-ensemble measurement (0: skip it)")
+    ap.add_argument("--ens-N", type=int, default=64, help="N1 = N2 of the ensemble")
+    ap.add_argument("--ens-steps", type=int, default=10)
+    ap.add_argument("--ens-plain-steps", type=int, default=2)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU; this build has no multi-GPU path "
@@ -223,6 +370,18 @@ def main():
                       cells_per_hw=args.cells_per_hw)
     sim.algo = 0 if args.algo == "binned" else 1
     log("cg init", init)
+    tuned = None
+    if args.algo == "binned" and not args.no_tune:
+        (cph, spl, th), tune_ms = tune_plain(sim)
+        tuned = {"cells_per_hw": cph, "spl": spl, "eval_threads": th, "candidates_ms": tune_ms}
+        log("tuned plain step: cells_per_hw=%d spl=%d eval_threads=%d (%.3f ms)"
+            % (cph, spl, th, tune_ms["%d,%d,%d" % (cph, spl, th)]))
+        from vortexparticlemc_cuda_amd.sim import Simulation
+        s2 = Simulation(sim.P, device=device, cells_per_hw=cph, mode=sim.mode, seed=sim.seed,
+                        spl=spl, eval_threads=th)
+        s2.set_particles(sim.pos.cpu().numpy(), sim.omega.cpu().numpy())
+        sim = s2
+        args.cells_per_hw = cph
     err = None
     if not args.no_error:
         err = velocity_error(sim)
@@ -251,6 +410,7 @@ def main():
     traffic = read_traffic(eval_kernel, args)
     if args.copy_probe:
         copy_probe(args.copy_probe << 20)
+    batched = ensemble_bench(args, device) if args.ens_sims > 0 else None
     cpu = None if args.no_cpu else cpu_baseline(args.Nx, args.hwr, args.N1, args.N2)
     value = N * args.steps / elapsed
     line = {
@@ -272,6 +432,7 @@ def main():
                                % (args.Nx, N, args.hwr, args.N1),
                    "particles": N, "N1": args.N1, "N2": args.N2, "hwr": args.hwr,
                    "cells_per_hw": args.cells_per_hw, "estimator_algo": args.algo,
+                   "tuned": tuned,
                    "parallelism": "single GPU"},
         "roofline": {"kernel": eval_kernel, "bound": "valu", "achieved": achieved,
                      "peak": PEAK_FP32, "unit": "FLOP/s", "frac": achieved / PEAK_FP32,
@@ -286,6 +447,7 @@ def main():
         "estimator_fraction_of_step": est_avg_s / (elapsed / args.steps),
         "error": err,
         "cg_init": init,
+        "batched": batched,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -189,6 +189,10 @@ typedef struct vpmc_mc_desc {
                                          2 or 4; 0 means 2).  A tuning knob only:
                                          every sample sums its candidates in the
                                          same order, so results are identical */
+    int          eval_threads;        /* binned evaluation workgroup size: 64,
+                                         128 or 256 (0: 256), a tuning knob with
+                                         identical results; the stencil's
+                                         columns must fit (VPMC_ERR_GRID)     */
 } vpmc_mc_desc;
 
 enum { VPMC_ALGO_BINNED = 0, VPMC_ALGO_DIRECT = 1 };
@@ -246,6 +250,7 @@ typedef struct vpmc_ensemble_desc {
     int          mode;                /* VPMC_MODE_*                            */
     int          cells_per_hw;        /* fine grid (vpmc_field_grid)            */
     int          spl;                 /* samples per lane of the evaluation     */
+    int          eval_threads;        /* evaluation workgroup size (64/128/256) */
     uint64_t     seed;
     float        dt;
     float       *pos_xy;              /* device: all particles (float2)         */

@@ -34,16 +34,16 @@ def _ensemble_data(sizes=(16, 24, 32), hwr=4, seed=5):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("spl", [1, 2, 4])
-def test_ensemble_equals_plain_path(device, spl):
+@pytest.mark.parametrize("spl,threads", [(1, 256), (2, 256), (4, 256), (2, 64), (1, 128)])
+def test_ensemble_equals_plain_path(device, spl, threads):
     from vortexparticlemc_cuda_amd.ensemble import Ensemble
     from vortexparticlemc_cuda_amd.sim import Simulation
-    _say("ensemble == plain, spl=%d" % spl)
+    _say("ensemble == plain, spl=%d threads=%d" % (spl, threads))
     N1, N2, steps, seed = 32, 32, 3, 77
     data = _ensemble_data()
     ens = Ensemble([d[1] for d in data], [d[2] for d in data], [d[0].h_w for d in data],
                    N1=N1, N2=N2, dt=float(data[0][0].dt), seed=seed, cells_per_hw=8, spl=spl,
-                   device=device)
+                   eval_threads=threads, device=device)
     ens.step(steps)
     torch.cuda.synchronize()
     for s, (P, pos, w) in enumerate(data):

@@ -54,13 +54,14 @@ class McDesc(C.Structure):
                 ("periodic", ci), ("seed", C.c_uint64), ("step", C.c_uint32),
                 ("sim", C.c_uint32), ("uniforms", vp), ("aux_draws", ci), ("n_owner", ci),
                 ("owner_xy", vp), ("set", SignSet * 2), ("vel_sign", vp),
-                ("eval_count", vp), ("algo", ci), ("eval_events", vp * 2), ("spl", ci)]
+                ("eval_count", vp), ("algo", ci), ("eval_events", vp * 2), ("spl", ci),
+                ("eval_threads", ci)]
 
 
 class EnsembleDesc(C.Structure):
     _fields_ = [("n_sim", ci), ("h_offset", vp), ("h_h_w", vp), ("h_L", vp), ("N1", ci),
                 ("N2", ci), ("mode", ci), ("cells_per_hw", ci), ("spl", ci),
-                ("seed", C.c_uint64), ("dt", cf), ("pos_xy", vp), ("omega", vp),
+                ("eval_threads", ci), ("seed", C.c_uint64), ("dt", cf), ("pos_xy", vp), ("omega", vp),
                 ("vel_xy", vp), ("eval_events", vp * 2)]
 
 

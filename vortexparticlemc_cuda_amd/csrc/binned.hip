@@ -51,8 +51,8 @@ __global__ void __launch_bounds__(BIN_THREADS)
     bin_gen_one<MODE, PER>(d, g, sid, sid, 0u, (uint32_t)g.ntile, key, q, r, om);
 }
 
-template <int SPL, bool COUNT>
-__global__ void __launch_bounds__(BIN_THREADS)
+template <int THREADS, int SPL, bool COUNT>
+__global__ void __launch_bounds__(THREADS)
     k_bin_eval(McD m, BinGeom g, const int *__restrict__ start, const float4 *__restrict__ srec,
                float *__restrict__ om, unsigned long long *__restrict__ eval_count)
 {
@@ -62,9 +62,9 @@ __global__ void __launch_bounds__(BIN_THREADS)
     if (tile >= g.ntile) return;
     const int beg = start[tile], end = start[tile + 1];
     if (beg >= end) return;   // uniform over the workgroup
-    __shared__ TileLds L;
+    __shared__ TileLds<THREADS> L;
     EvalCount ec;
-    bin_eval_tile<SPL, COUNT>(m, g, tile, beg, end, srec, om, L, ec);
+    bin_eval_tile<THREADS, SPL, COUNT>(m, g, tile, beg, end, srec, om, L, ec);
     if (COUNT) {
         unsigned a = wave_sum(ec.in_support), b = wave_sum(ec.visited);
         if ((threadIdx.x & (WAVE - 1)) == 0) {
@@ -83,18 +83,34 @@ __global__ void __launch_bounds__(BIN_THREADS)
     bin_final_pair<MODE, PER>(d, g, pair, r, om);
 }
 
-// eval launch for a runtime samples-per-lane choice
-template <bool COUNT>
-static void launch_eval(int spl, const McD &m, const BinGeom &g, const int *start,
-                        const float4 *srec, float *om, unsigned long long *cnt, hipStream_t s)
+// eval launch for runtime workgroup-size and samples-per-lane choices
+template <int THREADS, bool COUNT>
+static void launch_eval_t(int spl, const McD &m, const BinGeom &g, const int *start,
+                          const float4 *srec, float *om, unsigned long long *cnt, hipStream_t s)
 {
-    const dim3 grid((unsigned)(8 * g.per_xcd)), block(BIN_THREADS);
+    const dim3 grid((unsigned)(8 * g.per_xcd)), block(THREADS);
     if (spl == 1)
-        hipLaunchKernelGGL((k_bin_eval<1, COUNT>), grid, block, 0, s, m, g, start, srec, om, cnt);
+        hipLaunchKernelGGL((k_bin_eval<THREADS, 1, COUNT>), grid, block, 0, s, m, g, start, srec,
+                           om, cnt);
     else if (spl == 4)
-        hipLaunchKernelGGL((k_bin_eval<4, COUNT>), grid, block, 0, s, m, g, start, srec, om, cnt);
+        hipLaunchKernelGGL((k_bin_eval<THREADS, 4, COUNT>), grid, block, 0, s, m, g, start, srec,
+                           om, cnt);
     else
-        hipLaunchKernelGGL((k_bin_eval<2, COUNT>), grid, block, 0, s, m, g, start, srec, om, cnt);
+        hipLaunchKernelGGL((k_bin_eval<THREADS, 2, COUNT>), grid, block, 0, s, m, g, start, srec,
+                           om, cnt);
+}
+
+template <bool COUNT>
+static int launch_eval(int threads, int spl, const McD &m, const BinGeom &g, const int *start,
+                       const float4 *srec, float *om, unsigned long long *cnt, hipStream_t s)
+{
+    if (threads == 0) threads = 256;
+    if (threads != 64 && threads != 128 && threads != 256) return VPMC_ERR_ARG;
+    if (!tile_threads_ok(g, threads)) return VPMC_ERR_GRID;
+    if (threads == 64) launch_eval_t<64, COUNT>(spl, m, g, start, srec, om, cnt, s);
+    else if (threads == 128) launch_eval_t<128, COUNT>(spl, m, g, start, srec, om, cnt, s);
+    else launch_eval_t<256, COUNT>(spl, m, g, start, srec, om, cnt, s);
+    return VPMC_OK;
 }
 
 int bin_geom(const McD &m, int N1, int N2, int n_owner, BinGeom *g)
@@ -210,9 +226,11 @@ int mc_binned_impl(const vpmc_mc_desc *d, void *ws, size_t ws_bytes, hipStream_t
                                    B.start, B.sort_ws, B.sort_bytes, s));
     VPMC_TRY(record_event(d->eval_events[0], s));
     if (d->eval_count)
-        launch_eval<true>(d->spl, m, g, B.start, B.srec, B.om, d->eval_count, s);
+        VPMC_TRY(launch_eval<true>(d->eval_threads, d->spl, m, g, B.start, B.srec, B.om,
+                                   d->eval_count, s));
     else
-        launch_eval<false>(d->spl, m, g, B.start, B.srec, B.om, nullptr, s);
+        VPMC_TRY(launch_eval<false>(d->eval_threads, d->spl, m, g, B.start, B.srec, B.om, nullptr,
+                                    s));
     VPMC_CHECK_LAUNCH();
     VPMC_TRY(record_event(d->eval_events[1], s));
     launch_mode(d->mode, d->periodic, m, g, B, true, s);

@@ -52,12 +52,20 @@ __device__ __forceinline__ void bin_gen_one(const McD &d, const BinGeom &g, size
     key[out] = tile_base + (uint32_t)(si * g.ncell + cx * g.ny + cy);
 }
 
-constexpr int EVAL_CAP = 1024;                  // staged candidates per pass (16 KB)
-constexpr int MAX_PIECES = 3 * MAX_COL_EXT;     // <= 3 runs per stencil column
-constexpr int BIN_WAVES = BIN_THREADS / WAVE;
 constexpr float CBRT4 = 1.5874010519681994f;    // 4^(1/3)
 
+// Tile evaluation comes in workgroup sizes THREADS = 64, 128, 256 (chosen by
+// the host for the samples per tile); a pass stages up to CAP = 4 THREADS
+// candidates in LDS (more candidates take several passes).
+template <int THREADS> struct TileCfg {
+    static constexpr int WAVES = THREADS / WAVE;
+    static constexpr int CAP = 4 * THREADS;
+    static constexpr int MAXCOL = THREADS < MAX_COL_EXT ? THREADS : MAX_COL_EXT;
+    static constexpr int MAXP = 3 * MAXCOL;   // <= 3 runs per stencil column
+};
+
 // exclusive block-wide prefix of v (one value per thread); *total gets the sum
+template <int THREADS>
 __device__ __forceinline__ int block_excl_scan(int v, int *wsum, int *total)
 {
     const int lane = threadIdx.x & (WAVE - 1), wid = threadIdx.x / WAVE;
@@ -71,7 +79,7 @@ __device__ __forceinline__ int block_excl_scan(int v, int *wsum, int *total)
     __syncthreads();
     int base = 0, tot = 0;
 #pragma unroll
-    for (int w = 0; w < BIN_WAVES; ++w) {
+    for (int w = 0; w < TileCfg<THREADS>::WAVES; ++w) {
         base += (w < wid) ? wsum[w] : 0;
         tot += wsum[w];
     }
@@ -83,19 +91,22 @@ __device__ __forceinline__ int block_excl_scan(int v, int *wsum, int *total)
 // The tile's stencil as runs ("pieces") of the sorted particle array, in the
 // order column -cols..cols and, per column, [rows wrapped below] [rows
 // wrapped above] [rows in range]; p_off is the running candidate count.
-struct TileLds {
-    float4 cand[EVAL_CAP];
-    int p_beg[MAX_PIECES];
-    int p_off[MAX_PIECES + 1];
-    float p_sx[MAX_PIECES], p_sy[MAX_PIECES];
-    int wsum[BIN_WAVES];
-    int n_stage;
+template <int THREADS> struct TileLds {
+    using Cfg = TileCfg<THREADS>;
+    float4 cand[Cfg::CAP];
+    int p_beg[Cfg::MAXP];
+    int p_off[Cfg::MAXP + 1];
+    float p_sx[Cfg::MAXP], p_sy[Cfg::MAXP];
+    int wsum[Cfg::WAVES];
 };
 
+// 2 cols + 1 <= TileCfg<THREADS>::MAXCOL (checked on the host)
+template <int THREADS>
 __device__ __forceinline__ int tile_pieces(const BinGeom &g, const int *__restrict__ cell_start,
-                                           int cx, int cy, float Lx, float Ly, TileLds &L)
+                                           int cx, int cy, float Lx, float Ly,
+                                           TileLds<THREADS> &L)
 {
-    const int ncol = 2 * g.cols + 1;   // <= MAX_COL_EXT <= BIN_THREADS
+    const int ncol = 2 * g.cols + 1;
     const int j = threadIdx.x;
     int len[3] = {0, 0, 0}, pb[3] = {0, 0, 0};
     float sx = 0.f, sy[3] = {-Ly, Ly, 0.f};
@@ -122,7 +133,7 @@ __device__ __forceinline__ int tile_pieces(const BinGeom &g, const int *__restri
         }
     }
     int total;
-    int off = block_excl_scan(len[0] + len[1] + len[2], L.wsum, &total);
+    int off = block_excl_scan<THREADS>(len[0] + len[1] + len[2], L.wsum, &total);
     if (j < ncol) {
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
@@ -138,18 +149,19 @@ __device__ __forceinline__ int tile_pieces(const BinGeom &g, const int *__restri
     return total;
 }
 
-// stages the candidates with raw index in [r0, r1) (r1 - r0 <= EVAL_CAP) that
+// stages the candidates with raw index in [r0, r1) (r1 - r0 <= CAP) that
 // can reach the cell: cand[0 .. n) in raw order, padded with zero weights to
 // a multiple of 4; returns n (padded).  Starts and ends with a barrier.
+template <int THREADS>
 __device__ __forceinline__ int tile_stage(const BinGeom &g, const float4 *__restrict__ pts,
                                           int npieces, int r0, int r1, float ox, float oy,
-                                          TileLds &L)
+                                          TileLds<THREADS> &L)
 {
     const int lane = threadIdx.x & (WAVE - 1), wid = threadIdx.x / WAVE;
     const float hx = 0.5f * g.cw, hy = 0.5f * g.ch;
     int n = 0;
     __syncthreads();   // the previous pass's readers are done with cand
-    for (int i0 = r0; i0 < r1; i0 += BIN_THREADS) {
+    for (int i0 = r0; i0 < r1; i0 += THREADS) {
         const int i = i0 + threadIdx.x;
         bool keep = false;
         float4 rec = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -172,7 +184,7 @@ __device__ __forceinline__ int tile_stage(const BinGeom &g, const float4 *__rest
         __syncthreads();
         int base = n, tot = 0;
 #pragma unroll
-        for (int w = 0; w < BIN_WAVES; ++w) {
+        for (int w = 0; w < TileCfg<THREADS>::WAVES; ++w) {
             base += (w < wid) ? L.wsum[w] : 0;
             tot += L.wsum[w];
         }
@@ -219,25 +231,26 @@ __device__ __forceinline__ void tile_eval(const float4 *__restrict__ cand, int n
 
 // The samples srec[beg, end) of local tile `tile` (sign tile / ncell) of
 // the simulation described by (m, g): stage, evaluate, write omega to
-// om[sample id].  Called by the whole workgroup.
-template <int SPL, bool COUNT>
+// om[sample id].  Called by the whole workgroup of THREADS threads.
+template <int THREADS, int SPL, bool COUNT>
 __device__ __forceinline__ void bin_eval_tile(const McD &m, const BinGeom &g, int tile, int beg,
                                               int end, const float4 *__restrict__ srec,
-                                              float *__restrict__ om, TileLds &L,
+                                              float *__restrict__ om, TileLds<THREADS> &L,
                                               EvalCount &ec)
 {
+    constexpr int CAP = TileCfg<THREADS>::CAP;
     const int si = tile >= g.ncell ? 1 : 0;
     const int c = tile - si * g.ncell;
     const int cx = c / g.ny, cy = c - cx * g.ny;
     const float ox = ((float)cx + 0.5f) * g.cw, oy = ((float)cy + 0.5f) * g.ch;
     const float4 *__restrict__ pts = m.set[si].field.pts;
-    const int total = tile_pieces(g, m.set[si].field.cell_start, cx, cy, m.Lx, m.Ly, L);
+    const int total = tile_pieces<THREADS>(g, m.set[si].field.cell_start, cx, cy, m.Lx, m.Ly, L);
     const int npieces = 3 * (2 * g.cols + 1);
-    const int npass = (total + EVAL_CAP - 1) / EVAL_CAP;
+    const int npass = (total + CAP - 1) / CAP;
     int n = 0;
-    if (npass == 1) n = tile_stage(g, pts, npieces, 0, total, ox, oy, L);
+    if (npass == 1) n = tile_stage<THREADS>(g, pts, npieces, 0, total, ox, oy, L);
     const int lane = threadIdx.x & (WAVE - 1), wid = threadIdx.x / WAVE;
-    for (int bb = beg; bb < end; bb += BIN_THREADS * SPL) {   // uniform trip count
+    for (int bb = beg; bb < end; bb += THREADS * SPL) {   // uniform trip count
         float px[SPL], py[SPL], pp[SPL], acc[SPL];
         uint32_t sid[SPL];
         bool live[SPL];
@@ -254,14 +267,21 @@ __device__ __forceinline__ void bin_eval_tile(const McD &m, const BinGeom &g, in
         }
         for (int pass = 0; pass < npass; ++pass) {
             if (npass > 1)
-                n = tile_stage(g, pts, npieces, pass * EVAL_CAP,
-                               min(total, (pass + 1) * EVAL_CAP), ox, oy, L);
+                n = tile_stage<THREADS>(g, pts, npieces, pass * CAP, min(total, (pass + 1) * CAP),
+                                        ox, oy, L);
             tile_eval<SPL, COUNT>(L.cand, n, px, py, pp, m.inv_hw, acc, ec, live);
         }
 #pragma unroll
         for (int s = 0; s < SPL; ++s)
             if (live[s]) om[sid[s]] = 2.f * m.sig2 * acc[s];
     }
+}
+
+// host check: the stencil's columns fit a workgroup of `threads`
+inline bool tile_threads_ok(const BinGeom &g, int threads)
+{
+    const int maxcol = threads < MAX_COL_EXT ? threads : MAX_COL_EXT;
+    return 2 * g.cols + 1 <= maxcol;
 }
 
 // One wave: the estimate of (owner, sign) = pair from its N1+N2 samples at

@@ -111,8 +111,8 @@ __global__ void __launch_bounds__(BIN_THREADS)
     bin_gen_one<MODE, true>(d, g, sid, out, (uint32_t)t.toff[s], dead, key, q, r, om);
 }
 
-template <int SPL>
-__global__ void __launch_bounds__(BIN_THREADS)
+template <int THREADS, int SPL>
+__global__ void __launch_bounds__(THREADS)
     k_ens_eval(EnsTabs t, int n_sim, int ntile, int per_xcd, const int *__restrict__ start,
                const float4 *__restrict__ srec, float *__restrict__ om)
 {
@@ -121,9 +121,10 @@ __global__ void __launch_bounds__(BIN_THREADS)
     const int beg = start[tile], end = start[tile + 1];
     if (beg >= end) return;   // uniform over the workgroup
     const int s = seg_of(t.toff, n_sim, tile);
-    __shared__ TileLds L;
+    __shared__ TileLds<THREADS> L;
     EvalCount ec;
-    bin_eval_tile<SPL, false>(t.sim[s], t.geom[s], tile - t.toff[s], beg, end, srec, om, L, ec);
+    bin_eval_tile<THREADS, SPL, false>(t.sim[s], t.geom[s], tile - t.toff[s], beg, end, srec, om,
+                                       L, ec);
 }
 
 template <int MODE>
@@ -358,16 +359,22 @@ static int ens_estimate(const vpmc_ensemble_desc *e, EnsLayout &L, hipStream_t s
     VPMC_TRY(radix_sort_to_records(L.skey, (int)L.S, key_bits_for((uint32_t)L.T),
                                    (const float2 *)L.q, nullptr, L.srec, L.skeys_sorted, L.T,
                                    L.tstart, L.ssort_ws, L.ssort_bytes, st));
-    const dim3 grid((unsigned)(8 * L.per_xcd)), block(BIN_THREADS);
-    if (e->spl == 1)
-        hipLaunchKernelGGL((k_ens_eval<1>), grid, block, 0, st, L.t, ns, L.T, L.per_xcd,
-                           (const int *)L.tstart, (const float4 *)L.srec, L.om);
-    else if (e->spl == 4)
-        hipLaunchKernelGGL((k_ens_eval<4>), grid, block, 0, st, L.t, ns, L.T, L.per_xcd,
-                           (const int *)L.tstart, (const float4 *)L.srec, L.om);
-    else
-        hipLaunchKernelGGL((k_ens_eval<2>), grid, block, 0, st, L.t, ns, L.T, L.per_xcd,
-                           (const int *)L.tstart, (const float4 *)L.srec, L.om);
+    const int th = e->eval_threads ? e->eval_threads : 256;
+#define VPMC_ENS_EVAL(TH, SP)                                                                 \
+    hipLaunchKernelGGL((k_ens_eval<TH, SP>), dim3((unsigned)(8 * L.per_xcd)), dim3(TH), 0, st,  \
+                       L.t, ns, L.T, L.per_xcd, (const int *)L.tstart, (const float4 *)L.srec, \
+                       L.om)
+#define VPMC_ENS_EVAL_SPL(TH)                        \
+    do {                                             \
+        if (e->spl == 1) VPMC_ENS_EVAL(TH, 1);       \
+        else if (e->spl == 4) VPMC_ENS_EVAL(TH, 4);  \
+        else VPMC_ENS_EVAL(TH, 2);                   \
+    } while (0)
+    if (th == 64) VPMC_ENS_EVAL_SPL(64);
+    else if (th == 128) VPMC_ENS_EVAL_SPL(128);
+    else VPMC_ENS_EVAL_SPL(256);
+#undef VPMC_ENS_EVAL_SPL
+#undef VPMC_ENS_EVAL
     VPMC_CHECK_LAUNCH();
     const int npair = 2 * L.N;
     hipLaunchKernelGGL((k_ens_final<MODE>), dim3((unsigned)((npair + 3) / 4)), dim3(BIN_THREADS), 0,
@@ -415,6 +422,9 @@ int vpmc_ensemble_init(const vpmc_ensemble_desc *e, void *workspace, size_t work
         vpmc_mc_desc d = ens_sim_desc(e, L, s);
         VPMC_TRY(make_mcd(&d, &sims[s]));
         VPMC_TRY(bin_geom(sims[s], e->N1, e->N2, d.n_owner, &geoms[s]));
+        const int th = e->eval_threads ? e->eval_threads : 256;
+        if (th != 64 && th != 128 && th != 256) return VPMC_ERR_ARG;
+        if (!tile_threads_ok(geoms[s], th)) return VPMC_ERR_GRID;
     }
     struct Up { void *dst; const void *src; size_t bytes; } ups[] = {
         {L.t.sim, sims.data(), sizeof(McD) * ns},

@@ -17,7 +17,8 @@ from ._capi import MODE_CONSISTENT, EnsembleDesc, check, lib
 
 class Ensemble:
     def __init__(self, positions, strengths, h_w, L=1.0, N1=64, N2=64, dt=5e-4,
-                 mode=MODE_CONSISTENT, seed=0, cells_per_hw=8, spl=0, device="cuda"):
+                 mode=MODE_CONSISTENT, seed=0, cells_per_hw=8, spl=0, eval_threads=0,
+                 device="cuda"):
         """positions[s]: (N_s, 2) array, strengths[s]: (N_s,), h_w: per-simulation
         kernel radius (scalar or sequence), L: box side (scalar or sequence)"""
         self.device = torch.device(device)
@@ -43,6 +44,7 @@ class Ensemble:
         d.mode = int(mode)
         d.cells_per_hw = int(cells_per_hw)
         d.spl = int(spl)
+        d.eval_threads = int(eval_threads)
         d.seed = int(seed)
         d.dt = float(dt)
         d.pos_xy = ops._p(self.pos)
@@ -122,3 +124,72 @@ class Ensemble:
         if s is None:
             return v
         return v[self.offsets[s]:self.offsets[s + 1]]
+
+
+# ---- autotuner ------------------------------------------------------------------
+
+# (cells_per_hw, samples per lane, evaluation workgroup size)
+TUNE_CANDIDATES = tuple((c, s, t) for c in (8, 4) for s in (1, 2, 4) for t in (64, 128, 256))
+
+
+def _tune_cache_path():
+    import os
+    root = os.environ.get("VPMC_TUNE_CACHE",
+                          os.path.join(os.path.expanduser("~"), ".cache", "vortexparticlemc_cuda_amd"))
+    return os.path.join(root, "tune.json")
+
+
+def autotune(positions, strengths, h_w, L=1.0, N1=64, N2=64, mode=MODE_CONSISTENT,
+             candidates=TUNE_CANDIDATES, sample_sims=8, steps=2, use_cache=True, device="cuda"):
+    """(cells_per_hw, spl, eval_threads) with the shortest step on a subset of the ensemble
+    (the first `sample_sims` simulations), timed with HIP events after one
+    warm-up step; results are cached per GPU and workload shape"""
+    import json
+    import os
+    sizes = tuple(int(np.asarray(p).shape[0]) for p in positions)
+    key = "%s|sizes=%s|hw=%s|N1=%d|N2=%d|mode=%d" % (
+        torch.cuda.get_device_name(device), sizes[:sample_sims],
+        tuple(round(float(v), 7) for v in np.broadcast_to(np.asarray(h_w, np.float32),
+                                                            (len(sizes),))[:sample_sims]),
+        N1, N2, mode)
+    path = _tune_cache_path()
+    cache = {}
+    if use_cache and os.path.exists(path):
+        try:
+            cache = json.load(open(path))
+        except (OSError, ValueError):
+            cache = {}
+    if use_cache and key in cache:
+        c = cache[key]
+        return (int(c["cells_per_hw"]), int(c["spl"]), int(c["eval_threads"]),
+                c.get("times_ms", {}))
+    hw = np.broadcast_to(np.asarray(h_w, np.float32), (len(sizes),))[:sample_sims]
+    times = {}
+    for cph, spl, th in candidates:
+        try:
+            e = Ensemble(positions[:sample_sims], strengths[:sample_sims], hw, L=L, N1=N1, N2=N2,
+                         mode=mode, cells_per_hw=cph, spl=spl, eval_threads=th, device=device)
+        except Exception:   # grid or workgroup invalid for this candidate
+            continue
+        e.step(1)
+        a = torch.cuda.Event(enable_timing=True)
+        b = torch.cuda.Event(enable_timing=True)
+        a.record()
+        e.step(steps)
+        b.record()
+        torch.cuda.synchronize()
+        times["%d,%d,%d" % (cph, spl, th)] = a.elapsed_time(b) / steps
+        e.close()
+        del e
+    if not times:
+        raise RuntimeError("no valid tuning candidate")
+    best = min(times, key=times.get)
+    cph, spl, th = (int(v) for v in best.split(","))
+    if use_cache:
+        try:
+            os.makedirs(os.path.dirname(path), exist_ok=True)
+            cache[key] = {"cells_per_hw": cph, "spl": spl, "eval_threads": th, "times_ms": times}
+            json.dump(cache, open(path, "w"), indent=1)
+        except OSError:
+            pass
+    return cph, spl, th, times

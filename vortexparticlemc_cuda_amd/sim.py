@@ -57,10 +57,11 @@ class Simulation:
     include/vortex_particle_mc.hpp:136-192, for its GPU path)."""
 
     def __init__(self, P, device="cuda", cells_per_hw=8, mode=MODE_CONSISTENT, seed=0, sim=0,
-                 algo=ALGO_BINNED, spl=0):
+                 algo=ALGO_BINNED, spl=0, eval_threads=0):
         self.P = P
         self.algo = int(algo)
         self.spl = int(spl)
+        self.eval_threads = int(eval_threads)
         self.ws = None
         self.device = torch.device(device)
         self.cells_per_hw = int(cells_per_hw)
@@ -144,6 +145,7 @@ class Simulation:
         d.eval_count = ops._p(self.eval_count) if count else None
         d.algo = self.algo
         d.spl = self.spl
+        d.eval_threads = self.eval_threads
         return d
 
     def _ws_for(self, d):
