@@ -35,7 +35,7 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
-def setup(Nx, hwr, N1, N2, device, seed=0, cells_per_hw=8):
+def setup(Nx, hwr, N1, N2, device, seed=0, cells_per_hw=8, spl=0, eval_threads=0):
     import torch
 
     import vortexparticlemc_cuda_amd as v
@@ -48,7 +48,7 @@ def setup(Nx, hwr, N1, N2, device, seed=0, cells_per_hw=8):
     b = (4 * np.pi * np.sin(2 * np.pi * pos[:, 0].astype(np.float64)) *
          np.sin(2 * np.pi * pos[:, 1].astype(np.float64))).astype(np.float32)
     sim = Simulation(P, device=device, cells_per_hw=cells_per_hw, mode=v.MODE_CONSISTENT,
-                     seed=seed)
+                     seed=seed, spl=spl, eval_threads=eval_threads)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     it, rr, st = sim.init_strengths_cg(pos, b, tol=1e-6, refine=True)
@@ -190,6 +190,7 @@ def ensemble_bench(args, device):
     import torch
 
     from vortexparticlemc_cuda_amd import ops
+    from vortexparticlemc_cuda_amd._capi import ALGO_DIRECT
     from vortexparticlemc_cuda_amd.ensemble import Ensemble, autotune
     from vortexparticlemc_cuda_amd.sim import SimParam, Simulation
     from vortexparticlemc_cuda_amd.workloads import gaussian_ensemble
@@ -269,6 +270,20 @@ def ensemble_bench(args, device):
     torch.cuda.synchronize()
     t_plain = time.perf_counter() - t0
     plain = N * Kp / t_plain
+    # the same per-simulation loop with the reference's launch structure: one
+    # workgroup per (owner, sign) sampling and querying on its own
+    # (monte_carlo_bs_kernel_safe, src/cuda/monte_carlo_bs.cu:144-271)
+    for sim in sims:
+        sim.algo = ALGO_DIRECT
+    for sim in sims:
+        sim.step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for sim in sims:
+        sim.step()
+    torch.cuda.synchronize()
+    t_direct = time.perf_counter() - t0
+    plain_direct = N / t_direct
     del sims
     torch.cuda.empty_cache()
     log("batched %.3g particle-steps/s (eager %.1f ms/step, graph %.1f ms/step), plain %.3g"
@@ -282,9 +297,92 @@ def ensemble_bench(args, device):
             "batched_ms_per_step_graph": t_graph / K * 1e3,
             "plain_particle_steps_per_s": plain, "plain_ms_per_step": t_plain / Kp * 1e3,
             "batched_over_plain": batched / plain, "steps_timed": K, "plain_steps_timed": Kp,
+            "plain_direct_particle_steps_per_s": plain_direct,
+            "plain_direct_ms_per_step": t_direct * 1e3,
+            "batched_over_plain_direct": batched / plain_direct,
             "rms_err_vs_direct": err, "cells_per_hw": cph, "spl": spl, "eval_threads": th,
             "autotune_ms": tune_ms,
             "cg_init_s": t_cg}
+
+
+def large_bench(args, device):
+    """The large run: double shear layer (rho = 1/30, delta = 0.05) on the
+    periodic unit box, Nx = Ny = 1024 (1,048,576 particles), hwr = 8,
+    N1 = N2 = 256, dt = 2.5e-4; strengths from the RBF solve to 1e-10 (FP64
+    MINRES); plain steps with the plain path's tuned launch shape; RMS error
+    against the FP64 blob direct sum on 4,096 uniform probe points (seed
+    20261203), the estimator evaluated at those probes on the same fields."""
+    import torch
+
+    from vortexparticlemc_cuda_amd import ops
+    from vortexparticlemc_cuda_amd.sim import SimParam, Simulation, lattice_positions
+    from vortexparticlemc_cuda_amd.workloads import double_shear
+
+    Nx, hwr, N = args.large_Nx, 8, args.large_N
+    P = SimParam(dt=2.5e-4, resX=Nx, hwr=hwr)
+    P.N1 = P.N2 = N
+    pos = lattice_positions(Nx)
+    b = double_shear(pos[:, 0], pos[:, 1]).astype(np.float32)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    w, it, rr, st = ops.cg_rbf(torch.as_tensor(pos, device=device),
+                               torch.as_tensor(b, device=device), float(P.h_w), tol=1e-10,
+                               max_iter=3000, refine=True, cells_per_hw=8)
+    torch.cuda.synchronize()
+    t_cg = time.perf_counter() - t0
+    log("large: %d particles, RBF solve %d iterations, rel. residual %.3g (status %d), %.2f s"
+        % (pos.shape[0], it, rr, st, t_cg))
+    sim = Simulation(P, device=device, cells_per_hw=args.large_cells_per_hw, seed=0,
+                     spl=args.large_spl, eval_threads=args.large_threads)
+    sim.set_particles(pos, w.cpu().numpy())
+    del w
+    # error at probe points: the same fields, owners = probes
+    rng = np.random.default_rng(20261203)
+    probes = torch.as_tensor(rng.random((args.large_probes, 2)).astype(np.float32), device=device)
+    sim.rebuild_fields()
+    d = sim.desc(N, N)
+    d.n_owner = probes.shape[0]
+    d.owner_xy = ops._p(probes)
+    vs = torch.zeros((2 * probes.shape[0], 2), dtype=torch.float32, device=device)
+    d.vel_sign = ops._p(vs)
+    sim.launch_estimate(d)
+    um = (vs[:probes.shape[0]] - vs[probes.shape[0]:]).double()
+    ud = ops.direct_sum_blob(probes.double(), sim.pos.double(), sim.omega.double(),
+                             float(P.h_w))
+    err = float(torch.sqrt(((um - ud) ** 2).sum() / (ud ** 2).sum()))
+    log("large: RMS error vs FP64 blob direct sum at %d probes: %.4g" % (probes.shape[0], err))
+    run_steps(sim, 2)
+    torch.cuda.synchronize()
+    events = {"est": [], "eval": []}
+    K = args.large_steps
+    t0 = time.perf_counter()
+    run_steps(sim, K, events)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    est_s = float(np.mean([a.elapsed_time(b) for a, b in events["est"]])) / 1e3
+    eval_s = float(np.mean([a.elapsed_time(b) for a, b in events["eval"]])) / 1e3
+    n_in, n_vis = count_evals(sim)
+    flop = FLOP_PER_EVAL * n_in
+    n = sim.N
+    log("large: %.1f ms/step, estimator %.1f ms, evaluation %.1f ms, %.3g evals/launch"
+        % (el / K * 1e3, est_s * 1e3, eval_s * 1e3, n_in))
+    out = {"workload": "double shear layer (rho=1/30, delta=0.05) on periodic [0,1]^2, "
+                       "Nx=Ny=%d (%d particles), hwr=8, N1=N2=%d, dt=2.5e-4, consistent "
+                       "estimator, Philox seed 0, RBF strengths to 1e-10" % (Nx, n, N),
+           "particles": n, "steps_timed": K, "ms_per_step": el / K * 1e3,
+           "particle_steps_per_s": n * K / el, "estimator_ms": est_s * 1e3,
+           "rms_err_vs_direct": err, "probes": int(probes.shape[0]),
+           "rbf_iters": it, "rbf_rel_res": rr, "rbf_status": st, "rbf_seconds": t_cg,
+           "cells_per_hw": sim.cells_per_hw, "spl": sim.spl, "eval_threads": sim.eval_threads,
+           "roofline": {"kernel": "k_bin_eval", "bound": "valu", "achieved": flop / eval_s,
+                        "peak": PEAK_FP32, "unit": "FLOP/s", "frac": flop / eval_s / PEAK_FP32,
+                        "avg_launch_ms": eval_s * 1e3, "kernel_evals_per_launch": n_in,
+                        "candidates_per_launch": n_vis,
+                        "samples_per_launch": n * 2 * 2 * N,
+                        "frac_of_unpacked": flop / eval_s / (PEAK_FP32 / 2)}}
+    del sim
+    torch.cuda.empty_cache()
+    return out
 
 
 def read_traffic(kernel, args):
@@ -293,7 +391,8 @@ def read_traffic(kernel, args):
     path = os.path.join(ROOT, "profiles", "traffic_%s_tg%d.txt" % (kernel, args.Nx))
     if not os.path.exists(path):
         return None
-    want = "N1=%d N2=%d hwr=%d cells_per_hw=%d" % (args.N1, args.N2, args.hwr, args.cells_per_hw)
+    want = "N1=%d N2=%d hwr=%d cells_per_hw=%d spl=%d eval_threads=%d" % (
+        args.N1, args.N2, args.hwr, args.cells_per_hw, args.spl, args.eval_threads)
     ok = False
     for line in open(path):
         if line.startswith("workload:"):
@@ -344,7 +443,11 @@ def main():
     ap.add_argument("--hwr", type=int, default=16)
     ap.add_argument("--N1", type=int, default=128)
     ap.add_argument("--N2", type=int, default=128)
-    ap.add_argument("--cells-per-hw", type=int, default=8)
+    ap.add_argument("--cells-per-hw", type=int, default=8,
+                    help="fine-grid cells per h_w (with --no-tune; else tuned)")
+    ap.add_argument("--spl", type=int, default=0, help="samples per lane (with --no-tune)")
+    ap.add_argument("--eval-threads", type=int, default=0,
+                    help="evaluation workgroup size (with --no-tune)")
     ap.add_argument("--algo", choices=["binned", "direct"], default="binned")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-error", action="store_true")
@@ -357,6 +460,14 @@ def main():
     ap.add_argument("--ens-N", type=int, default=64, help="N1 = N2 of the ensemble")
     ap.add_argument("--ens-steps", type=int, default=10)
     ap.add_argument("--ens-plain-steps", type=int, default=2)
+    ap.add_argument("--large-Nx", type=int, default=1024,
+                    help="lattice side of the large double-shear run (0: skip it)")
+    ap.add_argument("--large-N", type=int, default=256, help="N1 = N2 of the large run")
+    ap.add_argument("--large-steps", type=int, default=10)
+    ap.add_argument("--large-probes", type=int, default=4096)
+    ap.add_argument("--large-cells-per-hw", type=int, default=12)
+    ap.add_argument("--large-spl", type=int, default=1)
+    ap.add_argument("--large-threads", type=int, default=256)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU; this build has no multi-GPU path "
@@ -367,7 +478,8 @@ def main():
     device = torch.device("cuda:0")
     log("setup Nx=%d hwr=%d N1=%d N2=%d" % (args.Nx, args.hwr, args.N1, args.N2))
     sim, init = setup(args.Nx, args.hwr, args.N1, args.N2, device,
-                      cells_per_hw=args.cells_per_hw)
+                      cells_per_hw=args.cells_per_hw, spl=args.spl,
+                      eval_threads=args.eval_threads)
     sim.algo = 0 if args.algo == "binned" else 1
     log("cg init", init)
     tuned = None
@@ -381,7 +493,7 @@ def main():
                         spl=spl, eval_threads=th)
         s2.set_particles(sim.pos.cpu().numpy(), sim.omega.cpu().numpy())
         sim = s2
-        args.cells_per_hw = cph
+        args.cells_per_hw, args.spl, args.eval_threads = cph, spl, th
     err = None
     if not args.no_error:
         err = velocity_error(sim)
@@ -411,6 +523,7 @@ def main():
     if args.copy_probe:
         copy_probe(args.copy_probe << 20)
     batched = ensemble_bench(args, device) if args.ens_sims > 0 else None
+    large = large_bench(args, device) if args.large_Nx > 0 else None
     cpu = None if args.no_cpu else cpu_baseline(args.Nx, args.hwr, args.N1, args.N2)
     value = N * args.steps / elapsed
     line = {
@@ -448,6 +561,7 @@ def main():
         "error": err,
         "cg_init": init,
         "batched": batched,
+        "large": large,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -280,6 +280,64 @@ int vpmc_graph_launch(void *graph_exec, void *stream);
 int vpmc_graph_destroy(void *graph_exec);
 
 /* ---------------------------------------------------------------------------
+ * Adjoint (reverse-mode derivative) of the consistent estimator and of the
+ * advection step under common random numbers (no reference counterpart: the
+ * reference has no derivatives; the forward is monte_carlo_bs_kernel_safe,
+ * src/cuda/monte_carlo_bs.cu:144-271, and step_cuda,
+ * src/cuda/simulation_step.cu:9-90).  Samples are frozen: the Philox draws,
+ * the particle each branch-1 sample was drawn from and the jitter / disk
+ * offsets stay fixed, so sample points move with their source particle
+ * (branch 1) or owner (branch 2).  The MIS denominator is the density the
+ * samples were drawn from: it follows the positions but not the strengths
+ * (the frozen indices do not follow them), so the estimate and its
+ * derivative stay unbiased for the exact velocity and its derivative.
+ * Periodic, consistent mode, Philox draws only; owners are the particles
+ * (n_owner = particle count, owner_xy = their positions) and the fields of
+ * `d` are built from those positions.  Gradients are accumulated (+=).
+ */
+size_t vpmc_mc_adjoint_workspace_bytes(const vpmc_mc_desc *d);
+/* adj_vel: n_owner float2, dL/du_i of u = v+ - v- (vel_sign[0] - vel_sign[1]);
+   grad_pos (float2) and grad_w receive dL/dx and dL/dw of every particle. */
+int vpmc_mc_adjoint(const vpmc_mc_desc *d, const int *sub_pos, const int *sub_neg,
+                    const float *adj_vel, float *grad_pos, float *grad_w, void *workspace,
+                    size_t workspace_bytes, void *stream);
+/* One step backwards through x' = wrap(x + dt u(x, w)) at the step index in
+   d->step: rebuilds the fields from pos_xy (the state the forward step
+   started from), then lambda (dL/dx', float2) becomes dL/dx and grad_w
+   accumulates dL/dw of the step. */
+size_t vpmc_step_adjoint_workspace_bytes(const vpmc_mc_desc *d);
+int vpmc_step_adjoint(vpmc_mc_desc *d, int n_total, const float *pos_xy, const float *omega,
+                      const int *sub_pos, const int *sub_neg, float dt, float *lambda,
+                      float *grad_w, void *workspace, size_t workspace_bytes, void *stream);
+/* FP64 forward of the same frozen-sample step (checks by finite differences):
+   positions pos_xy (double2, advanced in place) and strengths w (double) at
+   which the velocity is evaluated; the sampling tables of `d` (subsets, CDF,
+   guide, totals) and the MIS density come from the nominal strengths omega0. */
+size_t vpmc_step_f64_workspace_bytes(const vpmc_mc_desc *d);
+int vpmc_step_f64(vpmc_mc_desc *d, int n_total, double *pos_xy, const double *w,
+                  const float *omega0, const int *sub_pos, const int *sub_neg, double dt,
+                  void *workspace, size_t workspace_bytes, void *stream);
+
+/* ---------------------------------------------------------------------------
+ * Vorticity sampled by the B-spline kernel on a grid (no single reference
+ * entry point: the host loop of dump_vorticity_png, include/viz_utils.hpp:26-82,
+ * and the grid of test_vorticity_grid, src/vortex_particle_mc.cu:332-456):
+ * out[a*gny + b] = sum_j w_j W(|z_ab - x_j|), z_ab = ((a+1/2) Lx/gnx,
+ * (b+1/2) Ly/gny), signed strengths, periodic.
+ */
+size_t vpmc_grid_vorticity_workspace_bytes(int n, int nx, int ny);
+int vpmc_grid_vorticity(int n, const float *pos_xy, const float *w, float h_w, float Lx,
+                        float Ly, int cells_per_hw, int gnx, int gny, float *out, void *workspace,
+                        size_t workspace_bytes, void *stream);
+/* adjoint: grad_pos / grad_w += the gradient of sum_g grad_out[g] out[g] */
+int vpmc_grid_vorticity_adjoint(int n, const float *pos_xy, const float *w, float h_w, float Lx,
+                                float Ly, int gnx, int gny, const float *grad_out,
+                                float *grad_pos, float *grad_w, void *stream);
+/* FP64, all particles per grid point (checks) */
+int vpmc_grid_vorticity_f64(int n, const double *pos_xy, const double *w, double h_w, double Lx,
+                            double Ly, int gnx, int gny, double *out, void *stream);
+
+/* ---------------------------------------------------------------------------
  * RBF strength initialisation.
  * replaces Ax_cuda / matvec_kernel (src/cuda/cg_rbf_init.cu:11-50):
  *   y_i = sum_j x_j W(|x_i - x_j|) over all particles (diagonal included).

@@ -1,0 +1,152 @@
+"""GPU tests of the adjoint (vpmc_mc_adjoint / vpmc_step_adjoint /
+vpmc_grid_vorticity_adjoint behind vortexparticlemc_cuda_amd.adjoint).
+
+The FP32 adjoint gradients are checked against central finite differences of
+the FP64 forward of the same frozen-sample functions (vpmc_step_f64,
+vpmc_grid_vorticity_f64), and for determinism.  Tolerances are relative to
+the largest gradient component of the same kind (positions or strengths).
+"""
+import numpy as np
+import pytest
+
+from conftest import tg_problem
+
+pytestmark = pytest.mark.gpu
+
+
+def _sim(device, Nx=32, hwr=4, N1=16, N2=16, dt=2e-3, seed=3, jitter=0.25):
+    from vortexparticlemc_cuda_amd.sim import SimParam, Simulation
+    x, y, w, hw = tg_problem(Nx, hwr, jitter=jitter, seed=seed)
+    P = SimParam(dt=dt, resX=Nx, hwr=hwr)
+    P.N1, P.N2 = N1, N2
+    sim = Simulation(P, device=device, cells_per_hw=8, seed=seed)
+    pos = np.stack([x, y], 1).astype(np.float32)
+    sim.set_particles(pos, w)
+    return sim, pos, w
+
+
+def _fd_pick(rng, n, k, w, min_frac=0.2):
+    """k particle indices whose |w| is not small (no sign flip under eps)"""
+    big = np.nonzero(np.abs(w) > min_frac * np.abs(w).max())[0]
+    return rng.choice(big, size=k, replace=False)
+
+
+def test_grid_vorticity_forward_and_adjoint(device):
+    import torch
+
+    from vortexparticlemc_cuda_amd import adjoint as A
+    sim, pos, w = _sim(device)
+    h = float(sim.P.h_w)
+    G = 16
+    p = torch.tensor(pos, device=device, requires_grad=True)
+    wt = torch.tensor(w, device=device, requires_grad=True)
+    om = A.grid_vorticity(p, wt, h, G, G)
+    om64 = A.grid_vorticity_f64(p.detach().double(), wt.detach().double(), h, G, G)
+    scale = float(om64.abs().max())
+    assert float((om.detach().double() - om64).abs().max()) < 2e-5 * scale
+    rng = np.random.default_rng(0)
+    c = torch.tensor(rng.standard_normal(G * G), device=device)
+    (om.double() * c).sum().backward()
+    gp = p.grad.double().cpu().numpy()
+    gw = wt.grad.double().cpu().numpy()
+
+    def L(pp, ww):
+        return float((A.grid_vorticity_f64(pp, ww, h, G, G) * c).sum())
+
+    p64 = p.detach().double()
+    w64 = wt.detach().double()
+    eps_x, eps_w = 1e-6, 1e-6 * float(np.abs(w).max())
+    for i in _fd_pick(rng, len(w), 6, w):
+        for k in range(2):
+            a = p64.clone()
+            b = p64.clone()
+            a[i, k] += eps_x
+            b[i, k] -= eps_x
+            fd = (L(a, w64) - L(b, w64)) / (2 * eps_x)
+            assert abs(fd - gp[i, k]) < 1e-4 * np.abs(gp).max(), (i, k, fd, gp[i, k])
+        a = w64.clone()
+        b = w64.clone()
+        a[i] += eps_w
+        b[i] -= eps_w
+        fd = (L(p64, a) - L(p64, b)) / (2 * eps_w)
+        assert abs(fd - gw[i]) < 1e-4 * np.abs(gw).max(), (i, fd, gw[i])
+
+
+def _loss_setup(device, steps):
+    import torch
+
+    from vortexparticlemc_cuda_amd import adjoint as A
+    sim, pos, w = _sim(device)
+    h = float(sim.P.h_w)
+    G = 16
+    rng = np.random.default_rng(1)
+    ref = A.grid_vorticity_f64(torch.tensor(pos, device=device).double(),
+                               torch.tensor(w * (1 + 0.2 * rng.standard_normal(w.size)),
+                                            device=device).double(), h, G, G)
+    return sim, pos, w, h, G, ref
+
+
+def test_rollout_adjoint_matches_fp64_finite_differences(device):
+    import torch
+
+    from vortexparticlemc_cuda_amd import adjoint as A
+    steps = 4
+    sim, pos, w, h, G, ref = _loss_setup(device, steps)
+    p = torch.tensor(pos, device=device, requires_grad=True)
+    wt = torch.tensor(w, device=device, requires_grad=True)
+    XT = A.rollout(sim, p, wt, steps, checkpoint=3)
+    om = A.grid_vorticity(XT, wt, h, G, G)
+    loss = ((om.double() - ref) ** 2).mean()
+    loss.backward()
+    gp = p.grad.double().cpu().numpy()
+    gw = wt.grad.double().cpu().numpy()
+
+    # FP64 forward of the same frozen-sample rollout (sampling tables of w)
+    def L(pp, ww):
+        X = A.rollout_f64(sim, pp, ww, steps)
+        o = A.grid_vorticity_f64(X, ww, h, G, G)
+        return float(((o - ref) ** 2).mean())
+
+    p64 = torch.tensor(pos, device=device).double()
+    w64 = torch.tensor(w, device=device).double()
+    L0 = L(p64, w64)
+    assert abs(L0 - float(loss)) < 1e-4 * abs(L0)
+    rng = np.random.default_rng(2)
+    eps_x, eps_w = 1e-6, 1e-6 * float(np.abs(w).max())
+    errs_x, errs_w = [], []
+    for i in _fd_pick(rng, len(w), 6, w):
+        k = int(rng.integers(2))
+        a = p64.clone()
+        b = p64.clone()
+        a[i, k] += eps_x
+        b[i, k] -= eps_x
+        fd = (L(a, w64) - L(b, w64)) / (2 * eps_x)
+        errs_x.append(abs(fd - gp[i, k]) / np.abs(gp).max())
+        a = w64.clone()
+        b = w64.clone()
+        a[i] += eps_w
+        b[i] -= eps_w
+        fd = (L(p64, a) - L(p64, b)) / (2 * eps_w)
+        errs_w.append(abs(fd - gw[i]) / np.abs(gw).max())
+    print("rollout adjoint vs FD: max rel err positions %.3g strengths %.3g"
+          % (max(errs_x), max(errs_w)))
+    assert max(errs_x) < 1e-4 and max(errs_w) < 1e-4
+
+
+def test_adjoint_is_deterministic(device):
+    import torch
+
+    from vortexparticlemc_cuda_amd import adjoint as A
+    sim, pos, w = _sim(device)
+    rng = np.random.default_rng(4)
+    c = torch.tensor(rng.standard_normal((len(w), 2)).astype(np.float32), device=device)
+    out = []
+    for _ in range(2):
+        p = torch.tensor(pos, device=device, requires_grad=True)
+        wt = torch.tensor(w, device=device, requires_grad=True)
+        sim.step_index = 5
+        u = A.mc_velocity(sim, p, wt)
+        (u * c).sum().backward()
+        out.append((p.grad.cpu().numpy(), wt.grad.cpu().numpy()))
+    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
+    assert np.abs(out[0][0]).max() > 0 and np.abs(out[0][1]).max() > 0

@@ -124,6 +124,21 @@ def lib():
     L.vpmc_ensemble_graph_create.argtypes = [ep, vp, csz, ci, vp, C.POINTER(vp)]
     L.vpmc_graph_launch.argtypes = [vp, vp]
     L.vpmc_graph_destroy.argtypes = [vp]
+    mp = C.POINTER(McDesc)
+    L.vpmc_mc_adjoint_workspace_bytes.restype = csz
+    L.vpmc_mc_adjoint_workspace_bytes.argtypes = [mp]
+    L.vpmc_mc_adjoint.argtypes = [mp, vp, vp, vp, vp, vp, vp, csz, vp]
+    L.vpmc_step_adjoint_workspace_bytes.restype = csz
+    L.vpmc_step_adjoint_workspace_bytes.argtypes = [mp]
+    L.vpmc_step_adjoint.argtypes = [mp, ci, vp, vp, vp, vp, cf, vp, vp, vp, csz, vp]
+    L.vpmc_step_f64_workspace_bytes.restype = csz
+    L.vpmc_step_f64_workspace_bytes.argtypes = [mp]
+    L.vpmc_step_f64.argtypes = [mp, ci, vp, vp, vp, vp, vp, cd, vp, csz, vp]
+    L.vpmc_grid_vorticity_workspace_bytes.restype = csz
+    L.vpmc_grid_vorticity_workspace_bytes.argtypes = [ci, ci, ci]
+    L.vpmc_grid_vorticity.argtypes = [ci, vp, vp, cf, cf, cf, ci, ci, ci, vp, vp, csz, vp]
+    L.vpmc_grid_vorticity_adjoint.argtypes = [ci, vp, vp, cf, cf, cf, ci, ci, vp, vp, vp, vp]
+    L.vpmc_grid_vorticity_f64.argtypes = [ci, vp, vp, cd, cd, cd, ci, ci, vp, vp]
     _lib = L
     return L
 
@@ -146,5 +161,8 @@ EXPORTS = [
     "vpmc_cg_rbf", "vpmc_cg_workspace_bytes", "vpmc_direct_sum_eps", "vpmc_direct_sum_blob",
     "vpmc_ensemble_workspace_bytes", "vpmc_ensemble_init", "vpmc_ensemble_step",
     "vpmc_ensemble_set_step", "vpmc_ensemble_graph_create", "vpmc_graph_launch",
-    "vpmc_graph_destroy",
+    "vpmc_graph_destroy", "vpmc_mc_adjoint_workspace_bytes", "vpmc_mc_adjoint",
+    "vpmc_step_adjoint_workspace_bytes", "vpmc_step_adjoint", "vpmc_step_f64_workspace_bytes",
+    "vpmc_step_f64", "vpmc_grid_vorticity_workspace_bytes", "vpmc_grid_vorticity",
+    "vpmc_grid_vorticity_adjoint", "vpmc_grid_vorticity_f64",
 ]

@@ -1,0 +1,210 @@
+"""Differentiable Monte Carlo vortex-particle steps (torch.autograd over the
+library's hand-written adjoint kernels).
+
+    mc_velocity(sim, pos, w)                   u = v+ - v- at the particles
+    rollout(sim, pos0, w, steps, checkpoint)   positions after `steps` steps
+    grid_vorticity(pos, w, h_w, gnx, gny)      omega_h on a gnx x gny grid
+
+Derivatives are taken under common random numbers with the samples frozen
+(vpmc_mc_adjoint in include/vpmc.h states the estimator that is
+differentiated): the Philox stream of step t is keyed by (seed, t), so a
+recomputed step replays its samples exactly.  The sampling tables (sign
+subsets, |w| CDF, guide) are built from the strengths passed to the forward,
+which is where the derivative is taken.
+
+`rollout` keeps the positions every `checkpoint` steps; the backward pass
+recomputes each segment forwards from its checkpoint and then runs the step
+adjoints backwards through it, so memory grows with steps / checkpoint +
+checkpoint states, not with steps.
+
+rollout_f64 / grid_vorticity_f64 evaluate the same frozen-sample functions in
+FP64 (vpmc_step_f64, vpmc_grid_vorticity_f64), for finite-difference checks.
+"""
+import ctypes as C
+
+import torch
+
+from . import ops
+from ._capi import check, lib
+
+
+def _set_nominal(sim, w):
+    """make w the strengths the sampling tables and fields use"""
+    w = w.detach().to(torch.float32).contiguous()
+    if sim.omega is None or sim.omega.shape != w.shape:
+        raise ValueError("strengths must match the simulation's particle count")
+    sim.omega.copy_(w)
+    counts = sim.cnt
+    sim.build_subsets()
+    if sim.cnt != counts:   # sign subsets changed size: field buffers follow
+        sim._alloc()
+
+
+def _adj_ws(sim, d):
+    need = lib().vpmc_step_adjoint_workspace_bytes(C.byref(d))
+    if need == 0:
+        raise ValueError("the adjoint needs the consistent, periodic, Philox-driven estimator")
+    if getattr(sim, "adj_ws", None) is None or sim.adj_ws.numel() < need:
+        sim.adj_ws = ops.workspace(need, sim.device)
+    return sim.adj_ws
+
+
+class _McVelocity(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, pos, w, sim):
+        sim.pos.copy_(pos.detach())
+        _set_nominal(sim, w)
+        u = sim.estimate(sim.P.N1, sim.P.N2).clone()
+        ctx.sim = sim
+        ctx.step = sim.step_index
+        ctx.save_for_backward(pos.detach().clone())
+        return u
+
+    @staticmethod
+    def backward(ctx, gu):
+        sim = ctx.sim
+        (pos,) = ctx.saved_tensors
+        sim.pos.copy_(pos)
+        sim.rebuild_fields()
+        d = sim.desc(sim.P.N1, sim.P.N2)
+        d.step = ctx.step
+        gpos = torch.zeros_like(pos)
+        gw = torch.zeros(pos.shape[0], dtype=torch.float32, device=pos.device)
+        ws = _adj_ws(sim, d)
+        check(lib().vpmc_mc_adjoint(C.byref(d), ops._p(sim.sub["sub_pos"]),
+                                    ops._p(sim.sub["sub_neg"]),
+                                    ops._p(gu.to(torch.float32).contiguous()), ops._p(gpos),
+                                    ops._p(gw), ops._p(ws), ws.numel(), ops._stream()),
+              "vpmc_mc_adjoint")
+        return gpos, gw, None
+
+
+def mc_velocity(sim, pos, w):
+    """the estimator's velocity u = v+ - v- at the particles (owners = the
+    particles), differentiable in pos and w; uses sim's step index"""
+    return _McVelocity.apply(pos, w, sim)
+
+
+def _run(sim, t0, t1, keep=None):
+    """advance sim.pos from step t0 to t1 (forward steps); keep: list that
+    receives the state before each step"""
+    for t in range(t0, t1):
+        if keep is not None:
+            keep.append(sim.pos.clone())
+        sim.step_index = t
+        sim.step()
+    sim.step_index = t1
+
+
+class _Rollout(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, pos0, w, sim, steps, checkpoint):
+        sim.pos.copy_(pos0.detach())
+        _set_nominal(sim, w)
+        K = max(1, int(checkpoint))
+        ckpt = {}
+        for t0 in range(0, steps, K):
+            ckpt[t0] = sim.pos.clone()
+            _run(sim, t0, min(steps, t0 + K))
+        ctx.sim, ctx.steps, ctx.K, ctx.ckpt = sim, steps, K, ckpt
+        return sim.pos.clone()
+
+    @staticmethod
+    def backward(ctx, gX):
+        sim, steps, K = ctx.sim, ctx.steps, ctx.K
+        lam = gX.to(torch.float32).contiguous().clone()
+        gw = torch.zeros(sim.N, dtype=torch.float32, device=lam.device)
+        P = sim.P
+        for t0 in reversed(range(0, steps, K)):
+            t1 = min(steps, t0 + K)
+            sim.pos.copy_(ctx.ckpt[t0])
+            states = []
+            _run(sim, t0, t1, keep=states)
+            for t in reversed(range(t0, t1)):
+                sim.pos.copy_(states[t - t0])
+                d = sim.desc(P.N1, P.N2)
+                d.step = t
+                ws = _adj_ws(sim, d)
+                check(lib().vpmc_step_adjoint(C.byref(d), sim.N, ops._p(sim.pos),
+                                              ops._p(sim.omega), ops._p(sim.sub["sub_pos"]),
+                                              ops._p(sim.sub["sub_neg"]), float(P.dt),
+                                              ops._p(lam), ops._p(gw), ops._p(ws), ws.numel(),
+                                              ops._stream()), "vpmc_step_adjoint")
+        sim.step_index = steps
+        return lam, gw, None, None, None
+
+
+def rollout(sim, pos0, w, steps, checkpoint=10):
+    """positions after `steps` plain steps (Philox steps 0 .. steps-1) from
+    pos0 with strengths w; differentiable in pos0 and w"""
+    return _Rollout.apply(pos0, w, sim, int(steps), int(checkpoint))
+
+
+class _GridVorticity(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, pos, w, h_w, gnx, gny, L, cells_per_hw):
+        pos = pos.detach().to(torch.float32).contiguous()
+        w = w.detach().to(torch.float32).contiguous()
+        n = pos.shape[0]
+        nx, ny = ops.field_grid(h_w, L, L, cells_per_hw)
+        ws = ops.workspace(lib().vpmc_grid_vorticity_workspace_bytes(n, nx, ny), pos.device)
+        out = torch.empty(gnx * gny, dtype=torch.float32, device=pos.device)
+        check(lib().vpmc_grid_vorticity(n, ops._p(pos), ops._p(w), h_w, L, L, cells_per_hw,
+                                        gnx, gny, ops._p(out), ops._p(ws), ws.numel(),
+                                        ops._stream()), "vpmc_grid_vorticity")
+        ctx.save_for_backward(pos, w)
+        ctx.args = (h_w, gnx, gny, L)
+        return out
+
+    @staticmethod
+    def backward(ctx, g):
+        pos, w = ctx.saved_tensors
+        h_w, gnx, gny, L = ctx.args
+        gpos = torch.zeros_like(pos)
+        gw = torch.zeros_like(w)
+        check(lib().vpmc_grid_vorticity_adjoint(pos.shape[0], ops._p(pos), ops._p(w), h_w, L, L,
+                                                gnx, gny,
+                                                ops._p(g.to(torch.float32).contiguous()),
+                                                ops._p(gpos), ops._p(gw), ops._stream()),
+              "vpmc_grid_vorticity_adjoint")
+        return gpos, gw, None, None, None, None, None
+
+
+def grid_vorticity(pos, w, h_w, gnx, gny, L=1.0, cells_per_hw=8):
+    """omega_h(z) = sum_j w_j W(|z - x_j|) at the centres of a gnx x gny grid
+    (index a*gny + b), differentiable in pos and w"""
+    return _GridVorticity.apply(pos, w, float(h_w), int(gnx), int(gny), float(L),
+                                int(cells_per_hw))
+
+
+# ---- FP64 forwards of the same functions (finite-difference checks) ------------
+
+def rollout_f64(sim, pos0, w, steps):
+    """FP64 positions after `steps` frozen-sample steps from pos0 (float64,
+    (N, 2)) with strengths w (float64); the sampling tables and the MIS
+    density stay those of sim's nominal strengths"""
+    X = pos0.to(torch.float64).contiguous().clone()
+    w = w.to(torch.float64).contiguous()
+    P = sim.P
+    ws = None
+    for t in range(steps):
+        d = sim.desc(P.N1, P.N2)
+        d.step = t
+        need = lib().vpmc_step_f64_workspace_bytes(C.byref(d))
+        if ws is None or ws.numel() < need:
+            ws = ops.workspace(need, sim.device)
+        check(lib().vpmc_step_f64(C.byref(d), sim.N, ops._p(X), ops._p(w), ops._p(sim.omega),
+                                  ops._p(sim.sub["sub_pos"]), ops._p(sim.sub["sub_neg"]),
+                                  float(P.dt), ops._p(ws), ws.numel(), ops._stream()),
+              "vpmc_step_f64")
+    return X
+
+
+def grid_vorticity_f64(pos, w, h_w, gnx, gny, L=1.0):
+    pos = pos.to(torch.float64).contiguous()
+    w = w.to(torch.float64).contiguous()
+    out = torch.empty(gnx * gny, dtype=torch.float64, device=pos.device)
+    check(lib().vpmc_grid_vorticity_f64(pos.shape[0], ops._p(pos), ops._p(w), float(h_w),
+                                        float(L), float(L), int(gnx), int(gny), ops._p(out),
+                                        ops._stream()), "vpmc_grid_vorticity_f64")
+    return out

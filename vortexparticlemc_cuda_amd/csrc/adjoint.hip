@@ -1,0 +1,974 @@
+// adjoint.hip -- reverse-mode derivative of the Monte Carlo velocity and of
+// the advection step with respect to particle positions and strengths, under
+// common random numbers; the grid vorticity used as a loss, and its adjoint.
+//
+// The consistent estimator's velocity of owner i from sign set s is
+//   v_i = (1/NS) sum_k K(r_k) om(y_k) / D_k,
+//   D_k = w1 om0(y_k) / Omega0 + w2 alpha / |r_k|,
+// with om(y) = sum_{m in s} a_m W(|y - x_m|), a_m = |w_m|, K the doubly
+// periodic kernel, y_k the sample point and r_k the min-image displacement
+// x_i - y_k.  Samples are frozen: the Philox draws, the particle a branch-1
+// sample was drawn from and the jitter / disk offsets stay fixed, so y_k
+// moves with its source x_j (branch 1, y = x_j + jitter) or with the owner
+// x_i (branch 2, y = x_i + offset).  D_k is the density the samples were
+// drawn from: it follows the positions (om0 is om at the strengths the
+// sampling tables were built from) but not the strengths, because the frozen
+// indices do not follow them.  With the density held, the estimate remains
+// an unbiased estimate of the exact velocity for every strength vector, and
+// so does its derivative.  At the nominal strengths om0 = om and the forward
+// value is the product estimator's.
+//
+// Per sample, with a the adjoint of v_i (scaled by 1/NS), s = (a.K) om / D:
+//   beta   = ds/d om   = (a.K) / D
+//   beta0  = ds/d om0  = -(a.K) om w1 / (Omega0 D^2)
+//   ds/dr  = (om/D) grad_r(a.K) + (a.K) om w2 alpha r / (D^2 |r|^3)
+//   ds/dy  = (beta + beta0) grad om(y)
+// branch 1: x_i += ds/dr, x_j += ds/dy - ds/dr;  branch 2: x_i += ds/dy;
+// every particle m of the set within h_w of y: w_m += sign_m beta W,
+// x_m -= a_m (beta + beta0) grad_y W(y - x_m).
+//
+// Passes (one launch each; samples binned by fine cell as in the forward):
+//   gen      the forward's draws, plus the global index of the branch-1 source
+//   sort     stable radix sort by (sign, fine cell)
+//   query    one thread per sorted sample: om(y) and grad om(y)
+//   final    one wave per (owner, sign): adjoint weights per sample, the
+//            owner's gradient (wave butterfly in sample order) and each
+//            branch-1 sample's source gradient
+//   recs     samples in cell order as (y, beta, beta + beta0)
+//   field    one thread per particle of each set: the sums over samples
+//            within h_w -> d/dw_m and d/dx_m
+//   sources  stable radix sort of the source gradients by source particle,
+//            then one thread per particle adds its run in sample order
+// Every sum has a fixed order, so the gradients are deterministic.
+#include <math.h>
+
+#include <mutex>
+
+#include "binned_core.hpp"
+
+namespace vpmc {
+
+constexpr int ADJ_THREADS = 256;
+
+// ---- device helpers ------------------------------------------------------------
+
+// W(r)/sig2 and (dW/dr)/(r sig2) of the cubic spline (include/utilities.hpp:7-17)
+__device__ __forceinline__ void spline_wg(float r2, float inv_hw, float &w, float &gr)
+{
+    const float q = sqrtf(r2) * inv_hw;
+    w = 0.f;
+    gr = 0.f;
+    if (!(q < 1.f)) return;
+    if (q <= 0.5f) {
+        w = fmaf(6.f * q * q, q - 1.f, 1.f);
+        gr = fmaf(18.f, q, -12.f) * inv_hw * inv_hw;
+    } else {
+        const float t = 1.f - q;
+        w = 2.f * t * t * t;
+        gr = -6.f * t * t * inv_hw / (q / inv_hw);
+    }
+}
+
+__device__ __forceinline__ void spline_wg64(double r2, double inv_hw, double &w, double &gr)
+{
+    const double q = sqrt(r2) * inv_hw;
+    w = 0.0;
+    gr = 0.0;
+    if (!(q < 1.0)) return;
+    if (q <= 0.5) {
+        w = 6.0 * q * q * (q - 1.0) + 1.0;
+        gr = (18.0 * q - 12.0) * inv_hw * inv_hw;
+    } else {
+        const double t = 1.0 - q;
+        w = 2.0 * t * t * t;
+        gr = -6.0 * t * t * inv_hw * inv_hw / q;
+    }
+}
+
+// Visits every record of field f within the search radius of p (periodic
+// images by column / row wrap, as field_sum): fn(record, dx, dy) with
+// (dx, dy) = record position - p in the image frame.
+template <class F>
+__device__ __forceinline__ void field_visit(const FieldD &f, float px, float py, F &fn)
+{
+    const FieldGeom &g = f.g;
+    const int cx = min((int)(px * g.inv_cx), g.nx - 1);
+    for (int ox = -g.reach; ox <= g.reach; ++ox) {
+        const int col = cx + ox;
+        float gx = 0.f;
+        if (ox < 0) gx = px - (float)(col + 1) * g.cw;
+        else if (ox > 0) gx = (float)col * g.cw - px;
+        gx = fmaxf(gx, 0.f);
+        const float rem = g.rs2 - gx * gx;
+        if (rem < 0.f) continue;
+        const float ry = sqrtf(rem);
+        int r0 = (int)floorf((py - ry) * g.inv_cy);
+        int r1 = (int)floorf((py + ry) * g.inv_cy);
+        float qx = px;
+        int colw = col;
+        if (colw < 0) { colw += g.nx; qx = px + g.Lx; }
+        else if (colw >= g.nx) { colw -= g.nx; qx = px - g.Lx; }
+        const int *cs = f.cell_start + (size_t)colw * g.ny;
+        auto run = [&](int b, int e, float sx, float sy) {
+            for (int k = b; k < e; ++k) {
+                const float4 c = f.pts[k];
+                fn(c, c.x - sx, c.y - sy);
+            }
+        };
+        if (r0 < 0) { run(cs[g.ny + r0], cs[g.ny], qx, py + g.Ly); r0 = 0; }
+        if (r1 >= g.ny) { run(cs[0], cs[r1 - g.ny + 1], qx, py - g.Ly); r1 = g.ny - 1; }
+        run(cs[r0], cs[r1 + 1], qx, py);
+    }
+}
+
+// The same traversal over FP64 records (x, y, a, a0) kept in the float
+// field's cell order; cell ranges from the FP32 geometry, distances in FP64.
+template <class F>
+__device__ __forceinline__ void field_visit64(const FieldGeom &g, const int *__restrict__ cell_start,
+                                              const double4 *__restrict__ recs, double px,
+                                              double py, F &fn)
+{
+    const float pxf = (float)px, pyf = (float)py;
+    const int cx = min((int)(pxf * g.inv_cx), g.nx - 1);
+    for (int ox = -g.reach; ox <= g.reach; ++ox) {
+        const int col = cx + ox;
+        float gx = 0.f;
+        if (ox < 0) gx = pxf - (float)(col + 1) * g.cw;
+        else if (ox > 0) gx = (float)col * g.cw - pxf;
+        gx = fmaxf(gx, 0.f);
+        const float rem = g.rs2 - gx * gx;
+        if (rem < 0.f) continue;
+        const float ry = sqrtf(rem);
+        int r0 = (int)floorf((pyf - ry) * g.inv_cy);
+        int r1 = (int)floorf((pyf + ry) * g.inv_cy);
+        double qx = px;
+        int colw = col;
+        if (colw < 0) { colw += g.nx; qx = px + (double)g.Lx; }
+        else if (colw >= g.nx) { colw -= g.nx; qx = px - (double)g.Lx; }
+        const int *cs = cell_start + (size_t)colw * g.ny;
+        auto run = [&](int b, int e, double sx, double sy) {
+            for (int k = b; k < e; ++k) {
+                const double4 c = recs[k];
+                fn(c, c.x - sx, c.y - sy);
+            }
+        };
+        if (r0 < 0) { run(cs[g.ny + r0], cs[g.ny], qx, py + (double)g.Ly); r0 = 0; }
+        if (r1 >= g.ny) { run(cs[0], cs[r1 - g.ny + 1], qx, py - (double)g.Ly); r1 = g.ny - 1; }
+        run(cs[r0], cs[r1 + 1], qx, py);
+    }
+}
+
+// gradient of a . K_s(r) with respect to r, K_s the smooth part of the
+// periodic kernel (kper_smooth): with A = P(s) + pi conj(s), s = r/L,
+// P(s) = sum_m c_m s^(4m+3) and Q = P'(s):
+//   d/drx = -(ax Im Q + ay (Re Q + pi)) / (2 pi L^2)
+//   d/dry = -(ax (Re Q - pi) - ay Im Q) / (2 pi L^2)
+__device__ __forceinline__ void kper_smooth_grad(const McD &d, float rx, float ry, float ax,
+                                                 float ay, float &gx, float &gy)
+{
+    const float sr = rx * d.inv_L, si = ry * d.inv_L;
+    const float s2r = sr * sr - si * si, s2i = 2.f * sr * si;
+    const float wr = s2r * s2r - s2i * s2i, wi = 2.f * s2r * s2i;
+    const int nt = d.nterms;
+    float hr = d.kc[nt - 1] * (float)(4 * (nt - 1) + 3), hi = 0.f;
+    for (int m = nt - 2; m >= 0; --m) {
+        const float t = hr * wr - hi * wi + d.kc[m] * (float)(4 * m + 3);
+        hi = hr * wi + hi * wr;
+        hr = t;
+    }
+    const float Qr = hr * s2r - hi * s2i, Qi = hr * s2i + hi * s2r;
+    const float c = d.inv_2piL * d.inv_L;
+    gx = -(ax * Qi + ay * (Qr + PI_F)) * c;
+    gy = -(ax * (Qr - PI_F) - ay * Qi) * c;
+}
+
+__device__ __forceinline__ void kper_smooth_f64(const double *kc, int nt, double L, double rx,
+                                                double ry, double &kx, double &ky)
+{
+    const double sr = rx / L, si = ry / L;
+    const double s2r = sr * sr - si * si, s2i = 2.0 * sr * si;
+    const double wr = s2r * s2r - s2i * s2i, wi = 2.0 * s2r * s2i;
+    double hr = kc[nt - 1], hi = 0.0;
+    for (int m = nt - 2; m >= 0; --m) {
+        const double t = hr * wr - hi * wi + kc[m];
+        hi = hr * wi + hi * wr;
+        hr = t;
+    }
+    const double s3r = s2r * sr - s2i * si, s3i = s2r * si + s2i * sr;
+    const double Ar = hr * s3r - hi * s3i + PI_D * sr;
+    const double Ai = hr * s3i + hi * s3r - PI_D * si;
+    const double c = 1.0 / (2.0 * PI_D * L);
+    kx = -Ai * c;
+    ky = -Ar * c;
+}
+
+// FP64 inverse of the cubic spline's radial mass fraction (the FP32
+// mass_inverse's equation, bracketed Newton to convergence)
+__device__ __forceinline__ double mass_inverse_f64(double u)
+{
+    if (!(u > 0.0)) return 0.0;
+    if (u >= 1.0) return 1.0;
+    const double target = (7.0 / 80.0) * u;
+    double lo = 0.0, hi = 1.0, q = 0.5;
+    for (int it = 0; it < 60; ++it) {
+        double f, df;
+        const double q2 = q * q;
+        if (q <= 0.5) {
+            f = 0.5 * q2 - 1.5 * q2 * q2 + 1.2 * q2 * q2 * q;
+            df = q * (1.0 - 6.0 * q2 + 6.0 * q2 * q);
+        } else {
+            const double t = 1.0 - q, t2 = t * t;
+            f = 7.0 / 80.0 - (0.5 * t2 * t2 - 0.4 * t2 * t2 * t);
+            df = 2.0 * t2 * t * (1.0 - t);
+        }
+        f -= target;
+        if (f > 0.0) hi = q; else lo = q;
+        double qn = (df > 0.0) ? q - f / df : 0.5 * (lo + hi);
+        if (!(qn > lo && qn < hi)) qn = 0.5 * (lo + hi);
+        if (fabs(qn - q) < 1e-16) { q = qn; break; }
+        q = qn;
+    }
+    return q;
+}
+
+// ---- the estimator adjoint -------------------------------------------------------
+
+__global__ void __launch_bounds__(BIN_THREADS)
+    k_adj_gen(McD d, BinGeom g, const int *__restrict__ sub0, const int *__restrict__ sub1,
+              uint32_t *__restrict__ key, float2 *__restrict__ q, float2 *__restrict__ r,
+              int *__restrict__ src)
+{
+    const size_t sid = (size_t)blockIdx.x * BIN_THREADS + threadIdx.x;
+    if (sid >= g.S) return;
+    const uint32_t pair = (uint32_t)(sid / g.NS);
+    const int k = (int)(sid - (size_t)pair * g.NS);
+    const int owner = (int)(pair >> 1), si = (int)(pair & 1);
+    const SetD &S = d.set[si];
+    const int n_s = S.count_dev ? *S.count_dev : S.n;
+    float qx = 0.f, qy = 0.f, rx = 0.f, ry = 0.f;
+    int j = -1;
+    const bool live = n_s > 0 && gen_sample<VPMC_MODE_CONSISTENT, true>(
+                                     d, S, n_s, owner, si, k, d.owner[owner], qx, qy, rx, ry, &j);
+    r[sid] = make_float2(rx, ry);
+    q[sid] = make_float2(qx, qy);
+    const int *sub = si ? sub1 : sub0;
+    src[sid] = (live && j >= 0) ? sub[j] : -1;
+    if (!live) {
+        key[sid] = (uint32_t)g.ntile;
+        return;
+    }
+    const int cx = min(max((int)(qx * g.inv_cx), 0), g.nx - 1);
+    const int cy = min(max((int)(qy * g.inv_cy), 0), g.ny - 1);
+    key[sid] = (uint32_t)(si * g.ncell + cx * g.ny + cy);
+}
+
+// om(y) and grad om(y) for every live sample (sorted order, one thread each)
+__global__ void __launch_bounds__(ADJ_THREADS)
+    k_adj_query(McD d, BinGeom g, const int *__restrict__ start,
+                const uint32_t *__restrict__ keys_sorted, const float4 *__restrict__ srec,
+                float *__restrict__ om, float2 *__restrict__ G)
+{
+    const int p = blockIdx.x * ADJ_THREADS + threadIdx.x;
+    if (p >= start[g.ntile]) return;
+    const int si = keys_sorted[p] >= (uint32_t)g.ncell ? 1 : 0;
+    const float4 s = srec[p];
+    const uint32_t sid = __float_as_uint(s.z);
+    float acc = 0.f, gx = 0.f, gy = 0.f;
+    auto fn = [&](const float4 &c, float dx, float dy) {
+        float w, gr;
+        spline_wg(fmaf(dx, dx, dy * dy), d.inv_hw, w, gr);
+        acc = fmaf(c.z, w, acc);
+        // grad_y W(|y - x|) = W'(r)/r (y - x) and (dx, dy) = x - y
+        gx = fmaf(-c.z * gr, dx, gx);
+        gy = fmaf(-c.z * gr, dy, gy);
+    };
+    field_visit(d.set[si].field, s.x, s.y, fn);
+    om[sid] = d.sig2 * acc;
+    G[sid] = make_float2(d.sig2 * gx, d.sig2 * gy);
+}
+
+// one wave per (owner, sign)
+__global__ void __launch_bounds__(BIN_THREADS)
+    k_adj_final(McD d, BinGeom g, const uint32_t *__restrict__ key, const float2 *__restrict__ r,
+                const float *__restrict__ om, const float2 *__restrict__ G,
+                const int *__restrict__ src, const float2 *__restrict__ adj_vel,
+                float2 *__restrict__ own_grad, float2 *__restrict__ beta,
+                float4 *__restrict__ srcrec, uint32_t *__restrict__ srckey, uint32_t dead_src)
+{
+    const int pair = blockIdx.x * (BIN_THREADS / WAVE) + threadIdx.x / WAVE;
+    if (pair >= 2 * d.n_owner) return;
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int owner = pair >> 1, si = pair & 1;
+    const SetD &S = d.set[si];
+    const int n_s = S.count_dev ? *S.count_dev : S.n;
+    const float2 av = adj_vel[owner];
+    const float sg = si ? -d.inv_tot : d.inv_tot;   // u = v+ - v-, v = (1/NS) sum
+    const float ax = sg * av.x, ay = sg * av.y;
+    const float Om = n_s > 0 ? (float)(*S.total_dev) : 1.f;
+    float gxs = 0.f, gys = 0.f;
+    const size_t base = (size_t)pair * g.NS;
+    for (uint32_t k = lane; k < g.NS; k += WAVE) {
+        const size_t sid = base + k;
+        float b = 0.f, bt = 0.f, ex = 0.f, ey = 0.f;
+        uint32_t sk = dead_src;
+        if (n_s > 0 && key[sid] < (uint32_t)g.ntile) {
+            const float o = om[sid];
+            const float2 rr = r[sid];
+            const float r2 = rr.x * rr.x + rr.y * rr.y, rn = sqrtf(r2);
+            if (o > 0.f && rn > 0.f) {
+                const float f = 1.f / (2.f * PI_F * r2);
+                float ksx, ksy;
+                kper_smooth(d, rr.x, rr.y, ksx, ksy);
+                const float Kx = -rr.y * f + ksx, Ky = rr.x * f + ksy;
+                const float aK = ax * Kx + ay * Ky;
+                const float D = d.w1 * o / Om + d.w2 * d.alpha_c / rn;
+                const float iD = 1.f / D;
+                b = aK * iD;
+                bt = b - aK * o * d.w1 * iD * iD / Om;
+                // grad_r (a . K): free part, then the smooth part
+                const float P = -ax * rr.y + ay * rr.x;
+                float gfx = (ay - 2.f * P * rr.x / r2) * f;
+                float gfy = (-ax - 2.f * P * rr.y / r2) * f;
+                float gsx, gsy;
+                kper_smooth_grad(d, rr.x, rr.y, ax, ay, gsx, gsy);
+                const float c1 = o * iD, c2 = aK * o * d.w2 * d.alpha_c * iD * iD / (r2 * rn);
+                const float drx = c1 * (gfx + gsx) + c2 * rr.x;
+                const float dry = c1 * (gfy + gsy) + c2 * rr.y;
+                const float2 Gk = G[sid];
+                const float dyx = bt * Gk.x, dyy = bt * Gk.y;
+                const int j = src[sid];
+                if (j >= 0) {   // branch 1: r = x_i - (x_j + jitter)
+                    gxs += drx;
+                    gys += dry;
+                    ex = dyx - drx;
+                    ey = dyy - dry;
+                    sk = (uint32_t)j;
+                } else {   // branch 2: y = x_i + offset, r fixed
+                    gxs += dyx;
+                    gys += dyy;
+                }
+            }
+        }
+        beta[sid] = make_float2(b, bt);
+        srcrec[sid] = make_float4(ex, ey, 0.f, 0.f);
+        srckey[sid] = sk;
+    }
+    gxs = wave_sum(gxs);
+    gys = wave_sum(gys);
+    if (lane == 0) own_grad[pair] = make_float2(gxs, gys);
+}
+
+__global__ void k_adj_recs(const int *__restrict__ start, int ntile, const float4 *__restrict__ srec,
+                           const float2 *__restrict__ beta, float4 *__restrict__ rec2)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= start[ntile]) return;
+    const float4 s = srec[p];
+    const float2 b = beta[__float_as_uint(s.z)];
+    rec2[p] = make_float4(s.x, s.y, b.x, b.y);
+}
+
+// one thread per particle of set blockIdx.y: sums over the set's samples
+__global__ void __launch_bounds__(ADJ_THREADS)
+    k_adj_field(McD d, BinGeom g, const int *__restrict__ start, const float4 *__restrict__ rec2,
+                const int *__restrict__ order0, const int *__restrict__ order1,
+                const int *__restrict__ sub0, const int *__restrict__ sub1,
+                float2 *__restrict__ grad_pos, float *__restrict__ grad_w)
+{
+    const int si = blockIdx.y;
+    const SetD &S = d.set[si];
+    const int n_s = S.count_dev ? *S.count_dev : S.n;
+    const int t = blockIdx.x * ADJ_THREADS + threadIdx.x;
+    if (t >= n_s) return;
+    const float4 c = S.field.pts[t];
+    FieldD smp;
+    smp.pts = rec2;
+    smp.cell_start = start + (size_t)si * g.ncell;
+    smp.g = S.field.g;
+    float A = 0.f, bx = 0.f, by = 0.f;
+    auto fn = [&](const float4 &s, float dx, float dy) {
+        // (dx, dy) = y - x_m; grad_y W = W'(r)/r (y - x_m)
+        float w, gr;
+        spline_wg(fmaf(dx, dx, dy * dy), d.inv_hw, w, gr);
+        A = fmaf(s.z, w, A);
+        bx = fmaf(s.w * gr, dx, bx);
+        by = fmaf(s.w * gr, dy, by);
+    };
+    field_visit(smp, c.x, c.y, fn);
+    const int j = (si ? order1 : order0)[t];
+    const int gm = (si ? sub1 : sub0)[j];
+    const float sg = si ? -1.f : 1.f;   // d|w|/dw on the set
+    grad_w[gm] += sg * d.sig2 * A;
+    float2 gp = grad_pos[gm];
+    gp.x -= c.z * d.sig2 * bx;
+    gp.y -= c.z * d.sig2 * by;
+    grad_pos[gm] = gp;
+}
+
+// owner terms (both signs) and the branch-1 source terms, per particle
+__global__ void k_adj_sources(int n, const int *__restrict__ src_start,
+                              const float4 *__restrict__ srcrec_sorted,
+                              const float2 *__restrict__ own_grad, float2 *__restrict__ grad_pos)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float2 a = own_grad[2 * (size_t)i], b = own_grad[2 * (size_t)i + 1];
+    float sx = a.x + b.x, sy = a.y + b.y;
+    for (int p = src_start[i]; p < src_start[i + 1]; ++p) {
+        const float4 e = srcrec_sorted[p];
+        sx += e.x;
+        sy += e.y;
+    }
+    float2 gp = grad_pos[i];
+    gp.x += sx;
+    gp.y += sy;
+    grad_pos[i] = gp;
+}
+
+__global__ void k_scale_xy(int n, const float2 *__restrict__ x, float a, float2 *__restrict__ y)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) y[i] = make_float2(a * x[i].x, a * x[i].y);
+}
+
+struct AdjBufs {
+    uint32_t *key, *keys_sorted, *srckey, *srckey_sorted;
+    float2 *q, *r, *G, *beta, *own;
+    float *om;
+    int *src, *start, *src_start;
+    float4 *srec, *rec2, *srcrec, *srcrec_sorted;
+    void *sort_ws;
+    size_t sort_bytes;
+};
+
+static size_t adj_ws_bytes(const BinGeom &g, int n)
+{
+    const size_t S = g.S > 0 ? g.S : 1, T = (size_t)g.ntile, N = (size_t)(n > 0 ? n : 1);
+    return 4 * align256(S * 4) + 5 * align256(S * 8) + align256(S * 4) + align256(S * 4) +
+           align256((T + 1) * 4) + align256((N + 1) * 4) + 4 * align256(S * 16) +
+           align256(2 * N * 8) + radix_workspace_bytes((int)S) + 8192;
+}
+
+static int carve_adj(const BinGeom &g, int n, void *ws, size_t bytes, AdjBufs *B)
+{
+    Carve cv(ws, bytes);
+    const size_t S = g.S > 0 ? g.S : 1, T = (size_t)g.ntile, N = (size_t)(n > 0 ? n : 1);
+    B->key = cv.take<uint32_t>(S);
+    B->keys_sorted = cv.take<uint32_t>(S);
+    B->srckey = cv.take<uint32_t>(S);
+    B->srckey_sorted = cv.take<uint32_t>(S);
+    B->q = cv.take<float2>(S);
+    B->r = cv.take<float2>(S);
+    B->G = cv.take<float2>(S);
+    B->beta = cv.take<float2>(S);
+    B->own = cv.take<float2>(2 * N);
+    B->om = cv.take<float>(S);
+    B->src = cv.take<int>(S);
+    B->start = cv.take<int>(T + 1);
+    B->src_start = cv.take<int>(N + 1);
+    B->srec = cv.take<float4>(S);
+    B->rec2 = cv.take<float4>(S);
+    B->srcrec = cv.take<float4>(S);
+    B->srcrec_sorted = cv.take<float4>(S);
+    B->sort_bytes = radix_workspace_bytes((int)S);
+    B->sort_ws = cv.take<char>(B->sort_bytes);
+    if (!B->key || !B->keys_sorted || !B->srckey || !B->srckey_sorted || !B->q || !B->r ||
+        !B->G || !B->beta || !B->own || !B->om || !B->src || !B->start || !B->src_start ||
+        !B->srec || !B->rec2 || !B->srcrec || !B->srcrec_sorted || !B->sort_ws)
+        return VPMC_ERR_WORKSPACE;
+    return VPMC_OK;
+}
+
+static int adj_check(const vpmc_mc_desc *d, McD *m, BinGeom *g)
+{
+    if (!d) return VPMC_ERR_ARG;
+    if (d->mode != VPMC_MODE_CONSISTENT || !d->periodic || d->uniforms) return VPMC_ERR_ARG;
+    VPMC_TRY(make_mcd(d, m));
+    VPMC_TRY(bin_geom(*m, d->N1, d->N2, d->n_owner, g));
+    for (int s = 0; s < 2; ++s)
+        if ((d->set[s].n > 0 || d->set[s].count_dev) && !d->set[s].field.order)
+            return VPMC_ERR_ARG;
+    return VPMC_OK;
+}
+
+int mc_adjoint_impl(const vpmc_mc_desc *d, const int *sub_pos, const int *sub_neg,
+                    const float *adj_vel, float *grad_pos, float *grad_w, void *ws,
+                    size_t ws_bytes, hipStream_t s)
+{
+    McD m;
+    BinGeom g;
+    VPMC_TRY(adj_check(d, &m, &g));
+    const int n = d->n_owner;
+    if (n == 0) return VPMC_OK;
+    if (!adj_vel || !grad_pos || !grad_w || !sub_pos || !sub_neg) return VPMC_ERR_ARG;
+    AdjBufs B;
+    VPMC_TRY(carve_adj(g, n, ws, ws_bytes, &B));
+    const unsigned gs = (unsigned)((g.S + BIN_THREADS - 1) / BIN_THREADS);
+    hipLaunchKernelGGL(k_adj_gen, dim3(gs), dim3(BIN_THREADS), 0, s, m, g, sub_pos, sub_neg,
+                       B.key, B.q, B.r, B.src);
+    VPMC_CHECK_LAUNCH();
+    VPMC_TRY(radix_sort_to_records(B.key, (int)g.S, key_bits_for((uint32_t)g.ntile),
+                                   (const float2 *)B.q, nullptr, B.srec, B.keys_sorted, g.ntile,
+                                   B.start, B.sort_ws, B.sort_bytes, s));
+    const unsigned ga = (unsigned)((g.S + ADJ_THREADS - 1) / ADJ_THREADS);
+    hipLaunchKernelGGL(k_adj_query, dim3(ga), dim3(ADJ_THREADS), 0, s, m, g,
+                       (const int *)B.start, (const uint32_t *)B.keys_sorted,
+                       (const float4 *)B.srec, B.om, B.G);
+    const unsigned gf = (unsigned)((2 * (size_t)n + 3) / 4);
+    hipLaunchKernelGGL(k_adj_final, dim3(gf), dim3(BIN_THREADS), 0, s, m, g,
+                       (const uint32_t *)B.key, (const float2 *)B.r, (const float *)B.om,
+                       (const float2 *)B.G, (const int *)B.src, (const float2 *)adj_vel, B.own,
+                       B.beta, B.srcrec, B.srckey, (uint32_t)n);
+    hipLaunchKernelGGL(k_adj_recs, dim3(ga), dim3(ADJ_THREADS), 0, s, (const int *)B.start,
+                       g.ntile, (const float4 *)B.srec, (const float2 *)B.beta, B.rec2);
+    VPMC_CHECK_LAUNCH();
+    const int nmax = d->set[0].n > d->set[1].n ? d->set[0].n : d->set[1].n;
+    if (nmax > 0) {
+        hipLaunchKernelGGL(k_adj_field, dim3((unsigned)((nmax + ADJ_THREADS - 1) / ADJ_THREADS), 2),
+                           dim3(ADJ_THREADS), 0, s, m, g, (const int *)B.start,
+                           (const float4 *)B.rec2, d->set[0].field.order, d->set[1].field.order,
+                           sub_pos, sub_neg, (float2 *)grad_pos, grad_w);
+        VPMC_CHECK_LAUNCH();
+    }
+    VPMC_TRY(radix_sort_to_records(B.srckey, (int)g.S, key_bits_for((uint32_t)n), nullptr,
+                                   (const float4 *)B.srcrec, B.srcrec_sorted, B.srckey_sorted, n,
+                                   B.src_start, B.sort_ws, B.sort_bytes, s));
+    hipLaunchKernelGGL(k_adj_sources, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n,
+                       (const int *)B.src_start, (const float4 *)B.srcrec_sorted,
+                       (const float2 *)B.own, (float2 *)grad_pos);
+    VPMC_CHECK_LAUNCH();
+    return VPMC_OK;
+}
+
+// ---- FP64 forward of the same frozen-sample estimator (checks) -------------------
+
+constexpr int KPER_TERMS_ADJ64 = 24;
+struct Kper64 {
+    double c[KPER_TERMS_ADJ64];
+};
+const double *kper_coeffs_f64_host(int nterms);
+
+struct F64Args {
+    const double2 *pos;         // global positions
+    const double4 *rec[2];      // per set, cell order: (x, y, a, a0)
+    const double2 *set64[2];    // per set, local order: positions
+    const int *cell_start[2];
+    FieldGeom geom;
+    double h_w, inv_hw, sig2, L, Rc, alpha, w1, w2, inv_tot;
+    double2 *contrib;           // per sample
+    Kper64 kc;
+};
+
+__global__ void k_f64_recs(int n_s, const int *__restrict__ count_dev,
+                           const int *__restrict__ order, const int *__restrict__ sub,
+                           const double2 *__restrict__ pos, const double *__restrict__ w,
+                           const float *__restrict__ w0, double sg, double4 *__restrict__ rec,
+                           double2 *__restrict__ set64)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n = count_dev ? *count_dev : n_s;
+    if (t >= n) return;
+    const int gt = sub[order[t]];
+    rec[t] = make_double4(pos[gt].x, pos[gt].y, sg * w[gt], fabs((double)w0[gt]));
+    set64[t] = pos[sub[t]];
+}
+
+__global__ void __launch_bounds__(ADJ_THREADS) k_f64_sample(McD d, BinGeom g, F64Args a)
+{
+    const size_t sid = (size_t)blockIdx.x * ADJ_THREADS + threadIdx.x;
+    if (sid >= g.S) return;
+    const uint32_t pair = (uint32_t)(sid / g.NS);
+    const int k = (int)(sid - (size_t)pair * g.NS);
+    const int owner = (int)(pair >> 1), si = (int)(pair & 1);
+    const SetD &S = d.set[si];
+    const int n_s = S.count_dev ? *S.count_dev : S.n;
+    double2 out = make_double2(0.0, 0.0);
+    if (n_s > 0) {
+        const bool b1 = k < d.N1;
+        float u[3];
+        draws(d, owner, si, b1 ? 1 : 2, b1 ? k : k - d.N1, u);
+        const double2 xi = a.pos[owner];
+        const double L = a.L;
+        double yx, yy, rx, ry;
+        bool live = true;
+        if (b1) {
+            const int j = guided_upper_bound(u[0], S.cdf, S.guide, S.glog2, n_s);
+            const double2 xj = a.set64[si][j];
+            const double rho = mass_inverse_f64((double)u[1]) * a.h_w;
+            double sn, cs;
+            sincos(2.0 * PI_D * (double)u[2], &sn, &cs);
+            yx = xj.x + rho * cs;
+            yy = xj.y + rho * sn;
+            rx = xi.x - yx;
+            ry = xi.y - yy;
+            if (rx > 0.5 * L) rx -= L; else if (rx < -0.5 * L) rx += L;
+            if (ry > 0.5 * L) ry -= L; else if (ry < -0.5 * L) ry += L;
+        } else {
+            const double rho = a.Rc * (double)u[1];
+            double sn, cs;
+            sincos(2.0 * PI_D * (double)u[0], &sn, &cs);
+            yx = xi.x + rho * cs;
+            yy = xi.y + rho * sn;
+            rx = -rho * cs;
+            ry = -rho * sn;
+            live = fabs(rx) <= 0.5 * L && fabs(ry) <= 0.5 * L;
+        }
+        if (live) {
+            yx -= floor(yx / L) * L;
+            yy -= floor(yy / L) * L;
+            double om = 0.0, om0 = 0.0;
+            auto fn = [&](const double4 &c, double dx, double dy) {
+                double w, gr;
+                spline_wg64(dx * dx + dy * dy, a.inv_hw, w, gr);
+                om += c.z * w;
+                om0 += c.w * w;
+            };
+            field_visit64(a.geom, a.cell_start[si], a.rec[si], yx, yy, fn);
+            om *= a.sig2;
+            om0 *= a.sig2;
+            const double r2 = rx * rx + ry * ry, rn = sqrt(r2);
+            if (om0 > 0.0 && rn > 0.0) {
+                const double Om0 = *S.total_dev;
+                const double D = a.w1 * om0 / Om0 + a.w2 * a.alpha / rn;
+                double ksx, ksy;
+                kper_smooth_f64(a.kc.c, KPER_TERMS_ADJ64, L, rx, ry, ksx, ksy);
+                const double f = 1.0 / (2.0 * PI_D * r2);
+                const double gk = om / D;
+                out = make_double2((-ry * f + ksx) * gk, (rx * f + ksy) * gk);
+            }
+        }
+    }
+    a.contrib[sid] = out;
+}
+
+// per (owner, sign): sum in sample order; vel[si * n + owner]
+__global__ void k_f64_reduce(int n_owner, uint32_t NS, double inv_tot,
+                             const double2 *__restrict__ contrib, double2 *__restrict__ vel)
+{
+    const int pair = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pair >= 2 * n_owner) return;
+    double sx = 0.0, sy = 0.0;
+    const size_t base = (size_t)pair * NS;
+    for (uint32_t k = 0; k < NS; ++k) {
+        sx += contrib[base + k].x;
+        sy += contrib[base + k].y;
+    }
+    vel[(size_t)(pair & 1) * n_owner + (pair >> 1)] = make_double2(sx * inv_tot, sy * inv_tot);
+}
+
+__global__ void k_f64_advect(int n, double2 *__restrict__ pos, const double2 *__restrict__ vel,
+                             double dt, double L, float *__restrict__ pos32)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double2 p = pos[i];
+    p.x += dt * (vel[i].x - vel[(size_t)n + i].x);
+    p.y += dt * (vel[i].y - vel[(size_t)n + i].y);
+    p.x -= floor(p.x / L) * L;
+    p.y -= floor(p.y / L) * L;
+    pos[i] = p;
+    if (pos32) {
+        pos32[2 * (size_t)i] = (float)p.x;
+        pos32[2 * (size_t)i + 1] = (float)p.y;
+    }
+}
+
+__global__ void k_d2f_xy(int n, const double2 *__restrict__ p, float2 *__restrict__ q)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) q[i] = make_float2((float)p[i].x, (float)p[i].y);
+}
+
+// ---- grid vorticity (loss) ----------------------------------------------------------
+
+__global__ void k_grid_query(FieldD f, float inv_hw, float sig2, int gnx, int gny,
+                             float *__restrict__ out)
+{
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= gnx * gny) return;
+    const int a = idx / gny, b = idx - a * gny;
+    const float zx = ((float)a + 0.5f) * (f.g.Lx / (float)gnx);
+    const float zy = ((float)b + 0.5f) * (f.g.Ly / (float)gny);
+    float acc = 0.f;
+    auto fn = [&](const float4 &c, float dx, float dy) {
+        float w, gr;
+        spline_wg(fmaf(dx, dx, dy * dy), inv_hw, w, gr);
+        acc = fmaf(c.z, w, acc);
+    };
+    field_visit(f, zx, zy, fn);
+    out[idx] = sig2 * acc;
+}
+
+// grid points within h_w of each particle, by index range (periodic)
+__global__ void k_grid_adjoint(int n, const float2 *__restrict__ pos, const float *__restrict__ w,
+                               float h_w, float Lx, float Ly, int gnx, int gny,
+                               const float *__restrict__ gout, float2 *__restrict__ grad_pos,
+                               float *__restrict__ grad_w)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float2 p = pos[i];
+    const float hx = Lx / (float)gnx, hy = Ly / (float)gny;
+    const float inv_hw = 1.f / h_w, sig2 = 40.f / (7.f * PI_F * h_w * h_w);
+    const int a0 = (int)ceilf((p.x - h_w) / hx - 0.5f), a1 = (int)floorf((p.x + h_w) / hx - 0.5f);
+    const int b0 = (int)ceilf((p.y - h_w) / hy - 0.5f), b1 = (int)floorf((p.y + h_w) / hy - 0.5f);
+    float A = 0.f, bx = 0.f, by = 0.f;
+    for (int a = a0; a <= a1; ++a) {
+        const int aw = ((a % gnx) + gnx) % gnx;
+        const float dx = ((float)a + 0.5f) * hx - p.x;
+        for (int b = b0; b <= b1; ++b) {
+            const int bw = ((b % gny) + gny) % gny;
+            const float dy = ((float)b + 0.5f) * hy - p.y;
+            float wv, gr;
+            spline_wg(fmaf(dx, dx, dy * dy), inv_hw, wv, gr);
+            const float go = gout[aw * gny + bw];
+            A = fmaf(go, wv, A);
+            bx = fmaf(go * gr, dx, bx);
+            by = fmaf(go * gr, dy, by);
+        }
+    }
+    grad_w[i] += sig2 * A;
+    float2 gp = grad_pos[i];
+    const float wi = w[i];
+    gp.x -= wi * sig2 * bx;   // d/dx_i W(|z - x_i|) = -W'(r)/r (z - x_i)
+    gp.y -= wi * sig2 * by;
+    grad_pos[i] = gp;
+}
+
+// FP64, brute force over all particles (checks)
+__global__ void k_grid_f64(int n, const double2 *__restrict__ pos, const double *__restrict__ w,
+                           double h_w, double Lx, double Ly, int gnx, int gny,
+                           double *__restrict__ out)
+{
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= gnx * gny) return;
+    const int a = idx / gny, b = idx - a * gny;
+    const double zx = ((double)a + 0.5) * (Lx / gnx), zy = ((double)b + 0.5) * (Ly / gny);
+    const double inv_hw = 1.0 / h_w, hw2 = h_w * h_w;
+    double acc = 0.0;
+    for (int j = 0; j < n; ++j) {
+        double dx = zx - pos[j].x, dy = zy - pos[j].y;
+        dx -= rint(dx / Lx) * Lx;
+        dy -= rint(dy / Ly) * Ly;
+        const double r2 = dx * dx + dy * dy;
+        if (r2 < hw2) {
+            double wv, gr;
+            spline_wg64(r2, inv_hw, wv, gr);
+            acc += w[j] * wv;
+        }
+    }
+    out[idx] = 40.0 / (7.0 * PI_D * h_w * h_w) * acc;
+}
+
+}  // namespace vpmc
+
+using namespace vpmc;
+
+extern "C" {
+
+size_t vpmc_mc_adjoint_workspace_bytes(const vpmc_mc_desc *d)
+{
+    McD m;
+    BinGeom g;
+    if (adj_check(d, &m, &g) != VPMC_OK) return 0;
+    return adj_ws_bytes(g, d->n_owner);
+}
+
+int vpmc_mc_adjoint(const vpmc_mc_desc *d, const int *sub_pos, const int *sub_neg,
+                    const float *adj_vel, float *grad_pos, float *grad_w, void *workspace,
+                    size_t workspace_bytes, void *stream)
+{
+    return mc_adjoint_impl(d, sub_pos, sub_neg, adj_vel, grad_pos, grad_w, workspace,
+                           workspace_bytes, as_stream(stream));
+}
+
+size_t vpmc_step_adjoint_workspace_bytes(const vpmc_mc_desc *d)
+{
+    if (!d) return 0;
+    const size_t a = vpmc_mc_adjoint_workspace_bytes(d);
+    if (a == 0) return 0;
+    const int m = d->set[0].n > d->set[1].n ? d->set[0].n : d->set[1].n;
+    size_t f = vpmc_field_workspace_bytes(m, d->set[0].field.nx, d->set[0].field.ny);
+    const size_t n = (size_t)(d->n_owner > 0 ? d->n_owner : 1);
+    return align256(n * 8) + (a > f ? a : f);
+}
+
+int vpmc_step_adjoint(vpmc_mc_desc *d, int n_total, const float *pos_xy, const float *omega,
+                      const int *sub_pos, const int *sub_neg, float dt, float *lambda,
+                      float *grad_w, void *workspace, size_t workspace_bytes, void *stream)
+{
+    if (!d || n_total < 0 || d->n_owner != n_total || d->owner_xy != pos_xy) return VPMC_ERR_ARG;
+    if (n_total == 0) return VPMC_OK;
+    if (!lambda || !grad_w || !omega) return VPMC_ERR_ARG;
+    hipStream_t s = as_stream(stream);
+    Carve cv(workspace, workspace_bytes);
+    float2 *av = cv.take<float2>((size_t)n_total);
+    if (!av) return VPMC_ERR_WORKSPACE;
+    char *rest = cv.base + cv.used;
+    const size_t rest_bytes = cv.cap - cv.used;
+    const int *subs[2] = {sub_pos, sub_neg};
+    for (int k = 0; k < 2; ++k) {
+        vpmc_sign_set &S = d->set[k];
+        VPMC_TRY(field_build_impl(S.field.nx, S.field.ny, S.field.Lx, S.field.Ly, S.n, nullptr,
+                                  subs[k], pos_xy, omega, 1, const_cast<float *>(S.field.pts),
+                                  const_cast<int *>(S.field.cell_start),
+                                  const_cast<int *>(S.field.order),
+                                  const_cast<float *>(S.set_xy), rest, rest_bytes, s));
+    }
+    hipLaunchKernelGGL(k_scale_xy, dim3((unsigned)((n_total + 255) / 256)), dim3(256), 0, s,
+                       n_total, (const float2 *)lambda, dt, av);
+    VPMC_CHECK_LAUNCH();
+    return mc_adjoint_impl(d, sub_pos, sub_neg, (const float *)av, lambda, grad_w, rest,
+                           rest_bytes, s);
+}
+
+size_t vpmc_step_f64_workspace_bytes(const vpmc_mc_desc *d)
+{
+    McD m;
+    if (!d || make_mcd(d, &m) != VPMC_OK) return 0;
+    BinGeom g;
+    if (bin_geom(m, d->N1, d->N2, d->n_owner, &g) != VPMC_OK) return 0;
+    const size_t n = (size_t)(d->n_owner > 0 ? d->n_owner : 1);
+    const int mx = d->set[0].n > d->set[1].n ? d->set[0].n : d->set[1].n;
+    const size_t nm = (size_t)(mx > 0 ? mx : 1);
+    return align256(n * 8) + 2 * align256(nm * 32) + 2 * align256(nm * 16) +
+           align256((g.S > 0 ? g.S : 1) * 16) + align256(2 * n * 16) +
+           vpmc_field_workspace_bytes(mx, d->set[0].field.nx, d->set[0].field.ny) + 4096;
+}
+
+int vpmc_step_f64(vpmc_mc_desc *d, int n_total, double *pos_xy, const double *w,
+                  const float *omega0, const int *sub_pos, const int *sub_neg, double dt,
+                  void *workspace, size_t workspace_bytes, void *stream)
+{
+    if (!d || n_total < 0 || d->n_owner != n_total) return VPMC_ERR_ARG;
+    if (d->mode != VPMC_MODE_CONSISTENT || !d->periodic || d->uniforms || d->Lx != d->Ly)
+        return VPMC_ERR_ARG;
+    if (n_total == 0) return VPMC_OK;
+    if (!pos_xy || !w || !omega0 || !sub_pos || !sub_neg) return VPMC_ERR_ARG;
+    hipStream_t s = as_stream(stream);
+    McD m;
+    VPMC_TRY(make_mcd(d, &m));
+    BinGeom g;
+    VPMC_TRY(bin_geom(m, d->N1, d->N2, d->n_owner, &g));
+    Carve cv(workspace, workspace_bytes);
+    const int mx = d->set[0].n > d->set[1].n ? d->set[0].n : d->set[1].n;
+    const size_t nm = (size_t)(mx > 0 ? mx : 1);
+    float2 *pos32 = cv.take<float2>((size_t)n_total);
+    double4 *rec[2] = {cv.take<double4>(nm), cv.take<double4>(nm)};
+    double2 *set64[2] = {cv.take<double2>(nm), cv.take<double2>(nm)};
+    double2 *contrib = cv.take<double2>(g.S > 0 ? g.S : 1);
+    double2 *vel = cv.take<double2>(2 * (size_t)n_total);
+    const size_t fb = vpmc_field_workspace_bytes(mx, d->set[0].field.nx, d->set[0].field.ny);
+    void *fws = cv.take<char>(fb);
+    if (!pos32 || !rec[0] || !rec[1] || !set64[0] || !set64[1] || !contrib || !vel || !fws)
+        return VPMC_ERR_WORKSPACE;
+    const unsigned gn = (unsigned)((n_total + 255) / 256);
+    hipLaunchKernelGGL(k_d2f_xy, dim3(gn), dim3(256), 0, s, n_total, (const double2 *)pos_xy,
+                       pos32);
+    VPMC_CHECK_LAUNCH();
+    const int *subs[2] = {sub_pos, sub_neg};
+    for (int k = 0; k < 2; ++k) {
+        vpmc_sign_set &S = d->set[k];
+        VPMC_TRY(field_build_impl(S.field.nx, S.field.ny, S.field.Lx, S.field.Ly, S.n, nullptr,
+                                  subs[k], (const float *)pos32, omega0, 1,
+                                  const_cast<float *>(S.field.pts),
+                                  const_cast<int *>(S.field.cell_start),
+                                  const_cast<int *>(S.field.order),
+                                  const_cast<float *>(S.set_xy), fws, fb, s));
+        if (S.n > 0) {
+            hipLaunchKernelGGL(k_f64_recs, dim3((unsigned)((S.n + 255) / 256)), dim3(256), 0, s,
+                               S.n, S.count_dev, S.field.order, subs[k], (const double2 *)pos_xy,
+                               w, omega0, k ? -1.0 : 1.0, rec[k], set64[k]);
+            VPMC_CHECK_LAUNCH();
+        }
+    }
+    F64Args a;
+    a.pos = (const double2 *)pos_xy;
+    for (int k = 0; k < 2; ++k) {
+        a.rec[k] = rec[k];
+        a.set64[k] = set64[k];
+        a.cell_start[k] = d->set[k].field.cell_start;
+    }
+    a.geom = m.set[0].field.g;
+    a.h_w = (double)d->h_w;
+    a.inv_hw = 1.0 / a.h_w;
+    a.sig2 = 40.0 / (7.0 * PI_D * a.h_w * a.h_w);
+    a.L = (double)d->Lx;
+    a.Rc = 0.5 * sqrt(2.0) * a.L;
+    a.alpha = 1.0 / (2.0 * PI_D * a.Rc);
+    a.w1 = (double)d->N1 / (double)(d->N1 + d->N2);
+    a.w2 = (double)d->N2 / (double)(d->N1 + d->N2);
+    a.inv_tot = 1.0 / (double)(d->N1 + d->N2);
+    a.contrib = contrib;
+    const double *kc = kper_coeffs_f64_host(KPER_TERMS_ADJ64);
+    for (int i = 0; i < KPER_TERMS_ADJ64; ++i) a.kc.c[i] = kc[i];
+    const unsigned gs = (unsigned)((g.S + ADJ_THREADS - 1) / ADJ_THREADS);
+    hipLaunchKernelGGL(k_f64_sample, dim3(gs), dim3(ADJ_THREADS), 0, s, m, g, a);
+    hipLaunchKernelGGL(k_f64_reduce, dim3((unsigned)((2 * (size_t)n_total + 255) / 256)),
+                       dim3(256), 0, s, n_total, g.NS, a.inv_tot, (const double2 *)contrib, vel);
+    hipLaunchKernelGGL(k_f64_advect, dim3(gn), dim3(256), 0, s, n_total, (double2 *)pos_xy,
+                       (const double2 *)vel, dt, a.L, (float *)nullptr);
+    VPMC_CHECK_LAUNCH();
+    return VPMC_OK;
+}
+
+size_t vpmc_grid_vorticity_workspace_bytes(int n, int nx, int ny)
+{
+    const size_t m = (size_t)(n > 0 ? n : 1);
+    return align256(m * 16) + align256(((size_t)nx * ny + 1) * 4) +
+           vpmc_field_workspace_bytes(n, nx, ny) + 1024;
+}
+
+int vpmc_grid_vorticity(int n, const float *pos_xy, const float *w, float h_w, float Lx,
+                        float Ly, int cells_per_hw, int gnx, int gny, float *out, void *workspace,
+                        size_t workspace_bytes, void *stream)
+{
+    if (n < 0 || gnx <= 0 || gny <= 0 || !out || (n > 0 && (!pos_xy || !w))) return VPMC_ERR_ARG;
+    hipStream_t s = as_stream(stream);
+    int nx, ny;
+    VPMC_TRY(vpmc_field_grid(h_w, Lx, Ly, cells_per_hw, &nx, &ny));
+    Carve cv(workspace, workspace_bytes);
+    float *pts = cv.take<float>(4 * (size_t)(n > 0 ? n : 1));
+    int *cs = cv.take<int>((size_t)nx * ny + 1);
+    const size_t fb = vpmc_field_workspace_bytes(n, nx, ny);
+    void *fws = cv.take<char>(fb);
+    if (!pts || !cs || !fws) return VPMC_ERR_WORKSPACE;
+    VPMC_TRY(field_build_impl(nx, ny, Lx, Ly, n, nullptr, nullptr, pos_xy, w, 0, pts, cs, nullptr,
+                              nullptr, fws, fb, s));
+    vpmc_field fh{nx, ny, Lx, Ly, n, pts, cs, nullptr};
+    FieldGeom g;
+    VPMC_TRY(field_geom(&fh, h_w, &g));
+    FieldD f{(const float4 *)pts, cs, g};
+    hipLaunchKernelGGL(k_grid_query, dim3((unsigned)((gnx * gny + 255) / 256)), dim3(256), 0, s, f,
+                       1.f / h_w, 40.f / (7.f * PI_F * h_w * h_w), gnx, gny, out);
+    VPMC_CHECK_LAUNCH();
+    return VPMC_OK;
+}
+
+int vpmc_grid_vorticity_adjoint(int n, const float *pos_xy, const float *w, float h_w, float Lx,
+                                float Ly, int gnx, int gny, const float *grad_out,
+                                float *grad_pos, float *grad_w, void *stream)
+{
+    if (n < 0 || gnx <= 0 || gny <= 0 || !(h_w > 0.f) || 2.f * h_w >= Lx || 2.f * h_w >= Ly)
+        return VPMC_ERR_ARG;
+    if (n == 0) return VPMC_OK;
+    if (!pos_xy || !w || !grad_out || !grad_pos || !grad_w) return VPMC_ERR_ARG;
+    hipLaunchKernelGGL(k_grid_adjoint, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                       as_stream(stream), n, (const float2 *)pos_xy, w, h_w, Lx, Ly, gnx, gny,
+                       grad_out, (float2 *)grad_pos, grad_w);
+    VPMC_CHECK_LAUNCH();
+    return VPMC_OK;
+}
+
+int vpmc_grid_vorticity_f64(int n, const double *pos_xy, const double *w, double h_w, double Lx,
+                            double Ly, int gnx, int gny, double *out, void *stream)
+{
+    if (n < 0 || gnx <= 0 || gny <= 0 || !out || (n > 0 && (!pos_xy || !w))) return VPMC_ERR_ARG;
+    hipLaunchKernelGGL(k_grid_f64, dim3((unsigned)((gnx * gny + 255) / 256)), dim3(256), 0,
+                       as_stream(stream), n, (const double2 *)pos_xy, w, h_w, Lx, Ly, gnx, gny,
+                       out);
+    VPMC_CHECK_LAUNCH();
+    return VPMC_OK;
+}
+
+}  // extern "C"

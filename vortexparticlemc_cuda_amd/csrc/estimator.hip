@@ -128,6 +128,15 @@ static const float *kper_coeffs_f32()
     return kc;
 }
 
+// the first 24 coefficients in FP64 (FP64 forward of the frozen-sample estimator)
+const double *kper_coeffs_f64_host(int nterms)
+{
+    static double kc[24];
+    static std::once_flag once;
+    std::call_once(once, [] { kper_coeffs_host(24, kc); });
+    return nterms <= 24 ? kc : nullptr;
+}
+
 int make_mcd(const vpmc_mc_desc *in, McD *d)
 {
     if (!in || in->N1 < 0 || in->N2 < 0 || in->N1 + in->N2 <= 0 || in->n_owner < 0 ||

@@ -130,10 +130,12 @@ __device__ __forceinline__ void kper_smooth(const McD &d, float rx, float ry, fl
 // branch-2 point outside the min-image cell, or a non-finite point).
 // (qx, qy): the query point inside the box; (rx, ry): the displacement the
 // MIS contribution uses (reference: dst - sp unwrapped; consistent: min image).
+// src (optional) receives the set-local index of a branch-1 sample's
+// particle, -1 for a branch-2 sample.
 template <int MODE, bool PER>
 __device__ __forceinline__ bool gen_sample(const McD &d, const SetD &S, int n_s, int owner,
                                            int si, int k, float2 dst, float &qx, float &qy,
-                                           float &rx, float &ry)
+                                           float &rx, float &ry, int *src = nullptr)
 {
     const bool b1 = k < d.N1;
     float u[3];
@@ -165,6 +167,7 @@ __device__ __forceinline__ bool gen_sample(const McD &d, const SetD &S, int n_s,
     } else {
         if (b1) {   // particle from |w|/Omega, jitter from W (exact radial inverse)
             int j = guided_upper_bound(u[0], S.cdf, S.guide, S.glog2, n_s);
+            if (src) *src = j;
             float2 xj = S.set_xy[j];
             float rho = mass_inverse(u[1]) * d.h_w;
             float s, c;
@@ -178,6 +181,7 @@ __device__ __forceinline__ bool gen_sample(const McD &d, const SetD &S, int n_s,
                 if (ry > 0.5f * d.Ly) ry -= d.Ly; else if (ry < -0.5f * d.Ly) ry += d.Ly;
             }
         } else {   // disk radius uniform in [0, Rc): density alpha / r
+            if (src) *src = -1;
             float rho = d.Rc * u[1];
             float s, c;
             sincosf(2.f * PI_F * u[0], &s, &c);
