@@ -351,6 +351,30 @@ def large_bench(args, device):
                              float(P.h_w))
     err = float(torch.sqrt(((um - ud) ** 2).sum() / (ud ** 2).sum()))
     log("large: RMS error vs FP64 blob direct sum at %d probes: %.4g" % (probes.shape[0], err))
+    del vs, um, ud
+    # launch shape: a few candidates, one timed step each, on the one
+    # simulation (its buffers are re-laid for each grid)
+    tune = {}
+    if args.large_tune:
+        for cph, spl, th in ((6, 1, 256), (8, 1, 256), (8, 2, 256), (12, 1, 256)):
+            sim.cells_per_hw, sim.spl, sim.eval_threads = cph, spl, th
+            sim._alloc()
+            try:
+                run_steps(sim, 1)
+                a = torch.cuda.Event(enable_timing=True)
+                b = torch.cuda.Event(enable_timing=True)
+                a.record()
+                run_steps(sim, 1)
+                b.record()
+                torch.cuda.synchronize()
+            except Exception:   # stencil does not fit this workgroup / grid
+                continue
+            tune["%d,%d,%d" % (cph, spl, th)] = a.elapsed_time(b)
+        best = min(tune, key=tune.get)
+        sim.cells_per_hw, sim.spl, sim.eval_threads = (int(v) for v in best.split(","))
+        sim._alloc()
+        log("large: tuned cells_per_hw=%d spl=%d eval_threads=%d (%s)"
+            % (sim.cells_per_hw, sim.spl, sim.eval_threads, tune))
     run_steps(sim, 2)
     torch.cuda.synchronize()
     events = {"est": [], "eval": []}
@@ -374,12 +398,129 @@ def large_bench(args, device):
            "rms_err_vs_direct": err, "probes": int(probes.shape[0]),
            "rbf_iters": it, "rbf_rel_res": rr, "rbf_status": st, "rbf_seconds": t_cg,
            "cells_per_hw": sim.cells_per_hw, "spl": sim.spl, "eval_threads": sim.eval_threads,
+           "tune_ms": tune,
            "roofline": {"kernel": "k_bin_eval", "bound": "valu", "achieved": flop / eval_s,
                         "peak": PEAK_FP32, "unit": "FLOP/s", "frac": flop / eval_s / PEAK_FP32,
                         "avg_launch_ms": eval_s * 1e3, "kernel_evals_per_launch": n_in,
                         "candidates_per_launch": n_vis,
                         "samples_per_launch": n * 2 * 2 * N,
                         "frac_of_unpacked": flop / eval_s / (PEAK_FP32 / 2)}}
+    del sim
+    torch.cuda.empty_cache()
+    return out
+
+
+def adjoint_bench(args, device):
+    """The adjoint workload: Taylor-Green on the periodic unit box, Nx = Ny =
+    128 (16,384 particles), hwr = 16, N1 = N2 = 64, dt = 1e-3, T = 50 steps,
+    checkpoint every 10.  Strengths: the RBF solution w_TG of the Taylor-Green
+    vorticity plus 0.1 N(0,1) per particle in vorticity units (scaled by
+    rms(w_TG) / rms(omega_TG), the lattice's vorticity-to-strength factor).
+    Loss: mean squared difference on a 64 x 64 grid between the final
+    vorticity (B-spline sum) and that of the unperturbed w_TG evolved with
+    the same Philox stream.  Times the forward rollout and the checkpointed
+    backward (recompute + step adjoints), and checks dL/dx0 and dL/dw against
+    central differences of the FP64 forward of the same frozen-sample rollout
+    on `adj_fd` coordinates (half positions, half strengths; eps = 1e-4 for
+    positions, 1e-4 max|w| for strengths)."""
+    import torch
+
+    from vortexparticlemc_cuda_amd import adjoint as A
+    from vortexparticlemc_cuda_amd import ops
+    from vortexparticlemc_cuda_amd.sim import SimParam, Simulation, lattice_positions
+    from vortexparticlemc_cuda_amd.workloads import taylor_green
+
+    Nx, hwr, N, T, K, G = args.adj_Nx, 16, 64, args.adj_steps, 10, 64
+    P = SimParam(dt=1e-3, resX=Nx, hwr=hwr)
+    P.N1 = P.N2 = N
+    pos = lattice_positions(Nx)
+    om_tg = taylor_green(pos[:, 0], pos[:, 1])
+    pos_t = torch.as_tensor(pos, device=device)
+    w_tg, it, rr, st = ops.cg_rbf(pos_t, torch.as_tensor(om_tg.astype(np.float32), device=device),
+                                  float(P.h_w), tol=1e-6, refine=True)
+    w_tg = w_tg.clone()
+    scale = float(torch.sqrt((w_tg.double() ** 2).mean())) / float(np.sqrt((om_tg ** 2).mean()))
+    rng = np.random.default_rng(20261203)
+    w0 = (w_tg.double() + torch.as_tensor(0.1 * scale * rng.standard_normal(pos.shape[0]),
+                                          device=device)).float()
+    h = float(P.h_w)
+    sim = Simulation(P, device=device, cells_per_hw=8, seed=0)
+    sim.set_particles(pos, w_tg.cpu().numpy())
+    with torch.no_grad():
+        X_ref = A.rollout(sim, pos_t, w_tg, T, checkpoint=K)
+        ref = A.grid_vorticity(X_ref, w_tg, h, G, G, f64=True)
+    p = pos_t.clone().requires_grad_(True)
+    wt = w0.clone().requires_grad_(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    XT = A.rollout(sim, p, wt, T, checkpoint=K)
+    om = A.grid_vorticity(XT, wt, h, G, G, f64=True)
+    loss = ((om - ref) ** 2).mean()
+    torch.cuda.synchronize()
+    t_fwd = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    loss.backward()
+    torch.cuda.synchronize()
+    t_bwd = time.perf_counter() - t0
+    gp = p.grad.double().cpu().numpy()
+    gw = wt.grad.double().cpu().numpy()
+    n = pos.shape[0]
+    log("adjoint: loss %.6g, forward %.3f s, backward %.3f s (%d steps, %d particles)"
+        % (float(loss.detach()), t_fwd, t_bwd, T, n))
+    out = {"workload": "Taylor-Green + 0.1 N(0,1) strength noise, Nx=Ny=%d (%d particles), "
+                       "hwr=16, N1=N2=64, dt=1e-3, T=%d, checkpoint every %d, loss = MSE of "
+                       "the final B-spline vorticity on a 64x64 grid vs the unperturbed run"
+                       % (Nx, n, T, K),
+           "loss": float(loss.detach()), "forward_s": t_fwd, "backward_s": t_bwd,
+           "forward_particle_steps_per_s": n * T / t_fwd,
+           "adjoint_particle_steps_per_s": n * T / t_bwd,
+           "grad_pos_max": float(np.abs(gp).max()), "grad_w_max": float(np.abs(gw).max())}
+    if args.adj_fd > 0:
+        p64 = pos_t.double()
+        w64 = w0.double()
+
+        def L(pp, ww):
+            X = A.rollout_f64(sim, pp, ww, T)
+            o = A.grid_vorticity_f64(X, ww, h, G, G)
+            return float(((o - ref) ** 2).mean())
+
+        t0 = time.perf_counter()
+        L0 = L(p64, w64)
+        wn = w0.abs().cpu().numpy()
+        cand = np.nonzero(wn > 0.2 * wn.max())[0]
+        pick = rng.choice(cand, size=args.adj_fd, replace=False)
+        axis = rng.integers(2, size=args.adj_fd)
+        out.update({"fd_coordinates": int(args.adj_fd), "fd_loss_fp64": L0})
+        # eps = 1e-4 as specified, and 1e-6: the frozen-sample velocity has the
+        # 1/r Biot-Savart kernel at branch-1 samples close to their owner, so
+        # the wider difference carries truncation error of its own
+        for tag, eps_x, eps_w in (("", 1e-4, 1e-4 * float(w0.abs().max())),
+                                  ("_eps1e-6", 1e-6, 1e-6 * float(w0.abs().max()))):
+            ex, ew, rel = [], [], []
+            for c, i in enumerate(pick):
+                if c % 2 == 0:
+                    k = int(axis[c])
+                    a, b = p64.clone(), p64.clone()
+                    a[i, k] += eps_x
+                    b[i, k] -= eps_x
+                    fd = (L(a, w64) - L(b, w64)) / (2 * eps_x)
+                    g = gp[i, k]
+                    ex.append(abs(fd - g) / np.abs(gp).max())
+                else:
+                    a, b = w64.clone(), w64.clone()
+                    a[i] += eps_w
+                    b[i] -= eps_w
+                    fd = (L(p64, a) - L(p64, b)) / (2 * eps_w)
+                    g = gw[i]
+                    ew.append(abs(fd - g) / np.abs(gw).max())
+                rel.append(abs(fd - g) / max(abs(fd), 1e-300))
+            log("adjoint vs FP64 central differences (eps %.0e): max err/max|g| positions %.3g, "
+                "strengths %.3g; median per-coordinate rel err %.3g"
+                % (eps_x, max(ex), max(ew), np.median(rel)))
+            out.update({"fd_max_err_pos" + tag: max(ex), "fd_max_err_w" + tag: max(ew),
+                        "fd_median_rel_err" + tag: float(np.median(rel)),
+                        "fd_max_rel_err" + tag: float(np.max(rel))})
+        out["fd_seconds"] = time.perf_counter() - t0
     del sim
     torch.cuda.empty_cache()
     return out
@@ -468,6 +609,13 @@ def main():
     ap.add_argument("--large-cells-per-hw", type=int, default=12)
     ap.add_argument("--large-spl", type=int, default=1)
     ap.add_argument("--large-threads", type=int, default=256)
+    ap.add_argument("--large-tune", type=int, default=1,
+                    help="try a few launch shapes on the large run (0: use the flags)")
+    ap.add_argument("--adj-Nx", type=int, default=128,
+                    help="lattice side of the adjoint workload (0: skip it)")
+    ap.add_argument("--adj-steps", type=int, default=50)
+    ap.add_argument("--adj-fd", type=int, default=32,
+                    help="coordinates checked by FP64 central differences")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU; this build has no multi-GPU path "
@@ -524,6 +672,7 @@ def main():
         copy_probe(args.copy_probe << 20)
     batched = ensemble_bench(args, device) if args.ens_sims > 0 else None
     large = large_bench(args, device) if args.large_Nx > 0 else None
+    adj = adjoint_bench(args, device) if args.adj_Nx > 0 else None
     cpu = None if args.no_cpu else cpu_baseline(args.Nx, args.hwr, args.N1, args.N2)
     value = N * args.steps / elapsed
     line = {
@@ -562,6 +711,7 @@ def main():
         "cg_init": init,
         "batched": batched,
         "large": large,
+        "adjoint": adj,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

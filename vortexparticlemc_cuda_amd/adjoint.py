@@ -142,16 +142,19 @@ def rollout(sim, pos0, w, steps, checkpoint=10):
 
 class _GridVorticity(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, pos, w, h_w, gnx, gny, L, cells_per_hw):
+    def forward(ctx, pos, w, h_w, gnx, gny, L, cells_per_hw, f64):
         pos = pos.detach().to(torch.float32).contiguous()
         w = w.detach().to(torch.float32).contiguous()
         n = pos.shape[0]
-        nx, ny = ops.field_grid(h_w, L, L, cells_per_hw)
-        ws = ops.workspace(lib().vpmc_grid_vorticity_workspace_bytes(n, nx, ny), pos.device)
-        out = torch.empty(gnx * gny, dtype=torch.float32, device=pos.device)
-        check(lib().vpmc_grid_vorticity(n, ops._p(pos), ops._p(w), h_w, L, L, cells_per_hw,
-                                        gnx, gny, ops._p(out), ops._p(ws), ws.numel(),
-                                        ops._stream()), "vpmc_grid_vorticity")
+        if f64:
+            out = grid_vorticity_f64(pos, w, h_w, gnx, gny, L)
+        else:
+            nx, ny = ops.field_grid(h_w, L, L, cells_per_hw)
+            ws = ops.workspace(lib().vpmc_grid_vorticity_workspace_bytes(n, nx, ny), pos.device)
+            out = torch.empty(gnx * gny, dtype=torch.float32, device=pos.device)
+            check(lib().vpmc_grid_vorticity(n, ops._p(pos), ops._p(w), h_w, L, L, cells_per_hw,
+                                            gnx, gny, ops._p(out), ops._p(ws), ws.numel(),
+                                            ops._stream()), "vpmc_grid_vorticity")
         ctx.save_for_backward(pos, w)
         ctx.args = (h_w, gnx, gny, L)
         return out
@@ -167,14 +170,16 @@ class _GridVorticity(torch.autograd.Function):
                                                 ops._p(g.to(torch.float32).contiguous()),
                                                 ops._p(gpos), ops._p(gw), ops._stream()),
               "vpmc_grid_vorticity_adjoint")
-        return gpos, gw, None, None, None, None, None
+        return gpos, gw, None, None, None, None, None, None
 
 
-def grid_vorticity(pos, w, h_w, gnx, gny, L=1.0, cells_per_hw=8):
+def grid_vorticity(pos, w, h_w, gnx, gny, L=1.0, cells_per_hw=8, f64=False):
     """omega_h(z) = sum_j w_j W(|z - x_j|) at the centres of a gnx x gny grid
-    (index a*gny + b), differentiable in pos and w"""
+    (index a*gny + b), differentiable in pos and w.  f64: the values are
+    summed in FP64 (all particles per grid point), for losses that subtract
+    nearly equal fields; the derivative is the FP32 adjoint either way."""
     return _GridVorticity.apply(pos, w, float(h_w), int(gnx), int(gny), float(L),
-                                int(cells_per_hw))
+                                int(cells_per_hw), bool(f64))
 
 
 # ---- FP64 forwards of the same functions (finite-difference checks) ------------

@@ -30,13 +30,14 @@
 // Passes (one launch each; samples binned by fine cell as in the forward):
 //   gen      the forward's draws, plus the global index of the branch-1 source
 //   sort     stable radix sort by (sign, fine cell)
-//   query    one thread per sorted sample: om(y) and grad om(y)
+//   eval     one workgroup per tile of samples (the forward's LDS staging):
+//            om(y) and grad om(y)
 //   final    one wave per (owner, sign): adjoint weights per sample, the
 //            owner's gradient (wave butterfly in sample order) and each
 //            branch-1 sample's source gradient
 //   recs     samples in cell order as (y, beta, beta + beta0)
-//   field    one thread per particle of each set: the sums over samples
-//            within h_w -> d/dw_m and d/dx_m
+//   field    one workgroup per (sign, fine cell): the cell's particles
+//            against the samples in the cell's stencil -> d/dw_m, d/dx_m
 //   sources  stable radix sort of the source gradients by source particle,
 //            then one thread per particle adds its run in sample order
 // Every sum has a fixed order, so the gradients are deterministic.
@@ -52,21 +53,17 @@ constexpr int ADJ_THREADS = 256;
 
 // ---- device helpers ------------------------------------------------------------
 
-// W(r)/sig2 and (dW/dr)/(r sig2) of the cubic spline (include/utilities.hpp:7-17)
+// W(r)/sig2 and (dW/dr)/(r sig2) of the cubic spline (include/utilities.hpp:7-17),
+// branch-free: W/sig2 = 2 t^3 - 8 u^3 with t = max(1-q, 0), u = max(1/2-q, 0);
+// (dW/dq)/q = 18 q - 12 for q <= 1/2 and -6 t^2 / q beyond (0 past the support)
 __device__ __forceinline__ void spline_wg(float r2, float inv_hw, float &w, float &gr)
 {
-    const float q = sqrtf(r2) * inv_hw;
-    w = 0.f;
-    gr = 0.f;
-    if (!(q < 1.f)) return;
-    if (q <= 0.5f) {
-        w = fmaf(6.f * q * q, q - 1.f, 1.f);
-        gr = fmaf(18.f, q, -12.f) * inv_hw * inv_hw;
-    } else {
-        const float t = 1.f - q;
-        w = 2.f * t * t * t;
-        gr = -6.f * t * t * inv_hw / (q / inv_hw);
-    }
+    const float q = __builtin_amdgcn_sqrtf(r2) * inv_hw;
+    const float t = fmaxf(1.f - q, 0.f), u = fmaxf(0.5f - q, 0.f);
+    w = fmaf(2.f * t * t, t, -8.f * u * u * u);
+    const float gin = fmaf(18.f, q, -12.f);
+    const float gout = -6.f * t * t * __builtin_amdgcn_rcpf(q);
+    gr = (q <= 0.5f ? gin : gout) * (inv_hw * inv_hw);
 }
 
 __device__ __forceinline__ void spline_wg64(double r2, double inv_hw, double &w, double &gr)
@@ -262,29 +259,58 @@ __global__ void __launch_bounds__(BIN_THREADS)
     key[sid] = (uint32_t)(si * g.ncell + cx * g.ny + cy);
 }
 
-// om(y) and grad om(y) for every live sample (sorted order, one thread each)
+// om(y) and grad om(y) for every live sample: one workgroup per tile (sign,
+// fine cell) of samples, the cell's particle stencil staged in LDS as in the
+// forward's evaluation (tile_stage), one sample per thread.
 __global__ void __launch_bounds__(ADJ_THREADS)
-    k_adj_query(McD d, BinGeom g, const int *__restrict__ start,
-                const uint32_t *__restrict__ keys_sorted, const float4 *__restrict__ srec,
-                float *__restrict__ om, float2 *__restrict__ G)
+    k_adj_eval(McD m, BinGeom g, const int *__restrict__ start, const float4 *__restrict__ srec,
+               float *__restrict__ om, float2 *__restrict__ G)
 {
-    const int p = blockIdx.x * ADJ_THREADS + threadIdx.x;
-    if (p >= start[g.ntile]) return;
-    const int si = keys_sorted[p] >= (uint32_t)g.ncell ? 1 : 0;
-    const float4 s = srec[p];
-    const uint32_t sid = __float_as_uint(s.z);
-    float acc = 0.f, gx = 0.f, gy = 0.f;
-    auto fn = [&](const float4 &c, float dx, float dy) {
-        float w, gr;
-        spline_wg(fmaf(dx, dx, dy * dy), d.inv_hw, w, gr);
-        acc = fmaf(c.z, w, acc);
-        // grad_y W(|y - x|) = W'(r)/r (y - x) and (dx, dy) = x - y
-        gx = fmaf(-c.z * gr, dx, gx);
-        gy = fmaf(-c.z * gr, dy, gy);
-    };
-    field_visit(d.set[si].field, s.x, s.y, fn);
-    om[sid] = d.sig2 * acc;
-    G[sid] = make_float2(d.sig2 * gx, d.sig2 * gy);
+    constexpr int CAP = TileCfg<ADJ_THREADS>::CAP;
+    const int tile = (int)(blockIdx.x & 7) * g.per_xcd + (int)(blockIdx.x >> 3);
+    if (tile >= g.ntile) return;
+    const int beg = start[tile], end = start[tile + 1];
+    if (beg >= end) return;   // uniform over the workgroup
+    __shared__ TileLds<ADJ_THREADS> L;
+    const int si = tile >= g.ncell ? 1 : 0;
+    const int c = tile - si * g.ncell;
+    const int cx = c / g.ny, cy = c - cx * g.ny;
+    const float ox = ((float)cx + 0.5f) * g.cw, oy = ((float)cy + 0.5f) * g.ch;
+    const float4 *__restrict__ pts = m.set[si].field.pts;
+    const int total = tile_pieces<ADJ_THREADS>(g, m.set[si].field.cell_start, cx, cy, m.Lx, m.Ly,
+                                                L);
+    const int npieces = 3 * (2 * g.cols + 1);
+    const int npass = (total + CAP - 1) / CAP;
+    int n = 0;
+    if (npass == 1) n = tile_stage<ADJ_THREADS>(g, pts, npieces, 0, total, ox, oy, L);
+    for (int bb = beg; bb < end; bb += ADJ_THREADS) {   // uniform trip count
+        const int idx = bb + (int)threadIdx.x;
+        const bool live = idx < end;
+        const float4 p = live ? srec[idx] : make_float4(ox, oy, 0.f, 0.f);
+        const float px = p.x - ox, py = p.y - oy;
+        float acc = 0.f, gx = 0.f, gy = 0.f;
+        for (int pass = 0; pass < npass; ++pass) {
+            if (npass > 1)
+                n = tile_stage<ADJ_THREADS>(g, pts, npieces, pass * CAP,
+                                            min(total, (pass + 1) * CAP), ox, oy, L);
+            for (int k = 0; k < n; ++k) {
+                // staged record (-2 c', |c'|^2, w), c' = particle - cell centre
+                const float4 r = L.cand[k];
+                const float dx = fmaf(-0.5f, r.x, -px), dy = fmaf(-0.5f, r.y, -py);
+                float w, gr;
+                spline_wg(fmaf(dx, dx, dy * dy), m.inv_hw, w, gr);
+                acc = fmaf(r.w, w, acc);
+                // grad_y W(|y - x|) = W'(r)/r (y - x), (dx, dy) = x - y
+                gx = fmaf(-r.w * gr, dx, gx);
+                gy = fmaf(-r.w * gr, dy, gy);
+            }
+        }
+        if (live) {
+            const uint32_t sid = __float_as_uint(p.z);
+            om[sid] = m.sig2 * acc;
+            G[sid] = make_float2(m.sig2 * gx, m.sig2 * gy);
+        }
+    }
 }
 
 // one wave per (owner, sign)
@@ -368,41 +394,91 @@ __global__ void k_adj_recs(const int *__restrict__ start, int ntile, const float
     rec2[p] = make_float4(s.x, s.y, b.x, b.y);
 }
 
-// one thread per particle of set blockIdx.y: sums over the set's samples
+// One workgroup per (sign, fine cell): the particles of the cell, up to
+// ADJ_PCHUNK at a time, against every sample of the same sign in the cell's
+// stencil (the forward's tile stencil over the sample array), threads
+// striding over the samples; fixed-order block reduction per particle.
+constexpr int ADJ_PCHUNK = 8;
+
 __global__ void __launch_bounds__(ADJ_THREADS)
     k_adj_field(McD d, BinGeom g, const int *__restrict__ start, const float4 *__restrict__ rec2,
                 const int *__restrict__ order0, const int *__restrict__ order1,
                 const int *__restrict__ sub0, const int *__restrict__ sub1,
                 float2 *__restrict__ grad_pos, float *__restrict__ grad_w)
 {
-    const int si = blockIdx.y;
+    const int tile = (int)blockIdx.x;
+    const int si = tile >= g.ncell ? 1 : 0;
+    const int c = tile - si * g.ncell;
     const SetD &S = d.set[si];
-    const int n_s = S.count_dev ? *S.count_dev : S.n;
-    const int t = blockIdx.x * ADJ_THREADS + threadIdx.x;
-    if (t >= n_s) return;
-    const float4 c = S.field.pts[t];
-    FieldD smp;
-    smp.pts = rec2;
-    smp.cell_start = start + (size_t)si * g.ncell;
-    smp.g = S.field.g;
-    float A = 0.f, bx = 0.f, by = 0.f;
-    auto fn = [&](const float4 &s, float dx, float dy) {
-        // (dx, dy) = y - x_m; grad_y W = W'(r)/r (y - x_m)
-        float w, gr;
-        spline_wg(fmaf(dx, dx, dy * dy), d.inv_hw, w, gr);
-        A = fmaf(s.z, w, A);
-        bx = fmaf(s.w * gr, dx, bx);
-        by = fmaf(s.w * gr, dy, by);
-    };
-    field_visit(smp, c.x, c.y, fn);
-    const int j = (si ? order1 : order0)[t];
-    const int gm = (si ? sub1 : sub0)[j];
+    const int pb = S.field.cell_start[c], pe = S.field.cell_start[c + 1];
+    if (pb >= pe) return;   // uniform: no particle of this sign in the cell
+    const int cx = c / g.ny, cy = c - cx * g.ny;
+    __shared__ TileLds<ADJ_THREADS> L;
+    __shared__ float red[ADJ_THREADS / WAVE][3 * ADJ_PCHUNK];
+    const int total = tile_pieces<ADJ_THREADS>(g, start + (size_t)si * g.ncell, cx, cy, d.Lx,
+                                                d.Ly, L);
+    const int npieces = 3 * (2 * g.cols + 1);
+    const int lane = threadIdx.x & (WAVE - 1), wid = threadIdx.x / WAVE;
+    const int *order = si ? order1 : order0;
+    const int *sub = si ? sub1 : sub0;
     const float sg = si ? -1.f : 1.f;   // d|w|/dw on the set
-    grad_w[gm] += sg * d.sig2 * A;
-    float2 gp = grad_pos[gm];
-    gp.x -= c.z * d.sig2 * bx;
-    gp.y -= c.z * d.sig2 * by;
-    grad_pos[gm] = gp;
+    for (int p0 = pb; p0 < pe; p0 += ADJ_PCHUNK) {
+        const int np = min(ADJ_PCHUNK, pe - p0);
+        float px[ADJ_PCHUNK], py[ADJ_PCHUNK], A[ADJ_PCHUNK], Bx[ADJ_PCHUNK], By[ADJ_PCHUNK];
+#pragma unroll
+        for (int j = 0; j < ADJ_PCHUNK; ++j) {
+            const float4 q = S.field.pts[p0 + min(j, np - 1)];
+            px[j] = q.x;
+            py[j] = q.y;
+            A[j] = Bx[j] = By[j] = 0.f;
+        }
+        int lo = 0;   // last piece with p_off <= i (i only grows)
+        for (int i = threadIdx.x; i < total; i += ADJ_THREADS) {
+            while (lo + 1 < npieces && L.p_off[lo + 1] <= i) ++lo;
+            const float4 sm = rec2[L.p_beg[lo] + (i - L.p_off[lo])];
+            const float sx = sm.x + L.p_sx[lo], sy = sm.y + L.p_sy[lo];
+#pragma unroll
+            for (int j = 0; j < ADJ_PCHUNK; ++j) {
+                if (j >= np) break;   // uniform
+                // (dx, dy) = y - x_m; grad_y W = W'(r)/r (y - x_m)
+                const float dx = sx - px[j], dy = sy - py[j];
+                float w, gr;
+                spline_wg(fmaf(dx, dx, dy * dy), d.inv_hw, w, gr);
+                A[j] = fmaf(sm.z, w, A[j]);
+                Bx[j] = fmaf(sm.w * gr, dx, Bx[j]);
+                By[j] = fmaf(sm.w * gr, dy, By[j]);
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < ADJ_PCHUNK; ++j) {
+            const float a = wave_sum(A[j]), bx = wave_sum(Bx[j]), by = wave_sum(By[j]);
+            if (lane == 0) {
+                red[wid][3 * j] = a;
+                red[wid][3 * j + 1] = bx;
+                red[wid][3 * j + 2] = by;
+            }
+        }
+        __syncthreads();
+        if ((int)threadIdx.x < np) {
+            const int j = threadIdx.x;
+            float a = 0.f, bx = 0.f, by = 0.f;
+#pragma unroll
+            for (int w = 0; w < ADJ_THREADS / WAVE; ++w) {
+                a += red[w][3 * j];
+                bx += red[w][3 * j + 1];
+                by += red[w][3 * j + 2];
+            }
+            const int t = p0 + j;
+            const float aw = S.field.pts[t].z;
+            const int gm = sub[order[t]];
+            grad_w[gm] += sg * d.sig2 * a;
+            float2 gp = grad_pos[gm];
+            gp.x -= aw * d.sig2 * bx;
+            gp.y -= aw * d.sig2 * by;
+            grad_pos[gm] = gp;
+        }
+        __syncthreads();   // red is rewritten by the next chunk
+    }
 }
 
 // owner terms (both signs) and the branch-1 source terms, per particle
@@ -485,6 +561,7 @@ static int adj_check(const vpmc_mc_desc *d, McD *m, BinGeom *g)
     if (d->mode != VPMC_MODE_CONSISTENT || !d->periodic || d->uniforms) return VPMC_ERR_ARG;
     VPMC_TRY(make_mcd(d, m));
     VPMC_TRY(bin_geom(*m, d->N1, d->N2, d->n_owner, g));
+    if (!tile_threads_ok(*g, ADJ_THREADS)) return VPMC_ERR_GRID;
     for (int s = 0; s < 2; ++s)
         if ((d->set[s].n > 0 || d->set[s].count_dev) && !d->set[s].field.order)
             return VPMC_ERR_ARG;
@@ -511,9 +588,8 @@ int mc_adjoint_impl(const vpmc_mc_desc *d, const int *sub_pos, const int *sub_ne
                                    (const float2 *)B.q, nullptr, B.srec, B.keys_sorted, g.ntile,
                                    B.start, B.sort_ws, B.sort_bytes, s));
     const unsigned ga = (unsigned)((g.S + ADJ_THREADS - 1) / ADJ_THREADS);
-    hipLaunchKernelGGL(k_adj_query, dim3(ga), dim3(ADJ_THREADS), 0, s, m, g,
-                       (const int *)B.start, (const uint32_t *)B.keys_sorted,
-                       (const float4 *)B.srec, B.om, B.G);
+    hipLaunchKernelGGL(k_adj_eval, dim3((unsigned)(8 * g.per_xcd)), dim3(ADJ_THREADS), 0, s, m, g,
+                       (const int *)B.start, (const float4 *)B.srec, B.om, B.G);
     const unsigned gf = (unsigned)((2 * (size_t)n + 3) / 4);
     hipLaunchKernelGGL(k_adj_final, dim3(gf), dim3(BIN_THREADS), 0, s, m, g,
                        (const uint32_t *)B.key, (const float2 *)B.r, (const float *)B.om,
@@ -522,14 +598,10 @@ int mc_adjoint_impl(const vpmc_mc_desc *d, const int *sub_pos, const int *sub_ne
     hipLaunchKernelGGL(k_adj_recs, dim3(ga), dim3(ADJ_THREADS), 0, s, (const int *)B.start,
                        g.ntile, (const float4 *)B.srec, (const float2 *)B.beta, B.rec2);
     VPMC_CHECK_LAUNCH();
-    const int nmax = d->set[0].n > d->set[1].n ? d->set[0].n : d->set[1].n;
-    if (nmax > 0) {
-        hipLaunchKernelGGL(k_adj_field, dim3((unsigned)((nmax + ADJ_THREADS - 1) / ADJ_THREADS), 2),
-                           dim3(ADJ_THREADS), 0, s, m, g, (const int *)B.start,
-                           (const float4 *)B.rec2, d->set[0].field.order, d->set[1].field.order,
-                           sub_pos, sub_neg, (float2 *)grad_pos, grad_w);
-        VPMC_CHECK_LAUNCH();
-    }
+    hipLaunchKernelGGL(k_adj_field, dim3((unsigned)g.ntile), dim3(ADJ_THREADS), 0, s, m, g,
+                       (const int *)B.start, (const float4 *)B.rec2, d->set[0].field.order,
+                       d->set[1].field.order, sub_pos, sub_neg, (float2 *)grad_pos, grad_w);
+    VPMC_CHECK_LAUNCH();
     VPMC_TRY(radix_sort_to_records(B.srckey, (int)g.S, key_bits_for((uint32_t)n), nullptr,
                                    (const float4 *)B.srcrec, B.srcrec_sorted, B.srckey_sorted, n,
                                    B.src_start, B.sort_ws, B.sort_bytes, s));
