@@ -309,7 +309,7 @@ def large_bench(args, device):
     """The large run: double shear layer (rho = 1/30, delta = 0.05) on the
     periodic unit box, Nx = Ny = 1024 (1,048,576 particles), hwr = 8,
     N1 = N2 = 256, dt = 2.5e-4; strengths from the RBF solve to 1e-10 (FP64
-    MINRES); plain steps with the plain path's tuned launch shape; RMS error
+    MINRES on the FP64 vorticity); plain steps with a tuned launch shape; RMS error
     against the FP64 blob direct sum on 4,096 uniform probe points (seed
     20261203), the estimator evaluated at those probes on the same fields."""
     import torch
@@ -322,7 +322,7 @@ def large_bench(args, device):
     P = SimParam(dt=2.5e-4, resX=Nx, hwr=hwr)
     P.N1 = P.N2 = N
     pos = lattice_positions(Nx)
-    b = double_shear(pos[:, 0], pos[:, 1]).astype(np.float32)
+    b = double_shear(pos[:, 0], pos[:, 1])   # FP64 right-hand side
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     w, it, rr, st = ops.cg_rbf(torch.as_tensor(pos, device=device),

@@ -370,6 +370,16 @@ int vpmc_cg_rbf(int n, const float *pos_xy, const float *b, float *w_out, float 
                 int *iters, double *rel_res, void *workspace, size_t workspace_bytes,
                 void *stream);
 size_t vpmc_cg_workspace_bytes(int n, int nx, int ny);
+/* The same solve (MINRES on FP64 vectors, as refine != 0) for an FP64
+   right-hand side b, e.g. a vorticity field evaluated in FP64: an FP32 b
+   carries rounding noise into every lattice mode, which bounds the reachable
+   residual near 1e-8; an FP64 b of a smooth field does not.  w_out receives
+   the strengths in FP32, w64_out (optional) the FP64 iterate whose relative
+   residual *rel_res reports. */
+int vpmc_cg_rbf_f64(int n, const float *pos_xy, const double *b, float *w_out, double *w64_out,
+                    float h_w, float Lx, float Ly, int cells_per_hw, double tol, int max_iter,
+                    int *iters, double *rel_res, void *workspace, size_t workspace_bytes,
+                    void *stream);
 
 /* ---------------------------------------------------------------------------
  * FP64 direct sums (error oracles on the device).

@@ -113,6 +113,8 @@ def lib():
     L.vpmc_cg_workspace_bytes.argtypes = [ci, ci, ci]
     L.vpmc_cg_rbf.argtypes = [ci, vp, vp, vp, cf, cf, cf, ci, cd, ci, ci, C.POINTER(ci),
                               C.POINTER(cd), vp, csz, vp]
+    L.vpmc_cg_rbf_f64.argtypes = [ci, vp, vp, vp, vp, cf, cf, cf, ci, cd, ci, C.POINTER(ci),
+                                  C.POINTER(cd), vp, csz, vp]
     L.vpmc_direct_sum_eps.argtypes = [ci, vp, ci, ci, vp, vp, cd, ci, cd, vp, vp]
     L.vpmc_direct_sum_blob.argtypes = [ci, vp, ci, vp, vp, cd, cd, vp, vp]
     ep = C.POINTER(EnsembleDesc)
@@ -158,7 +160,7 @@ EXPORTS = [
     "vpmc_field_query", "vpmc_mc_estimate", "vpmc_mc_binned_workspace_bytes",
     "vpmc_mc_estimate_binned", "vpmc_advect", "vpmc_step_workspace_bytes",
     "vpmc_step", "vpmc_rbf_workspace_bytes", "vpmc_rbf_matvec", "vpmc_rbf_matvec_f64", "vpmc_rbf_matvec_particles",
-    "vpmc_cg_rbf", "vpmc_cg_workspace_bytes", "vpmc_direct_sum_eps", "vpmc_direct_sum_blob",
+    "vpmc_cg_rbf", "vpmc_cg_rbf_f64", "vpmc_cg_workspace_bytes", "vpmc_direct_sum_eps", "vpmc_direct_sum_blob",
     "vpmc_ensemble_workspace_bytes", "vpmc_ensemble_init", "vpmc_ensemble_step",
     "vpmc_ensemble_set_step", "vpmc_ensemble_graph_create", "vpmc_graph_launch",
     "vpmc_graph_destroy", "vpmc_mc_adjoint_workspace_bytes", "vpmc_mc_adjoint",

@@ -283,7 +283,9 @@ __global__ void __launch_bounds__(ADJ_THREADS)
     const int npass = (total + CAP - 1) / CAP;
     int n = 0;
     if (npass == 1) n = tile_stage<ADJ_THREADS>(g, pts, npieces, 0, total, ox, oy, L);
+    const int wid = (int)threadIdx.x / WAVE;
     for (int bb = beg; bb < end; bb += ADJ_THREADS) {   // uniform trip count
+        if (npass == 1 && bb + wid * WAVE >= end) break;   // no sample left for this wave
         const int idx = bb + (int)threadIdx.x;
         const bool live = idx < end;
         const float4 p = live ? srec[idx] : make_float4(ox, oy, 0.f, 0.f);

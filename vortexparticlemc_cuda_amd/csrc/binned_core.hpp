@@ -251,6 +251,9 @@ __device__ __forceinline__ void bin_eval_tile(const McD &m, const BinGeom &g, in
     if (npass == 1) n = tile_stage<THREADS>(g, pts, npieces, 0, total, ox, oy, L);
     const int lane = threadIdx.x & (WAVE - 1), wid = threadIdx.x / WAVE;
     for (int bb = beg; bb < end; bb += THREADS * SPL) {   // uniform trip count
+        // a wave past the tile's last sample stops (no barrier follows when
+        // the stencil is staged once), freeing its SIMD slot for other waves
+        if (npass == 1 && bb + wid * SPL * WAVE >= end) break;
         float px[SPL], py[SPL], pp[SPL], acc[SPL];
         uint32_t sid[SPL];
         bool live[SPL];

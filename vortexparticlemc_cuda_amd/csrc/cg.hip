@@ -82,12 +82,23 @@ __global__ void k_permute_in(int n, const int *__restrict__ order, const float *
     b64[t] = (double)v;
 }
 
+__global__ void k_permute_in64(int n, const int *__restrict__ order, const double *__restrict__ b,
+                               float *__restrict__ bs, double *__restrict__ b64)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    double v = b[order[t]];
+    bs[t] = (float)v;
+    b64[t] = v;
+}
+
 __global__ void k_permute_out(int n, const int *__restrict__ order, const double *__restrict__ ws,
-                              float *__restrict__ w)
+                              float *__restrict__ w, double *__restrict__ w64)
 {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
     w[order[t]] = (float)ws[t];
+    if (w64) w64[order[t]] = ws[t];
 }
 
 // elementwise helpers, one kernel per CG update
@@ -393,15 +404,20 @@ int vpmc_rbf_matvec_particles(int n, const float *pos_xy, const float *x, float 
     return VPMC_OK;
 }
 
-int vpmc_cg_rbf(int n, const float *pos_xy, const float *b, float *w_out, float h_w, float Lx,
-                float Ly, int cells_per_hw, double tol, int max_iter, int refine, int *iters,
-                double *rel_res, void *workspace, size_t workspace_bytes, void *stream)
+}  // extern "C"
+
+namespace vpmc {
+
+static int cg_rbf_impl(int n, const float *pos_xy, const float *b, const double *b_f64,
+                       float *w_out, double *w64_out, float h_w, float Lx, float Ly,
+                       int cells_per_hw, double tol, int max_iter, int refine, int *iters,
+                       double *rel_res, void *workspace, size_t workspace_bytes, void *stream)
 {
     if (n < 0 || !(h_w > 0.f) || max_iter < 0) return VPMC_ERR_ARG;
     if (iters) *iters = 0;
     if (rel_res) *rel_res = 0.0;
     if (n == 0) return VPMC_OK;
-    if (!pos_xy || !b || !w_out) return VPMC_ERR_ARG;
+    if (!pos_xy || (!b && !b_f64) || !w_out) return VPMC_ERR_ARG;
     hipStream_t s = as_stream(stream);
     int nx, ny;
     VPMC_TRY(vpmc_field_grid(h_w, Lx, Ly, cells_per_hw, &nx, &ny));
@@ -416,8 +432,12 @@ int vpmc_cg_rbf(int n, const float *pos_xy, const float *b, float *w_out, float 
     const float inv_hw = 1.f / h_w, sig2 = 40.f / (7.f * PI_F * h_w * h_w);
     const float diag = sig2;   // W(0)
     unsigned gb = (unsigned)((n + 255) / 256);
-    hipLaunchKernelGGL(k_permute_in, dim3(gb), dim3(256), 0, s, n, (const int *)B.order, b, B.bs,
-                       B.b64);
+    if (b_f64)
+        hipLaunchKernelGGL(k_permute_in64, dim3(gb), dim3(256), 0, s, n, (const int *)B.order,
+                           b_f64, B.bs, B.b64);
+    else
+        hipLaunchKernelGGL(k_permute_in, dim3(gb), dim3(256), 0, s, n, (const int *)B.order, b,
+                           B.bs, B.b64);
     VPMC_CHECK_LAUNCH();
     double bb;
     VPMC_TRY(dot_host(B.b64, B.b64, n, B, s, &bb));
@@ -442,7 +462,7 @@ int vpmc_cg_rbf(int n, const float *pos_xy, const float *b, float *w_out, float 
     hipLaunchKernelGGL(k_res64, dim3(gb), dim3(256), 0, s, n, (const double *)B.b64,
                        (const double *)B.Aw64, B.r64);
     hipLaunchKernelGGL(k_permute_out, dim3(gb), dim3(256), 0, s, n, (const int *)B.order,
-                       (const double *)B.w64, w_out);
+                       (const double *)B.w64, w_out, w64_out);
     VPMC_CHECK_LAUNCH();
     double r2;
     VPMC_TRY(dot_host(B.r64, B.r64, n, B, s, &r2));
@@ -457,6 +477,27 @@ int vpmc_cg_rbf(int n, const float *pos_xy, const float *b, float *w_out, float 
         if (refine || total_it >= max_iter) return VPMC_ERR_NOT_CONVERGED;
     }
     return VPMC_OK;
+}
+
+}  // namespace vpmc
+
+extern "C" {
+
+int vpmc_cg_rbf(int n, const float *pos_xy, const float *b, float *w_out, float h_w, float Lx,
+                float Ly, int cells_per_hw, double tol, int max_iter, int refine, int *iters,
+                double *rel_res, void *workspace, size_t workspace_bytes, void *stream)
+{
+    return cg_rbf_impl(n, pos_xy, b, nullptr, w_out, nullptr, h_w, Lx, Ly, cells_per_hw, tol,
+                       max_iter, refine, iters, rel_res, workspace, workspace_bytes, stream);
+}
+
+int vpmc_cg_rbf_f64(int n, const float *pos_xy, const double *b, float *w_out, double *w64_out,
+                    float h_w, float Lx, float Ly, int cells_per_hw, double tol, int max_iter,
+                    int *iters, double *rel_res, void *workspace, size_t workspace_bytes,
+                    void *stream)
+{
+    return cg_rbf_impl(n, pos_xy, nullptr, b, w_out, w64_out, h_w, Lx, Ly, cells_per_hw, tol,
+                       max_iter, 1, iters, rel_res, workspace, workspace_bytes, stream);
 }
 
 }  // extern "C"

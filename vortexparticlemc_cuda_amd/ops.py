@@ -181,15 +181,23 @@ def advect(pos, vel_sign, dt, Lx=1.0, Ly=1.0, periodic=True):
 # RBF / CG
 
 def cg_rbf(pos, b, h_w, Lx=1.0, Ly=1.0, tol=1e-6, max_iter=1000, refine=True, cells_per_hw=8):
+    """strengths w with A w = b (vpmc_cg_rbf); a float64 b selects the FP64
+    right-hand side (vpmc_cg_rbf_f64, MINRES).  Returns (w, iterations,
+    relative FP64 residual, status)."""
     pos = _xy(pos)
     n = pos.shape[0]
     nx, ny = field_grid(h_w, Lx, Ly, cells_per_hw)
     ws = workspace(lib().vpmc_cg_workspace_bytes(n, nx, ny), pos.device)
     w = torch.empty(max(n, 1), dtype=torch.float32, device=pos.device)
     it, rr = C.c_int(), C.c_double()
-    st = lib().vpmc_cg_rbf(n, _p(pos), _p(b.to(torch.float32).contiguous()), _p(w), h_w, Lx, Ly,
-                           cells_per_hw, tol, max_iter, int(refine), C.byref(it), C.byref(rr),
-                           _p(ws), ws.numel(), _stream())
+    if b.dtype == torch.float64:
+        st = lib().vpmc_cg_rbf_f64(n, _p(pos), _p(b.contiguous()), _p(w), None, h_w, Lx, Ly,
+                                   cells_per_hw, tol, max_iter, C.byref(it), C.byref(rr), _p(ws),
+                                   ws.numel(), _stream())
+    else:
+        st = lib().vpmc_cg_rbf(n, _p(pos), _p(b.to(torch.float32).contiguous()), _p(w), h_w, Lx,
+                               Ly, cells_per_hw, tol, max_iter, int(refine), C.byref(it),
+                               C.byref(rr), _p(ws), ws.numel(), _stream())
     return w[:n], it.value, rr.value, st
 
 
