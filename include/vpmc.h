@@ -193,6 +193,17 @@ typedef struct vpmc_mc_desc {
                                          128 or 256 (0: 256), a tuning knob with
                                          identical results; the stencil's
                                          columns must fit (VPMC_ERR_GRID)     */
+    /* auxiliary targets (src/cuda/monte_carlo_bs.cu:166-169, 213-259;
+       src/cpu/monte_carlo_bs.cpp:60-119): with aux != 0 the binned estimator
+       also estimates at dst -/+ aux_dx e_x (L, R) and dst +/- aux_dx e_y (T, B).
+       Branch-1 samples are shared by the five targets (same point and
+       vorticity); each aux target has N2 disk samples of its own (Philox
+       branch ids 3..6; in a replay stream, draws 2..9 of each branch-2
+       sample's ten, aux_draws = 1).  vel_aux: 8 * n_owner float2, index
+       (sign * 4 + target) * n_owner + owner, targets L, R, T, B. */
+    int          aux;
+    float        aux_dx;
+    float       *vel_aux;
 } vpmc_mc_desc;
 
 enum { VPMC_ALGO_BINNED = 0, VPMC_ALGO_DIRECT = 1 };
@@ -210,6 +221,14 @@ int vpmc_mc_estimate(const vpmc_mc_desc *d, void *stream);
 size_t vpmc_mc_binned_workspace_bytes(const vpmc_mc_desc *d);
 int vpmc_mc_estimate_binned(const vpmc_mc_desc *d, void *workspace, size_t workspace_bytes,
                             void *stream);
+
+/* replaces update_jacobian_from5 + renorm_det1 (src/cuda/monte_carlo_bs.cu:274-288,
+   include/monte_carlo_bs.cuh:140-144; src/cuda/simulation_step.cu:69-73):
+   from the centre velocities vel_sign and the aux targets' vel_aux of one
+   estimate (u = v+ - v- per target), F = I + (dt / 2dx) [uR - uL | uT - uB]
+   rescaled to det F = 1; jac_out: n float4 (F.x F.y; F.z F.w) row-major. */
+int vpmc_update_jacobian(int n, const float *vel_sign, const float *vel_aux, float dt, float dx,
+                         float *jac_out, void *stream);
 
 /* replaces the combine + Euler + wrap of step_cuda
    (src/cuda/simulation_step.cu:63-67,77-89) and step_cpu

@@ -185,6 +185,47 @@ void ref_mc_velocities(int n, const float *px, const float *py, const float *w, 
     delete S;
 }
 
+// The reference's estimates with real auxiliary targets (dst -/+ dx along x,
+// dst +/- dx along y, the CUDA step's placement, src/cuda/monte_carlo_bs.cu:166-169)
+// for owners [o0, o1): sign +1 then -1 per owner on one std::mt19937.
+// out[((i - o0) * 2 + sign_index) * 10 + 2 * target + c], targets main, L, R, T, B.
+void ref_mc_aux(int n, const float *px, const float *py, const float *w, int rshx, int rshy,
+                float Lx, float Ly, float h_w, int periodic, int N1, int N2, float dx,
+                uint32_t seed, int o0, int o1, float *out)
+{
+    float cum[2];
+    RefSet *S = build_set(n, px, py, w, rshx, rshy, Lx, Ly, cum);
+    ParticleSet P;
+    P.resize(n);
+    for (int i = 0; i < n; ++i) P.pos[i] = S->pos[i];
+    P.omega = S->w;
+    MCBSCtx C;
+    C.sub_pos = &S->sub_pos;
+    C.sub_neg = &S->sub_neg;
+    C.cdf_pos = &S->cdf_pos;
+    C.cdf_neg = &S->cdf_neg;
+    C.h_w = h_w;
+    C.periodic = periodic != 0;
+    C.Lx = Lx; C.Ly = Ly; C.InvLx = 1.f / Lx; C.InvLy = 1.f / Ly;
+    C.cum_pos = cum[0];
+    C.cum_neg = cum[1];
+    C.pos_view = make_view(*S, true, rshx, rshy, Lx, Ly);
+    C.neg_view = make_view(*S, false, rshx, rshy, Lx, Ly);
+    std::mt19937 rng(seed);
+    for (int i = o0; i < o1; ++i) {
+        float2 d = P.pos[i];
+        float2 dl{d.x - dx, d.y}, dr{d.x + dx, d.y}, dt{d.x, d.y + dx}, db{d.x, d.y - dx};
+        for (int sg = 0; sg < 2; ++sg) {
+            MCBSEstimate e = monte_carlo_bs_cpu(d, sg == 0 ? +1 : -1, 1, dl, dr, dt, db, P, C, N1,
+                                                N2, rng);
+            float2 v[5] = {e.main, e.left, e.right, e.top, e.bottom};
+            float *o = out + ((size_t)(i - o0) * 2 + sg) * 10;
+            for (int t = 0; t < 5; ++t) { o[2 * t] = v[t].x; o[2 * t + 1] = v[t].y; }
+        }
+    }
+    delete S;
+}
+
 // The draws a std::mt19937(seed) + uniform_real_distribution<float>(0,1)
 // produce, in order: the injected-uniform stream for replay parity.
 void ref_uniform_stream(uint32_t seed, int64_t count, float *out)

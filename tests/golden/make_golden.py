@@ -7,7 +7,9 @@ oracle/build.py in this container) to record inputs and outputs:
   - evalVorticityAbs_cpu at random points for both signs;
   - monte_carlo_bs_cpu velocities (sign +1 then -1 per owner, one
     std::mt19937 as src/cpu/simulation_step.cpp:28-44 uses it) together with
-    the draws that generator produced, for aux = 0 and aux = 1.
+    the draws that generator produced, for aux = 0 and aux = 1;
+  - the five monte_carlo_bs_cpu estimates (owner and the four auxiliary
+    targets at dst -/+ dx) per owner and sign, with their draws.
 Run: python tests/golden/make_golden.py
 """
 import os
@@ -83,6 +85,15 @@ def main():
         out["mc_aux%d_vel" % aux] = vel.reshape(-1, 2)
         out["mc_aux%d_U" % aux] = U
     out.update(mc_N1=np.int32(N1), mc_N2=np.int32(N2), mc_owners=np.int32(n_own))
+    # ---- the five estimates with real auxiliary targets (dst -/+ dx, the CUDA
+    # step's placement), per owner and sign, and the draws
+    dx = np.float32(0.5 * hw)
+    est = np.zeros(n_own * 2 * 10, np.float32)
+    R.ref_mc_aux(x.size, x, y, w, rsh, rsh, 1.0, 1.0, hw, 1, N1, N2, dx, seed, 0, n_own, est)
+    count = n_own * 2 * (N1 * 3 + N2 * 10)
+    U = np.zeros(count, np.float32)
+    R.ref_uniform_stream(seed, count, U)
+    out.update(mcaux_est=est.reshape(n_own, 2, 5, 2), mcaux_U=U, mcaux_dx=dx)
     path = os.path.join(HERE, "ref_cpu_vectors.npz")
     np.savez_compressed(path, **out)
     print("wrote", path, os.path.getsize(path), "bytes")

@@ -55,7 +55,7 @@ class McDesc(C.Structure):
                 ("sim", C.c_uint32), ("uniforms", vp), ("aux_draws", ci), ("n_owner", ci),
                 ("owner_xy", vp), ("set", SignSet * 2), ("vel_sign", vp),
                 ("eval_count", vp), ("algo", ci), ("eval_events", vp * 2), ("spl", ci),
-                ("eval_threads", ci)]
+                ("eval_threads", ci), ("aux", ci), ("aux_dx", cf), ("vel_aux", vp)]
 
 
 class EnsembleDesc(C.Structure):
@@ -101,6 +101,7 @@ def lib():
     L.vpmc_mc_binned_workspace_bytes.argtypes = [C.POINTER(McDesc)]
     L.vpmc_mc_estimate_binned.argtypes = [C.POINTER(McDesc), vp, csz, vp]
     L.vpmc_advect.argtypes = [ci, vp, vp, vp, cf, cf, cf, ci, vp]
+    L.vpmc_update_jacobian.argtypes = [ci, vp, vp, cf, cf, vp, vp]
     L.vpmc_step_workspace_bytes.restype = csz
     L.vpmc_step_workspace_bytes.argtypes = [C.POINTER(McDesc)]
     L.vpmc_step.argtypes = [C.POINTER(McDesc), ci, vp, vp, vp, vp, vp, cf, vp, csz, vp]
@@ -158,7 +159,7 @@ EXPORTS = [
     "vpmc_subsets_workspace_bytes", "vpmc_build_subsets", "vpmc_build_guide", "vpmc_sample_idx",
     "vpmc_query_view", "vpmc_field_grid", "vpmc_field_workspace_bytes", "vpmc_field_build",
     "vpmc_field_query", "vpmc_mc_estimate", "vpmc_mc_binned_workspace_bytes",
-    "vpmc_mc_estimate_binned", "vpmc_advect", "vpmc_step_workspace_bytes",
+    "vpmc_mc_estimate_binned", "vpmc_advect", "vpmc_update_jacobian", "vpmc_step_workspace_bytes",
     "vpmc_step", "vpmc_rbf_workspace_bytes", "vpmc_rbf_matvec", "vpmc_rbf_matvec_f64", "vpmc_rbf_matvec_particles",
     "vpmc_cg_rbf", "vpmc_cg_rbf_f64", "vpmc_cg_workspace_bytes", "vpmc_direct_sum_eps", "vpmc_direct_sum_blob",
     "vpmc_ensemble_workspace_bytes", "vpmc_ensemble_init", "vpmc_ensemble_step",

@@ -560,7 +560,8 @@ static int carve_adj(const BinGeom &g, int n, void *ws, size_t bytes, AdjBufs *B
 static int adj_check(const vpmc_mc_desc *d, McD *m, BinGeom *g)
 {
     if (!d) return VPMC_ERR_ARG;
-    if (d->mode != VPMC_MODE_CONSISTENT || !d->periodic || d->uniforms) return VPMC_ERR_ARG;
+    if (d->mode != VPMC_MODE_CONSISTENT || !d->periodic || d->uniforms || d->aux)
+        return VPMC_ERR_ARG;
     VPMC_TRY(make_mcd(d, m));
     VPMC_TRY(bin_geom(*m, d->N1, d->N2, d->n_owner, g));
     if (!tile_threads_ok(*g, ADJ_THREADS)) return VPMC_ERR_GRID;
@@ -915,7 +916,8 @@ int vpmc_step_f64(vpmc_mc_desc *d, int n_total, double *pos_xy, const double *w,
                   void *workspace, size_t workspace_bytes, void *stream)
 {
     if (!d || n_total < 0 || d->n_owner != n_total) return VPMC_ERR_ARG;
-    if (d->mode != VPMC_MODE_CONSISTENT || !d->periodic || d->uniforms || d->Lx != d->Ly)
+    if (d->mode != VPMC_MODE_CONSISTENT || !d->periodic || d->uniforms || d->Lx != d->Ly ||
+        d->aux)
         return VPMC_ERR_ARG;
     if (n_total == 0) return VPMC_OK;
     if (!pos_xy || !w || !omega0 || !sub_pos || !sub_neg) return VPMC_ERR_ARG;

@@ -140,7 +140,7 @@ int bin_geom(const McD &m, int N1, int N2, int n_owner, BinGeom *g)
         if (e > emax) emax = e;
     }
     if (2 * emax + 1 > f.ny) return VPMC_ERR_GRID;
-    g->NS = (uint32_t)(N1 + N2);
+    g->NS = (uint32_t)samples_per_pair(N1, N2, m.aux);
     g->S = (size_t)n_owner * 2 * g->NS;
     if (g->S >= (size_t)1 << 31) return VPMC_ERR_ARG;   // int32 sample slots
     return VPMC_OK;

@@ -287,8 +287,11 @@ inline bool tile_threads_ok(const BinGeom &g, int threads)
     return 2 * g.cols + 1 <= maxcol;
 }
 
-// One wave: the estimate of (owner, sign) = pair from its N1+N2 samples at
-// r[base..], om[base..] in sample order (deterministic butterfly sum).
+// One wave: the estimate of (owner, sign) = pair from its samples at
+// r[base..], om[base..] in sample order (deterministic butterfly sum).  With
+// auxiliary targets, branch-1 samples also contribute at each target t with
+// the displacement r + offset(t) (min image in the periodic consistent
+// mode), and the disk samples of target t only there.
 template <int MODE, bool PER>
 __device__ __forceinline__ void bin_final_pair(const McD &d, const BinGeom &g, int pair,
                                                const float2 *__restrict__ r,
@@ -297,7 +300,7 @@ __device__ __forceinline__ void bin_final_pair(const McD &d, const BinGeom &g, i
     const int lane = threadIdx.x & (WAVE - 1);
     const int owner = pair >> 1, si = pair & 1;
     const SetD &S = d.set[si];
-    float sx = 0.f, sy = 0.f;
+    float sx[5] = {0.f, 0.f, 0.f, 0.f, 0.f}, sy[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
     const int n_s = S.count_dev ? *S.count_dev : S.n;
     if (n_s > 0) {
         const float Om = (float)(*S.total_dev);
@@ -305,14 +308,42 @@ __device__ __forceinline__ void bin_final_pair(const McD &d, const BinGeom &g, i
         if (MODE == VPMC_MODE_REFERENCE && PER) cum *= 9.f;
         const size_t base = (size_t)pair * g.NS;
         for (uint32_t k = lane; k < g.NS; k += WAVE) {
-            float2 rr = r[base + k];
-            mis_contrib<MODE, PER>(d, om[base + k], rr.x, rr.y, Om, cum, sx, sy);
+            const float2 rr = r[base + k];
+            const float o = om[base + k];
+            if ((int)k < d.N1) {
+                mis_contrib<MODE, PER>(d, o, rr.x, rr.y, Om, cum, sx[0], sy[0]);
+                if (d.aux) {
+#pragma unroll
+                    for (int t = 1; t < 5; ++t) {
+                        const float2 of = aux_offset(t, d.aux_dx);
+                        float rx = rr.x + of.x, ry = rr.y + of.y;
+                        if (MODE == VPMC_MODE_CONSISTENT && PER) {
+                            if (rx > 0.5f * d.Lx) rx -= d.Lx; else if (rx < -0.5f * d.Lx) rx += d.Lx;
+                            if (ry > 0.5f * d.Ly) ry -= d.Ly; else if (ry < -0.5f * d.Ly) ry += d.Ly;
+                        }
+                        mis_contrib<MODE, PER>(d, o, rx, ry, Om, cum, sx[t], sy[t]);
+                    }
+                }
+            } else {
+                const int kb = (int)k - d.N1;
+                const int t = kb < d.N2 ? 0 : 1 + (kb - d.N2) / d.N2;
+#pragma unroll
+                for (int tt = 0; tt < 5; ++tt)   // unrolled: the accumulators stay in registers
+                    if (tt == t) mis_contrib<MODE, PER>(d, o, rr.x, rr.y, Om, cum, sx[tt], sy[tt]);
+            }
         }
     }
-    sx = wave_sum(sx);
-    sy = wave_sum(sy);
-    if (lane == 0)
-        d.vel_sign[(size_t)si * d.n_owner + owner] = make_float2(sx * d.inv_tot, sy * d.inv_tot);
+    const int nt = d.aux ? 5 : 1;
+#pragma unroll
+    for (int t = 0; t < 5; ++t) {
+        if (t >= nt) break;
+        const float ax = wave_sum(sx[t]), ay = wave_sum(sy[t]);
+        if (lane == 0) {
+            const float2 v = make_float2(ax * d.inv_tot, ay * d.inv_tot);
+            if (t == 0) d.vel_sign[(size_t)si * d.n_owner + owner] = v;
+            else d.vel_aux[((size_t)si * 4 + (t - 1)) * d.n_owner + owner] = v;
+        }
+    }
 }
 
 }  // namespace vpmc

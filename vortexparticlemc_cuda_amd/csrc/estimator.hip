@@ -73,6 +73,31 @@ __global__ void __launch_bounds__(MC_THREADS) k_mc_estimate(McD d)
     }
 }
 
+// update_jacobian_from5 + renorm_det1 (src/cuda/monte_carlo_bs.cu:274-288,
+// include/monte_carlo_bs.cuh:140-144)
+__global__ void k_update_jacobian(int n, const float2 *__restrict__ vs,
+                                  const float2 *__restrict__ va, float c,
+                                  float4 *__restrict__ F)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float2 u[4];   // L, R, T, B: v+ - v-
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        const float2 p = va[(size_t)t * n + i], q = va[(size_t)(4 + t) * n + i];
+        u[t] = make_float2(p.x - q.x, p.y - q.y);
+    }
+    float4 f;
+    f.x = 1.f + (u[1].x - u[0].x) * c;
+    f.y = (u[2].x - u[3].x) * c;
+    f.z = (u[1].y - u[0].y) * c;
+    f.w = 1.f + (u[2].y - u[3].y) * c;
+    const float det = f.x * f.w - f.y * f.z;
+    const float sc = 1.f / sqrtf(fmaxf(det, 1e-12f));   // det F -> 1 (incompressible)
+    F[i] = make_float4(f.x * sc, f.y * sc, f.z * sc, f.w * sc);
+    (void)vs;
+}
+
 __global__ void k_advect(int n, float2 *__restrict__ pos, const float2 *__restrict__ vs,
                          float2 *__restrict__ vel, float dt, float Lx, float Ly, int per)
 {
@@ -205,6 +230,14 @@ int make_mcd(const vpmc_mc_desc *in, McD *d)
     }
     m.vel_sign = (float2 *)in->vel_sign;
     m.eval_count = in->eval_count;
+    m.aux = in->aux ? 1 : 0;
+    m.aux_dx = in->aux_dx;
+    m.vel_aux = (float2 *)in->vel_aux;
+    if (m.aux) {
+        if (!(in->aux_dx > 0.f) || (in->n_owner > 0 && !in->vel_aux)) return VPMC_ERR_ARG;
+        if (in->uniforms && !in->aux_draws) return VPMC_ERR_ARG;   // no aux draws to replay
+        if ((size_t)in->N1 + 5 * (size_t)in->N2 > (size_t)1 << 30) return VPMC_ERR_ARG;
+    }
     m.nterms = KPER_TERMS_F32;
     const float *kc = kper_coeffs_f32();
     for (int i = 0; i < KPER_TERMS_F32; ++i) m.kc[i] = kc[i];
@@ -228,6 +261,7 @@ int mc_estimate_impl(const vpmc_mc_desc *desc, hipStream_t s)
 {
     McD m;
     VPMC_TRY(make_mcd(desc, &m));
+    if (m.aux) return VPMC_ERR_ARG;   // auxiliary targets: the binned algorithm
     if (m.n_owner == 0) return VPMC_OK;
     VPMC_TRY(record_event(desc->eval_events[0], s));
     if (desc->eval_count) launch_mc<true>(m, desc->mode, desc->periodic, s);
@@ -246,6 +280,19 @@ extern "C" {
 int vpmc_mc_estimate(const vpmc_mc_desc *d, void *stream)
 {
     return mc_estimate_impl(d, as_stream(stream));
+}
+
+int vpmc_update_jacobian(int n, const float *vel_sign, const float *vel_aux, float dt, float dx,
+                         float *jac_out, void *stream)
+{
+    if (n < 0 || !(dx > 0.f)) return VPMC_ERR_ARG;
+    if (n == 0) return VPMC_OK;
+    if (!vel_aux || !jac_out) return VPMC_ERR_ARG;
+    hipLaunchKernelGGL(k_update_jacobian, dim3((n + 255) / 256), dim3(256), 0, as_stream(stream),
+                       n, (const float2 *)vel_sign, (const float2 *)vel_aux, 0.5f * dt / dx,
+                       (float4 *)jac_out);
+    VPMC_CHECK_LAUNCH();
+    return VPMC_OK;
 }
 
 int vpmc_advect(int n, float *pos_xy, const float *vel_sign, float *vel_xy, float dt, float Lx,

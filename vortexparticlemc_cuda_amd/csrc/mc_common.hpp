@@ -38,7 +38,22 @@ struct McD {
     unsigned long long *eval_count;
     int nterms;
     float kc[KPER_TERMS_F32];
+    int aux;                  // 1: four auxiliary targets (N2 disk samples each)
+    float aux_dx;
+    float2 *vel_aux;
 };
+
+// offset of auxiliary target t (1..4: L, R, T, B) from the owner
+__device__ __forceinline__ float2 aux_offset(int t, float dx)
+{
+    return make_float2(t == 1 ? -dx : (t == 2 ? dx : 0.f), t == 3 ? dx : (t == 4 ? -dx : 0.f));
+}
+
+// samples per (owner, sign): N1 + N2 (1 + 4 aux)
+__device__ __host__ __forceinline__ int samples_per_pair(int N1, int N2, int aux)
+{
+    return N1 + N2 * (aux ? 5 : 1);
+}
 
 int make_mcd(const vpmc_mc_desc *in, McD *d);
 
@@ -88,8 +103,8 @@ __device__ __forceinline__ void draws(const McD &d, int owner, int si, int branc
             u[0] = q[0];
             u[1] = d.b1 > 1 ? q[1] : 0.f;
             u[2] = d.b1 > 2 ? q[2] : 0.f;
-        } else {
-            const float *q = base + (size_t)d.N1 * d.b1 + (size_t)k * d.b2;
+        } else {   // branch 2 + t for the disk around target t: draws 2t, 2t + 1
+            const float *q = base + (size_t)d.N1 * d.b1 + (size_t)k * d.b2 + 2 * (branch - 2);
             u[0] = q[0];
             u[1] = q[1];
             u[2] = 0.f;
@@ -138,8 +153,17 @@ __device__ __forceinline__ bool gen_sample(const McD &d, const SetD &S, int n_s,
                                            float &rx, float &ry, int *src = nullptr)
 {
     const bool b1 = k < d.N1;
+    int t = 0, kb = b1 ? k : k - d.N1;   // t: target of a disk sample (0 = owner)
+    if (!b1 && kb >= d.N2) {
+        t = 1 + (kb - d.N2) / d.N2;
+        kb = (kb - d.N2) - (t - 1) * d.N2;
+    }
     float u[3];
-    draws(d, owner, si, b1 ? 1 : 2, b1 ? k : k - d.N1, u);
+    draws(d, owner, si, b1 ? 1 : 2 + t, kb, u);
+    if (t) {   // the auxiliary target, as the reference forms it (dst.x - dx, ...)
+        const float2 o = aux_offset(t, d.aux_dx);
+        dst = make_float2(__fadd_rn(dst.x, o.x), __fadd_rn(dst.y, o.y));
+    }
     float spx, spy;
     if (MODE == VPMC_MODE_REFERENCE) {
         if (b1) {   // src/cpu/monte_carlo_bs.cpp:44-58

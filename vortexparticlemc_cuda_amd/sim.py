@@ -115,8 +115,9 @@ class Simulation:
                                     device=self.device)
         self.vel = torch.zeros((max(self.N, 1), 2), dtype=torch.float32, device=self.device)
         self.eval_count = torch.zeros(2, dtype=torch.int64, device=self.device)
+        self.vel_aux = None
 
-    def desc(self, N1, N2, uniforms=None, aux_draws=0, count=False):
+    def desc(self, N1, N2, uniforms=None, aux_draws=0, count=False, aux=False):
         P = self.P
         sets = []
         names = ("pos", "neg")
@@ -146,6 +147,13 @@ class Simulation:
         d.algo = self.algo
         d.spl = self.spl
         d.eval_threads = self.eval_threads
+        if aux:   # the four targets at dst -/+ dx (SimParam::dx = h_w / 2)
+            if self.vel_aux is None:
+                self.vel_aux = torch.zeros((8 * max(self.N, 1), 2), dtype=torch.float32,
+                                           device=self.device)
+            d.aux = 1
+            d.aux_dx = float(P.dx)
+            d.vel_aux = ops._p(self.vel_aux)
         return d
 
     def _ws_for(self, d):
@@ -171,13 +179,30 @@ class Simulation:
             ops.field_build(self.fields[k], self.pos, self.omega, sub=self.sub[key], abs_w=True,
                             ws=ws)
 
-    def estimate(self, N1, N2, uniforms=None, aux_draws=0, count=False):
-        """velocity estimate at the current positions (no advection)"""
+    def estimate(self, N1, N2, uniforms=None, aux_draws=0, count=False, aux=False):
+        """velocity estimate at the current positions (no advection); with aux
+        also at the four targets dst -/+ dx (self.vel_aux, [sign][L,R,T,B][owner])"""
         self.rebuild_fields()
-        d = self.desc(N1, N2, uniforms, aux_draws, count)
+        d = self.desc(N1, N2, uniforms, aux_draws, count, aux)
         self.launch_estimate(d)
         vs = self.vel_sign[:2 * self.N].view(2, self.N, 2)
         return vs[0] - vs[1]
+
+    def aux_velocities(self):
+        """u = v+ - v- at the targets L, R, T, B of the last aux estimate: (4, N, 2)"""
+        va = self.vel_aux[:8 * self.N].view(2, 4, self.N, 2)
+        return va[0] - va[1]
+
+    def update_jacobian(self, dt=None, jac=None):
+        """update_jacobian_from5 (src/cuda/monte_carlo_bs.cu:274-288) from the
+        last aux estimate: F = renorm(I + dt/(2 dx) [uR - uL | uT - uB]), (N, 4)"""
+        dt = float(self.P.dt if dt is None else dt)
+        if jac is None:
+            jac = torch.empty((max(self.N, 1), 4), dtype=torch.float32, device=self.device)
+        check(lib().vpmc_update_jacobian(self.N, ops._p(self.vel_sign), ops._p(self.vel_aux), dt,
+                                         float(self.P.dx), ops._p(jac), ops._stream()),
+              "vpmc_update_jacobian")
+        return jac[:self.N]
 
     def step(self, N1=None, N2=None, dt=None, uniforms=None, aux_draws=0):
         """one plain step: fields, estimator, advection (step_cuda)"""
