@@ -122,6 +122,14 @@ typedef struct vpmc_field {
     const float *pts;                 /* n float4 records, cell order          */
     const int   *cell_start;          /* nx*ny + 1                             */
     const int   *order;               /* sorted position -> local index        */
+    /* deformed kernels (F in the query, include/vorticity_query.cuh:52-66):
+       gq (optional) holds per record the quadratic form G = F^-T F^-1 of
+       its particle's deformation F as float4 (G11, 2 G12, G22, 0), so that
+       the kernel is W(sqrt(d^T G d)) = W(|F^-1 d|); queries then search
+       gq_bound * h_w (0: 2, the reference's 2 h_w cutoff), the largest
+       stretch sigma_max(F) the records may carry. */
+    const float *gq;
+    float        gq_bound;
 } vpmc_field;
 
 int vpmc_field_grid(float h_w, float Lx, float Ly, int cells_per_hw, int *nx, int *ny);
@@ -132,6 +140,13 @@ int vpmc_field_build(int nx, int ny, float Lx, float Ly, int n, const int *sub,
                      const float *pos_xy, const float *w, int abs_w, float *pts,
                      int *cell_start, int *order, float *set_xy, void *workspace,
                      size_t workspace_bytes, void *stream);
+/* vpmc_field_build with deformations: jac (n float4 F per particle, global
+   index) gives gq[k] = G of record k (field order) and set_jac[j] = F of
+   local index j (either output may be NULL). */
+int vpmc_field_build_jac(int nx, int ny, float Lx, float Ly, int n, const int *sub,
+                         const float *pos_xy, const float *w, int abs_w, const float *jac,
+                         float *pts, float *gq, int *cell_start, int *order, float *set_xy,
+                         float *set_jac, void *workspace, size_t workspace_bytes, void *stream);
 /* periodic query of the field's weighted kernel sum at npts points */
 int vpmc_field_query(const vpmc_field *f, float h_w, int npts, const float *pts_xy,
                      float *out, void *stream);
@@ -161,6 +176,9 @@ typedef struct vpmc_sign_set {
     int           guide_log2;
     const float  *set_xy;             /* positions of the set in local order   */
     vpmc_field    field;              /* the set on the fine grid, weight |w|  */
+    const float  *set_jac;            /* optional: F of the set in local order
+                                         (float4); branch-1 jitter F_j u
+                                         (src/cuda/monte_carlo_bs.cu:210-212)  */
 } vpmc_sign_set;
 
 typedef struct vpmc_mc_desc {
@@ -204,6 +222,11 @@ typedef struct vpmc_mc_desc {
     int          aux;
     float        aux_dx;
     float       *vel_aux;
+    /* vpmc_step only: per-particle deformation F (n float4, row-major
+       F.x F.y; F.z F.w) or NULL.  The step builds the fields with it (the
+       sets' set_jac and field.gq buffers must be given) and, with aux,
+       replaces it by update_jacobian_from5 of this step's estimates. */
+    float       *jac;
 } vpmc_mc_desc;
 
 enum { VPMC_ALGO_BINNED = 0, VPMC_ALGO_DIRECT = 1 };

@@ -119,3 +119,243 @@ def test_update_jacobian_formula(device):
     ref = f / np.sqrt(np.maximum(det, 1e-12))[:, None]
     assert np.abs(F - ref).max() < 1e-6
     assert np.abs(F[:, 0] * F[:, 3] - F[:, 1] * F[:, 2] - 1).max() < 1e-5
+
+
+# --------------------------------------------------------------------------- deformed kernels
+# F enters the field query as W(|F^-1 d|) (include/vorticity_query.cuh:52-66)
+# and the branch-1 jitter as F u (src/cuda/monte_carlo_bs.cu:210-212).
+
+def _rand_F(n, rng, smax=1.6):
+    """det-1 deformations R(a) diag(s, 1/s) R(b), sigma_max = s <= smax"""
+    a, b = rng.uniform(0, 2 * np.pi, n), rng.uniform(0, 2 * np.pi, n)
+    s = rng.uniform(1.0, smax, n)
+    ca, sa, cb, sb = np.cos(a), np.sin(a), np.cos(b), np.sin(b)
+    F = np.empty((n, 2, 2))
+    F[:, 0, 0] = ca * s * cb - sa * sb / s
+    F[:, 0, 1] = -ca * s * sb - sa * cb / s
+    F[:, 1, 0] = sa * s * cb + ca * sb / s
+    F[:, 1, 1] = -sa * s * sb + ca * cb / s
+    return F.reshape(n, 4).astype(np.float32)
+
+
+def _W(q, hw):
+    sig = 40.0 / (7 * np.pi * hw * hw)
+    return np.where(q <= 0.5, sig * (1 - 6 * q ** 2 + 6 * q ** 3),
+                    np.where(q < 1, 2 * sig * (1 - q) ** 3, 0.0))
+
+
+def _brute_deformed(qx, qy, px, py, w, F, hw):
+    """FP64 sum_j |w_j| W(|F_j^-1 (x_j - q)|) over the 3 x 3 periodic images"""
+    qx = qx.astype(np.float64)[:, None]
+    qy = qy.astype(np.float64)[:, None]
+    F = F.astype(np.float64)
+    a, b, c, d = (F[None, :, k] for k in range(4))
+    det = a * d - b * c
+    aw = np.abs(w.astype(np.float64))
+    out = np.zeros(qx.shape[0])
+    for ix in (-1, 0, 1):
+        for iy in (-1, 0, 1):
+            dx = px.astype(np.float64)[None, :] + ix - qx
+            dy = py.astype(np.float64)[None, :] + iy - qy
+            lx, ly = (d * dx - b * dy) / det, (-c * dx + a * dy) / det
+            out += _W(np.hypot(lx, ly) / hw, hw) @ aw
+    return out
+
+
+def _deformed_field(device, x, y, w, F, hw, cph=8, gq_bound=0.0):
+    import torch
+
+    from vortexparticlemc_cuda_amd import ops
+    nx, ny = ops.field_grid(hw, 1.0, 1.0, cph)
+    fb = ops.FieldBuffers(nx, ny, 1.0, 1.0, x.size, device, deformed=True, gq_bound=gq_bound)
+    pos = torch.as_tensor(np.stack([x, y], 1), device=device)
+    ops.field_build(fb, pos, torch.as_tensor(w, device=device), abs_w=True,
+                    jac=torch.as_tensor(F, device=device))
+    return fb
+
+
+def test_deformed_query_matches_brute_force(device):
+    """field query with F == FP64 sum of |w| W(|F^-1 d|) over periodic images"""
+    import torch
+
+    from vortexparticlemc_cuda_amd import ops
+    x, y, w, hw = tg_problem(32, 4, seed=21)
+    rng = np.random.default_rng(5)
+    F = _rand_F(x.size, rng, 1.9)
+    fb = _deformed_field(device, x, y, w, F, hw)
+    # set_jac is F in the field's local order (identity subset here)
+    assert np.array_equal(fb.set_jac[:x.size].cpu().numpy(), F)
+    q = rng.random((3000, 2)).astype(np.float32)
+    q[:100] = np.stack([x[:100], y[:100]], 1)
+    got = ops.field_query(fb, hw, torch.as_tensor(q, device=device)).cpu().numpy()
+    bf = _brute_deformed(q[:, 0], q[:, 1], x, y, w, F, hw)
+    assert rel_l2(got, bf) < 2e-6, rel_l2(got, bf)
+    np.testing.assert_allclose(got, bf, rtol=2e-5, atol=4e-6 * np.abs(bf).max())
+
+
+def test_deformed_query_single_blob_closed_form(device):
+    """one particle of strength w with F: the field at x_j + F u is |w| W(|u|)
+    (a transposed or inverted F fails this)"""
+    import torch
+
+    from vortexparticlemc_cuda_amd import ops
+    hw = 0.125
+    rng = np.random.default_rng(9)
+    for trial in range(4):
+        F = _rand_F(1, rng, 1.8)
+        xj = rng.random(2).astype(np.float32)
+        fb = _deformed_field(device, xj[:1], xj[1:], np.array([-0.7], np.float32), F, hw)
+        u = rng.uniform(-1, 1, (2000, 2)) * hw
+        Fm = F.reshape(2, 2).astype(np.float64)
+        p = (xj[None, :] + u @ Fm.T) % 1.0
+        got = ops.field_query(fb, hw, torch.as_tensor(p.astype(np.float32),
+                                                      device=device)).cpu().numpy()
+        exp = 0.7 * _W(np.hypot(u[:, 0], u[:, 1]) / hw, hw)
+        np.testing.assert_allclose(got, exp, rtol=0, atol=2e-5 * exp.max())
+
+
+def test_deformed_query_rotation_and_identity_equal_undeformed(device):
+    """F a rotation (and F = I): the blob is unchanged, the deformed query
+    equals the undeformed one"""
+    import torch
+
+    from vortexparticlemc_cuda_amd import ops
+    x, y, w, hw = tg_problem(32, 4, seed=22)
+    rng = np.random.default_rng(6)
+    th = rng.uniform(0, 2 * np.pi, x.size)
+    R = np.stack([np.cos(th), -np.sin(th), np.sin(th), np.cos(th)], 1).astype(np.float32)
+    I = np.tile(np.array([1, 0, 0, 1], np.float32), (x.size, 1))
+    nx, ny = ops.field_grid(hw, 1.0, 1.0, 8)
+    fb0 = ops.FieldBuffers(nx, ny, 1.0, 1.0, x.size, device)
+    ops.field_build(fb0, torch.as_tensor(np.stack([x, y], 1), device=device),
+                    torch.as_tensor(w, device=device), abs_w=True)
+    q = torch.as_tensor(rng.random((3000, 2)).astype(np.float32), device=device)
+    ref = ops.field_query(fb0, hw, q).cpu().numpy()
+    for F in (R, I):
+        got = ops.field_query(_deformed_field(device, x, y, w, F, hw), hw, q).cpu().numpy()
+        np.testing.assert_allclose(got, ref, rtol=1e-5, atol=2e-6 * np.abs(ref).max())
+
+
+def _deformed_sim(device, x, y, w, F, Nx, hwr, algo, seed=3):
+    from vortexparticlemc_cuda_amd._capi import MODE_CONSISTENT
+    from vortexparticlemc_cuda_amd.sim import SimParam, Simulation
+    P = SimParam(dt=1e-3, resX=Nx, hwr=hwr)
+    sim = Simulation(P, device=device, cells_per_hw=8, mode=MODE_CONSISTENT, seed=seed,
+                     algo=algo, deform=True)
+    sim.set_particles(np.stack([x, y], 1), w)
+    sim.set_jacobian(F)
+    return sim
+
+
+def test_deformed_binned_matches_per_owner(device):
+    """the binned estimator's deformed tile path == the per-owner estimator
+    (same Philox samples, the field query with F) on the owner and aux targets"""
+    from vortexparticlemc_cuda_amd._capi import ALGO_BINNED, ALGO_DIRECT
+    x, y, w, hw = tg_problem(32, 4, seed=23)
+    F = _rand_F(x.size, np.random.default_rng(7), 1.7)
+    out = []
+    for algo in (ALGO_BINNED, ALGO_DIRECT):
+        sim = _deformed_sim(device, x, y, w, F, 32, 4, algo)
+        sim.step_index = 2
+        out.append(sim.estimate(24, 24).cpu().numpy())
+    assert rel_l2(out[0], out[1]) < 1e-5, rel_l2(out[0], out[1])
+    sim = _deformed_sim(device, x, y, w, F, 32, 4, ALGO_BINNED)
+    sim.step_index = 2
+    u_aux = sim.estimate(24, 24, aux=True).cpu().numpy()
+    assert np.array_equal(u_aux, out[0])
+
+
+def _spectral_velocity(x, y, w, F, hw, M):
+    """FP64 velocity of the deformed blob field on the M x M grid m / M:
+    omega_F sampled on the grid, psi = omega / |k|^2 (mean dropped, as the
+    doubly periodic kernel does), u = (d psi / dy, -d psi / dx)"""
+    g = np.arange(M) / M
+    gx, gy = np.meshgrid(g, g, indexing="ij")
+    F = F.astype(np.float64)
+    om = np.zeros(M * M)
+    qx, qy = gx.ravel(), gy.ravel()
+    for j in range(x.size):   # min image: the support 2 h_w < 1/2
+        dx = qx - float(x[j])
+        dy = qy - float(y[j])
+        dx -= np.round(dx)
+        dy -= np.round(dy)
+        a, b, c, d = F[j]
+        det = a * d - b * c
+        lx, ly = (d * dx - b * dy) / det, (-c * dx + a * dy) / det
+        om += float(w[j]) * _W(np.hypot(lx, ly) / hw, hw)
+    oh = np.fft.fft2(om.reshape(M, M))
+    k = 2 * np.pi * np.fft.fftfreq(M, 1.0 / M)
+    kx, ky = np.meshgrid(k, k, indexing="ij")
+    k2 = kx * kx + ky * ky
+    k2[0, 0] = 1.0
+    ph = oh / k2
+    ph[0, 0] = 0.0
+    ux = np.real(np.fft.ifft2(1j * ky * ph))
+    uy = np.real(np.fft.ifft2(-1j * kx * ph))
+    return ux, uy
+
+
+def test_deformed_estimate_is_unbiased(device):
+    """consistent mode with F: the mean over Philox seeds converges to the
+    velocity of the deformed blob field (FP64 spectral Biot-Savart on a
+    fine grid; the same construction reproduces the undeformed closed-form
+    blob sum to 1e-6 first)"""
+    import torch
+
+    from vortexparticlemc_cuda_amd import ops
+    from vortexparticlemc_cuda_amd._capi import ALGO_BINNED
+    Nx, hwr, M = 24, 4, 192          # owners on the lattice (i + 1/2)/24 = (8 i + 4)/192
+    x, y, w, hw = tg_problem(Nx, hwr, jitter=0)
+    idx = (np.round(x * M).astype(int), np.round(y * M).astype(int))
+    I = np.tile(np.array([1, 0, 0, 1], np.float32), (x.size, 1))
+    ux, uy = _spectral_velocity(x, y, w, I, hw, M)
+    pos = np.stack([x, y], 1).astype(np.float64)
+    ud = ops.direct_sum_blob(torch.as_tensor(pos, device=device),
+                             torch.as_tensor(pos, device=device),
+                             torch.as_tensor(w.astype(np.float64), device=device),
+                             hw).cpu().numpy()
+    sp = np.stack([ux[idx], uy[idx]], 1)
+    assert rel_l2(sp, ud) < 1e-6, rel_l2(sp, ud)
+    F = _rand_F(x.size, np.random.default_rng(8), 1.6)
+    ux, uy = _spectral_velocity(x, y, w, F, hw, M)
+    ref = np.stack([ux[idx], uy[idx]], 1)
+    assert rel_l2(ref, ud) > 1e-3   # the deformation changes the velocity
+    sim = _deformed_sim(device, x, y, w, F, Nx, hwr, ALGO_BINNED)
+    K = 64
+    est = []
+    for k in range(K):
+        sim.seed = 900 + k
+        est.append(sim.estimate(4096, 4096).double().cpu().numpy())
+    est = np.array(est)
+    err2 = ((est.mean(0) - ref) ** 2).sum()
+    var = est.var(0, ddof=1).sum() / K
+    assert err2 / var < 2.0, err2 / var
+    # the undeformed velocity (2.7 % away) is rejected by the same statistic
+    assert ((est.mean(0) - ud) ** 2).sum() / var > 5.0, ((est.mean(0) - ud) ** 2).sum() / var
+
+
+def test_deformed_step_updates_F_from_the_five_estimates(device):
+    """step() with deform: the estimate includes the aux targets, F becomes
+    update_jacobian_from5 of them before the move (step_cuda order,
+    src/cuda/simulation_step.cu:51-86); the next step builds deformed fields"""
+    import torch
+
+    from vortexparticlemc_cuda_amd._capi import ALGO_BINNED
+    x, y, w, hw = tg_problem(32, 4, seed=24)
+    F0 = _rand_F(x.size, np.random.default_rng(10), 1.3)
+    a = _deformed_sim(device, x, y, w, F0, 32, 4, ALGO_BINNED)
+    b = _deformed_sim(device, x, y, w, F0, 32, 4, ALGO_BINNED)
+    dt = 2e-3
+    u = a.estimate(20, 20, aux=True).clone()
+    Fa = a.update_jacobian(dt).cpu().numpy()
+    p0 = b.pos.clone()
+    v = b.step(20, 20, dt=dt)
+    assert np.array_equal(v.cpu().numpy(), u.cpu().numpy())
+    assert np.array_equal(b.jac[:b.N].cpu().numpy(), Fa)
+    moved = (p0 + dt * v) % 1.0
+    assert torch.allclose(b.pos, moved, atol=1e-6)
+    for _ in range(3):
+        v = b.step(20, 20, dt=dt)
+    Fb = b.jac[:b.N].double().cpu().numpy()
+    assert np.isfinite(v.cpu().numpy()).all()
+    assert np.abs(Fb[:, 0] * Fb[:, 3] - Fb[:, 1] * Fb[:, 2] - 1).max() < 1e-5

@@ -41,12 +41,12 @@ class View(C.Structure):
 
 class Field(C.Structure):
     _fields_ = [("nx", ci), ("ny", ci), ("Lx", cf), ("Ly", cf), ("n", ci), ("pts", vp),
-                ("cell_start", vp), ("order", vp)]
+                ("cell_start", vp), ("order", vp), ("gq", vp), ("gq_bound", cf)]
 
 
 class SignSet(C.Structure):
     _fields_ = [("n", ci), ("count_dev", vp), ("cdf", vp), ("total_dev", vp), ("guide", vp),
-                ("guide_log2", ci), ("set_xy", vp), ("field", Field)]
+                ("guide_log2", ci), ("set_xy", vp), ("field", Field), ("set_jac", vp)]
 
 
 class McDesc(C.Structure):
@@ -55,7 +55,8 @@ class McDesc(C.Structure):
                 ("sim", C.c_uint32), ("uniforms", vp), ("aux_draws", ci), ("n_owner", ci),
                 ("owner_xy", vp), ("set", SignSet * 2), ("vel_sign", vp),
                 ("eval_count", vp), ("algo", ci), ("eval_events", vp * 2), ("spl", ci),
-                ("eval_threads", ci), ("aux", ci), ("aux_dx", cf), ("vel_aux", vp)]
+                ("eval_threads", ci), ("aux", ci), ("aux_dx", cf), ("vel_aux", vp),
+                ("jac", vp)]
 
 
 class EnsembleDesc(C.Structure):
@@ -95,6 +96,8 @@ def lib():
     L.vpmc_field_workspace_bytes.argtypes = [ci, ci, ci]
     L.vpmc_field_build.argtypes = [ci, ci, cf, cf, ci, vp, vp, vp, ci, vp, vp, vp, vp, vp, csz,
                                    vp]
+    L.vpmc_field_build_jac.argtypes = [ci, ci, cf, cf, ci, vp, vp, vp, ci, vp, vp, vp, vp, vp,
+                                       vp, vp, vp, csz, vp]
     L.vpmc_field_query.argtypes = [C.POINTER(Field), cf, ci, vp, vp, vp]
     L.vpmc_mc_estimate.argtypes = [C.POINTER(McDesc), vp]
     L.vpmc_mc_binned_workspace_bytes.restype = csz
@@ -158,7 +161,7 @@ EXPORTS = [
     "vpmc_hash_workspace_bytes", "vpmc_fill_cells", "vpmc_counting_sort",
     "vpmc_subsets_workspace_bytes", "vpmc_build_subsets", "vpmc_build_guide", "vpmc_sample_idx",
     "vpmc_query_view", "vpmc_field_grid", "vpmc_field_workspace_bytes", "vpmc_field_build",
-    "vpmc_field_query", "vpmc_mc_estimate", "vpmc_mc_binned_workspace_bytes",
+    "vpmc_field_build_jac", "vpmc_field_query", "vpmc_mc_estimate", "vpmc_mc_binned_workspace_bytes",
     "vpmc_mc_estimate_binned", "vpmc_advect", "vpmc_update_jacobian", "vpmc_step_workspace_bytes",
     "vpmc_step", "vpmc_rbf_workspace_bytes", "vpmc_rbf_matvec", "vpmc_rbf_matvec_f64", "vpmc_rbf_matvec_particles",
     "vpmc_cg_rbf", "vpmc_cg_rbf_f64", "vpmc_cg_workspace_bytes", "vpmc_direct_sum_eps", "vpmc_direct_sum_blob",

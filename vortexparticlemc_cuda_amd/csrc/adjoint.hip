@@ -560,8 +560,9 @@ static int carve_adj(const BinGeom &g, int n, void *ws, size_t bytes, AdjBufs *B
 static int adj_check(const vpmc_mc_desc *d, McD *m, BinGeom *g)
 {
     if (!d) return VPMC_ERR_ARG;
-    if (d->mode != VPMC_MODE_CONSISTENT || !d->periodic || d->uniforms || d->aux)
-        return VPMC_ERR_ARG;
+    if (d->mode != VPMC_MODE_CONSISTENT || !d->periodic || d->uniforms || d->aux || d->jac ||
+        d->set[0].field.gq || d->set[1].field.gq)
+        return VPMC_ERR_ARG;   // the adjoint covers undeformed kernels only
     VPMC_TRY(make_mcd(d, m));
     VPMC_TRY(bin_geom(*m, d->N1, d->N2, d->n_owner, g));
     if (!tile_threads_ok(*g, ADJ_THREADS)) return VPMC_ERR_GRID;

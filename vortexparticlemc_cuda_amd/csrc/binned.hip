@@ -51,7 +51,7 @@ __global__ void __launch_bounds__(BIN_THREADS)
     bin_gen_one<MODE, PER>(d, g, sid, sid, 0u, (uint32_t)g.ntile, key, q, r, om);
 }
 
-template <int THREADS, int SPL, bool COUNT>
+template <int THREADS, int SPL, bool COUNT, bool DEF = false>
 __global__ void __launch_bounds__(THREADS)
     k_bin_eval(McD m, BinGeom g, const int *__restrict__ start, const float4 *__restrict__ srec,
                float *__restrict__ om, unsigned long long *__restrict__ eval_count)
@@ -62,9 +62,9 @@ __global__ void __launch_bounds__(THREADS)
     if (tile >= g.ntile) return;
     const int beg = start[tile], end = start[tile + 1];
     if (beg >= end) return;   // uniform over the workgroup
-    __shared__ TileLds<THREADS> L;
+    __shared__ TileLds<THREADS, DEF> L;
     EvalCount ec;
-    bin_eval_tile<THREADS, SPL, COUNT>(m, g, tile, beg, end, srec, om, L, ec);
+    bin_eval_tile<THREADS, SPL, COUNT, DEF>(m, g, tile, beg, end, srec, om, L, ec);
     if (COUNT) {
         unsigned a = wave_sum(ec.in_support), b = wave_sum(ec.visited);
         if ((threadIdx.x & (WAVE - 1)) == 0) {
@@ -84,22 +84,33 @@ __global__ void __launch_bounds__(BIN_THREADS)
 }
 
 // eval launch for runtime workgroup-size and samples-per-lane choices
-template <int THREADS, bool COUNT>
+template <int THREADS, bool COUNT, bool DEF>
 static void launch_eval_t(int spl, const McD &m, const BinGeom &g, const int *start,
                           const float4 *srec, float *om, unsigned long long *cnt, hipStream_t s)
 {
     const dim3 grid((unsigned)(8 * g.per_xcd)), block(THREADS);
     if (spl == 1)
-        hipLaunchKernelGGL((k_bin_eval<THREADS, 1, COUNT>), grid, block, 0, s, m, g, start, srec,
-                           om, cnt);
+        hipLaunchKernelGGL((k_bin_eval<THREADS, 1, COUNT, DEF>), grid, block, 0, s, m, g, start,
+                           srec, om, cnt);
     else if (spl == 4)
-        hipLaunchKernelGGL((k_bin_eval<THREADS, 4, COUNT>), grid, block, 0, s, m, g, start, srec,
-                           om, cnt);
+        hipLaunchKernelGGL((k_bin_eval<THREADS, 4, COUNT, DEF>), grid, block, 0, s, m, g, start,
+                           srec, om, cnt);
     else
-        hipLaunchKernelGGL((k_bin_eval<THREADS, 2, COUNT>), grid, block, 0, s, m, g, start, srec,
-                           om, cnt);
+        hipLaunchKernelGGL((k_bin_eval<THREADS, 2, COUNT, DEF>), grid, block, 0, s, m, g, start,
+                           srec, om, cnt);
 }
 
+template <bool COUNT, bool DEF>
+static void launch_eval_d(int threads, int spl, const McD &m, const BinGeom &g,
+                          const int *start, const float4 *srec, float *om,
+                          unsigned long long *cnt, hipStream_t s)
+{
+    if (threads == 64) launch_eval_t<64, COUNT, DEF>(spl, m, g, start, srec, om, cnt, s);
+    else if (threads == 128) launch_eval_t<128, COUNT, DEF>(spl, m, g, start, srec, om, cnt, s);
+    else launch_eval_t<256, COUNT, DEF>(spl, m, g, start, srec, om, cnt, s);
+}
+
+// deformed sets (field.gq != NULL) take the quadratic-form records
 template <bool COUNT>
 static int launch_eval(int threads, int spl, const McD &m, const BinGeom &g, const int *start,
                        const float4 *srec, float *om, unsigned long long *cnt, hipStream_t s)
@@ -107,9 +118,8 @@ static int launch_eval(int threads, int spl, const McD &m, const BinGeom &g, con
     if (threads == 0) threads = 256;
     if (threads != 64 && threads != 128 && threads != 256) return VPMC_ERR_ARG;
     if (!tile_threads_ok(g, threads)) return VPMC_ERR_GRID;
-    if (threads == 64) launch_eval_t<64, COUNT>(spl, m, g, start, srec, om, cnt, s);
-    else if (threads == 128) launch_eval_t<128, COUNT>(spl, m, g, start, srec, om, cnt, s);
-    else launch_eval_t<256, COUNT>(spl, m, g, start, srec, om, cnt, s);
+    if (m.set[0].field.gq) launch_eval_d<COUNT, true>(threads, spl, m, g, start, srec, om, cnt, s);
+    else launch_eval_d<COUNT, false>(threads, spl, m, g, start, srec, om, cnt, s);
     return VPMC_OK;
 }
 

@@ -91,9 +91,10 @@ __device__ __forceinline__ int block_excl_scan(int v, int *wsum, int *total)
 // The tile's stencil as runs ("pieces") of the sorted particle array, in the
 // order column -cols..cols and, per column, [rows wrapped below] [rows
 // wrapped above] [rows in range]; p_off is the running candidate count.
-template <int THREADS> struct TileLds {
+template <int THREADS, bool DEF = false> struct TileLds {
     using Cfg = TileCfg<THREADS>;
     float4 cand[Cfg::CAP];
+    float4 cand_g[DEF ? Cfg::CAP : 1];   // deformed records: quadratic forms
     int p_beg[Cfg::MAXP];
     int p_off[Cfg::MAXP + 1];
     float p_sx[Cfg::MAXP], p_sy[Cfg::MAXP];
@@ -101,10 +102,10 @@ template <int THREADS> struct TileLds {
 };
 
 // 2 cols + 1 <= TileCfg<THREADS>::MAXCOL (checked on the host)
-template <int THREADS>
+template <int THREADS, bool DEF = false>
 __device__ __forceinline__ int tile_pieces(const BinGeom &g, const int *__restrict__ cell_start,
                                            int cx, int cy, float Lx, float Ly,
-                                           TileLds<THREADS> &L)
+                                           TileLds<THREADS, DEF> &L)
 {
     const int ncol = 2 * g.cols + 1;
     const int j = threadIdx.x;
@@ -152,10 +153,12 @@ __device__ __forceinline__ int tile_pieces(const BinGeom &g, const int *__restri
 // stages the candidates with raw index in [r0, r1) (r1 - r0 <= CAP) that
 // can reach the cell: cand[0 .. n) in raw order, padded with zero weights to
 // a multiple of 4; returns n (padded).  Starts and ends with a barrier.
-template <int THREADS>
+// DEF: records are (c'x, c'y, w, 0) with the quadratic form in cand_g (gq).
+template <int THREADS, bool DEF = false>
 __device__ __forceinline__ int tile_stage(const BinGeom &g, const float4 *__restrict__ pts,
                                           int npieces, int r0, int r1, float ox, float oy,
-                                          TileLds<THREADS> &L)
+                                          TileLds<THREADS, DEF> &L,
+                                          const float4 *__restrict__ gq = nullptr)
 {
     const int lane = threadIdx.x & (WAVE - 1), wid = threadIdx.x / WAVE;
     const float hx = 0.5f * g.cw, hy = 0.5f * g.ch;
@@ -164,18 +167,24 @@ __device__ __forceinline__ int tile_stage(const BinGeom &g, const float4 *__rest
     for (int i0 = r0; i0 < r1; i0 += THREADS) {
         const int i = i0 + threadIdx.x;
         bool keep = false;
-        float4 rec = make_float4(0.f, 0.f, 0.f, 0.f);
+        float4 rec = make_float4(0.f, 0.f, 0.f, 0.f), recg = rec;
         if (i < r1) {
             int lo = 0, hi = npieces;   // last piece with p_off <= i
             while (hi - lo > 1) {
                 int mid = (lo + hi) >> 1;
                 if (L.p_off[mid] <= i) lo = mid; else hi = mid;
             }
-            const float4 c = pts[L.p_beg[lo] + (i - L.p_off[lo])];
+            const int src = L.p_beg[lo] + (i - L.p_off[lo]);
+            const float4 c = pts[src];
             const float x = (c.x + L.p_sx[lo]) - ox, y = (c.y + L.p_sy[lo]) - oy;
             const float gx = fmaxf(fabsf(x) - hx, 0.f), gy = fmaxf(fabsf(y) - hy, 0.f);
             keep = fmaf(gx, gx, gy * gy) <= g.rs2;
-            rec = make_float4(-2.f * x, -2.f * y, fmaf(x, x, y * y), c.z);
+            if (DEF) {
+                rec = make_float4(x, y, c.z, 0.f);
+                recg = gq[src];
+            } else {
+                rec = make_float4(-2.f * x, -2.f * y, fmaf(x, x, y * y), c.z);
+            }
         }
         const unsigned long long m = __ballot(keep);
         const int pre = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
@@ -188,12 +197,18 @@ __device__ __forceinline__ int tile_stage(const BinGeom &g, const float4 *__rest
             base += (w < wid) ? L.wsum[w] : 0;
             tot += L.wsum[w];
         }
-        if (keep) L.cand[base + pre] = rec;
+        if (keep) {
+            L.cand[base + pre] = rec;
+            if (DEF) L.cand_g[base + pre] = recg;
+        }
         n += tot;
         __syncthreads();   // wsum is rewritten by the next round
     }
     const int npad = (n + 3) & ~3;
-    if ((int)threadIdx.x < npad - n) L.cand[n + threadIdx.x] = make_float4(0.f, 0.f, 0.f, 0.f);
+    if ((int)threadIdx.x < npad - n) {
+        L.cand[n + threadIdx.x] = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (DEF) L.cand_g[n + threadIdx.x] = make_float4(1.f, 0.f, 1.f, 0.f);
+    }
     __syncthreads();
     return npad;
 }
@@ -229,14 +244,42 @@ __device__ __forceinline__ void tile_eval(const float4 *__restrict__ cand, int n
     }
 }
 
+// deformed records: r^2 = d^T G d with d = p' - c' (records (c'x, c'y, w, 0),
+// forms (G11, 2 G12, G22, 0)), then the same spline
+template <int SPL, bool COUNT>
+__device__ __forceinline__ void tile_eval_def(const float4 *__restrict__ cand,
+                                              const float4 *__restrict__ cand_g, int n,
+                                              const float (&px)[SPL], const float (&py)[SPL],
+                                              float inv_hw, float (&acc)[SPL], EvalCount &ec,
+                                              const bool (&live)[SPL])
+{
+    for (int k = 0; k < n; ++k) {
+        const float4 c = cand[k], G = cand_g[k];
+#pragma unroll
+        for (int s = 0; s < SPL; ++s) {
+            const float dx = px[s] - c.x, dy = py[s] - c.y;
+            const float r2 = fmaf(dx, fmaf(G.y, dy, G.x * dx), G.z * dy * dy);
+            float q = __builtin_amdgcn_sqrtf(fabsf(r2));
+            float t = __builtin_amdgcn_fmed3f(fmaf(-inv_hw, q, 1.f), 0.f, 1.f);
+            float v = __builtin_amdgcn_fmed3f(fmaf(t, CBRT4, -0.5f * CBRT4), 0.f, 1.f);
+            float t2 = t * t, v3 = v * v * v;
+            acc[s] = fmaf(c.z, fmaf(t2, t, -v3), acc[s]);
+            if (COUNT && live[s] && c.z > 0.f) {
+                ec.in_support += (t > 0.f) ? 1u : 0u;
+                ec.visited += 1u;
+            }
+        }
+    }
+}
+
 // The samples srec[beg, end) of local tile `tile` (sign tile / ncell) of
 // the simulation described by (m, g): stage, evaluate, write omega to
 // om[sample id].  Called by the whole workgroup of THREADS threads.
-template <int THREADS, int SPL, bool COUNT>
+template <int THREADS, int SPL, bool COUNT, bool DEF = false>
 __device__ __forceinline__ void bin_eval_tile(const McD &m, const BinGeom &g, int tile, int beg,
                                               int end, const float4 *__restrict__ srec,
-                                              float *__restrict__ om, TileLds<THREADS> &L,
-                                              EvalCount &ec)
+                                              float *__restrict__ om,
+                                              TileLds<THREADS, DEF> &L, EvalCount &ec)
 {
     constexpr int CAP = TileCfg<THREADS>::CAP;
     const int si = tile >= g.ncell ? 1 : 0;
@@ -244,11 +287,13 @@ __device__ __forceinline__ void bin_eval_tile(const McD &m, const BinGeom &g, in
     const int cx = c / g.ny, cy = c - cx * g.ny;
     const float ox = ((float)cx + 0.5f) * g.cw, oy = ((float)cy + 0.5f) * g.ch;
     const float4 *__restrict__ pts = m.set[si].field.pts;
-    const int total = tile_pieces<THREADS>(g, m.set[si].field.cell_start, cx, cy, m.Lx, m.Ly, L);
+    const float4 *__restrict__ gq = m.set[si].field.gq;
+    const int total = tile_pieces<THREADS, DEF>(g, m.set[si].field.cell_start, cx, cy, m.Lx,
+                                                m.Ly, L);
     const int npieces = 3 * (2 * g.cols + 1);
     const int npass = (total + CAP - 1) / CAP;
     int n = 0;
-    if (npass == 1) n = tile_stage<THREADS>(g, pts, npieces, 0, total, ox, oy, L);
+    if (npass == 1) n = tile_stage<THREADS, DEF>(g, pts, npieces, 0, total, ox, oy, L, gq);
     const int lane = threadIdx.x & (WAVE - 1), wid = threadIdx.x / WAVE;
     for (int bb = beg; bb < end; bb += THREADS * SPL) {   // uniform trip count
         // a wave past the tile's last sample stops (no barrier follows when
@@ -270,9 +315,12 @@ __device__ __forceinline__ void bin_eval_tile(const McD &m, const BinGeom &g, in
         }
         for (int pass = 0; pass < npass; ++pass) {
             if (npass > 1)
-                n = tile_stage<THREADS>(g, pts, npieces, pass * CAP, min(total, (pass + 1) * CAP),
-                                        ox, oy, L);
-            tile_eval<SPL, COUNT>(L.cand, n, px, py, pp, m.inv_hw, acc, ec, live);
+                n = tile_stage<THREADS, DEF>(g, pts, npieces, pass * CAP,
+                                             min(total, (pass + 1) * CAP), ox, oy, L, gq);
+            if constexpr (DEF)
+                tile_eval_def<SPL, COUNT>(L.cand, L.cand_g, n, px, py, m.inv_hw, acc, ec, live);
+            else
+                tile_eval<SPL, COUNT>(L.cand, n, px, py, pp, m.inv_hw, acc, ec, live);
         }
 #pragma unroll
         for (int s = 0; s < SPL; ++s)

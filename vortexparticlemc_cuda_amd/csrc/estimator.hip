@@ -226,6 +226,9 @@ int make_mcd(const vpmc_mc_desc *in, McD *d)
         VPMC_TRY(field_geom(&S.field, in->h_w, &D.field.g));
         D.field.pts = (const float4 *)S.field.pts;
         D.field.cell_start = S.field.cell_start;
+        D.field.gq = (const float4 *)S.field.gq;
+        D.set_jac = (const float4 *)S.set_jac;
+        if ((S.field.gq != nullptr) != (in->set[0].field.gq != nullptr)) return VPMC_ERR_ARG;
         if (!D.field.cell_start) return VPMC_ERR_ARG;
     }
     m.vel_sign = (float2 *)in->vel_sign;
@@ -329,16 +332,27 @@ int vpmc_step(vpmc_mc_desc *d, int n_total, float *pos_xy, const float *omega,
     const int *subs[2] = {sub_pos, sub_neg};
     for (int k = 0; k < 2; ++k) {
         vpmc_sign_set &S = d->set[k];
+        // with a deformation: the fields' quadratic forms and the sets' F
+        // (src/cuda/monte_carlo_bs.cu:210-212, include/vorticity_query.cuh:52-66)
+        if (d->jac && (!S.field.gq || !S.set_jac)) return VPMC_ERR_ARG;
+        if (!d->jac && (S.field.gq || S.set_jac)) return VPMC_ERR_ARG;
         VPMC_TRY(field_build_impl(S.field.nx, S.field.ny, S.field.Lx, S.field.Ly, S.n, nullptr,
                                   subs[k], pos_xy, omega, 1, const_cast<float *>(S.field.pts),
                                   const_cast<int *>(S.field.cell_start),
                                   const_cast<int *>(S.field.order),
-                                  const_cast<float *>(S.set_xy), workspace, workspace_bytes, s));
+                                  const_cast<float *>(S.set_xy), workspace, workspace_bytes, s,
+                                  d->jac, const_cast<float *>(S.field.gq),
+                                  const_cast<float *>(S.set_jac)));
     }
     if (d->algo == VPMC_ALGO_BINNED)
         VPMC_TRY(vpmc_mc_estimate_binned(d, workspace, workspace_bytes, stream));
     else
         VPMC_TRY(mc_estimate_impl(d, s));
+    // step_cuda: F from this step's five estimates, before the move
+    // (src/cuda/simulation_step.cu:68-72)
+    if (d->jac && d->aux)
+        VPMC_TRY(vpmc_update_jacobian(n_total, d->vel_sign, d->vel_aux, dt, d->aux_dx, d->jac,
+                                      stream));
     return vpmc_advect(n_total, pos_xy, d->vel_sign, vel_xy, dt, d->Lx, d->Ly, d->periodic,
                        stream);
 }

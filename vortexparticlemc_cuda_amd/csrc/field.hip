@@ -11,12 +11,14 @@ namespace vpmc {
 
 __global__ void k_field_keys(int n, const int *__restrict__ sub, const float2 *__restrict__ pos,
                              int nx, int ny, float inv_cx, float inv_cy, int *__restrict__ key,
-                             int *__restrict__ count, float2 *__restrict__ set_xy)
+                             int *__restrict__ count, float2 *__restrict__ set_xy,
+                             const float4 *__restrict__ jac, float4 *__restrict__ set_jac)
 {
     int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= n) return;
     int I = sub ? sub[j] : j;
     float2 p = pos[I];
+    if (set_jac) set_jac[j] = jac[I];
     int ix = min(max((int)floorf(p.x * inv_cx), 0), nx - 1);
     int iy = min(max((int)floorf(p.y * inv_cy), 0), ny - 1);
     int c = ix * ny + iy;
@@ -38,7 +40,8 @@ __global__ void k_field_scatter(int n, const int *__restrict__ key, int *__restr
 __global__ void k_field_fixup(int n, const int *__restrict__ key, const int *__restrict__ cell_start,
                               const int *__restrict__ tmp, const int *__restrict__ sub,
                               const float2 *__restrict__ pos, const float *__restrict__ w,
-                              int abs_w, int *__restrict__ order, float4 *__restrict__ pts)
+                              int abs_w, int *__restrict__ order, float4 *__restrict__ pts,
+                              const float4 *__restrict__ jac, float4 *__restrict__ gq)
 {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
@@ -54,6 +57,7 @@ __global__ void k_field_fixup(int n, const int *__restrict__ key, const int *__r
     if (abs_w) wv = fabsf(wv);
     if (order) order[dst] = j;
     pts[dst] = make_float4(p.x, p.y, wv, 0.f);
+    if (gq) gq[dst] = quad_form(jac[I]);
 }
 
 int field_geom(const vpmc_field *f, float h_w, FieldGeom *g)
@@ -68,9 +72,12 @@ int field_geom(const vpmc_field *f, float h_w, FieldGeom *g)
     g->inv_cy = (float)f->ny / f->Ly;
     g->cw = f->Lx / (float)f->nx;
     g->ch = f->Ly / (float)f->ny;
-    // margin covers FP32 rounding of keys and coordinates (a few ulp of L)
+    // margin covers FP32 rounding of keys and coordinates (a few ulp of L);
+    // deformed records reach sigma_max(F) h_w <= gq_bound h_w
     float Lm = f->Lx > f->Ly ? f->Lx : f->Ly;
-    g->rs = h_w * (1.f + 1e-5f) + 5e-7f * Lm;
+    const float stretch = f->gq ? (f->gq_bound > 0.f ? f->gq_bound : 2.f) : 1.f;
+    if (!(stretch >= 1.f)) return VPMC_ERR_ARG;
+    g->rs = stretch * h_w * (1.f + 1e-5f) + 5e-7f * Lm;
     g->rs2 = g->rs * g->rs;
     int rx = (int)ceilf(g->rs / g->cw), ry = (int)ceilf(g->rs / g->ch);
     g->reach = rx;
@@ -81,8 +88,9 @@ int field_geom(const vpmc_field *f, float h_w, FieldGeom *g)
 int field_build_impl(int nx, int ny, float Lx, float Ly, int n, const int *n_dev,
                      const int *sub, const float *pos_xy, const float *w, int abs_w,
                      float *pts, int *cell_start, int *order, float *set_xy, void *ws,
-                     size_t ws_bytes, hipStream_t s)
+                     size_t ws_bytes, hipStream_t s, const float *jac, float *gq, float *set_jac)
 {
+    if (!jac && (gq || set_jac)) return VPMC_ERR_ARG;
     (void)n_dev;
     if (nx <= 0 || ny <= 0 || n < 0 || !cell_start || (n > 0 && (!pos_xy || !pts)))
         return VPMC_ERR_ARG;
@@ -99,7 +107,7 @@ int field_build_impl(int nx, int ny, float Lx, float Ly, int n, const int *n_dev
     if (n > 0) {
         hipLaunchKernelGGL(k_field_keys, dim3(gb), dim3(256), 0, s, n, sub,
                            (const float2 *)pos_xy, nx, ny, inv_cx, inv_cy, key, count,
-                           (float2 *)set_xy);
+                           (float2 *)set_xy, (const float4 *)jac, (float4 *)set_jac);
         VPMC_CHECK_LAUNCH();
     }
     VPMC_TRY(scan_i32(count, cell_start, cells, false, cell_start + cells, scan_ws,
@@ -109,7 +117,8 @@ int field_build_impl(int nx, int ny, float Lx, float Ly, int n, const int *n_dev
                            (const int *)cell_start, tmp);
         hipLaunchKernelGGL(k_field_fixup, dim3(gb), dim3(256), 0, s, n, (const int *)key,
                            (const int *)cell_start, (const int *)tmp, sub,
-                           (const float2 *)pos_xy, w, abs_w, order, (float4 *)pts);
+                           (const float2 *)pos_xy, w, abs_w, order, (float4 *)pts,
+                           (const float4 *)jac, (float4 *)gq);
         VPMC_CHECK_LAUNCH();
     }
     return VPMC_OK;
@@ -173,6 +182,17 @@ int vpmc_field_build(int nx, int ny, float Lx, float Ly, int n, const int *sub,
                             order, set_xy, workspace, workspace_bytes, as_stream(stream));
 }
 
+int vpmc_field_build_jac(int nx, int ny, float Lx, float Ly, int n, const int *sub,
+                         const float *pos_xy, const float *w, int abs_w, const float *jac,
+                         float *pts, float *gq, int *cell_start, int *order, float *set_xy,
+                         float *set_jac, void *workspace, size_t workspace_bytes, void *stream)
+{
+    if (!jac) return VPMC_ERR_ARG;
+    return field_build_impl(nx, ny, Lx, Ly, n, nullptr, sub, pos_xy, w, abs_w, pts, cell_start,
+                            order, set_xy, workspace, workspace_bytes, as_stream(stream), jac, gq,
+                            set_jac);
+}
+
 int vpmc_field_query(const vpmc_field *f, float h_w, int npts, const float *pts_xy, float *out,
                      void *stream)
 {
@@ -181,7 +201,7 @@ int vpmc_field_query(const vpmc_field *f, float h_w, int npts, const float *pts_
     if (npts < 0) return VPMC_ERR_ARG;
     if (npts == 0) return VPMC_OK;
     if (!pts_xy || !out || !f->cell_start || (f->n > 0 && !f->pts)) return VPMC_ERR_ARG;
-    FieldD d{(const float4 *)f->pts, f->cell_start, g};
+    FieldD d{(const float4 *)f->pts, f->cell_start, g, (const float4 *)f->gq};
     hipLaunchKernelGGL(k_field_query, dim3((npts + 255) / 256), dim3(256), 0, as_stream(stream),
                        d, 1.f / h_w, 40.f / (7.f * PI_F * h_w * h_w), npts,
                        (const float2 *)pts_xy, out);

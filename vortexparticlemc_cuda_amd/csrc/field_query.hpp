@@ -19,6 +19,7 @@ struct FieldD {
     const float4 *pts;
     const int *cell_start;
     FieldGeom g;
+    const float4 *gq;   // deformed records: quadratic forms (G11, 2 G12, G22, 0), or NULL
 };
 
 struct EvalCount {
@@ -28,12 +29,17 @@ struct EvalCount {
 template <bool COUNT>
 __device__ __forceinline__ void seg_accum(const float4 *__restrict__ pts, int beg, int end,
                                           float qx, float qy, float inv_hw, float &a1,
-                                          float &a2, EvalCount &ec)
+                                          float &a2, EvalCount &ec,
+                                          const float4 *__restrict__ gq = nullptr)
 {
     for (int k = beg; k < end; ++k) {
         float4 c = pts[k];
         float dx = c.x - qx, dy = c.y - qy;
         float r2 = fmaf(dx, dx, dy * dy);
+        if (gq) {   // |F^-1 d|^2
+            const float4 G = gq[k];
+            r2 = fmaxf(fmaf(dx, fmaf(G.y, dy, G.x * dx), G.z * dy * dy), 0.f);
+        }
         float q = __builtin_amdgcn_sqrtf(r2) * inv_hw;
         float t = fmaxf(1.f - q, 0.f);   // q >= 0: the upper clamp never binds
         float u = fmaxf(0.5f - q, 0.f);
@@ -80,16 +86,16 @@ __device__ __forceinline__ float field_sum(const FieldD &f, float px, float py, 
         if (PERIODIC) {
             if (r0 < 0) {   // rows ny+r0 .. ny-1 seen through the image y - Ly
                 seg_accum<COUNT>(f.pts, cs[g.ny + r0], cs[g.ny], qx, py + g.Ly, inv_hw, a1, a2,
-                                 ec);
+                                 ec, f.gq);
                 r0 = 0;
             }
             if (r1 >= g.ny) {   // rows 0 .. r1-ny seen through the image y + Ly
                 seg_accum<COUNT>(f.pts, cs[0], cs[r1 - g.ny + 1], qx, py - g.Ly, inv_hw, a1, a2,
-                                 ec);
+                                 ec, f.gq);
                 r1 = g.ny - 1;
             }
         }
-        seg_accum<COUNT>(f.pts, cs[r0], cs[r1 + 1], qx, py, inv_hw, a1, a2, ec);
+        seg_accum<COUNT>(f.pts, cs[r0], cs[r1 + 1], qx, py, inv_hw, a1, a2, ec, f.gq);
     }
     return 2.f * a1 - 8.f * a2;
 }

@@ -19,6 +19,7 @@ struct SetD {
     int glog2;
     const float2 *set_xy;
     FieldD field;
+    const float4 *set_jac;    // deformation F by local index, or NULL
 };
 
 struct McD {
@@ -196,8 +197,15 @@ __device__ __forceinline__ bool gen_sample(const McD &d, const SetD &S, int n_s,
             float rho = mass_inverse(u[1]) * d.h_w;
             float s, c;
             sincosf(2.f * PI_F * u[2], &s, &c);
-            spx = xj.x + rho * c;
-            spy = xj.y + rho * s;
+            float jx = rho * c, jy = rho * s;
+            if (S.set_jac) {   // the deformed blob: F_j u (src/cuda/monte_carlo_bs.cu:210-212)
+                const float4 F = S.set_jac[j];
+                const float ex = fmaf(F.x, jx, F.y * jy), ey = fmaf(F.z, jx, F.w * jy);
+                jx = ex;
+                jy = ey;
+            }
+            spx = xj.x + jx;
+            spy = xj.y + jy;
             rx = dst.x - spx;
             ry = dst.y - spy;
             if (PER) {

@@ -85,13 +85,25 @@ struct FieldGeom {           // device-side copy of vpmc_field's geometry
     float inv_cx, inv_cy;    // nx / Lx, ny / Ly
     float cw, ch;            // Lx / nx, Ly / ny
     int reach;               // columns each side
-    float rs, rs2;           // search radius (h_w plus rounding margin)
+    float rs, rs2;           // search radius (h_w, times the deformation bound
+                             // with deformed records, plus rounding margin)
 };
+
+// G = F^-T F^-1 of F = (a b; c d), stored (G11, 2 G12, G22, 0):
+// |F^-1 d|^2 = G11 dx^2 + 2 G12 dx dy + G22 dy^2
+__device__ __forceinline__ float4 quad_form(float4 F)
+{
+    const float det = F.x * F.w - F.y * F.z;
+    const float i2 = 1.f / (det * det);
+    return make_float4((F.z * F.z + F.w * F.w) * i2, -2.f * (F.x * F.z + F.y * F.w) * i2,
+                       (F.x * F.x + F.y * F.y) * i2, 0.f);
+}
 int field_geom(const vpmc_field *f, float h_w, FieldGeom *g);
 int field_build_impl(int nx, int ny, float Lx, float Ly, int n, const int *n_dev,
                      const int *sub, const float *pos_xy, const float *w, int abs_w,
                      float *pts, int *cell_start, int *order, float *set_xy, void *ws,
-                     size_t ws_bytes, hipStream_t s);
+                     size_t ws_bytes, hipStream_t s, const float *jac = nullptr,
+                     float *gq = nullptr, float *set_jac = nullptr);
 
 // ---- device math --------------------------------------------------------------
 __device__ __forceinline__ float u01(uint32_t x) { return (float)(x >> 8) * 0x1p-24f; }

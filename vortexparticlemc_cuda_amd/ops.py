@@ -136,27 +136,48 @@ def field_grid(h_w, Lx=1.0, Ly=1.0, cells_per_hw=8):
 class FieldBuffers:
     """device buffers of one fine-grid field"""
 
-    def __init__(self, nx, ny, Lx, Ly, n, device):
+    def __init__(self, nx, ny, Lx, Ly, n, device, deformed=False, gq_bound=0.0):
         self.nx, self.ny, self.Lx, self.Ly, self.n = nx, ny, float(Lx), float(Ly), int(n)
         m = max(n, 1)
         self.pts = torch.zeros((m, 4), dtype=torch.float32, device=device)
         self.cell_start = torch.zeros(nx * ny + 1, dtype=torch.int32, device=device)
         self.order = torch.zeros(m, dtype=torch.int32, device=device)
         self.set_xy = torch.zeros((m, 2), dtype=torch.float32, device=device)
+        # deformed kernels: per-record quadratic forms, per-local-index F
+        self.gq = self.set_jac = None
+        self.gq_bound = float(gq_bound)
+        if deformed:
+            self.gq = torch.zeros((m, 4), dtype=torch.float32, device=device)
+            self.set_jac = torch.zeros((m, 4), dtype=torch.float32, device=device)
 
     def struct(self):
         return Field(self.nx, self.ny, self.Lx, self.Ly, self.n, _p(self.pts),
-                     _p(self.cell_start), _p(self.order))
+                     _p(self.cell_start), _p(self.order), _p(self.gq), self.gq_bound)
 
 
-def field_build(fb, pos, w, sub=None, abs_w=True, ws=None):
+def field_build(fb, pos, w, sub=None, abs_w=True, ws=None, jac=None):
+    """vpmc_field_build; with jac (n x 4 F per particle) the deformed
+    build vpmc_field_build_jac (fb must have been made deformed=True)"""
     pos = _xy(pos)
     if ws is None:
         ws = workspace(lib().vpmc_field_workspace_bytes(fb.n, fb.nx, fb.ny), pos.device)
-    check(lib().vpmc_field_build(fb.nx, fb.ny, fb.Lx, fb.Ly, fb.n, _p(sub), _p(pos),
-                                 _p(None if w is None else w.contiguous()), int(abs_w),
-                                 _p(fb.pts), _p(fb.cell_start), _p(fb.order), _p(fb.set_xy),
-                                 _p(ws), ws.numel(), _stream()), "vpmc_field_build")
+    wp = _p(None if w is None else w.contiguous())
+    if jac is not None:
+        if fb.gq is None:
+            raise ValueError("field_build: jac needs FieldBuffers(..., deformed=True)")
+        jac = jac.to(torch.float32).contiguous()
+        check(lib().vpmc_field_build_jac(fb.nx, fb.ny, fb.Lx, fb.Ly, fb.n, _p(sub), _p(pos), wp,
+                                         int(abs_w), _p(jac), _p(fb.pts), _p(fb.gq),
+                                         _p(fb.cell_start), _p(fb.order), _p(fb.set_xy),
+                                         _p(fb.set_jac), _p(ws), ws.numel(), _stream()),
+              "vpmc_field_build_jac")
+        return fb
+    if fb.gq is not None:
+        raise ValueError("field_build: a deformed FieldBuffers needs jac")
+    check(lib().vpmc_field_build(fb.nx, fb.ny, fb.Lx, fb.Ly, fb.n, _p(sub), _p(pos), wp,
+                                 int(abs_w), _p(fb.pts), _p(fb.cell_start), _p(fb.order),
+                                 _p(fb.set_xy), _p(ws), ws.numel(), _stream()),
+          "vpmc_field_build")
     return fb
 
 

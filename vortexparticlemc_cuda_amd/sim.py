@@ -57,8 +57,14 @@ class Simulation:
     include/vortex_particle_mc.hpp:136-192, for its GPU path)."""
 
     def __init__(self, P, device="cuda", cells_per_hw=8, mode=MODE_CONSISTENT, seed=0, sim=0,
-                 algo=ALGO_BINNED, spl=0, eval_threads=0):
+                 algo=ALGO_BINNED, spl=0, eval_threads=0, deform=False, gq_bound=0.0):
         self.P = P
+        # kernel deformation (step_cuda's particles.F): F per particle, the
+        # fields carry its quadratic forms, each step refreshes it from the
+        # five estimates (src/cuda/simulation_step.cu:51-72)
+        self.deform = bool(deform)
+        self.gq_bound = float(gq_bound)
+        self.jac = None
         self.algo = int(algo)
         self.spl = int(spl)
         self.eval_threads = int(eval_threads)
@@ -109,8 +115,13 @@ class Simulation:
         P = self.P
         self.nx, self.ny = ops.field_grid(float(P.h_w), float(P.Lx), float(P.Ly),
                                           self.cells_per_hw)
-        self.fields = [ops.FieldBuffers(self.nx, self.ny, P.Lx, P.Ly, self.cnt[k], self.device)
+        self.fields = [ops.FieldBuffers(self.nx, self.ny, P.Lx, P.Ly, self.cnt[k], self.device,
+                                        deformed=self.deform, gq_bound=self.gq_bound)
                        for k in range(2)]
+        if self.deform and (self.jac is None or self.jac.shape[0] != max(self.N, 1)):
+            self.jac = torch.zeros((max(self.N, 1), 4), dtype=torch.float32, device=self.device)
+            self.jac[:, 0] = 1.0   # F = I
+            self.jac[:, 3] = 1.0
         self.vel_sign = torch.zeros((2 * max(self.N, 1), 2), dtype=torch.float32,
                                     device=self.device)
         self.vel = torch.zeros((max(self.N, 1), 2), dtype=torch.float32, device=self.device)
@@ -126,7 +137,8 @@ class Simulation:
             sets.append(SignSet(self.cnt[k], ops._p(self.sub["counts"][k:k + 1]),
                                 ops._p(self.sub["cdf_" + names[k]]),
                                 ops._p(self.sub["totals"][k:k + 1]), ops._p(self.guide[k]),
-                                self.glog2, ops._p(fb.set_xy), fb.struct()))
+                                self.glog2, ops._p(fb.set_xy), fb.struct(),
+                                ops._p(fb.set_jac)))
         self._U = None if uniforms is None else torch.as_tensor(
             uniforms, dtype=torch.float32, device=self.device).contiguous()
         d = McDesc()
@@ -154,6 +166,7 @@ class Simulation:
             d.aux = 1
             d.aux_dx = float(P.dx)
             d.vel_aux = ops._p(self.vel_aux)
+        d.jac = ops._p(self.jac) if self.deform else None
         return d
 
     def _ws_for(self, d):
@@ -177,7 +190,7 @@ class Simulation:
                            self.device)
         for k, key in enumerate(("sub_pos", "sub_neg")):
             ops.field_build(self.fields[k], self.pos, self.omega, sub=self.sub[key], abs_w=True,
-                            ws=ws)
+                            ws=ws, jac=self.jac if self.deform else None)
 
     def estimate(self, N1, N2, uniforms=None, aux_draws=0, count=False, aux=False):
         """velocity estimate at the current positions (no advection); with aux
@@ -204,12 +217,23 @@ class Simulation:
               "vpmc_update_jacobian")
         return jac[:self.N]
 
+    def set_jacobian(self, jac):
+        """the deformation F per particle, (N, 4) row-major (F.x F.y; F.z F.w)"""
+        if not self.deform:
+            raise ValueError("set_jacobian: Simulation(deform=True) carries F")
+        j = torch.as_tensor(np.asarray(jac, np.float32), device=self.device).reshape(-1, 4)
+        if j.shape[0] != self.N:
+            raise ValueError("set_jacobian: one F per particle")
+        self.jac[:self.N] = j
+
     def step(self, N1=None, N2=None, dt=None, uniforms=None, aux_draws=0):
-        """one plain step: fields, estimator, advection (step_cuda)"""
+        """one step: fields, estimator, advection (step_cuda); a deformed
+        simulation also estimates the four aux targets and replaces F by
+        update_jacobian_from5 of them before the move"""
         N1 = self.P.N1 if N1 is None else N1
         N2 = self.P.N2 if N2 is None else N2
         dt = float(self.P.dt if dt is None else dt)
-        d = self.desc(N1, N2, uniforms, aux_draws)
+        d = self.desc(N1, N2, uniforms, aux_draws, aux=self.deform)
         self._ws_for(d)
         check(lib().vpmc_step(C.byref(d), self.N, ops._p(self.pos), ops._p(self.omega),
                               ops._p(self.sub["sub_pos"]), ops._p(self.sub["sub_neg"]),
