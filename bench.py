@@ -180,6 +180,67 @@ def velocity_error(sim):
                 rms_err_vs_tg=rms(u - ut) / rms(ut))
 
 
+def deform_bench(args, device, base):
+    """the reference's CUDA step semantics (step_cuda): five estimates per
+    owner and sign (owner + the aux targets at dst -/+ dx, branch-1 samples
+    shared), F = update_jacobian_from5 of them, deformed kernels W(|F^-1 d|)
+    in every query and F u in the branch-1 jitter; same workload and launch
+    shape as the plain step"""
+    import ctypes as C
+
+    import torch
+
+    from vortexparticlemc_cuda_amd import ops
+    from vortexparticlemc_cuda_amd._capi import check, lib
+    from vortexparticlemc_cuda_amd.sim import Simulation
+
+    P = base.P
+    sim = Simulation(P, device=device, cells_per_hw=args.cells_per_hw, mode=base.mode,
+                     seed=base.seed, spl=args.spl, eval_threads=args.eval_threads, deform=True)
+    sim.set_particles(base.pos.cpu().numpy(), base.omega.cpu().numpy())
+    for _ in range(args.warmup):
+        sim.step(P.N1, P.N2)
+    s = torch.cuda.current_stream()
+    K = args.deform_steps
+    pairs = [_event_pair(s) for _ in range(K)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(K):
+        d = sim.desc(P.N1, P.N2, aux=True)
+        d.eval_events[0] = pairs[i][0].cuda_event
+        d.eval_events[1] = pairs[i][1].cuda_event
+        ws = sim._ws_for(d)
+        check(lib().vpmc_step(C.byref(d), sim.N, ops._p(sim.pos), ops._p(sim.omega),
+                              ops._p(sim.sub["sub_pos"]), ops._p(sim.sub["sub_neg"]),
+                              ops._p(sim.vel), float(P.dt), ops._p(ws), ws.numel(),
+                              ops._stream()), "vpmc_step")
+        sim.step_index += 1
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    eval_s = float(np.mean([a.elapsed_time(b) for a, b in pairs])) / 1e3
+    sim.eval_count.zero_()
+    sim.rebuild_fields()
+    d = sim.desc(P.N1, P.N2, count=True, aux=True)
+    sim.launch_estimate(d)
+    torch.cuda.synchronize()
+    n_in, n_vis = (int(v) for v in sim.eval_count.cpu().numpy())
+    F = sim.jac[:sim.N].double()
+    det = F[:, 0] * F[:, 3] - F[:, 1] * F[:, 2]
+    sv = torch.linalg.svdvals(F.view(-1, 2, 2))
+    out = {"workload": "the plain workload with the CUDA step's aux targets and kernel "
+                       "deformation (F reset from the five estimates every step)",
+           "particle_steps_per_s": sim.N * K / el, "ms_per_step": el / K * 1e3, "steps": K,
+           "samples_per_step": sim.N * 2 * (P.N1 + 5 * P.N2),
+           "eval_kernel": "k_bin_eval (deformed records)", "eval_avg_ms": eval_s * 1e3,
+           "kernel_evals_per_launch": n_in, "candidates_per_launch": n_vis,
+           "achieved_flops": FLOP_PER_EVAL * n_in / eval_s,
+           "frac_of_fp32_peak": FLOP_PER_EVAL * n_in / eval_s / PEAK_FP32,
+           "max_abs_det_minus_1": float((det - 1).abs().max()),
+           "max_sigma_max_F": float(sv[:, 0].max())}
+    log("deform", out)
+    return out
+
+
 def ensemble_bench(args, device):
     """The batched ensemble (64 periodic simulations, Nx drawn from
     {64, 96, 128, 192, 256}, 8 Gaussian vortices each, hwr = 8, N1 = N2 = 64,
@@ -596,6 +657,8 @@ def main():
                     help="skip the launch-shape tuning of the plain step")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="also run a plain copy of MIB MiB (HBM counter calibration)")
+    ap.add_argument("--deform-steps", type=int, default=10,
+                    help="steps of the deformed (aux + F) step on the plain workload (0: skip)")
     ap.add_argument("--ens-sims", type=int, default=64,
                     help="simulations in the batched-ensemble measurement (0: skip it)")
     ap.add_argument("--ens-N", type=int, default=64, help="N1 = N2 of the ensemble")
@@ -670,6 +733,7 @@ def main():
     traffic = read_traffic(eval_kernel, args)
     if args.copy_probe:
         copy_probe(args.copy_probe << 20)
+    deform = deform_bench(args, device, sim) if args.deform_steps > 0 else None
     batched = ensemble_bench(args, device) if args.ens_sims > 0 else None
     large = large_bench(args, device) if args.large_Nx > 0 else None
     adj = adjoint_bench(args, device) if args.adj_Nx > 0 else None
@@ -709,6 +773,7 @@ def main():
         "estimator_fraction_of_step": est_avg_s / (elapsed / args.steps),
         "error": err,
         "cg_init": init,
+        "deform": deform,
         "batched": batched,
         "large": large,
         "adjoint": adj,

@@ -173,6 +173,11 @@ public:
     uint32_t sim_id = 0;      // ensemble member (Philox counter)
     uint32_t step_index = 0;  // advanced by step_cuda
     int cells_per_hw = 8;     // fine grid of the estimator
+    // step_cuda with the CUDA step's aux targets and kernel deformation
+    // (particles.F: W(|F^-1 d|) in the query, F u in the jitter, F reset
+    // from the five estimates every step); off: aux = 0, F = I
+    bool kernel_deformation = false;
+    float deformation_bound = 0.f;   // largest sigma_max(F) searched (0: 2, the 2 h_w cutoff)
     hipStream_t stream = nullptr;
     int cg_iterations = 0;
     double cg_rel_residual = 0.0;

@@ -84,6 +84,20 @@ int main(int argc, char **argv)
         (void)hipDeviceSynchronize();
         dump(out + "/pos2.bin", sim.particles_.pos.to_host());
 
+        // two steps of the CUDA step with aux targets and kernel deformation
+        {
+            Simulation sd(P);
+            sd.mode = EstimatorMode::Consistent;
+            sd.seed = 7;
+            sd.kernel_deformation = true;
+            sd.set_particles(pos, sim.particles_.omega.to_host());
+            sd.step_cuda(true);
+            sd.step_cuda(true);
+            (void)hipDeviceSynchronize();
+            dump(out + "/pos_def2.bin", sd.particles_.pos.to_host());
+            dump(out + "/F_def2.bin", sd.particles_.F.to_host());
+        }
+
         sim.test_vorticity_grid(16);
 
         bool threw = false;

@@ -67,3 +67,16 @@ def test_cxx_api_end_to_end(device, tmp_path):
     d = rd("pos2.bin").reshape(-1, 2) - np.stack([px, py], 1)
     d -= np.rint(d)
     assert np.abs(d).max() < 1e-6
+    # kernel_deformation: step_cuda with aux targets and F == the Python mirror
+    # (same library entry point, same Philox streams) bit for bit
+    from vortexparticlemc_cuda_amd._capi import MODE_CONSISTENT
+    from vortexparticlemc_cuda_amd.sim import SimParam, Simulation
+    sd = Simulation(SimParam(dt=1e-3, resX=Nx, hwr=hwr), device=device, cells_per_hw=8,
+                    mode=MODE_CONSISTENT, seed=7, deform=True)
+    sd.set_particles(np.stack([x, y], 1), w)
+    sd.step(64, 64)
+    sd.step(64, 64)
+    assert np.array_equal(rd("pos_def2.bin").reshape(-1, 2), sd.pos.cpu().numpy())
+    F = rd("F_def2.bin").reshape(-1, 4)[:N]
+    assert np.array_equal(F, sd.jac[:N].cpu().numpy())
+    assert not np.array_equal(F, np.tile([1, 0, 0, 1], (N, 1)).astype(np.float32))

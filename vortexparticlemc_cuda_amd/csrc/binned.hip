@@ -136,6 +136,7 @@ int bin_geom(const McD &m, int N1, int N2, int n_owner, BinGeom *g)
     g->cw = f.cw;
     g->ch = f.ch;
     g->rs2 = f.rs2;
+    g->rs2_1 = f.rs1 * f.rs1;
     g->ntile = 2 * g->ncell;
     g->per_xcd = (g->ntile + 7) / 8;
     g->cols = 1 + (int)floorf(f.rs / f.cw);

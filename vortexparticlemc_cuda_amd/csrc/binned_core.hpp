@@ -13,6 +13,7 @@ struct BinGeom {
     int ncell, nx, ny;
     float inv_cx, inv_cy;
     float cw, ch, rs2;        // cell size, squared search radius (field geometry)
+    float rs2_1;              // squared h_w reach (deformed records scale it by sigma_max^2)
     int ntile, per_xcd;       // 2 * ncell tiles; tiles per XCD in the launch order
     int cols;                 // stencil half-width (columns)
     int ext[MAX_COL_EXT];     // rows each side for column offset ox (index ox+cols), -1: none
@@ -179,9 +180,13 @@ __device__ __forceinline__ int tile_stage(const BinGeom &g, const float4 *__rest
             const float x = (c.x + L.p_sx[lo]) - ox, y = (c.y + L.p_sy[lo]) - oy;
             const float gx = fmaxf(fabsf(x) - hx, 0.f), gy = fmaxf(fabsf(y) - hy, 0.f);
             keep = fmaf(gx, gx, gy * gy) <= g.rs2;
-            if (DEF) {
-                rec = make_float4(x, y, c.z, 0.f);
-                recg = gq[src];
+            if (DEF) {   // r^2 = p'^T G p' - 2 (G c')^T p' + c'^T G c'
+                const float4 G = gq[src];
+                keep = keep && fmaf(gx, gx, gy * gy) <= G.w * g.rs2_1;   // own reach
+                const float g12 = 0.5f * G.y;
+                const float gcx = fmaf(G.x, x, g12 * y), gcy = fmaf(g12, x, G.z * y);
+                rec = make_float4(G.x, G.y, G.z, -2.f * gcx);
+                recg = make_float4(-2.f * gcy, fmaf(x, gcx, y * gcy), c.z, 0.f);
             } else {
                 rec = make_float4(-2.f * x, -2.f * y, fmaf(x, x, y * y), c.z);
             }
@@ -207,7 +212,7 @@ __device__ __forceinline__ int tile_stage(const BinGeom &g, const float4 *__rest
     const int npad = (n + 3) & ~3;
     if ((int)threadIdx.x < npad - n) {
         L.cand[n + threadIdx.x] = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (DEF) L.cand_g[n + threadIdx.x] = make_float4(1.f, 0.f, 1.f, 0.f);
+        if (DEF) L.cand_g[n + threadIdx.x] = make_float4(0.f, 0.f, 0.f, 0.f);
     }
     __syncthreads();
     return npad;
@@ -244,8 +249,9 @@ __device__ __forceinline__ void tile_eval(const float4 *__restrict__ cand, int n
     }
 }
 
-// deformed records: r^2 = d^T G d with d = p' - c' (records (c'x, c'y, w, 0),
-// forms (G11, 2 G12, G22, 0)), then the same spline
+// deformed records: r^2 = |F^-1 (p' - c')|^2 as five FMAs on the sample's
+// monomials (px^2, px py, py^2, px, py); records (G11, 2 G12, G22, -2 (Gc')x)
+// and (-2 (Gc')y, c'^T G c', w, 0), then the same spline
 template <int SPL, bool COUNT>
 __device__ __forceinline__ void tile_eval_def(const float4 *__restrict__ cand,
                                               const float4 *__restrict__ cand_g, int n,
@@ -253,22 +259,33 @@ __device__ __forceinline__ void tile_eval_def(const float4 *__restrict__ cand,
                                               float inv_hw, float (&acc)[SPL], EvalCount &ec,
                                               const bool (&live)[SPL])
 {
-    for (int k = 0; k < n; ++k) {
-        const float4 c = cand[k], G = cand_g[k];
+    float xx[SPL], xy[SPL], yy[SPL];
+#pragma unroll
+    for (int s = 0; s < SPL; ++s) {
+        xx[s] = px[s] * px[s];
+        xy[s] = px[s] * py[s];
+        yy[s] = py[s] * py[s];
+    }
+    auto one = [&](const float4 &c, const float4 &e) {
 #pragma unroll
         for (int s = 0; s < SPL; ++s) {
-            const float dx = px[s] - c.x, dy = py[s] - c.y;
-            const float r2 = fmaf(dx, fmaf(G.y, dy, G.x * dx), G.z * dy * dy);
+            const float r2 = fmaf(c.x, xx[s], fmaf(c.y, xy[s], fmaf(c.z, yy[s],
+                                  fmaf(c.w, px[s], fmaf(e.x, py[s], e.y)))));
             float q = __builtin_amdgcn_sqrtf(fabsf(r2));
             float t = __builtin_amdgcn_fmed3f(fmaf(-inv_hw, q, 1.f), 0.f, 1.f);
             float v = __builtin_amdgcn_fmed3f(fmaf(t, CBRT4, -0.5f * CBRT4), 0.f, 1.f);
             float t2 = t * t, v3 = v * v * v;
-            acc[s] = fmaf(c.z, fmaf(t2, t, -v3), acc[s]);
-            if (COUNT && live[s] && c.z > 0.f) {
+            acc[s] = fmaf(e.z, fmaf(t2, t, -v3), acc[s]);
+            if (COUNT && live[s] && e.z > 0.f) {
                 ec.in_support += (t > 0.f) ? 1u : 0u;
                 ec.visited += 1u;
             }
         }
+    };
+    for (int k = 0; k < n; k += 2) {   // n is a multiple of 4 (zero-weight padding)
+        const float4 c0 = cand[k], e0 = cand_g[k], c1 = cand[k + 1], e1 = cand_g[k + 1];
+        one(c0, e0);
+        one(c1, e1);
     }
 }
 

@@ -169,6 +169,8 @@ struct Simulation::Impl {
     dvec<int> cell_start[2], order[2];
     dvec<float2> set_xy[2];
     dvec<float2> vel_sign;
+    dvec<float4> gq[2], set_jac[2];   // kernel deformation: forms and F per set
+    dvec<float2> vel_aux;             // [sign][L, R, T, B][owner]
     Scratch ws;
 };
 
@@ -310,11 +312,28 @@ void Simulation::do_spatial_hashing()
     vcheck(vpmc_field_grid(P_.h_w, P_.Lx, P_.Ly, cells_per_hw, &I.nx, &I.ny), "field_grid");
     const int cnt[2] = {cnt_pos_, cnt_neg_};
     size_t b = vpmc_field_workspace_bytes(std::max(cnt_pos_, cnt_neg_), I.nx, I.ny);
+    if (kernel_deformation && particles_.F.size() < (size_t)particles_.N_cur) {
+        std::vector<float4> F(particles_.N_cur, make_float4(1.f, 0.f, 0.f, 1.f));
+        particles_.F.assign_host(F.data(), F.size());
+    }
     for (int s = 0; s < 2; ++s) {
         I.pts[s].resize(std::max(cnt[s], 1));
         I.order[s].resize(std::max(cnt[s], 1));
         I.set_xy[s].resize(std::max(cnt[s], 1));
         I.cell_start[s].resize((size_t)I.nx * I.ny + 1);
+        if (kernel_deformation) {   // F in the query (include/vorticity_query.cuh:52-66)
+            I.gq[s].resize(std::max(cnt[s], 1));
+            I.set_jac[s].resize(std::max(cnt[s], 1));
+            vcheck(vpmc_field_build_jac(
+                       I.nx, I.ny, P_.Lx, P_.Ly, cnt[s], s ? raw_ptr(sub_neg_) : raw_ptr(sub_pos_),
+                       (const float *)particles_.d_p(), particles_.d_omega(), 1,
+                       (const float *)raw_ptr(particles_.F), (float *)raw_ptr(I.pts[s]),
+                       (float *)raw_ptr(I.gq[s]), raw_ptr(I.cell_start[s]), raw_ptr(I.order[s]),
+                       (float *)raw_ptr(I.set_xy[s]), (float *)raw_ptr(I.set_jac[s]),
+                       I.ws.get(b), b, stream),
+                   "field_build");
+            continue;
+        }
         vcheck(vpmc_field_build(I.nx, I.ny, P_.Lx, P_.Ly, cnt[s],
                                 s ? raw_ptr(sub_neg_) : raw_ptr(sub_pos_),
                                 (const float *)particles_.d_p(), particles_.d_omega(), 1,
@@ -381,6 +400,16 @@ static vpmc_mc_desc make_desc(Simulation &S, int N1, int N2, int cnt_pos, int cn
     return d;
 }
 
+// the CUDA step's F in the fields and the jitter (src/cuda/monte_carlo_bs.cu:210-212)
+static void attach_deformation(vpmc_mc_desc &d, Simulation::Impl &I, float bound)
+{
+    for (int s = 0; s < 2; ++s) {
+        d.set[s].field.gq = (const float *)raw_ptr(I.gq[s]);
+        d.set[s].field.gq_bound = bound;
+        d.set[s].set_jac = (const float *)raw_ptr(I.set_jac[s]);
+    }
+}
+
 void Simulation::estimate_velocity(bool periodic)
 {
     Impl &I = *impl_;
@@ -391,6 +420,7 @@ void Simulation::estimate_velocity(bool periodic)
                                raw_ptr(I.totals), I.guide, I.glog2, raw_ptr(cdf_pos_),
                                raw_ptr(cdf_neg_), I.pts, I.cell_start, I.order, I.set_xy, I.nx,
                                I.ny, raw_ptr(I.vel_sign), periodic);
+    if (kernel_deformation) attach_deformation(d, I, deformation_bound);
     size_t wb = vpmc_mc_binned_workspace_bytes(&d);
     vcheck(vpmc_mc_estimate_binned(&d, I.ws.get(wb), wb, stream), "monte_carlo_bs");
     // vel = v+ - v- (dt = 0 leaves the positions unchanged)
@@ -417,6 +447,16 @@ void Simulation::step_cuda(bool periodic)
                                raw_ptr(I.totals), I.guide, I.glog2, raw_ptr(cdf_pos_),
                                raw_ptr(cdf_neg_), I.pts, I.cell_start, I.order, I.set_xy, I.nx,
                                I.ny, raw_ptr(I.vel_sign), periodic);
+    if (kernel_deformation) {
+        attach_deformation(d, I, deformation_bound);
+        // aux = 1: estimates at dst -/+ dx, F = update_jacobian_from5 of them
+        // before the move (src/cuda/simulation_step.cu:51-73)
+        I.vel_aux.resize(8 * (size_t)N);
+        d.aux = 1;
+        d.aux_dx = P_.dx;
+        d.vel_aux = (float *)raw_ptr(I.vel_aux);
+        d.jac = (float *)raw_ptr(particles_.F);
+    }
     size_t b = vpmc_step_workspace_bytes(&d);
     vcheck(vpmc_step(&d, N, (float *)particles_.d_p(), particles_.d_omega(), raw_ptr(sub_pos_),
                      raw_ptr(sub_neg_), (float *)particles_.d_v(), P_.dt, I.ws.get(b), b, stream),

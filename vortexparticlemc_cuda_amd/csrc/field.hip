@@ -77,6 +77,7 @@ int field_geom(const vpmc_field *f, float h_w, FieldGeom *g)
     float Lm = f->Lx > f->Ly ? f->Lx : f->Ly;
     const float stretch = f->gq ? (f->gq_bound > 0.f ? f->gq_bound : 2.f) : 1.f;
     if (!(stretch >= 1.f)) return VPMC_ERR_ARG;
+    g->rs1 = h_w * (1.f + 1e-5f) + 5e-7f * Lm;
     g->rs = stretch * h_w * (1.f + 1e-5f) + 5e-7f * Lm;
     g->rs2 = g->rs * g->rs;
     int rx = (int)ceilf(g->rs / g->cw), ry = (int)ceilf(g->rs / g->ch);
