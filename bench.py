@@ -69,10 +69,11 @@ def _event_pair(s):
 PLAIN_TUNE = tuple((c, s, t) for c in (8, 12, 16) for s in (1, 2, 4) for t in (64, 128, 256))
 
 
-def tune_plain(sim, steps=3):
+def tune_plain(sim, steps=3, deform=False):
     """(cells_per_hw, spl, eval_threads) with the shortest plain step on this
     workload; every candidate computes identical velocities (the knobs change
-    only the launch shape and the grid), timed with HIP events"""
+    only the launch shape and the grid), timed with HIP events.  deform: the
+    same for the step with aux targets and kernel deformation."""
     import torch
 
     from vortexparticlemc_cuda_amd.sim import Simulation
@@ -82,13 +83,15 @@ def tune_plain(sim, steps=3):
     for cph, spl, th in PLAIN_TUNE:
         try:
             t = Simulation(sim.P, device=sim.device, cells_per_hw=cph, mode=sim.mode,
-                           seed=sim.seed, spl=spl, eval_threads=th)
+                           seed=sim.seed, spl=spl, eval_threads=th, deform=deform)
             t.set_particles(pos, om)
-            run_steps(t, 1)
+            stepper = (lambda k: [t.step() for _ in range(k)]) if deform else \
+                (lambda k: run_steps(t, k))
+            stepper(1)
             a = torch.cuda.Event(enable_timing=True)
             b = torch.cuda.Event(enable_timing=True)
             a.record()
-            run_steps(t, steps)
+            stepper(steps)
             b.record()
             torch.cuda.synchronize()
         except Exception:   # stencil does not fit this workgroup / grid
@@ -195,8 +198,15 @@ def deform_bench(args, device, base):
     from vortexparticlemc_cuda_amd.sim import Simulation
 
     P = base.P
-    sim = Simulation(P, device=device, cells_per_hw=args.cells_per_hw, mode=base.mode,
-                     seed=base.seed, spl=args.spl, eval_threads=args.eval_threads, deform=True)
+    cph, spl, th = args.cells_per_hw, args.spl, args.eval_threads
+    tuned = None
+    if not args.no_tune:
+        (cph, spl, th), tune_ms = tune_plain(base, deform=True)
+        tuned = {"cells_per_hw": cph, "spl": spl, "eval_threads": th, "candidates_ms": tune_ms}
+        log("tuned deformed step: cells_per_hw=%d spl=%d eval_threads=%d (%.3f ms)"
+            % (cph, spl, th, tune_ms["%d,%d,%d" % (cph, spl, th)]))
+    sim = Simulation(P, device=device, cells_per_hw=cph, mode=base.mode, seed=base.seed,
+                     spl=spl, eval_threads=th, deform=True)
     sim.set_particles(base.pos.cpu().numpy(), base.omega.cpu().numpy())
     for _ in range(args.warmup):
         sim.step(P.N1, P.N2)
@@ -235,6 +245,7 @@ def deform_bench(args, device, base):
            "kernel_evals_per_launch": n_in, "candidates_per_launch": n_vis,
            "achieved_flops": FLOP_PER_EVAL * n_in / eval_s,
            "frac_of_fp32_peak": FLOP_PER_EVAL * n_in / eval_s / PEAK_FP32,
+           "tuned": tuned,
            "max_abs_det_minus_1": float((det - 1).abs().max()),
            "max_sigma_max_F": float(sv[:, 0].max())}
     log("deform", out)

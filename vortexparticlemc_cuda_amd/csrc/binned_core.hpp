@@ -282,10 +282,13 @@ __device__ __forceinline__ void tile_eval_def(const float4 *__restrict__ cand,
             }
         }
     };
-    for (int k = 0; k < n; k += 2) {   // n is a multiple of 4 (zero-weight padding)
+    for (int k = 0; k < n; k += 4) {   // n is a multiple of 4 (zero-weight padding)
         const float4 c0 = cand[k], e0 = cand_g[k], c1 = cand[k + 1], e1 = cand_g[k + 1];
+        const float4 c2 = cand[k + 2], e2 = cand_g[k + 2], c3 = cand[k + 3], e3 = cand_g[k + 3];
         one(c0, e0);
         one(c1, e1);
+        one(c2, e2);
+        one(c3, e3);
     }
 }
 
