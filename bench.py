@@ -615,13 +615,15 @@ def read_traffic(kernel, args):
     return None
 
 
-def cpu_baseline(Nx, hwr, N1, N2, target_s=12.0):
+def cpu_baseline(Nx, hwr, N1, N2, target_s=10.0):
     """the reference's monte_carlo_bs_cpu (its compiled src/cpu sources,
-    oracle/_ref) over a bounded subset of owners of the same workload,
-    single-threaded like its step_cpu; aux = 0 (the work the GPU path does)"""
+    oracle/_ref) over a bounded subset of owners of the same workload, aux = 0
+    (the work the GPU path does): single-threaded like its step_cpu, and over
+    the host's cores (OpenMP over owners, one generator per thread)"""
+    import os
+
     import oracle as O
     R = O.ref_lib()
-    kind = "reference"
     if R is None:
         return None
     x, y = O.lattice(Nx)
@@ -631,20 +633,40 @@ def cpu_baseline(Nx, hwr, N1, N2, target_s=12.0):
     rsh = O.rsh_of(Nx, hwr)
     hw = np.float32(hwr / Nx)
     n = x.size
-    probe = 40
-    vel = np.zeros(2 * probe, np.float32)
-    t0 = time.perf_counter()
-    R.ref_mc_velocities(n, x, y, w, rsh, rsh, 1.0, 1.0, hw, 1, N1, N2, 0, 5489, 0, probe, vel)
-    t1 = time.perf_counter() - t0
-    m = int(max(probe, min(n, target_s / max(t1 / probe, 1e-9))))
-    vel = np.zeros(2 * m, np.float32)
-    t0 = time.perf_counter()
-    R.ref_mc_velocities(n, x, y, w, rsh, rsh, 1.0, 1.0, hw, 1, N1, N2, 0, 5489, 0, m, vel)
-    t = time.perf_counter() - t0
-    return {"value": m / t, "unit": "particle-steps/s", "cores": 1, "kind": kind,
+
+    def timed(m, threads):
+        vel = np.zeros(2 * m, np.float32)
+        t0 = time.perf_counter()
+        if threads == 1:
+            R.ref_mc_velocities(n, x, y, w, rsh, rsh, 1.0, 1.0, hw, 1, N1, N2, 0, 5489, 0, m,
+                                vel)
+        else:
+            R.ref_mc_velocities_mt(n, x, y, w, rsh, rsh, 1.0, 1.0, hw, 1, N1, N2, 5489, 0, m,
+                                   threads, vel)
+        return time.perf_counter() - t0
+
+    def run(threads):
+        probe = 8 * threads
+        t1 = timed(probe, threads)
+        m = int(max(probe, min(n, target_s / max(t1 / probe, 1e-9))))
+        t = timed(m, threads)
+        return m, t
+
+    threads = max(1, min(16, os.cpu_count() or 1))
+    m1, t1 = run(1)
+    mt, tt = run(threads)
+    cpu = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            cpu = next(l.split(":", 1)[1].strip() for l in f if l.startswith("model name"))
+    except (OSError, StopIteration):
+        pass
+    return {"value": mt / tt, "unit": "particle-steps/s", "cores": threads, "kind": "reference",
             "sample": "reference monte_carlo_bs_cpu (src/cpu, compiled from source) for %d of "
-                      "%d owners, both signs, N1=N2=%d, aux=0, then scaled per particle; "
-                      "%.1f s single-threaded" % (m, n, N1, t)}
+                      "%d owners, both signs, N1=N2=%d, aux=0, OpenMP over owners on %d threads "
+                      "(%s), %.1f s, scaled per particle" % (mt, n, N1, threads, cpu, tt),
+            "single_thread": {"value": m1 / t1, "cores": 1,
+                              "sample": "%d owners, %.1f s" % (m1, t1)}}
 
 
 def main():

@@ -116,6 +116,9 @@ def ref_lib():
         R.ref_mc_velocities.argtypes = [C.c_int, f32p, f32p, f32p, C.c_int, C.c_int, C.c_float,
                                         C.c_float, C.c_float, C.c_int, C.c_int, C.c_int,
                                         C.c_int, C.c_uint32, C.c_int, C.c_int, f32p]
+        R.ref_mc_velocities_mt.argtypes = [C.c_int, f32p, f32p, f32p, C.c_int, C.c_int,
+                                           C.c_float, C.c_float, C.c_float, C.c_int, C.c_int,
+                                           C.c_int, C.c_uint32, C.c_int, C.c_int, C.c_int, f32p]
         R.ref_mc_aux.argtypes = [C.c_int, f32p, f32p, f32p, C.c_int, C.c_int, C.c_float,
                                  C.c_float, C.c_float, C.c_int, C.c_int, C.c_int, C.c_float,
                                  C.c_uint32, C.c_int, C.c_int, f32p]

@@ -5,6 +5,7 @@
 // oracle and by bench.py as the "reference" CPU baseline.  Nothing here
 // restates the algorithm: each function builds the reference's own types and
 // calls its functions.
+#include <omp.h>
 #include <cstdint>
 #include <random>
 #include <span>
@@ -181,6 +182,46 @@ void ref_mc_velocities(int n, const float *px, const float *py, const float *w, 
         MCBSEstimate vn = monte_carlo_bs_cpu(d, -1, aux, d, d, d, d, P, C, N1, N2, rng);
         vel[2 * (i - o0)] = vp.main.x - vn.main.x;
         vel[2 * (i - o0) + 1] = vp.main.y - vn.main.y;
+    }
+    delete S;
+}
+
+// ref_mc_velocities over nthreads OpenMP threads (owners split dynamically),
+// one std::mt19937 per thread seeded seed + 1000003 * thread: the same
+// estimator on other draws, for timing the reference's CPU path on all cores
+void ref_mc_velocities_mt(int n, const float *px, const float *py, const float *w, int rshx,
+                          int rshy, float Lx, float Ly, float h_w, int periodic, int N1, int N2,
+                          uint32_t seed, int o0, int o1, int nthreads, float *vel)
+{
+    float cum[2];
+    RefSet *S = build_set(n, px, py, w, rshx, rshy, Lx, Ly, cum);
+    ParticleSet P;
+    P.resize(n);
+    for (int i = 0; i < n; ++i) P.pos[i] = S->pos[i];
+    P.omega = S->w;
+    MCBSCtx C;
+    C.sub_pos = &S->sub_pos;
+    C.sub_neg = &S->sub_neg;
+    C.cdf_pos = &S->cdf_pos;
+    C.cdf_neg = &S->cdf_neg;
+    C.h_w = h_w;
+    C.periodic = periodic != 0;
+    C.Lx = Lx; C.Ly = Ly; C.InvLx = 1.f / Lx; C.InvLy = 1.f / Ly;
+    C.cum_pos = cum[0];
+    C.cum_neg = cum[1];
+    C.pos_view = make_view(*S, true, rshx, rshy, Lx, Ly);
+    C.neg_view = make_view(*S, false, rshx, rshy, Lx, Ly);
+#pragma omp parallel num_threads(nthreads)
+    {
+        std::mt19937 rng(seed + 1000003u * (uint32_t)omp_get_thread_num());
+#pragma omp for schedule(dynamic, 8)
+        for (int i = o0; i < o1; ++i) {
+            float2 d = P.pos[i];
+            MCBSEstimate vp = monte_carlo_bs_cpu(d, +1, 0, d, d, d, d, P, C, N1, N2, rng);
+            MCBSEstimate vn = monte_carlo_bs_cpu(d, -1, 0, d, d, d, d, P, C, N1, N2, rng);
+            vel[2 * (i - o0)] = vp.main.x - vn.main.x;
+            vel[2 * (i - o0) + 1] = vp.main.y - vn.main.y;
+        }
     }
     delete S;
 }

@@ -227,6 +227,11 @@ def test_mc_against_live_reference():
     R.ref_uniform_stream(seed, U.size, U)
     v = prob.velocities(prob.params(O.MODE_REFERENCE, N1, N2, hw, U=U), owners=n_own)
     assert np.array_equal(v, vel.reshape(-1, 2))
+    # the multi-threaded timing harness on one thread is the same call sequence
+    vm = np.zeros(2 * n_own, np.float32)
+    R.ref_mc_velocities_mt(x.size, x, y, w, rsh, rsh, 1.0, 1.0, hw, 1, N1, N2, seed, 0, n_own,
+                           1, vm)
+    assert np.array_equal(vm, vel)
 
 
 def test_consistent_estimator_is_unbiased():
