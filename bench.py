@@ -149,6 +149,27 @@ def copy_probe(nbytes, reps=3):
     log("copy probe: %d x %d bytes" % (reps, n * 4))
 
 
+def fma_probe(blocks=2048, iters=20000, reps=3):
+    """measured unpacked FP32 FMA rate (FLOP/s) of this device: the best of
+    reps launches of vpmc_probe_fma, timed with HIP events on the stream"""
+    import torch
+
+    from vortexparticlemc_cuda_amd import ops
+    from vortexparticlemc_cuda_amd._capi import check, lib
+    out = torch.zeros(blocks, dtype=torch.float32, device="cuda:0")
+    best = None
+    for _ in range(reps + 1):
+        a = torch.cuda.Event(enable_timing=True)
+        b = torch.cuda.Event(enable_timing=True)
+        a.record()
+        check(lib().vpmc_probe_fma(blocks, iters, ops._p(out), ops._stream()), "vpmc_probe_fma")
+        b.record()
+        torch.cuda.synchronize()
+        t = a.elapsed_time(b) / 1e3
+        best = t if best is None else min(best, t)   # the first launch warms up
+    return blocks * 256.0 * iters * 32.0 / best
+
+
 def count_evals(sim):
     import torch
 
@@ -764,6 +785,8 @@ def main():
     flop = FLOP_PER_EVAL * n_in + (FLOP_PER_SAMPLE * S if args.algo == "direct" else 0)
     achieved = flop / eval_avg_s
     traffic = read_traffic(eval_kernel, args)
+    fma_rate = fma_probe()
+    log("measured unpacked FP32 FMA rate %.3e FLOP/s" % fma_rate)
     if args.copy_probe:
         copy_probe(args.copy_probe << 20)
     deform = deform_bench(args, device, sim) if args.deform_steps > 0 else None
@@ -800,7 +823,9 @@ def main():
                      "candidates_per_launch": n_vis, "samples_per_launch": S,
                      "avg_launch_ms": eval_avg_s * 1e3,
                      "unpacked_fp32_issue_peak": PEAK_FP32 / 2,
-                     "frac_of_unpacked": achieved / (PEAK_FP32 / 2)},
+                     "frac_of_unpacked": achieved / (PEAK_FP32 / 2),
+                     "measured_unpacked_fma_flops": fma_rate,
+                     "frac_of_measured_unpacked": achieved / fma_rate},
         "kernel_evals_per_s": n_in / eval_avg_s,
         "estimator_ms": est_avg_s * 1e3,
         "estimator_fraction_of_step": est_avg_s / (elapsed / args.steps),

@@ -181,6 +181,11 @@ public:
     hipStream_t stream = nullptr;
     int cg_iterations = 0;
     double cg_rel_residual = 0.0;
+    // reinitialisation (README.md:319-321, 416-420): particles to the
+    // nx_l x ny_l cell-centred lattice (0: ResolutionX x ResolutionY),
+    // strengths re-fitted to the current field (deformed when
+    // kernel_deformation), F = I
+    void reinitialize(int nx_l = 0, int ny_l = 0, float tol = 1e-6f, int max_iter = 1000);
     // velocity estimate at the current positions without advancing
     void estimate_velocity(bool periodic);
     int subset_count(int sign) const { return sign > 0 ? cnt_pos_ : cnt_neg_; }

@@ -37,6 +37,12 @@ enum vpmc_status {
 const char *vpmc_status_string(int status);
 int vpmc_last_hip_error(void);          /* hipError_t of the last VPMC_ERR_HIP */
 int vpmc_version(void);                 /* major*10000 + minor*100 + patch     */
+/* Measurement aid, no counterpart in the reference: blocks x 256 lanes each
+   run iters x 16 dependent-chain FP32 FMAs (32 * iters FLOP per lane) on
+   the stream, for reading the unpacked FP32 issue rate of the device the
+   rooflines are quoted against; out (blocks floats) is written only in a
+   case that cannot occur. */
+int vpmc_probe_fma(int blocks, int iters, float *out, void *stream);
 
 /* ---------------------------------------------------------------------------
  * Spatial hash on the reference's grid (rsh = ResolutionX / int(hwr) / 2,
@@ -422,6 +428,22 @@ int vpmc_cg_rbf_f64(int n, const float *pos_xy, const double *b, float *w_out, d
                     float h_w, float Lx, float Ly, int cells_per_hw, double tol, int max_iter,
                     int *iters, double *rel_res, void *workspace, size_t workspace_bytes,
                     void *stream);
+
+/* Reinitialisation, described in the reference's README.md:319-321 and
+   :416-420 (step 16: positions to a uniform grid, Jacobians to I, strengths
+   re-fitted by the RBF solve) with no code behind it.  The current field
+   sum_j w_j W(|F_j^-1 (x - x_j)|) (jac NULL: F = I; signed strengths, FP64
+   sums, deformed search radius 2 h_w) is evaluated at the nx_l x ny_l
+   cell-centred lattice ((i+1/2) Lx/nx_l, (j+1/2) Ly/ny_l), index i*ny_l + j,
+   written to pos_out (2 nx_l ny_l floats); w_out receives strengths on that
+   lattice from the FP64 MINRES solve of vpmc_cg_rbf_f64 to relative
+   residual tol.  The caller resets its F to I. */
+size_t vpmc_reinit_workspace_bytes(int n, int nx_l, int ny_l, float h_w, float Lx, float Ly,
+                                   int cells_per_hw, int deformed);
+int vpmc_reinit(int n, const float *pos_xy, const float *w, const float *jac, float h_w,
+                float Lx, float Ly, int cells_per_hw, int nx_l, int ny_l, double tol,
+                int max_iter, float *pos_out, float *w_out, int *iters, double *rel_res,
+                void *workspace, size_t workspace_bytes, void *stream);
 
 /* ---------------------------------------------------------------------------
  * FP64 direct sums (error oracles on the device).

@@ -96,6 +96,9 @@ int main(int argc, char **argv)
             (void)hipDeviceSynchronize();
             dump(out + "/pos_def2.bin", sd.particles_.pos.to_host());
             dump(out + "/F_def2.bin", sd.particles_.F.to_host());
+            sd.reinitialize(0, 0, 1e-6f, 1000);
+            dump(out + "/pos_reinit.bin", sd.particles_.pos.to_host());
+            dump(out + "/omega_reinit.bin", sd.particles_.omega.to_host());
         }
 
         sim.test_vorticity_grid(16);

@@ -80,3 +80,7 @@ def test_cxx_api_end_to_end(device, tmp_path):
     F = rd("F_def2.bin").reshape(-1, 4)[:N]
     assert np.array_equal(F, sd.jac[:N].cpu().numpy())
     assert not np.array_equal(F, np.tile([1, 0, 0, 1], (N, 1)).astype(np.float32))
+    # reinitialize() == the Python mirror's reinit (same entry point, same state)
+    sd.reinit(tol=1e-6)
+    assert np.array_equal(rd("pos_reinit.bin").reshape(-1, 2)[:sd.N], sd.pos.cpu().numpy())
+    assert np.array_equal(rd("omega_reinit.bin")[:sd.N], sd.omega.cpu().numpy())

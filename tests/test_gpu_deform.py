@@ -144,14 +144,15 @@ def _W(q, hw):
                     np.where(q < 1, 2 * sig * (1 - q) ** 3, 0.0))
 
 
-def _brute_deformed(qx, qy, px, py, w, F, hw):
-    """FP64 sum_j |w_j| W(|F_j^-1 (x_j - q)|) over the 3 x 3 periodic images"""
+def _brute_deformed(qx, qy, px, py, w, F, hw, signed=False):
+    """FP64 sum_j |w_j| W(|F_j^-1 (x_j - q)|) over the 3 x 3 periodic images
+    (w_j itself when signed)"""
     qx = qx.astype(np.float64)[:, None]
     qy = qy.astype(np.float64)[:, None]
     F = F.astype(np.float64)
     a, b, c, d = (F[None, :, k] for k in range(4))
     det = a * d - b * c
-    aw = np.abs(w.astype(np.float64))
+    aw = w.astype(np.float64) if signed else np.abs(w.astype(np.float64))
     out = np.zeros(qx.shape[0])
     for ix in (-1, 0, 1):
         for iy in (-1, 0, 1):

@@ -501,3 +501,99 @@ int vpmc_cg_rbf_f64(int n, const float *pos_xy, const double *b, float *w_out, d
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Reinitialisation (README.md:319-321, 416-420; the reference describes it
+// and has no code for it): the current particles' signed field -- deformed
+// kernels when F is given -- evaluated in FP64 at a fresh cell-centred
+// lattice, then lattice strengths fitted to it by the FP64-RHS RBF solve.
+
+namespace vpmc {
+
+__global__ void k_reinit_rhs(FieldD f, double h_w, int nxl, int nyl, float Lx, float Ly,
+                             float2 *__restrict__ pos_out, double *__restrict__ b)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nxl * nyl) return;
+    const int ix = i / nyl, iy = i - ix * nyl;   // index i*Ny + j (x-major)
+    const float px = ((float)ix + 0.5f) * (Lx / (float)nxl);
+    const float py = ((float)iy + 0.5f) * (Ly / (float)nyl);
+    pos_out[i] = make_float2(px, py);
+    b[i] = field_sum_f64(f, nullptr, px, py, h_w);
+}
+
+struct ReinitBufs {
+    float *pts, *gq;
+    int *cell_start, *order;
+    double *b;
+    void *rest;
+    size_t rest_bytes;
+};
+
+static int carve_reinit(int n, int n_new, int nx, int ny, bool def, void *ws, size_t bytes,
+                        ReinitBufs *B)
+{
+    Carve cv(ws, bytes);
+    const size_t m = (size_t)(n > 0 ? n : 1);
+    B->pts = cv.take<float>(4 * m);
+    B->gq = def ? cv.take<float>(4 * m) : nullptr;
+    B->cell_start = cv.take<int>((size_t)nx * ny + 1);
+    B->order = cv.take<int>(m);
+    B->b = cv.take<double>((size_t)(n_new > 0 ? n_new : 1));
+    B->rest_bytes = std::max(vpmc_field_workspace_bytes(n, nx, ny),
+                             vpmc_cg_workspace_bytes(n_new, nx, ny));
+    B->rest = cv.take<char>(B->rest_bytes);
+    if (!B->pts || (def && !B->gq) || !B->cell_start || !B->order || !B->b || !B->rest)
+        return VPMC_ERR_WORKSPACE;
+    return VPMC_OK;
+}
+
+}  // namespace vpmc
+
+extern "C" {
+
+size_t vpmc_reinit_workspace_bytes(int n, int nx_l, int ny_l, float h_w, float Lx, float Ly,
+                                   int cells_per_hw, int deformed)
+{
+    int nx, ny;
+    if (nx_l <= 0 || ny_l <= 0 || vpmc_field_grid(h_w, Lx, Ly, cells_per_hw, &nx, &ny) != VPMC_OK)
+        return 0;
+    const int n_new = nx_l * ny_l;
+    const size_t m = (size_t)(n > 0 ? n : 1), nn = (size_t)n_new;
+    return align256(16 * m) * (deformed ? 2 : 1) + align256(((size_t)nx * ny + 1) * 4) +
+           align256(4 * m) + align256(8 * nn) +
+           std::max(vpmc_field_workspace_bytes(n, nx, ny), vpmc_cg_workspace_bytes(n_new, nx, ny)) +
+           4096;
+}
+
+int vpmc_reinit(int n, const float *pos_xy, const float *w, const float *jac, float h_w,
+                float Lx, float Ly, int cells_per_hw, int nx_l, int ny_l, double tol,
+                int max_iter, float *pos_out, float *w_out, int *iters, double *rel_res,
+                void *workspace, size_t workspace_bytes, void *stream)
+{
+    if (n < 0 || nx_l <= 0 || ny_l <= 0 || !(h_w > 0.f) || !pos_out || !w_out ||
+        (n > 0 && (!pos_xy || !w)))
+        return VPMC_ERR_ARG;
+    if ((long long)nx_l * ny_l >= (1LL << 31)) return VPMC_ERR_ARG;
+    hipStream_t s = as_stream(stream);
+    int nx, ny;
+    VPMC_TRY(vpmc_field_grid(h_w, Lx, Ly, cells_per_hw, &nx, &ny));
+    const int n_new = nx_l * ny_l;
+    ReinitBufs B;
+    VPMC_TRY(carve_reinit(n, n_new, nx, ny, jac != nullptr, workspace, workspace_bytes, &B));
+    // the field of every particle with its signed strength
+    VPMC_TRY(field_build_impl(nx, ny, Lx, Ly, n, nullptr, nullptr, pos_xy, w, 0, B.pts,
+                              B.cell_start, B.order, nullptr, B.rest, B.rest_bytes, s, jac,
+                              B.gq, nullptr));
+    vpmc_field fh{nx, ny, Lx, Ly, n, B.pts, B.cell_start, B.order, B.gq, 0.f};
+    FieldGeom g;
+    VPMC_TRY(field_geom(&fh, h_w, &g));
+    FieldD f{(const float4 *)B.pts, B.cell_start, g, (const float4 *)B.gq};
+    hipLaunchKernelGGL(k_reinit_rhs, dim3((n_new + 255) / 256), dim3(256), 0, s, f,
+                       (double)h_w, nx_l, ny_l, Lx, Ly, (float2 *)pos_out, B.b);
+    VPMC_CHECK_LAUNCH();
+    return cg_rbf_impl(n_new, pos_out, nullptr, B.b, w_out, nullptr, h_w, Lx, Ly, cells_per_hw,
+                       tol, max_iter, 1, iters, rel_res, B.rest, B.rest_bytes, stream);
+}
+
+}  // extern "C"

@@ -235,6 +235,34 @@ void Simulation::set_particles(const std::vector<float2> &pos, const std::vector
     do_spatial_hashing();
 }
 
+void Simulation::reinitialize(int nx_l, int ny_l, float tol, int max_iter)
+{
+    // README.md:319-321, 416-420: positions to a uniform grid, F to I,
+    // strengths re-fitted to the current field
+    if (nx_l <= 0) nx_l = P_.ResolutionX;
+    if (ny_l <= 0) ny_l = P_.ResolutionY;
+    const int N = particles_.N_cur;
+    const int n_new = nx_l * ny_l;
+    const float *jac = nullptr;
+    if (kernel_deformation && particles_.F.size() >= (size_t)N)
+        jac = (const float *)raw_ptr(particles_.F);
+    dvec<float2> pos_out(n_new);
+    dvec<float> w_out(n_new);
+    size_t b = vpmc_reinit_workspace_bytes(N, nx_l, ny_l, P_.h_w, P_.Lx, P_.Ly, cells_per_hw,
+                                           jac != nullptr);
+    Impl &I = *impl_;
+    int it = 0;
+    double rr = 0.0;
+    int st = vpmc_reinit(N, (const float *)particles_.d_p(), particles_.d_omega(), jac, P_.h_w,
+                         P_.Lx, P_.Ly, cells_per_hw, nx_l, ny_l, tol, max_iter,
+                         (float *)raw_ptr(pos_out), raw_ptr(w_out), &it, &rr, I.ws.get(b), b,
+                         stream);
+    if (st != VPMC_OK && st != VPMC_ERR_NOT_CONVERGED) vcheck(st, "reinitialize");
+    cg_iterations = it;
+    cg_rel_residual = rr;
+    set_particles(pos_out.to_host(), w_out.to_host());   // F = I, subsets, fields
+}
+
 void Simulation::prepare_rhs_b()
 {
     // src/vortex_particle_mc.cu:237-260: b_i = omega_TG(x_i)

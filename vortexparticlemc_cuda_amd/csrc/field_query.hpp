@@ -100,7 +100,9 @@ __device__ __forceinline__ float field_sum(const FieldD &f, float px, float py, 
     return 2.f * a1 - 8.f * a2;
 }
 
-// FP64 variant with weights from a separate array (RBF residual refresh)
+// FP64 variant with weights from a separate array (RBF residual refresh), or
+// with the records' own weights when wts is NULL; deformed records (f.gq)
+// use r^2 = d^T G d
 __device__ __forceinline__ double field_sum_f64(const FieldD &f, const double *__restrict__ wts,
                                                 double px, double py, double h_w)
 {
@@ -135,7 +137,11 @@ __device__ __forceinline__ double field_sum_f64(const FieldD &f, const double *_
                 float4 c = f.pts[k];
                 double dx = (double)c.x - qx, dy = (double)c.y - sy[p];
                 double r2 = dx * dx + dy * dy;
-                if (r2 < hw2) acc += wts[k] * spline_d(sqrt(r2), h_w);
+                if (f.gq) {
+                    const float4 G = f.gq[k];
+                    r2 = (double)G.x * dx * dx + (double)G.y * dx * dy + (double)G.z * dy * dy;
+                }
+                if (r2 < hw2) acc += (wts ? wts[k] : (double)c.z) * spline_d(sqrt(r2), h_w);
             }
     }
     return acc;

@@ -190,6 +190,28 @@ def field_query(fb, h_w, pts):
     return out[:pts.shape[0]]
 
 
+def reinit(pos, w, h_w, nx, ny, jac=None, Lx=1.0, Ly=1.0, tol=1e-6, max_iter=1000,
+           cells_per_hw=8):
+    """vpmc_reinit: the field of (pos, w[, F]) at the nx x ny cell-centred
+    lattice, strengths fitted there.  Returns (lattice positions, strengths,
+    iterations, relative FP64 residual, status)."""
+    pos = _xy(pos)
+    n = pos.shape[0]
+    ws = workspace(lib().vpmc_reinit_workspace_bytes(n, nx, ny, h_w, Lx, Ly, cells_per_hw,
+                                                     int(jac is not None)), pos.device)
+    pos_out = torch.empty((nx * ny, 2), dtype=torch.float32, device=pos.device)
+    w_out = torch.empty(nx * ny, dtype=torch.float32, device=pos.device)
+    if jac is not None:
+        jac = jac.to(torch.float32).contiguous()
+    it, rr = C.c_int(), C.c_double()
+    st = lib().vpmc_reinit(n, _p(pos), _p(w.to(torch.float32).contiguous()), _p(jac), h_w, Lx,
+                           Ly, cells_per_hw, nx, ny, tol, max_iter, _p(pos_out), _p(w_out),
+                           C.byref(it), C.byref(rr), _p(ws), ws.numel(), _stream())
+    if st not in (0, 5, 6):   # ok, CG breakdown, not converged: reported, not raised
+        check(st, "vpmc_reinit")
+    return pos_out, w_out, it.value, rr.value, st
+
+
 def advect(pos, vel_sign, dt, Lx=1.0, Ly=1.0, periodic=True):
     n = pos.shape[0]
     vel = torch.empty((max(n, 1), 2), dtype=torch.float32, device=pos.device)

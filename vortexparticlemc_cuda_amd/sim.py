@@ -217,6 +217,23 @@ class Simulation:
               "vpmc_update_jacobian")
         return jac[:self.N]
 
+    def reinit(self, nx=None, ny=None, tol=1e-6, max_iter=1000):
+        """reinitialisation (README.md:319-321, 416-420): particles to the
+        nx x ny cell-centred lattice (default: ResolutionX x ResolutionY),
+        strengths re-fitted to the current (deformed) field by the RBF solve,
+        F reset to I; subsets and fields rebuilt.  Returns (iterations,
+        relative FP64 residual, status)."""
+        P = self.P
+        nx = P.ResolutionX if nx is None else int(nx)
+        ny = P.ResolutionY if ny is None else int(ny)
+        pos, w, it, rr, st = ops.reinit(self.pos, self.omega, float(P.h_w), nx, ny,
+                                        jac=self.jac[:self.N] if self.deform else None,
+                                        Lx=float(P.Lx), Ly=float(P.Ly), tol=tol,
+                                        max_iter=max_iter, cells_per_hw=self.cells_per_hw)
+        self.jac = None   # _alloc makes F = I for the new particle count
+        self.set_particles(pos.cpu().numpy(), w.cpu().numpy())
+        return it, rr, st
+
     def set_jacobian(self, jac):
         """the deformation F per particle, (N, 4) row-major (F.x F.y; F.z F.w)"""
         if not self.deform:
