@@ -429,6 +429,21 @@ int vpmc_cg_rbf_f64(int n, const float *pos_xy, const double *b, float *w_out, d
                     int *iters, double *rel_res, void *workspace, size_t workspace_bytes,
                     void *stream);
 
+/* vpmc_cg_rbf_f64 for particles on the cell-centred nx_l x ny_l lattice
+   (particle i*ny_l + j near ((i+1/2) Lx/nx_l, (j+1/2) Ly/ny_l); the matvec
+   uses pos_xy as given): MINRES preconditioned by the lattice's circulant
+   Gram matrix, |lambda(k)| in the lattice Fourier basis (floored at
+   1e-10 lambda(0)), applied by direct DFTs along lattice lines (nx_l, ny_l
+   <= 2048).  Replaces conjugate_gradient_rbf (src/cuda/cg_rbf_init.cu:53-116)
+   where the reference initialises on its lattice; reaches 1e-10 relative
+   residuals that the unpreconditioned solve does not. */
+size_t vpmc_cg_lattice_workspace_bytes(int nx_l, int ny_l, float h_w, float Lx, float Ly,
+                                       int cells_per_hw);
+int vpmc_cg_rbf_lattice(int nx_l, int ny_l, const float *pos_xy, const double *b, float *w_out,
+                        double *w64_out, float h_w, float Lx, float Ly, int cells_per_hw,
+                        double tol, int max_iter, int *iters, double *rel_res, void *workspace,
+                        size_t workspace_bytes, void *stream);
+
 /* Reinitialisation, described in the reference's README.md:319-321 and
    :416-420 (step 16: positions to a uniform grid, Jacobians to I, strengths
    re-fitted by the RBF solve) with no code behind it.  The current field

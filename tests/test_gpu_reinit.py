@@ -80,3 +80,44 @@ def test_simulation_reinit_resets_F_and_steps(device):
     v = sim.step(16, 16).cpu().numpy()
     assert np.isfinite(v).all()
     assert not np.allclose(sim.jac[:sim.N].cpu().numpy(), [1, 0, 0, 1])
+
+
+@pytest.mark.parametrize("Nx,Ny,hwr", [(32, 32, 4), (30, 24, 4), (48, 48, 8)])
+def test_lattice_preconditioned_solve_matches_dense(device, Nx, Ny, hwr):
+    """vpmc_cg_rbf_lattice on a smooth FP64 field with a sharp layer (double
+    shear): FP64 residual 1e-11 in few iterations, strengths == the dense
+    FP64 solve of the Gram matrix of the same FP32 positions"""
+    import torch
+
+    from vortexparticlemc_cuda_amd import ops
+    from vortexparticlemc_cuda_amd.sim import lattice_positions
+    from vortexparticlemc_cuda_amd.workloads import double_shear
+    print("[case] lattice solve %dx%d hwr=%d" % (Nx, Ny, hwr), flush=True)
+    ix, iy = np.meshgrid(np.arange(Nx), np.arange(Ny), indexing="ij")
+    pos = np.stack([((ix + 0.5) / Nx).ravel(), ((iy + 0.5) / Ny).ravel()], 1).astype(np.float32)
+    if Nx == Ny:
+        assert np.array_equal(pos, lattice_positions(Nx))
+    hw = hwr / Nx
+    b = double_shear(pos[:, 0], pos[:, 1])
+    pt = torch.as_tensor(pos, device=device)
+    w, it, rr, st = ops.cg_rbf(pt, torch.as_tensor(b, device=device), hw, tol=1e-11,
+                               max_iter=500, lattice=(Nx, Ny))
+    assert st == 0 and rr < 1e-11, (st, rr, it)
+    assert it < 100, it
+    w0, it0, rr0, st0 = ops.cg_rbf(pt, torch.as_tensor(b, device=device), hw, tol=1e-11,
+                                   max_iter=500)
+    assert rr0 > 1e-11 or it0 > it   # the plain solve is slower or stalls
+    p64 = pos.astype(np.float64)
+    d = p64[:, None, :] - p64[None, :, :]
+    d -= np.round(d)
+    q = np.hypot(d[..., 0], d[..., 1]) / hw
+    sig = 40.0 / (7 * np.pi * hw * hw)
+    A = np.where(q <= 0.5, sig * (1 - 6 * q ** 2 + 6 * q ** 3),
+                 np.where(q < 1, 2 * sig * (1 - q) ** 3, 0.0))
+    wd = np.linalg.solve(A, b)
+    # the strengths come back rounded to FP32 (the FP64 iterate's residual is
+    # rr); on the 48 x 48, hwr = 8 lattice the shear layer is thinner than a
+    # blob and the strengths oscillate at ~15 |b|, so the rounding alone
+    # leaves |A w32 - b| / |b| ~ 4e-6
+    wg = w.cpu().numpy().astype(np.float64)
+    assert rel_l2(wg, wd) < 1e-5, rel_l2(wg, wd)

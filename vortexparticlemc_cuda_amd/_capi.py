@@ -97,6 +97,10 @@ def lib():
     L.vpmc_field_build.argtypes = [ci, ci, cf, cf, ci, vp, vp, vp, ci, vp, vp, vp, vp, vp, csz,
                                    vp]
     L.vpmc_probe_fma.argtypes = [ci, ci, vp, vp]
+    L.vpmc_cg_lattice_workspace_bytes.restype = csz
+    L.vpmc_cg_lattice_workspace_bytes.argtypes = [ci, ci, cf, cf, cf, ci]
+    L.vpmc_cg_rbf_lattice.argtypes = [ci, ci, vp, vp, vp, vp, cf, cf, cf, ci, C.c_double, ci,
+                                      C.POINTER(ci), C.POINTER(C.c_double), vp, csz, vp]
     L.vpmc_reinit_workspace_bytes.restype = csz
     L.vpmc_reinit_workspace_bytes.argtypes = [ci, ci, ci, cf, cf, cf, ci, ci]
     L.vpmc_reinit.argtypes = [ci, vp, vp, vp, cf, cf, cf, ci, ci, ci, C.c_double, ci, vp, vp,
@@ -169,7 +173,8 @@ EXPORTS = [
     "vpmc_field_build_jac", "vpmc_field_query", "vpmc_mc_estimate", "vpmc_mc_binned_workspace_bytes",
     "vpmc_mc_estimate_binned", "vpmc_advect", "vpmc_update_jacobian", "vpmc_step_workspace_bytes",
     "vpmc_step", "vpmc_rbf_workspace_bytes", "vpmc_rbf_matvec", "vpmc_rbf_matvec_f64", "vpmc_rbf_matvec_particles",
-    "vpmc_cg_rbf", "vpmc_cg_rbf_f64", "vpmc_cg_workspace_bytes", "vpmc_reinit_workspace_bytes",
+    "vpmc_cg_rbf", "vpmc_cg_rbf_f64", "vpmc_cg_workspace_bytes", "vpmc_cg_lattice_workspace_bytes",
+    "vpmc_cg_rbf_lattice", "vpmc_reinit_workspace_bytes",
     "vpmc_reinit", "vpmc_direct_sum_eps", "vpmc_direct_sum_blob",
     "vpmc_ensemble_workspace_bytes", "vpmc_ensemble_init", "vpmc_ensemble_step",
     "vpmc_ensemble_set_step", "vpmc_ensemble_graph_create", "vpmc_graph_launch",

@@ -250,17 +250,213 @@ __global__ void k_minres_dir(int n, const T *__restrict__ v, const T *__restrict
     x[i] += phi * wn;
 }
 
+// ---- circulant preconditioner for particles on a uniform periodic lattice ----
+// On the cell-centred nx_l x ny_l lattice, particle i*ny_l + j at
+// ((i+1/2) hx, (j+1/2) hy), the Gram matrix is block-circulant with circulant
+// blocks (up to the FP32 rounding of the positions): its eigenvectors are the
+// lattice Fourier modes, with eigenvalues
+//   lambda(k1, k2) = sum_o W(|(o1 hx, o2 hy)|) cos(2 pi k1 o1 / nx_l) cos(2 pi k2 o2 / ny_l)
+// (o over the offsets within h_w; W is even in each component, so the sines
+// cancel).  lambda changes sign (the matrix is indefinite) and is tiny at
+// high |k|, which is where unpreconditioned MINRES stalls.  MINRES takes the
+// SPD preconditioner M = F^-1 diag(max(|lambda|, 1e-10 lambda(0))) F, applied
+// with four passes of a direct DFT along lattice lines (FP64, twiddles in LDS).
+
+constexpr int DFT_THREADS = 256;
+constexpr int DFT_LDS_BYTES = 65536;
+constexpr int DFT_MAX_LINE = DFT_LDS_BYTES / 32;   // twiddles + one line
+
+__global__ void k_lattice_lambda(int nxl, int nyl, double hx, double hy, double h_w, int rx,
+                                 int ry, double *__restrict__ lam)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nxl * nyl) return;
+    const int k1 = i / nyl, k2 = i - k1 * nyl;
+    double s = 0.0;
+    for (int o1 = -rx; o1 <= rx; ++o1) {
+        const long long m1 = ((long long)k1 * (o1 < 0 ? -o1 : o1)) % nxl;
+        const double c1 = cospi(2.0 * (double)m1 / (double)nxl);
+        for (int o2 = -ry; o2 <= ry; ++o2) {
+            const double r = hypot(o1 * hx, o2 * hy);
+            if (!(r < h_w)) continue;
+            const long long m2 = ((long long)k2 * (o2 < 0 ? -o2 : o2)) % nyl;
+            s += spline_d(r, h_w) * c1 * cospi(2.0 * (double)m2 / (double)nyl);
+        }
+    }
+    lam[i] = s;
+}
+
+// scale[i] = 1 / (max(|lambda_i|, 1e-10 lambda(0)) nx ny); lambda(0) is the
+// largest eigenvalue (W >= 0)
+__global__ void k_lattice_scale(int n, const double *__restrict__ lam, double *__restrict__ scale)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double floor_ = 1e-10 * fabs(lam[0]);
+    scale[i] = 1.0 / (fmax(fabs(lam[i]), floor_) * (double)n);
+}
+
+// out[k * rows + r] = sum_j in[r * L + j] exp(sign 2 pi i k j / L) for the
+// rows lines of length L (REAL_IN: in holds doubles; REAL_OUT: the real
+// part only).  A workgroup takes `per` lines; LDS: L twiddles + per lines.
+template <bool REAL_IN, bool REAL_OUT>
+__global__ void __launch_bounds__(DFT_THREADS)
+    k_dft_lines(int rows, int L, int per, double sign, const void *__restrict__ in,
+                void *__restrict__ out)
+{
+    extern __shared__ double2 dft_lds[];
+    double2 *tw = dft_lds, *line = dft_lds + L;
+    for (int m = threadIdx.x; m < L; m += DFT_THREADS) {
+        double sn, cs;
+        sincospi(2.0 * (double)m / (double)L, &sn, &cs);
+        tw[m] = make_double2(cs, sign * sn);
+    }
+    const int r0 = blockIdx.x * per;
+    const int nr = min(per, rows - r0);
+    for (int t = threadIdx.x; t < nr * L; t += DFT_THREADS) {
+        const size_t src = (size_t)r0 * L + t;
+        line[t] = REAL_IN ? make_double2(((const double *)in)[src], 0.0)
+                          : ((const double2 *)in)[src];
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < nr * L; t += DFT_THREADS) {
+        const int k = t / nr, rr = t - k * nr;   // consecutive lanes: consecutive lines
+        const double2 *ln = line + (size_t)rr * L;
+        double ar = 0.0, ai = 0.0;
+        int idx = 0;
+        for (int j = 0; j < L; ++j) {
+            const double2 x = ln[j], w = tw[idx];
+            ar = fma(x.x, w.x, fma(-x.y, w.y, ar));
+            ai = fma(x.x, w.y, fma(x.y, w.x, ai));
+            idx += k;
+            if (idx >= L) idx -= L;
+        }
+        const size_t dst = (size_t)k * rows + r0 + rr;
+        if (REAL_OUT) ((double *)out)[dst] = ar;
+        else ((double2 *)out)[dst] = make_double2(ar, ai);
+    }
+}
+
+__global__ void k_to_lattice(int n, const int *__restrict__ order, const double *__restrict__ x,
+                             double *__restrict__ lat)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) lat[order[t]] = x[t];
+}
+__global__ void k_from_lattice(int n, const int *__restrict__ order,
+                               const double *__restrict__ lat, double *__restrict__ x)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) x[t] = lat[order[t]];
+}
+__global__ void k_scale_spectrum(int n, const double *__restrict__ scale, double2 *__restrict__ c)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double a = scale[i];
+    c[i] = make_double2(c[i].x * a, c[i].y * a);
+}
+
+struct LatticePc {
+    int nxl = 0, nyl = 0;
+    const int *order = nullptr;   // field position -> particle (= lattice) index
+    double *scale = nullptr;      // [k1 * ny + k2]
+    double *lat = nullptr;
+    double *z = nullptr;          // MINRES: M^-1 of the newest Lanczos vector
+    double2 *c1 = nullptr, *c2 = nullptr;
+};
+
+inline bool lattice_pc_ok(int nxl, int nyl)
+{
+    return nxl >= 1 && nyl >= 1 && nxl <= DFT_MAX_LINE && nyl <= DFT_MAX_LINE;
+}
+
+static size_t lattice_pc_bytes(int nxl, int nyl)
+{
+    const size_t n = (size_t)nxl * nyl;
+    return 3 * align256(8 * n) + 2 * align256(16 * n);
+}
+
+static int lattice_dft(int rows, int L, bool real_in, bool real_out, double sign, const void *in,
+                       void *out, hipStream_t s)
+{
+    int per = DFT_LDS_BYTES / (16 * L) - 1;
+    per = per < 1 ? 1 : (per > 16 ? 16 : per);
+    const size_t lds = (size_t)16 * L * (1 + per);
+    const dim3 grid((unsigned)((rows + per - 1) / per)), block(DFT_THREADS);
+    if (real_in)
+        hipLaunchKernelGGL((k_dft_lines<true, false>), grid, block, lds, s, rows, L, per, sign,
+                           in, out);
+    else if (real_out)
+        hipLaunchKernelGGL((k_dft_lines<false, true>), grid, block, lds, s, rows, L, per, sign,
+                           in, out);
+    else
+        hipLaunchKernelGGL((k_dft_lines<false, false>), grid, block, lds, s, rows, L, per, sign,
+                           in, out);
+    VPMC_CHECK_LAUNCH();
+    return VPMC_OK;
+}
+
+static int lattice_pc_init(LatticePc &P, int nxl, int nyl, float Lx, float Ly, float h_w,
+                           const int *order, void *ws, size_t bytes, hipStream_t s)
+{
+    Carve cv(ws, bytes);
+    const size_t n = (size_t)nxl * nyl;
+    P.nxl = nxl;
+    P.nyl = nyl;
+    P.order = order;
+    P.scale = cv.take<double>(n);
+    P.lat = cv.take<double>(n);
+    P.z = cv.take<double>(n);
+    P.c1 = cv.take<double2>(n);
+    P.c2 = cv.take<double2>(n);
+    if (!P.scale || !P.lat || !P.z || !P.c1 || !P.c2) return VPMC_ERR_WORKSPACE;
+    const double hx = (double)Lx / nxl, hy = (double)Ly / nyl;
+    const int rx = (int)floor((double)h_w / hx), ry = (int)floor((double)h_w / hy);
+    const unsigned gb = (unsigned)((n + 255) / 256);
+    hipLaunchKernelGGL(k_lattice_lambda, dim3(gb), dim3(256), 0, s, nxl, nyl, hx, hy,
+                       (double)h_w, rx, ry, P.lat);
+    hipLaunchKernelGGL(k_lattice_scale, dim3(gb), dim3(256), 0, s, (int)n,
+                       (const double *)P.lat, P.scale);
+    VPMC_CHECK_LAUNCH();
+    return VPMC_OK;
+}
+
+// out = M^-1 in, both in the field's cell order
+static int lattice_pc_apply(const LatticePc &P, const double *in, double *out, hipStream_t s)
+{
+    const int n = P.nxl * P.nyl;
+    const unsigned gb = (unsigned)((n + 255) / 256);
+    hipLaunchKernelGGL(k_to_lattice, dim3(gb), dim3(256), 0, s, n, P.order, in, P.lat);
+    VPMC_CHECK_LAUNCH();
+    // [ix][iy] -> [k2][ix] -> [k1][k2], scale, -> [iy][k1] -> [ix][iy]
+    VPMC_TRY(lattice_dft(P.nxl, P.nyl, true, false, -1.0, P.lat, P.c1, s));
+    VPMC_TRY(lattice_dft(P.nyl, P.nxl, false, false, -1.0, P.c1, P.c2, s));
+    hipLaunchKernelGGL(k_scale_spectrum, dim3(gb), dim3(256), 0, s, n,
+                       (const double *)P.scale, P.c2);
+    VPMC_CHECK_LAUNCH();
+    VPMC_TRY(lattice_dft(P.nxl, P.nyl, false, false, 1.0, P.c2, P.c1, s));
+    VPMC_TRY(lattice_dft(P.nyl, P.nxl, false, true, 1.0, P.c1, P.lat, s));
+    hipLaunchKernelGGL(k_from_lattice, dim3(gb), dim3(256), 0, s, n, P.order,
+                       (const double *)P.lat, out);
+    VPMC_CHECK_LAUNCH();
+    return VPMC_OK;
+}
+
 // MINRES (Paige & Saunders, SIAM J. Numer. Anal. 12, 1975) on A x = rhs from
 // x = 0, FP64 vectors and FP64 matvec: the Gram matrix is symmetric but
 // indefinite, where CG can meet p.Ap <= 0.  Stops when the residual estimate
 // phibar < tol * ||rhs|| or after max_iter iterations.
 static int minres_f64(const FieldD &f, float h_w, int n, const double *rhs, double *x,
-                      CgBufs &B, double tol, int max_iter, hipStream_t s, int *it)
+                      CgBufs &B, double tol, int max_iter, hipStream_t s, int *it,
+                      const LatticePc *pc = nullptr)
 {
     unsigned gb = (unsigned)((n + 255) / 256);
     *it = 0;
     double *r1 = B.m[0], *r2 = B.m[1], *y = B.m[2], *v = B.m[3], *w = B.m[4], *w1 = B.m[5],
            *w2 = B.m[6];
+    // z = M^-1 r2 (the preconditioned Lanczos vector); without M, z is r2
+    double *z = pc ? pc->z : nullptr;
     hipLaunchKernelGGL(k_scale<double>, dim3(gb), dim3(256), 0, s, n, rhs, 0.0, x);
     hipLaunchKernelGGL(k_scale<double>, dim3(gb), dim3(256), 0, s, n, rhs, 0.0, w);
     hipLaunchKernelGGL(k_scale<double>, dim3(gb), dim3(256), 0, s, n, rhs, 0.0, w1);
@@ -270,13 +466,22 @@ static int minres_f64(const FieldD &f, float h_w, int n, const double *rhs, doub
     VPMC_CHECK_LAUNCH();
     double b2;
     VPMC_TRY(dot_host(rhs, rhs, n, B, s, &b2));
+    const double bnorm = sqrt(b2);
+    if (bnorm == 0.0) return VPMC_OK;
+    if (pc) {
+        VPMC_TRY(lattice_pc_apply(*pc, rhs, z, s));
+        VPMC_TRY(dot_host(rhs, (const double *)z, n, B, s, &b2));
+        if (!(b2 > 0.0)) return VPMC_ERR_ARG;
+    }
     const double beta1 = sqrt(b2);
-    if (beta1 == 0.0) return VPMC_OK;
     double oldb = 0.0, beta = beta1, dbar = 0.0, epsln = 0.0, phibar = beta1, cs = -1.0, sn = 0.0;
+    // preconditioned: phibar estimates |r|_{M^-1}; the FP64 residual |b - A x|
+    // is checked whenever the estimate passes the next decade below tol
+    double check_at = tol * beta1;
     // r1, r2: the last two Lanczos residuals; y: free buffer the matvec fills
     for (int itn = 1; itn <= max_iter; ++itn) {
-        hipLaunchKernelGGL(k_scale<double>, dim3(gb), dim3(256), 0, s, n, (const double *)r2,
-                           1.0 / beta, v);
+        hipLaunchKernelGGL(k_scale<double>, dim3(gb), dim3(256), 0, s, n,
+                           (const double *)(pc ? z : r2), 1.0 / beta, v);
         hipLaunchKernelGGL(k_rbf_matvec_f64, dim3(gb), dim3(256), 0, s, f, h_w, n,
                            (const double *)v, y);
         if (itn >= 2)
@@ -294,8 +499,13 @@ static int minres_f64(const FieldD &f, float h_w, int n, const double *rhs, doub
         y = t;
         oldb = beta;
         double bb;
-        VPMC_TRY(dot_host(r2, r2, n, B, s, &bb));
-        beta = sqrt(bb);
+        if (pc) {
+            VPMC_TRY(lattice_pc_apply(*pc, r2, z, s));
+            VPMC_TRY(dot_host(r2, (const double *)z, n, B, s, &bb));
+        } else {
+            VPMC_TRY(dot_host(r2, r2, n, B, s, &bb));
+        }
+        beta = sqrt(fmax(bb, 0.0));
         double oldeps = epsln;
         double delta = cs * dbar + sn * alfa;
         double gbar = sn * dbar - cs * alfa;
@@ -315,7 +525,20 @@ static int minres_f64(const FieldD &f, float h_w, int n, const double *rhs, doub
                            delta, 1.0 / gamma, phi, w, x);
         VPMC_CHECK_LAUNCH();
         *it = itn;
-        if (phibar < tol * beta1 || beta == 0.0) break;
+        if (beta == 0.0) break;
+        if (!pc) {
+            if (phibar < tol * beta1) break;
+        } else if (phibar < check_at) {
+            hipLaunchKernelGGL(k_rbf_matvec_f64, dim3(gb), dim3(256), 0, s, f, h_w, n,
+                               (const double *)x, B.Aw64);
+            hipLaunchKernelGGL(k_res64, dim3(gb), dim3(256), 0, s, n, rhs,
+                               (const double *)B.Aw64, B.r64);
+            VPMC_CHECK_LAUNCH();
+            double r2n;
+            VPMC_TRY(dot_host(B.r64, B.r64, n, B, s, &r2n));
+            if (sqrt(r2n) < tol * bnorm) break;
+            check_at *= 0.1;
+        }
     }
     return VPMC_OK;
 }
@@ -411,7 +634,8 @@ namespace vpmc {
 static int cg_rbf_impl(int n, const float *pos_xy, const float *b, const double *b_f64,
                        float *w_out, double *w64_out, float h_w, float Lx, float Ly,
                        int cells_per_hw, double tol, int max_iter, int refine, int *iters,
-                       double *rel_res, void *workspace, size_t workspace_bytes, void *stream)
+                       double *rel_res, void *workspace, size_t workspace_bytes, void *stream,
+                       int nx_l = 0, int ny_l = 0)
 {
     if (n < 0 || !(h_w > 0.f) || max_iter < 0) return VPMC_ERR_ARG;
     if (iters) *iters = 0;
@@ -422,7 +646,14 @@ static int cg_rbf_impl(int n, const float *pos_xy, const float *b, const double 
     int nx, ny;
     VPMC_TRY(vpmc_field_grid(h_w, Lx, Ly, cells_per_hw, &nx, &ny));
     CgBufs B;
-    VPMC_TRY(carve_cg(n, nx, ny, workspace, workspace_bytes, &B));
+    const bool lattice = nx_l > 0 && refine;
+    if (lattice && ((long long)nx_l * ny_l != n || !lattice_pc_ok(nx_l, ny_l)))
+        return VPMC_ERR_ARG;
+    const size_t pc_bytes = lattice ? lattice_pc_bytes(nx_l, ny_l) : 0;
+    if (workspace_bytes < pc_bytes) return VPMC_ERR_WORKSPACE;
+    // [0, pc_off): the solve's buffers; [pc_off, end): the preconditioner's
+    const size_t pc_off = lattice ? (workspace_bytes - pc_bytes) / 256 * 256 : workspace_bytes;
+    VPMC_TRY(carve_cg(n, nx, ny, workspace, pc_off, &B));
     VPMC_TRY(field_build_impl(nx, ny, Lx, Ly, n, nullptr, nullptr, pos_xy, nullptr, 0, B.pts,
                               B.cell_start, B.order, nullptr, B.fws, B.fws_bytes, s));
     vpmc_field fh{nx, ny, Lx, Ly, n, B.pts, B.cell_start, B.order};
@@ -452,6 +683,14 @@ static int cg_rbf_impl(int n, const float *pos_xy, const float *b, const double 
         total_it = it;
         hipLaunchKernelGGL(k_f2d, dim3(gb), dim3(256), 0, s, n, (const float *)B.w, B.w64);
         VPMC_CHECK_LAUNCH();
+    } else if (lattice) {
+        // the preconditioner's buffers sit at the end of the workspace
+        LatticePc P;
+        VPMC_TRY(lattice_pc_init(P, nx_l, ny_l, Lx, Ly, h_w, B.order,
+                                 static_cast<char *>(workspace) + pc_off,
+                                 workspace_bytes - pc_off, s));
+        VPMC_TRY(minres_f64(f, h_w, n, (const double *)B.b64, B.w64, B, tol, max_iter, s,
+                            &total_it, &P));
     } else {
         VPMC_TRY(minres_f64(f, h_w, n, (const double *)B.b64, B.w64, B, tol, max_iter, s,
                             &total_it));
@@ -500,6 +739,26 @@ int vpmc_cg_rbf_f64(int n, const float *pos_xy, const double *b, float *w_out, d
                        max_iter, 1, iters, rel_res, workspace, workspace_bytes, stream);
 }
 
+size_t vpmc_cg_lattice_workspace_bytes(int nx_l, int ny_l, float h_w, float Lx, float Ly,
+                                       int cells_per_hw)
+{
+    int nx, ny;
+    if (!lattice_pc_ok(nx_l, ny_l) || vpmc_field_grid(h_w, Lx, Ly, cells_per_hw, &nx, &ny))
+        return 0;
+    return vpmc_cg_workspace_bytes(nx_l * ny_l, nx, ny) + lattice_pc_bytes(nx_l, ny_l) + 256;
+}
+
+int vpmc_cg_rbf_lattice(int nx_l, int ny_l, const float *pos_xy, const double *b, float *w_out,
+                        double *w64_out, float h_w, float Lx, float Ly, int cells_per_hw,
+                        double tol, int max_iter, int *iters, double *rel_res, void *workspace,
+                        size_t workspace_bytes, void *stream)
+{
+    if (!lattice_pc_ok(nx_l, ny_l)) return VPMC_ERR_ARG;
+    return cg_rbf_impl(nx_l * ny_l, pos_xy, nullptr, b, w_out, w64_out, h_w, Lx, Ly,
+                       cells_per_hw, tol, max_iter, 1, iters, rel_res, workspace,
+                       workspace_bytes, stream, nx_l, ny_l);
+}
+
 }  // extern "C"
 
 // ---------------------------------------------------------------------------
@@ -530,11 +789,12 @@ struct ReinitBufs {
     size_t rest_bytes;
 };
 
-static int carve_reinit(int n, int n_new, int nx, int ny, bool def, void *ws, size_t bytes,
-                        ReinitBufs *B)
+static int carve_reinit(int n, int nx_l, int ny_l, int nx, int ny, bool def, void *ws,
+                        size_t bytes, ReinitBufs *B)
 {
     Carve cv(ws, bytes);
     const size_t m = (size_t)(n > 0 ? n : 1);
+    const int n_new = nx_l * ny_l;
     B->pts = cv.take<float>(4 * m);
     B->gq = def ? cv.take<float>(4 * m) : nullptr;
     B->cell_start = cv.take<int>((size_t)nx * ny + 1);
@@ -542,6 +802,9 @@ static int carve_reinit(int n, int n_new, int nx, int ny, bool def, void *ws, si
     B->b = cv.take<double>((size_t)(n_new > 0 ? n_new : 1));
     B->rest_bytes = std::max(vpmc_field_workspace_bytes(n, nx, ny),
                              vpmc_cg_workspace_bytes(n_new, nx, ny));
+    if (lattice_pc_ok(nx_l, ny_l))
+        B->rest_bytes = std::max(B->rest_bytes, vpmc_cg_workspace_bytes(n_new, nx, ny) +
+                                                    lattice_pc_bytes(nx_l, ny_l) + 256);
     B->rest = cv.take<char>(B->rest_bytes);
     if (!B->pts || (def && !B->gq) || !B->cell_start || !B->order || !B->b || !B->rest)
         return VPMC_ERR_WORKSPACE;
@@ -560,10 +823,13 @@ size_t vpmc_reinit_workspace_bytes(int n, int nx_l, int ny_l, float h_w, float L
         return 0;
     const int n_new = nx_l * ny_l;
     const size_t m = (size_t)(n > 0 ? n : 1), nn = (size_t)n_new;
+    size_t rest = std::max(vpmc_field_workspace_bytes(n, nx, ny),
+                           vpmc_cg_workspace_bytes(n_new, nx, ny));
+    if (lattice_pc_ok(nx_l, ny_l))
+        rest = std::max(rest, vpmc_cg_workspace_bytes(n_new, nx, ny) +
+                                  lattice_pc_bytes(nx_l, ny_l) + 256);
     return align256(16 * m) * (deformed ? 2 : 1) + align256(((size_t)nx * ny + 1) * 4) +
-           align256(4 * m) + align256(8 * nn) +
-           std::max(vpmc_field_workspace_bytes(n, nx, ny), vpmc_cg_workspace_bytes(n_new, nx, ny)) +
-           4096;
+           align256(4 * m) + align256(8 * nn) + align256(rest) + 4096;
 }
 
 int vpmc_reinit(int n, const float *pos_xy, const float *w, const float *jac, float h_w,
@@ -580,7 +846,8 @@ int vpmc_reinit(int n, const float *pos_xy, const float *w, const float *jac, fl
     VPMC_TRY(vpmc_field_grid(h_w, Lx, Ly, cells_per_hw, &nx, &ny));
     const int n_new = nx_l * ny_l;
     ReinitBufs B;
-    VPMC_TRY(carve_reinit(n, n_new, nx, ny, jac != nullptr, workspace, workspace_bytes, &B));
+    VPMC_TRY(carve_reinit(n, nx_l, ny_l, nx, ny, jac != nullptr, workspace, workspace_bytes,
+                          &B));
     // the field of every particle with its signed strength
     VPMC_TRY(field_build_impl(nx, ny, Lx, Ly, n, nullptr, nullptr, pos_xy, w, 0, B.pts,
                               B.cell_start, B.order, nullptr, B.rest, B.rest_bytes, s, jac,
@@ -592,8 +859,11 @@ int vpmc_reinit(int n, const float *pos_xy, const float *w, const float *jac, fl
     hipLaunchKernelGGL(k_reinit_rhs, dim3((n_new + 255) / 256), dim3(256), 0, s, f,
                        (double)h_w, nx_l, ny_l, Lx, Ly, (float2 *)pos_out, B.b);
     VPMC_CHECK_LAUNCH();
+    // the lattice's circulant preconditioner where its lines fit the DFT kernel
+    const bool pc = lattice_pc_ok(nx_l, ny_l);
     return cg_rbf_impl(n_new, pos_out, nullptr, B.b, w_out, nullptr, h_w, Lx, Ly, cells_per_hw,
-                       tol, max_iter, 1, iters, rel_res, B.rest, B.rest_bytes, stream);
+                       tol, max_iter, 1, iters, rel_res, B.rest, B.rest_bytes, stream,
+                       pc ? nx_l : 0, pc ? ny_l : 0);
 }
 
 }  // extern "C"

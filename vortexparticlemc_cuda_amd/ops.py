@@ -223,12 +223,29 @@ def advect(pos, vel_sign, dt, Lx=1.0, Ly=1.0, periodic=True):
 # ---------------------------------------------------------------------------
 # RBF / CG
 
-def cg_rbf(pos, b, h_w, Lx=1.0, Ly=1.0, tol=1e-6, max_iter=1000, refine=True, cells_per_hw=8):
+def cg_rbf(pos, b, h_w, Lx=1.0, Ly=1.0, tol=1e-6, max_iter=1000, refine=True, cells_per_hw=8,
+           lattice=None):
     """strengths w with A w = b (vpmc_cg_rbf); a float64 b selects the FP64
-    right-hand side (vpmc_cg_rbf_f64, MINRES).  Returns (w, iterations,
-    relative FP64 residual, status)."""
+    right-hand side (vpmc_cg_rbf_f64, MINRES); lattice=(nx, ny) for particles
+    on that cell-centred lattice adds its circulant preconditioner
+    (vpmc_cg_rbf_lattice, FP64 b).  Returns (w, iterations, relative FP64
+    residual, status)."""
     pos = _xy(pos)
     n = pos.shape[0]
+    if lattice is not None:
+        nxl, nyl = (int(v) for v in lattice)
+        if nxl * nyl != n:
+            raise ValueError("cg_rbf: lattice %dx%d for %d particles" % (nxl, nyl, n))
+        ws = workspace(lib().vpmc_cg_lattice_workspace_bytes(nxl, nyl, h_w, Lx, Ly,
+                                                             cells_per_hw), pos.device)
+        w = torch.empty(max(n, 1), dtype=torch.float32, device=pos.device)
+        it, rr = C.c_int(), C.c_double()
+        st = lib().vpmc_cg_rbf_lattice(nxl, nyl, _p(pos), _p(b.to(torch.float64).contiguous()),
+                                       _p(w), None, h_w, Lx, Ly, cells_per_hw, tol, max_iter,
+                                       C.byref(it), C.byref(rr), _p(ws), ws.numel(), _stream())
+        if st not in (0, 5, 6):
+            check(st, "vpmc_cg_rbf_lattice")
+        return w[:n], it.value, rr.value, st
     nx, ny = field_grid(h_w, Lx, Ly, cells_per_hw)
     ws = workspace(lib().vpmc_cg_workspace_bytes(n, nx, ny), pos.device)
     w = torch.empty(max(n, 1), dtype=torch.float32, device=pos.device)

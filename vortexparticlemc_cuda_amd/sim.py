@@ -87,16 +87,20 @@ class Simulation:
         self.build_subsets()
         self._alloc()
 
-    def init_strengths_cg(self, pos, b, tol=1e-6, max_iter=1000, refine=True):
+    def init_strengths_cg(self, pos, b, tol=1e-6, max_iter=1000, refine=True, lattice=None):
         """Simulation::init + conjugate_gradient_rbf (src/vortex_particle_mc.cu:77-120,
         src/cuda/cg_rbf_init.cu:53-116): strengths w solving A w = b.  With refine
-        the solve is FP64 MINRES; strengths are FP32, so tol below ~1e-7 only
-        amplifies the rounding noise of b in the Gram matrix's near-null modes."""
+        the solve is FP64 MINRES; strengths are FP32, so with an FP32 b, tol below
+        ~1e-7 only amplifies the rounding noise of b in the Gram matrix's
+        near-null modes.  lattice=(nx, ny): the particles are that cell-centred
+        lattice; b is taken in FP64 and MINRES uses the lattice's circulant
+        preconditioner."""
         pos_t = torch.as_tensor(np.asarray(pos, np.float32), device=self.device).contiguous()
-        b_t = torch.as_tensor(np.asarray(b, np.float32), device=self.device).contiguous()
+        dt = np.float64 if lattice is not None else np.float32
+        b_t = torch.as_tensor(np.asarray(b, dt), device=self.device).contiguous()
         w, it, rr, st = ops.cg_rbf(pos_t, b_t, float(self.P.h_w), float(self.P.Lx),
                                    float(self.P.Ly), tol=tol, max_iter=max_iter, refine=refine,
-                                   cells_per_hw=self.cells_per_hw)
+                                   cells_per_hw=self.cells_per_hw, lattice=lattice)
         self.cg_iters, self.cg_rel_res, self.cg_status = it, rr, st
         self.set_particles(pos, w.cpu().numpy())
         return it, rr, st
