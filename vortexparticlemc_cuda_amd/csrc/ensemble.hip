@@ -92,7 +92,7 @@ __global__ void k_ens_field_keys(EnsTabs t, int E, uint32_t dead, const int *__r
 
 template <int MODE>
 __global__ void __launch_bounds__(BIN_THREADS)
-    k_ens_gen(EnsTabs t, int n_sim, uint32_t dead, uint32_t *__restrict__ key,
+    k_ens_gen(EnsTabs t, int n_sim, uint32_t *__restrict__ key,
               float2 *__restrict__ q, float2 *__restrict__ r, float *__restrict__ om)
 {
     const int first = blockIdx.x * BIN_THREADS;
@@ -101,14 +101,15 @@ __global__ void __launch_bounds__(BIN_THREADS)
     const McD &d = t.sim[s];
     const BinGeom &g = t.geom[s];
     const size_t sid = out - (size_t)t.soff[s];
+    const uint32_t dk = (uint32_t)(t.toff[s] + g.ntile);   // the simulation's dead tile
     if (sid >= g.S) {   // padding of the simulation's sample range
-        key[out] = dead;
+        key[out] = dk;
         q[out] = make_float2(0.f, 0.f);
         r[out] = make_float2(0.f, 0.f);
         om[out] = 0.f;
         return;
     }
-    bin_gen_one<MODE, true>(d, g, sid, out, (uint32_t)t.toff[s], dead, key, q, r, om);
+    bin_gen_one<MODE, true>(d, g, sid, out, (uint32_t)t.toff[s], dk, key, q, r, om);
 }
 
 template <int THREADS, int SPL>
@@ -121,6 +122,7 @@ __global__ void __launch_bounds__(THREADS)
     const int beg = start[tile], end = start[tile + 1];
     if (beg >= end) return;   // uniform over the workgroup
     const int s = seg_of(t.toff, n_sim, tile);
+    if (tile - t.toff[s] >= t.geom[s].ntile) return;   // the dead tile: no query
     __shared__ TileLds<THREADS> L;
     EvalCount ec;
     bin_eval_tile<THREADS, SPL, false>(t.sim[s], t.geom[s], tile - t.toff[s], beg, end, srec, om,
@@ -163,7 +165,10 @@ __global__ void k_ens_step_add(McD *__restrict__ sims, int n_sim, uint32_t add, 
 
 struct EnsLayout {
     int n_sim = 0, N = 0, E = 0, C = 0, T = 0, per_xcd = 0;
+    int local_bits = 1, nblk = 0;   // the sample sort: bits of a local tile key, blocks
     size_t S = 0;
+    std::vector<int4> blk;          // the sample sort's block table (host copy)
+    std::vector<uint32_t> blk_base;
     std::vector<int> poff, soff, toff, eoff, coff, nx, ny, glog2, goff;
     // workspace
     EnsTabs t{};
@@ -181,6 +186,8 @@ struct EnsLayout {
     float *om = nullptr;
     float4 *srec = nullptr;
     int *tstart = nullptr;
+    int4 *sblk = nullptr;
+    uint32_t *sblk_base = nullptr;
     void *ssort_ws = nullptr, *init_ws = nullptr;
     size_t ssort_bytes = 0, init_bytes = 0;
 };
@@ -214,7 +221,11 @@ static int ens_plan(const vpmc_ensemble_desc *e, EnsLayout *L)
         if (so >= ((size_t)1 << 31)) return VPMC_ERR_ARG;
         L->soff[s + 1] = (int)so;
         const int cells = L->nx[s] * L->ny[s];
-        L->toff[s + 1] = L->toff[s] + 2 * cells;
+        // 2 * cells tiles, then the simulation's dead tile (samples without a
+        // query), so that every key of simulation s lies in [toff[s], toff[s+1])
+        L->toff[s + 1] = L->toff[s] + 2 * cells + 1;
+        const int lb = key_bits_for((uint32_t)(2 * cells));
+        if (lb > L->local_bits) L->local_bits = lb;
         for (int g = 0; g < 2; ++g) {
             L->eoff[2 * s + g + 1] = L->eoff[2 * s + g] + n;
             L->coff[2 * s + g + 1] = L->coff[2 * s + g] + cells;
@@ -231,6 +242,10 @@ static int ens_plan(const vpmc_ensemble_desc *e, EnsLayout *L)
     L->E = L->eoff[2 * ns];
     L->C = L->coff[2 * ns];
     L->per_xcd = (L->T + 7) / 8;
+    std::vector<uint32_t> base(ns);
+    for (int s = 0; s < ns; ++s) base[s] = (uint32_t)L->toff[s];
+    VPMC_TRY(radix_seg_blocks(L->soff, base, &L->blk, &L->blk_base));
+    L->nblk = (int)L->blk.size();
     return VPMC_OK;
 }
 
@@ -279,7 +294,9 @@ static size_t ens_carve(EnsLayout &L, void *ws, size_t bytes)
     L.srec = cv.take<float4>(S);
     L.skeys_sorted = cv.take<uint32_t>(S);
     L.tstart = cv.take<int>((size_t)L.T + 1);
-    L.ssort_bytes = radix_workspace_bytes((int)S);
+    L.sblk = cv.take<int4>((size_t)(L.nblk > 0 ? L.nblk : 1));
+    L.sblk_base = cv.take<uint32_t>((size_t)(L.nblk > 0 ? L.nblk : 1));
+    L.ssort_bytes = radix_seg_workspace_bytes((int)S, L.nblk);
     L.ssort_ws = cv.take<char>(L.ssort_bytes);
     L.vel_sign = cv.take<float2>(2 * N);
     L.init_bytes = subsets_ws_max(L);
@@ -354,11 +371,12 @@ static int ens_estimate(const vpmc_ensemble_desc *e, EnsLayout &L, hipStream_t s
 {
     const int ns = L.n_sim;
     hipLaunchKernelGGL((k_ens_gen<MODE>), dim3((unsigned)(L.S / BIN_THREADS)), dim3(BIN_THREADS), 0,
-                       st, L.t, ns, (uint32_t)L.T, L.skey, L.q, L.r, L.om);
+                       st, L.t, ns, L.skey, L.q, L.r, L.om);
     VPMC_CHECK_LAUNCH();
-    VPMC_TRY(radix_sort_to_records(L.skey, (int)L.S, key_bits_for((uint32_t)L.T),
-                                   (const float2 *)L.q, nullptr, L.srec, L.skeys_sorted, L.T,
-                                   L.tstart, L.ssort_ws, L.ssort_bytes, st));
+    // segmented by simulation: samples are grouped by simulation already
+    VPMC_TRY(radix_sort_segmented(L.skey, (int)L.S, L.local_bits, (const float2 *)L.q, L.srec,
+                                  L.skeys_sorted, L.T, L.tstart, L.sblk, L.sblk_base, L.nblk,
+                                  L.ssort_ws, L.ssort_bytes, st));
     const int th = e->eval_threads ? e->eval_threads : 256;
 #define VPMC_ENS_EVAL(TH, SP)                                                                 \
     hipLaunchKernelGGL((k_ens_eval<TH, SP>), dim3((unsigned)(8 * L.per_xcd)), dim3(TH), 0, st,  \
@@ -434,6 +452,8 @@ int vpmc_ensemble_init(const vpmc_ensemble_desc *e, void *workspace, size_t work
         {L.t.toff, L.toff.data(), sizeof(int) * (ns + 1)},
         {L.t.eoff, L.eoff.data(), sizeof(int) * (2 * ns + 1)},
         {L.t.coff, L.coff.data(), sizeof(int) * (2 * ns + 1)},
+        {L.sblk, L.blk.data(), sizeof(int4) * L.blk.size()},
+        {L.sblk_base, L.blk_base.data(), sizeof(uint32_t) * L.blk_base.size()},
     };
     hipError_t err = hipStreamSynchronize(st);
     for (const Up &u : ups)

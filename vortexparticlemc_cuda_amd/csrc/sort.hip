@@ -12,6 +12,16 @@
 //            LDS, so the pass is stable and the result is the unique stable
 //            order.
 // The 16-byte sample records move with the keys in every pass.
+//
+// Segmented form: when the input is already grouped into segments (an
+// ensemble's simulations, each owning the key range [base_s, base_s + 2^b)),
+// blocks are cut per segment and the histogram is laid out [segment][digit]
+// [block of the segment]; one global exclusive scan of it then yields
+// offsets that keep every segment in place, and only the b local bits need
+// passes (two 8-bit passes instead of three for 2^20 global keys).
+#include <algorithm>
+#include <vector>
+
 #include "vpmc_internal.hpp"
 
 namespace vpmc {
@@ -23,19 +33,38 @@ constexpr int RS_BINS = 256;                     // 8-bit digits
 constexpr int RS_WAVES = RS_THREADS / WAVE;
 static_assert(RS_THREADS == RS_BINS, "one thread per digit");
 
+// a block's input range, histogram column (digit d at h0 + d * hstride) and
+// key base; blk == NULL: block b of the plain sort
+struct RsBlk {
+    int base, cnt, h0, hstride;
+    uint32_t kbase;
+};
+__device__ __forceinline__ RsBlk rs_block(const int4 *__restrict__ blk,
+                                          const uint32_t *__restrict__ blk_base, int n, int nblk)
+{
+    if (blk) {
+        const int4 v = blk[blockIdx.x];
+        return RsBlk{v.x, v.y, v.z, v.w, blk_base[blockIdx.x]};
+    }
+    const int base = (int)blockIdx.x * RS_TILE;
+    return RsBlk{base, min(RS_TILE, n - base), (int)blockIdx.x, nblk, 0u};
+}
+
 __global__ void __launch_bounds__(RS_THREADS)
-    k_rs_hist(const uint32_t *__restrict__ keys, int n, int shift, int nblk, int *__restrict__ hist)
+    k_rs_hist(const uint32_t *__restrict__ keys, int n, int shift, int nblk,
+              const int4 *__restrict__ blk, const uint32_t *__restrict__ blk_base,
+              int *__restrict__ hist)
 {
     __shared__ int h[RS_BINS];
     h[threadIdx.x] = 0;
     __syncthreads();
-    const int base = blockIdx.x * RS_TILE;
+    const RsBlk B = rs_block(blk, blk_base, n, nblk);
     for (int r = 0; r < RS_ITEMS; ++r) {
-        const int i = base + r * RS_THREADS + threadIdx.x;
-        if (i < n) atomicAdd(&h[(keys[i] >> shift) & (RS_BINS - 1)], 1);
+        const int e = r * RS_THREADS + threadIdx.x;
+        if (e < B.cnt) atomicAdd(&h[((keys[B.base + e] - B.kbase) >> shift) & (RS_BINS - 1)], 1);
     }
     __syncthreads();
-    hist[(size_t)threadIdx.x * nblk + blockIdx.x] = h[threadIdx.x];
+    hist[(size_t)B.h0 + (size_t)threadIdx.x * B.hstride] = h[threadIdx.x];
 }
 
 // exclusive prefix over the block of one value per thread; *total = sum
@@ -75,6 +104,7 @@ constexpr int RS_WITEMS = RS_TILE / RS_WAVES / WAVE;   // keys per lane
 __global__ void __launch_bounds__(RS_THREADS)
     k_rs_scatter(const uint32_t *__restrict__ keys_in, const float4 *__restrict__ recs_in,
                  const float2 *__restrict__ q, int n, int shift, int nblk,
+                 const int4 *__restrict__ blk, const uint32_t *__restrict__ blk_base,
                  const int *__restrict__ offs, uint32_t *__restrict__ keys_out,
                  float4 *__restrict__ recs_out)
 {
@@ -84,8 +114,9 @@ __global__ void __launch_bounds__(RS_THREADS)
     __shared__ int gofs[RS_BINS];           // global position - local position
     __shared__ int wsum[RS_WAVES];
     const int t = threadIdx.x, lane = t & (WAVE - 1), wid = t / WAVE;
-    const int base = blockIdx.x * RS_TILE;
-    const int cnt = min(RS_TILE, n - base);
+    const RsBlk B = rs_block(blk, blk_base, n, nblk);
+    const int base = B.base, cnt = B.cnt;
+    const uint32_t kb = B.kbase;
 #pragma unroll
     for (int w = 0; w < RS_WAVES; ++w) wh[w][t] = 0;
     __syncthreads();
@@ -96,8 +127,8 @@ __global__ void __launch_bounds__(RS_THREADS)
     for (int r = 0; r < RS_WITEMS; ++r) {
         const int e = (wid * RS_WITEMS + r) * WAVE + lane;
         const bool valid = e < cnt;
-        kr[r] = valid ? keys_in[base + e] : 0u;
-        const int d = (int)((kr[r] >> shift) & (RS_BINS - 1));
+        kr[r] = valid ? keys_in[base + e] : kb;
+        const int d = (int)(((kr[r] - kb) >> shift) & (RS_BINS - 1));
         unsigned long long peers = __ballot(valid);
 #pragma unroll
         for (int b = 0; b < 8; ++b) {
@@ -122,7 +153,7 @@ __global__ void __launch_bounds__(RS_THREADS)
     }
     int tot;
     const int ls = rs_block_excl_scan(sum, wsum, &tot);
-    gofs[t] = offs[(size_t)t * nblk + blockIdx.x] - ls;
+    gofs[t] = offs[(size_t)B.h0 + (size_t)t * B.hstride] - ls;
     int acc = ls;
 #pragma unroll
     for (int w = 0; w < RS_WAVES; ++w) {
@@ -134,7 +165,7 @@ __global__ void __launch_bounds__(RS_THREADS)
     for (int r = 0; r < RS_WITEMS; ++r) {
         const int e = (wid * RS_WITEMS + r) * WAVE + lane;
         if (e < cnt) {
-            const int pos = wh[wid][(kr[r] >> shift) & (RS_BINS - 1)] + rk[r];
+            const int pos = wh[wid][((kr[r] - kb) >> shift) & (RS_BINS - 1)] + rk[r];
             lk[pos] = kr[r];
             li[pos] = (uint16_t)e;
         }
@@ -151,7 +182,7 @@ __global__ void __launch_bounds__(RS_THREADS)
         if (i < cnt) {
             ko[j] = lk[i];
             const int src = base + li[i];
-            po[j] = gofs[(ko[j] >> shift) & (RS_BINS - 1)] + i;
+            po[j] = gofs[((ko[j] - kb) >> shift) & (RS_BINS - 1)] + i;
             if (recs_in) {
                 rv[j] = recs_in[src];
             } else {
@@ -198,13 +229,13 @@ size_t radix_workspace_bytes(int n)
 // key must lie in [0, nkeys]).  keys and recs0 are left unchanged.  Pass p
 // writes the final buffers when P-1-p is even, the workspace copies
 // otherwise, so the last pass always ends in the outputs.
-int radix_sort_to_records(const uint32_t *keys, int n, int key_bits, const float2 *q,
-                          const float4 *recs0, float4 *rec, uint32_t *keys_sorted, int nkeys,
-                          int *start, void *ws, size_t ws_bytes, hipStream_t s)
+static int radix_sort_impl(const uint32_t *keys, int n, int key_bits, const float2 *q,
+                           const float4 *recs0, float4 *rec, uint32_t *keys_sorted, int nkeys,
+                           int *start, const int4 *blk, const uint32_t *blk_base, int nblk,
+                           void *ws, size_t ws_bytes, hipStream_t s)
 {
     if (n < 0 || key_bits < 1 || key_bits > 32) return VPMC_ERR_ARG;
     if (n == 0) return fill_i32(start, 0, (size_t)nkeys + 1, s);
-    const int nblk = rs_nblk(n);
     const size_t h = (size_t)RS_BINS * nblk;
     if (h >= ((size_t)1 << 31)) return VPMC_ERR_ARG;
     Carve cv(ws, ws_bytes);
@@ -223,11 +254,11 @@ int radix_sort_to_records(const uint32_t *keys, int n, int key_bits, const float
         uint32_t *kout = to_out ? keys_sorted : kb;
         float4 *rout = to_out ? rec : rb;
         hipLaunchKernelGGL(k_rs_hist, dim3(nblk), dim3(RS_THREADS), 0, s, kin, n, 8 * p, nblk,
-                           hist);
+                           blk, blk_base, hist);
         VPMC_CHECK_LAUNCH();
         VPMC_TRY(scan_i32(hist, offs, (int)h, false, nullptr, sws, sb, s));
         hipLaunchKernelGGL(k_rs_scatter, dim3(nblk), dim3(RS_THREADS), 0, s, kin, rin, q, n,
-                           8 * p, nblk, (const int *)offs, kout, rout);
+                           8 * p, nblk, blk, blk_base, (const int *)offs, kout, rout);
         VPMC_CHECK_LAUNCH();
         kin = kout;
         rin = rout;
@@ -236,6 +267,55 @@ int radix_sort_to_records(const uint32_t *keys, int n, int key_bits, const float
                        (const uint32_t *)keys_sorted, n, nkeys, start);
     VPMC_CHECK_LAUNCH();
     return VPMC_OK;
+}
+
+int radix_sort_to_records(const uint32_t *keys, int n, int key_bits, const float2 *q,
+                          const float4 *recs0, float4 *rec, uint32_t *keys_sorted, int nkeys,
+                          int *start, void *ws, size_t ws_bytes, hipStream_t s)
+{
+    return radix_sort_impl(keys, n, key_bits, q, recs0, rec, keys_sorted, nkeys, start, nullptr,
+                           nullptr, rs_nblk(n), ws, ws_bytes, s);
+}
+
+int radix_seg_blocks(const std::vector<int> &seg_off, const std::vector<uint32_t> &seg_base,
+                     std::vector<int4> *blk, std::vector<uint32_t> *blk_base)
+{
+    blk->clear();
+    blk_base->clear();
+    const int ns = (int)seg_base.size();
+    if ((int)seg_off.size() != ns + 1) return VPMC_ERR_ARG;
+    long long hbase = 0;
+    for (int g = 0; g < ns; ++g) {
+        const int b0 = seg_off[g], b1 = seg_off[g + 1];
+        if (b1 < b0) return VPMC_ERR_ARG;
+        const int nb = (b1 - b0 + RS_TILE - 1) / RS_TILE;
+        for (int k = 0; k < nb; ++k) {
+            const int beg = b0 + k * RS_TILE;
+            blk->push_back(make_int4(beg, std::min(RS_TILE, b1 - beg), (int)(hbase + k), nb));
+            blk_base->push_back(seg_base[g]);
+        }
+        hbase += (long long)RS_BINS * nb;
+        if (hbase >= ((long long)1 << 31)) return VPMC_ERR_ARG;
+    }
+    return VPMC_OK;
+}
+
+size_t radix_seg_workspace_bytes(int n, int nblk)
+{
+    const size_t m = (size_t)(n > 0 ? n : 1);
+    const size_t h = (size_t)RS_BINS * (nblk > 0 ? nblk : 1);
+    return align256(m * 4) + align256(m * 16) + 2 * align256(h * 4) +
+           scan_workspace_bytes((int)h, sizeof(int)) + 1024;
+}
+
+int radix_sort_segmented(const uint32_t *keys, int n, int local_bits, const float2 *q,
+                         float4 *rec, uint32_t *keys_sorted, int nkeys, int *start,
+                         const int4 *blk, const uint32_t *blk_base, int nblk, void *ws,
+                         size_t ws_bytes, hipStream_t s)
+{
+    if (!blk || !blk_base || nblk <= 0) return VPMC_ERR_ARG;
+    return radix_sort_impl(keys, n, local_bits, q, nullptr, rec, keys_sorted, nkeys, start, blk,
+                           blk_base, nblk, ws, ws_bytes, s);
 }
 
 }  // namespace vpmc

@@ -1,5 +1,6 @@
 // vpmc_internal.hpp -- shared device helpers and host plumbing for libvpmc.
 #pragma once
+#include <vector>
 
 #include <hip/hip_runtime.h>
 #include <stddef.h>
@@ -70,6 +71,16 @@ size_t radix_workspace_bytes(int n);
 int radix_sort_to_records(const uint32_t *keys, int n, int key_bits, const float2 *q,
                           const float4 *recs0, float4 *rec, uint32_t *keys_sorted, int nkeys,
                           int *start, void *ws, size_t ws_bytes, hipStream_t s);
+// segmented variant: blocks from radix_seg_blocks (segments = contiguous
+// ranges seg_off[g] .. seg_off[g+1] whose keys lie in [seg_base[g],
+// seg_base[g] + 2^local_bits)); keys globally sorted on return
+int radix_seg_blocks(const std::vector<int> &seg_off, const std::vector<uint32_t> &seg_base,
+                     std::vector<int4> *blk, std::vector<uint32_t> *blk_base);
+size_t radix_seg_workspace_bytes(int n, int nblk);
+int radix_sort_segmented(const uint32_t *keys, int n, int local_bits, const float2 *q,
+                         float4 *rec, uint32_t *keys_sorted, int nkeys, int *start,
+                         const int4 *blk, const uint32_t *blk_base, int nblk, void *ws,
+                         size_t ws_bytes, hipStream_t s);
 inline int key_bits_for(uint32_t max_key)
 {
     int b = 1;
