@@ -166,8 +166,9 @@ __device__ __forceinline__ void kper_smooth_grad(const McD &d, float rx, float r
     const float sr = rx * d.inv_L, si = ry * d.inv_L;
     const float s2r = sr * sr - si * si, s2i = 2.f * sr * si;
     const float wr = s2r * s2r - s2i * s2i, wi = 2.f * s2r * s2i;
-    const int nt = d.nterms;
+    constexpr int nt = KPER_TERMS_F32;   // make_mcd's term count; unrolled
     float hr = d.kc[nt - 1] * (float)(4 * (nt - 1) + 3), hi = 0.f;
+#pragma unroll
     for (int m = nt - 2; m >= 0; --m) {
         const float t = hr * wr - hi * wi + d.kc[m] * (float)(4 * m + 3);
         hi = hr * wi + hi * wr;

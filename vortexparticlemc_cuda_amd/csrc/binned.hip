@@ -78,7 +78,9 @@ template <int MODE, bool PER>
 __global__ void __launch_bounds__(BIN_THREADS)
     k_bin_final(McD d, BinGeom g, const float2 *__restrict__ r, const float *__restrict__ om)
 {
-    const int pair = blockIdx.x * (BIN_THREADS / WAVE) + threadIdx.x / WAVE;
+    // wave-uniform, stated so that the set's fields load through the scalar unit
+    const int pair = __builtin_amdgcn_readfirstlane(blockIdx.x * (BIN_THREADS / WAVE) +
+                                                    threadIdx.x / WAVE);
     if (pair >= 2 * d.n_owner) return;
     bin_final_pair<MODE, PER>(d, g, pair, r, om);
 }

@@ -134,7 +134,10 @@ __global__ void __launch_bounds__(BIN_THREADS)
     k_ens_final(EnsTabs t, int n_sim, int npair, const float2 *__restrict__ r,
                 const float *__restrict__ om)
 {
-    const int pair = blockIdx.x * (BIN_THREADS / WAVE) + threadIdx.x / WAVE;   // wave-uniform
+    // wave-uniform, stated so that the simulation's tables load through the
+    // scalar unit (the Horner coefficients of K_s among them)
+    const int pair = __builtin_amdgcn_readfirstlane(blockIdx.x * (BIN_THREADS / WAVE) +
+                                                    threadIdx.x / WAVE);
     if (pair >= npair) return;
     const int s = seg_of(t.poff, n_sim, pair >> 1);
     const size_t so = (size_t)t.soff[s];

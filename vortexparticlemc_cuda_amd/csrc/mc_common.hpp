@@ -128,8 +128,11 @@ __device__ __forceinline__ void kper_smooth(const McD &d, float rx, float ry, fl
     float sr = rx * d.inv_L, si = ry * d.inv_L;
     float s2r = sr * sr - si * si, s2i = 2.f * sr * si;
     float wr = s2r * s2r - s2i * s2i, wi = 2.f * s2r * s2i;
-    float hr = d.kc[d.nterms - 1], hi = 0.f;
-    for (int m = d.nterms - 2; m >= 0; --m) {
+    // KPER_TERMS_F32 terms (make_mcd sets nterms to it), unrolled so the
+    // coefficients load once, ahead of the chain
+    float hr = d.kc[KPER_TERMS_F32 - 1], hi = 0.f;
+#pragma unroll
+    for (int m = KPER_TERMS_F32 - 2; m >= 0; --m) {
         float t = hr * wr - hi * wi + d.kc[m];
         hi = hr * wi + hi * wr;
         hr = t;
