@@ -186,6 +186,28 @@ def test_philox_known_answers():
                                                           0x24126ea1]
 
 
+def test_philox_stream_uniformity():
+    """the estimator's counter layout (k, owner, step, sim/sign/branch tag):
+    the four words' uniforms pass chi-square (256 bins, and 16 x 16 bins on
+    pairs of words) and Kolmogorov-Smirnov tests at p > 1e-3"""
+    from scipy import stats
+    L = O.lib()
+    words = []
+    for owner in range(2):
+        for k in range(16384):
+            words.append(O.philox([k, owner, 7, (3 << 8) | (1 << 7) | (2 << 4)],
+                                  [0x1234ABCD, 0x5678]))
+    w = np.array(words, np.uint32)                     # (32768, 4)
+    u = np.vectorize(lambda v: L.orc_u01(int(v)), otypes=[np.float32])(w)
+    assert (u >= 0).all() and (u < 1).all()
+    for c in range(4):
+        h = np.histogram(u[:, c], bins=256, range=(0, 1))[0]
+        assert stats.chisquare(h).pvalue > 1e-3, c
+        assert stats.kstest(u[:, c], "uniform").pvalue > 1e-3, c
+    h2 = np.histogram2d(u[:, 0], u[:, 1], bins=16, range=[[0, 1], [0, 1]])[0].ravel()
+    assert stats.chisquare(h2).pvalue > 1e-3
+
+
 def test_uniform_conversion():
     L = O.lib()
     assert L.orc_u01(0) == 0.0
