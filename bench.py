@@ -149,6 +149,52 @@ def copy_probe(nbytes, reps=3):
     log("copy probe: %d x %d bytes" % (reps, n * 4))
 
 
+# algorithmic HBM bytes per particle of the HBM-bound stages (hash/field build,
+# |omega| CDF, advection), as the survey of the reference counts them
+HBM_BYTES_FIELD = 28.0
+HBM_BYTES_CDF = 16.0
+HBM_BYTES_ADVECT = 24.0
+
+
+def hbm_stages(sim, reps=5):
+    """achieved GB/s of the HBM-bound stages on sim's particles: the two
+    sign fields' builds (both together, 28 B per particle), the subsets with
+    FP64 CDFs (16 B per particle) and advection (24 B per particle); HIP
+    events around reps launches of each on the current stream"""
+    import torch
+
+    from vortexparticlemc_cuda_amd import ops
+    from vortexparticlemc_cuda_amd._capi import lib
+    N = sim.N
+    ws = ops.workspace(lib().vpmc_field_workspace_bytes(max(sim.cnt), sim.nx, sim.ny), sim.device)
+    vel_sign = torch.zeros((2 * N, 2), dtype=torch.float32, device=sim.device)
+    pos = sim.pos.clone()
+
+    def fields():
+        for k, key in enumerate(("sub_pos", "sub_neg")):
+            ops.field_build(sim.fields[k], sim.pos, sim.omega, sub=sim.sub[key], abs_w=True,
+                            ws=ws, jac=sim.jac if sim.deform else None)
+
+    stages = (("field_build", fields, HBM_BYTES_FIELD),
+              ("subsets_cdf", lambda: ops.build_subsets(sim.omega), HBM_BYTES_CDF),
+              ("advect", lambda: ops.advect(pos, vel_sign, 0.0), HBM_BYTES_ADVECT))
+    out = {}
+    for name, fn, bpp in stages:
+        fn()
+        a = torch.cuda.Event(enable_timing=True)
+        b = torch.cuda.Event(enable_timing=True)
+        a.record()
+        for _ in range(reps):
+            fn()
+        b.record()
+        torch.cuda.synchronize()
+        t = a.elapsed_time(b) / 1e3 / reps
+        gbs = bpp * N / t
+        out[name] = {"ms": t * 1e3, "bytes_per_particle": bpp, "achieved_GBps": gbs / 1e9,
+                     "frac_of_hbm": gbs / PEAK_HBM}
+    return out
+
+
 def fma_probe(blocks=2048, iters=20000, reps=3):
     """measured unpacked FP32 FMA rate (FLOP/s) of this device: the best of
     reps launches of vpmc_probe_fma, timed with HIP events on the stream"""
@@ -458,6 +504,8 @@ def large_bench(args, device):
     err = float(torch.sqrt(((um - ud) ** 2).sum() / (ud ** 2).sum()))
     log("large: RMS error vs FP64 blob direct sum at %d probes: %.4g" % (probes.shape[0], err))
     del vs, um, ud
+    hbm = hbm_stages(sim)
+    log("large: HBM-bound stages", hbm)
     # launch shape: a few candidates, one timed step each, on the one
     # simulation (its buffers are re-laid for each grid)
     tune = {}
@@ -503,6 +551,7 @@ def large_bench(args, device):
            "particle_steps_per_s": n * K / el, "estimator_ms": est_s * 1e3,
            "rms_err_vs_direct": err, "probes": int(probes.shape[0]),
            "rbf_iters": it, "rbf_rel_res": rr, "rbf_status": st, "rbf_seconds": t_cg,
+           "hbm_stages": hbm,
            "cells_per_hw": sim.cells_per_hw, "spl": sim.spl, "eval_threads": sim.eval_threads,
            "tune_ms": tune,
            "roofline": {"kernel": "k_bin_eval", "bound": "valu", "achieved": flop / eval_s,
@@ -798,6 +847,8 @@ def main():
     flop = FLOP_PER_EVAL * n_in + (FLOP_PER_SAMPLE * S if args.algo == "direct" else 0)
     achieved = flop / eval_avg_s
     traffic = read_traffic(eval_kernel, args)
+    hbm = hbm_stages(sim)
+    log("HBM-bound stages (plain workload)", hbm)
     fma_rate = fma_probe()
     log("measured unpacked FP32 FMA rate %.3e FLOP/s" % fma_rate)
     if args.copy_probe:
@@ -840,6 +891,7 @@ def main():
                      "measured_unpacked_fma_flops": fma_rate,
                      "frac_of_measured_unpacked": achieved / fma_rate},
         "kernel_evals_per_s": n_in / eval_avg_s,
+        "hbm_stages": hbm,
         "estimator_ms": est_avg_s * 1e3,
         "estimator_fraction_of_step": est_avg_s / (elapsed / args.steps),
         "error": err,
