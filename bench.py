@@ -622,6 +622,23 @@ def adjoint_bench(args, device):
     n = pos.shape[0]
     log("adjoint: loss %.6g, forward %.3f s, backward %.3f s (%d steps, %d particles)"
         % (float(loss.detach()), t_fwd, t_bwd, T, n))
+    # the same gradient along the FP64 trajectory of the frozen-sample step
+    p2 = pos_t.clone().requires_grad_(True)
+    wt2 = w0.clone().requires_grad_(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    XT2 = A.rollout(sim, p2, wt2, T, checkpoint=K, trajectory="f64")
+    loss2 = ((A.grid_vorticity(XT2, wt2, h, G, G, f64=True) - ref) ** 2).mean()
+    torch.cuda.synchronize()
+    t_fwd64 = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    loss2.backward()
+    torch.cuda.synchronize()
+    t_bwd64 = time.perf_counter() - t0
+    grads = {"": (p2.grad.double().cpu().numpy(), wt2.grad.double().cpu().numpy()),
+             "_f32traj": (gp, gw)}
+    log("adjoint (FP64 trajectory): loss %.9g, forward %.3f s, backward %.3f s"
+        % (float(loss2.detach()), t_fwd64, t_bwd64))
     out = {"workload": "Taylor-Green + 0.1 N(0,1) strength noise, Nx=Ny=%d (%d particles), "
                        "hwr=16, N1=N2=64, dt=1e-3, T=%d, checkpoint every %d, loss = MSE of "
                        "the final B-spline vorticity on a 64x64 grid vs the unperturbed run"
@@ -629,7 +646,9 @@ def adjoint_bench(args, device):
            "loss": float(loss.detach()), "forward_s": t_fwd, "backward_s": t_bwd,
            "forward_particle_steps_per_s": n * T / t_fwd,
            "adjoint_particle_steps_per_s": n * T / t_bwd,
-           "grad_pos_max": float(np.abs(gp).max()), "grad_w_max": float(np.abs(gw).max())}
+           "grad_pos_max": float(np.abs(gp).max()), "grad_w_max": float(np.abs(gw).max()),
+           "f64_trajectory": {"loss": float(loss2.detach()), "forward_s": t_fwd64,
+                              "backward_s": t_bwd64}}
     if args.adj_fd > 0:
         p64 = pos_t.double()
         w64 = w0.double()
@@ -649,32 +668,41 @@ def adjoint_bench(args, device):
         # eps = 1e-4 as specified, and 1e-6: the frozen-sample velocity has the
         # 1/r Biot-Savart kernel at branch-1 samples close to their owner, so
         # the wider difference carries truncation error of its own
+        # headline: the gradient along the FP64 trajectory ("" keys); the
+        # FP32-trajectory gradient ("_f32traj") beside it
         for tag, eps_x, eps_w in (("", 1e-4, 1e-4 * float(w0.abs().max())),
                                   ("_eps1e-6", 1e-6, 1e-6 * float(w0.abs().max()))):
-            ex, ew, rel = [], [], []
+            fds = []
             for c, i in enumerate(pick):
                 if c % 2 == 0:
                     k = int(axis[c])
                     a, b = p64.clone(), p64.clone()
                     a[i, k] += eps_x
                     b[i, k] -= eps_x
-                    fd = (L(a, w64) - L(b, w64)) / (2 * eps_x)
-                    g = gp[i, k]
-                    ex.append(abs(fd - g) / np.abs(gp).max())
+                    fds.append((L(a, w64) - L(b, w64)) / (2 * eps_x))
                 else:
                     a, b = w64.clone(), w64.clone()
                     a[i] += eps_w
                     b[i] -= eps_w
-                    fd = (L(p64, a) - L(p64, b)) / (2 * eps_w)
-                    g = gw[i]
-                    ew.append(abs(fd - g) / np.abs(gw).max())
-                rel.append(abs(fd - g) / max(abs(fd), 1e-300))
-            log("adjoint vs FP64 central differences (eps %.0e): max err/max|g| positions %.3g, "
-                "strengths %.3g; median per-coordinate rel err %.3g"
-                % (eps_x, max(ex), max(ew), np.median(rel)))
-            out.update({"fd_max_err_pos" + tag: max(ex), "fd_max_err_w" + tag: max(ew),
-                        "fd_median_rel_err" + tag: float(np.median(rel)),
-                        "fd_max_rel_err" + tag: float(np.max(rel))})
+                    fds.append((L(p64, a) - L(p64, b)) / (2 * eps_w))
+            for gtag, (gpx, gwx) in grads.items():
+                ex, ew, rel = [], [], []
+                for c, i in enumerate(pick):
+                    fd = fds[c]
+                    if c % 2 == 0:
+                        g = gpx[i, int(axis[c])]
+                        ex.append(abs(fd - g) / np.abs(gpx).max())
+                    else:
+                        g = gwx[i]
+                        ew.append(abs(fd - g) / np.abs(gwx).max())
+                    rel.append(abs(fd - g) / max(abs(fd), 1e-300))
+                log("adjoint%s vs FP64 central differences (eps %.0e): max err/max|g| "
+                    "positions %.3g, strengths %.3g; median per-coordinate rel err %.3g"
+                    % (gtag, eps_x, max(ex), max(ew), np.median(rel)))
+                k2 = tag + gtag
+                out.update({"fd_max_err_pos" + k2: max(ex), "fd_max_err_w" + k2: max(ew),
+                            "fd_median_rel_err" + k2: float(np.median(rel)),
+                            "fd_max_rel_err" + k2: float(np.max(rel))})
         out["fd_seconds"] = time.perf_counter() - t0
     del sim
     torch.cuda.empty_cache()

@@ -86,7 +86,10 @@ def _loss_setup(device, steps):
     return sim, pos, w, h, G, ref
 
 
-def test_rollout_adjoint_matches_fp64_finite_differences(device):
+@pytest.mark.parametrize("traj", ["f32", "f64"])
+def test_rollout_adjoint_matches_fp64_finite_differences(device, traj):
+    """dL/dx0, dL/dw of a checkpointed rollout vs central differences of the
+    FP64 forward; along the FP64 trajectory the bound is 10x tighter"""
     import torch
 
     from vortexparticlemc_cuda_amd import adjoint as A
@@ -94,8 +97,8 @@ def test_rollout_adjoint_matches_fp64_finite_differences(device):
     sim, pos, w, h, G, ref = _loss_setup(device, steps)
     p = torch.tensor(pos, device=device, requires_grad=True)
     wt = torch.tensor(w, device=device, requires_grad=True)
-    XT = A.rollout(sim, p, wt, steps, checkpoint=3)
-    om = A.grid_vorticity(XT, wt, h, G, G)
+    XT = A.rollout(sim, p, wt, steps, checkpoint=3, trajectory=traj)
+    om = A.grid_vorticity(XT, wt, h, G, G, f64=(traj == "f64"))
     loss = ((om.double() - ref) ** 2).mean()
     loss.backward()
     gp = p.grad.double().cpu().numpy()
@@ -128,9 +131,10 @@ def test_rollout_adjoint_matches_fp64_finite_differences(device):
         b[i] -= eps_w
         fd = (L(p64, a) - L(p64, b)) / (2 * eps_w)
         errs_w.append(abs(fd - gw[i]) / np.abs(gw).max())
-    print("rollout adjoint vs FD: max rel err positions %.3g strengths %.3g"
-          % (max(errs_x), max(errs_w)))
-    assert max(errs_x) < 1e-4 and max(errs_w) < 1e-4
+    print("rollout adjoint (%s trajectory) vs FD: max rel err positions %.3g strengths %.3g"
+          % (traj, max(errs_x), max(errs_w)))
+    tol = 1e-4 if traj == "f32" else 1e-5
+    assert max(errs_x) < tol and max(errs_w) < tol
 
 
 def test_adjoint_is_deterministic(device):

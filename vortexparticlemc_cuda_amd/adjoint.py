@@ -2,7 +2,8 @@
 library's hand-written adjoint kernels).
 
     mc_velocity(sim, pos, w)                   u = v+ - v- at the particles
-    rollout(sim, pos0, w, steps, checkpoint)   positions after `steps` steps
+    rollout(sim, pos0, w, steps, checkpoint, trajectory)
+                                               positions after `steps` steps
     grid_vorticity(pos, w, h_w, gnx, gny)      omega_h on a gnx x gny grid
 
 Derivatives are taken under common random numbers with the samples frozen
@@ -96,18 +97,53 @@ def _run(sim, t0, t1, keep=None):
     sim.step_index = t1
 
 
+def _step_f64(sim, X, w64, t, ws):
+    """one frozen-sample step of the FP64 positions X in place (vpmc_step_f64)"""
+    P = sim.P
+    d = sim.desc(P.N1, P.N2)
+    d.step = t
+    need = lib().vpmc_step_f64_workspace_bytes(C.byref(d))
+    if ws is None or ws.numel() < need:
+        ws = ops.workspace(need, sim.device)
+    check(lib().vpmc_step_f64(C.byref(d), sim.N, ops._p(X), ops._p(w64), ops._p(sim.omega),
+                              ops._p(sim.sub["sub_pos"]), ops._p(sim.sub["sub_neg"]),
+                              float(P.dt), ops._p(ws), ws.numel(), ops._stream()),
+          "vpmc_step_f64")
+    return ws
+
+
+def _run_f64(sim, X, w64, t0, t1, keep=None):
+    ws = None
+    for t in range(t0, t1):
+        if keep is not None:
+            keep.append(X.clone())
+        ws = _step_f64(sim, X, w64, t, ws)
+    sim.step_index = t1
+
+
 class _Rollout(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, pos0, w, sim, steps, checkpoint):
-        sim.pos.copy_(pos0.detach())
+    def forward(ctx, pos0, w, sim, steps, checkpoint, traj64):
         _set_nominal(sim, w)
         K = max(1, int(checkpoint))
         ckpt = {}
-        for t0 in range(0, steps, K):
-            ckpt[t0] = sim.pos.clone()
-            _run(sim, t0, min(steps, t0 + K))
-        ctx.sim, ctx.steps, ctx.K, ctx.ckpt = sim, steps, K, ckpt
-        return sim.pos.clone()
+        if traj64:
+            X = pos0.detach().to(torch.float64).contiguous().clone()
+            w64 = w.detach().to(torch.float64).contiguous()
+            for t0 in range(0, steps, K):
+                ckpt[t0] = X.clone()
+                _run_f64(sim, X, w64, t0, min(steps, t0 + K))
+            ctx.w64 = w64
+            out = X
+        else:
+            sim.pos.copy_(pos0.detach())
+            for t0 in range(0, steps, K):
+                ckpt[t0] = sim.pos.clone()
+                _run(sim, t0, min(steps, t0 + K))
+            out = sim.pos.clone()
+        ctx.sim, ctx.steps, ctx.K, ctx.ckpt, ctx.traj64 = sim, steps, K, ckpt, traj64
+        ctx.in_dtype = pos0.dtype
+        return out
 
     @staticmethod
     def backward(ctx, gX):
@@ -117,9 +153,13 @@ class _Rollout(torch.autograd.Function):
         P = sim.P
         for t0 in reversed(range(0, steps, K)):
             t1 = min(steps, t0 + K)
-            sim.pos.copy_(ctx.ckpt[t0])
             states = []
-            _run(sim, t0, t1, keep=states)
+            if ctx.traj64:   # recompute in FP64, linearise at the rounded states
+                X = ctx.ckpt[t0].clone()
+                _run_f64(sim, X, ctx.w64, t0, t1, keep=states)
+            else:
+                sim.pos.copy_(ctx.ckpt[t0])
+                _run(sim, t0, t1, keep=states)
             for t in reversed(range(t0, t1)):
                 sim.pos.copy_(states[t - t0])
                 d = sim.desc(P.N1, P.N2)
@@ -131,24 +171,35 @@ class _Rollout(torch.autograd.Function):
                                               ops._p(lam), ops._p(gw), ops._p(ws), ws.numel(),
                                               ops._stream()), "vpmc_step_adjoint")
         sim.step_index = steps
-        return lam, gw, None, None, None
+        return lam.to(ctx.in_dtype), gw, None, None, None, None
 
 
-def rollout(sim, pos0, w, steps, checkpoint=10):
+def rollout(sim, pos0, w, steps, checkpoint=10, trajectory="f32"):
     """positions after `steps` plain steps (Philox steps 0 .. steps-1) from
-    pos0 with strengths w; differentiable in pos0 and w"""
-    return _Rollout.apply(pos0, w, sim, int(steps), int(checkpoint))
+    pos0 with strengths w; differentiable in pos0 and w.
+
+    trajectory="f32": the FP32 step (vpmc_step); the positions carry FP32
+    rounding accumulated over the steps, which bounds how well the gradient
+    matches the exact derivative (about 1e-4 of its largest entry on a loss
+    that compares nearly equal fields).  trajectory="f64": the positions
+    advance by the FP64 forward of the same frozen-sample step
+    (vpmc_step_f64) and are returned in FP64; the step adjoints then
+    linearise at those states rounded to FP32 (about 1e-5)."""
+    if trajectory not in ("f32", "f64"):
+        raise ValueError("trajectory must be 'f32' or 'f64'")
+    return _Rollout.apply(pos0, w, sim, int(steps), int(checkpoint), trajectory == "f64")
 
 
 class _GridVorticity(torch.autograd.Function):
     @staticmethod
     def forward(ctx, pos, w, h_w, gnx, gny, L, cells_per_hw, f64):
+        in_dtype = pos.dtype
+        if f64:   # FP64 sums at the positions as given (FP64 ones stay FP64)
+            out = grid_vorticity_f64(pos.detach(), w.detach(), h_w, gnx, gny, L)
         pos = pos.detach().to(torch.float32).contiguous()
         w = w.detach().to(torch.float32).contiguous()
         n = pos.shape[0]
-        if f64:
-            out = grid_vorticity_f64(pos, w, h_w, gnx, gny, L)
-        else:
+        if not f64:
             nx, ny = ops.field_grid(h_w, L, L, cells_per_hw)
             ws = ops.workspace(lib().vpmc_grid_vorticity_workspace_bytes(n, nx, ny), pos.device)
             out = torch.empty(gnx * gny, dtype=torch.float32, device=pos.device)
@@ -157,6 +208,7 @@ class _GridVorticity(torch.autograd.Function):
                                             ops._stream()), "vpmc_grid_vorticity")
         ctx.save_for_backward(pos, w)
         ctx.args = (h_w, gnx, gny, L)
+        ctx.in_dtype = in_dtype
         return out
 
     @staticmethod
@@ -170,7 +222,7 @@ class _GridVorticity(torch.autograd.Function):
                                                 ops._p(g.to(torch.float32).contiguous()),
                                                 ops._p(gpos), ops._p(gw), ops._stream()),
               "vpmc_grid_vorticity_adjoint")
-        return gpos, gw, None, None, None, None, None, None
+        return gpos.to(ctx.in_dtype), gw, None, None, None, None, None, None
 
 
 def grid_vorticity(pos, w, h_w, gnx, gny, L=1.0, cells_per_hw=8, f64=False):
