@@ -27,7 +27,7 @@ class View(C.Structure):
     _fields_ = [("px", C.c_void_p), ("py", C.c_void_p), ("w", C.c_void_p), ("N", C.c_int),
                 ("vp_sort", C.c_void_p), ("cell_acc", C.c_void_p), ("rshx", C.c_int),
                 ("rshy", C.c_int), ("Lx", C.c_float), ("Ly", C.c_float),
-                ("cnt_view", C.c_int), ("sub", C.c_void_p)]
+                ("cnt_view", C.c_int), ("sub", C.c_void_p), ("F", C.c_void_p)]
 
 
 class McParams(C.Structure):
@@ -217,7 +217,8 @@ class Problem:
     """A particle set with its sign subsets and reference-grid hashes, ready
     for the oracle's query and estimator.  Arrays are kept alive here."""
 
-    def __init__(self, px, py, w, rshx, rshy, Lx=1.0, Ly=1.0, cdf_override=None):
+    def __init__(self, px, py, w, rshx, rshy, Lx=1.0, Ly=1.0, cdf_override=None, F=None):
+        self.F = None if F is None else np.ascontiguousarray(F, np.float32).reshape(-1, 4)
         self.px = np.ascontiguousarray(px, np.float32)
         self.py = np.ascontiguousarray(py, np.float32)
         self.w = np.ascontiguousarray(w, np.float32)
@@ -240,7 +241,8 @@ class Problem:
             self._keep += [sub_, vps_, acc, cdf_]
             v = View(self.px.ctypes.data, self.py.ctypes.data, self.w.ctypes.data, self.n,
                      vps_.ctypes.data, acc.ctypes.data, rshx, rshy, self.Lx, self.Ly,
-                     int(sub.size), sub_.ctypes.data)
+                     int(sub.size), sub_.ctypes.data,
+                     None if self.F is None else self.F.ctypes.data)
             self.sets.append(SignSet(int(sub.size), sub_.ctypes.data, cdf_.ctypes.data,
                                      float(s["cum_" + key]), v))
 

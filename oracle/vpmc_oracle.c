@@ -23,6 +23,9 @@
  *   orc_kernel_smooth_d Eisenstein closed form G4 = Gamma(1/4)^8/(960 pi^2),
  *                       brute-force lattice sums, and convergence of
  *                       orc_direct_blob to the analytic Taylor-Green velocity.
+ *   orc_eval_vorticity  with F (the CUDA query's deformed kernel): F = I and
+ *   (deformed)          rotations equal the undeformed FP64 brute force; one
+ *                       blob at x_j + F u equals |w| W(|u|) (closed form).
  *   orc_mc_velocity     (reference mode) oracle/_ref monte_carlo_bs_cpu replayed
  *                       on the same std::mt19937 draws; (consistent mode) its
  *                       Monte Carlo mean converges to orc_direct_blob.
@@ -196,7 +199,56 @@ float orc_eval_vorticity(const orc_view *v, float qx, float qy, int sign, int us
         }                                                                           \
     } while (0)
 
-    if (v->vp_sort && v->cell_acc) {
+    /* the CUDA step's deformed kernel: d = x_I - p by minimum image,
+       dloc = inv(F_I) d, rad = |dloc|, skipped beyond 2 h_w
+       (include/vorticity_query.cuh:52-76) */
+#define ORC_VISIT_F(I)                                                              \
+    do {                                                                            \
+        const float *Fi = v->F + 4 * (size_t)(I);                                   \
+        float det = Fi[0] * Fi[3] - Fi[1] * Fi[2];                                  \
+        float dx = v->px[I] - px, dy = v->py[I] - py;                               \
+        if (periodic) {                                                             \
+            if (dx > 0.5f * v->Lx) dx -= v->Lx;                                     \
+            if (dx < -0.5f * v->Lx) dx += v->Lx;                                    \
+            if (dy > 0.5f * v->Ly) dy -= v->Ly;                                     \
+            if (dy < -0.5f * v->Ly) dy += v->Ly;                                    \
+        }                                                                           \
+        float lx = (Fi[3] * dx - Fi[1] * dy) / det;                                 \
+        float ly = (-Fi[2] * dx + Fi[0] * dy) / det;                                \
+        float rad = hypotf(lx, ly);                                                 \
+        if (!(rad > 2.f * h_w) && !(exclude_center && rad < 1e-6f)) {               \
+            float wv = v->w[I];                                                     \
+            float wval = use_signed ? wv : fabsf(wv);                               \
+            if (!((sign > 0 && wv <= 0) || (sign < 0 && wv >= 0)))                  \
+                sum += wval * orc_spline_f(rad, h_w);                               \
+        }                                                                           \
+    } while (0)
+
+    if (v->F && v->vp_sort && v->cell_acc) {   /* include/vorticity_query.cuh:31-49 */
+        int cx = (int)(px / v->Lx * v->rshx);
+        int cy = (int)(py / v->Ly * v->rshy);
+        if (cx > v->rshx - 1) cx = v->rshx - 1;
+        if (cy > v->rshy - 1) cy = v->rshy - 1;
+        for (int ox = -1; ox <= 1; ++ox)
+            for (int oy = -1; oy <= 1; ++oy) {
+                int gx = cx + ox, gy = cy + oy;
+                if (periodic) {
+                    gx = (gx % v->rshx + v->rshx) % v->rshx;
+                    gy = (gy % v->rshy + v->rshy) % v->rshy;
+                } else if (gx < 0 || gy < 0 || gx >= v->rshx || gy >= v->rshy) {
+                    continue;
+                }
+                int c = gx * v->rshy + gy;
+                int beg = v->cell_acc[c];
+                int end = (c + 1 < v->rshx * v->rshy) ? v->cell_acc[c + 1] : v->cnt_view;
+                for (int k = beg; k < end; ++k) {
+                    int I = v->sub[v->vp_sort[k]];
+                    ORC_VISIT_F(I);
+                }
+            }
+    } else if (v->F) {
+        for (int I = 0; I < v->N; ++I) ORC_VISIT_F(I);
+    } else if (v->vp_sort && v->cell_acc) {
         int cx = (int)(px * v->rshx / v->Lx);
         int cy = (int)(py * v->rshy / v->Ly);
         if (cx > v->rshx - 1) cx = v->rshx - 1;
@@ -222,6 +274,7 @@ float orc_eval_vorticity(const orc_view *v, float qx, float qy, int sign, int us
         for (int I = 0; I < v->N; ++I) ORC_VISIT(I);
     }
 #undef ORC_VISIT
+#undef ORC_VISIT_F
     return sum;
 }
 
@@ -480,8 +533,15 @@ void orc_mc_velocity(const orc_mc_params *P, const orc_sign_set *S, int sign_idx
                 int pI = S->sub[idx];
                 float rho = (float)orc_mass_inverse((double)u[1]) * P->h_w;
                 float th = 2.f * PI_F * u[2];
-                spx = px[pI] + rho * cosf(th);
-                spy = py[pI] + rho * sinf(th);
+                float jx = rho * cosf(th), jy = rho * sinf(th);
+                if (S->view.F) {   /* F_j u, src/cuda/monte_carlo_bs.cu:210-212 */
+                    const float *F = S->view.F + 4 * (size_t)pI;
+                    float ex = F[0] * jx + F[1] * jy, ey = F[2] * jx + F[3] * jy;
+                    jx = ex;
+                    jy = ey;
+                }
+                spx = px[pI] + jx;
+                spy = py[pI] + jy;
                 rx = dstx - spx; ry = dsty - spy;
                 if (per) {
                     if (rx > 0.5f * P->Lx) rx -= P->Lx; else if (rx < -0.5f * P->Lx) rx += P->Lx;

@@ -56,6 +56,10 @@ typedef struct {
     float        Lx, Ly;
     int          cnt_view;      /* end of the last cell                          */
     const int   *sub;           /* local -> global                              */
+    const float *F;             /* NULL, or per global particle the deformation
+                                   F = (a b; c d) as 4 floats: the CUDA query's
+                                   inv(F) d with minimum image and 2 h_w cutoff
+                                   (include/vorticity_query.cuh:52-66)           */
 } orc_view;
 
 float orc_eval_vorticity(const orc_view *v, float px, float py, int sign, int use_signed,

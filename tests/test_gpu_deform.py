@@ -360,3 +360,25 @@ def test_deformed_step_updates_F_from_the_five_estimates(device):
     Fb = b.jac[:b.N].double().cpu().numpy()
     assert np.isfinite(v.cpu().numpy()).all()
     assert np.abs(Fb[:, 0] * Fb[:, 3] - Fb[:, 1] * Fb[:, 2] - 1).max() < 1e-5
+
+
+@pytest.mark.parametrize("algo", [0, 1])
+def test_deformed_estimator_matches_oracle(device, algo):
+    """consistent estimator with F on the same Philox stream: GPU (binned and
+    per-owner) == the oracle's restatement of the deformed query and jitter
+    (include/vorticity_query.cuh:52-66, src/cuda/monte_carlo_bs.cu:210-212)"""
+    import oracle as O
+    from vortexparticlemc_cuda_amd._capi import MODE_CONSISTENT
+    print("[case] deformed estimator vs oracle algo=%d" % algo, flush=True)
+    x, y, w, hw = tg_problem(32, 4, seed=71)
+    F = _rand_F(x.size, np.random.default_rng(72), 1.7)
+    sim = _deformed_sim(device, x, y, w, F, 32, 4, algo, seed=0x5EED)
+    sim.step_index = 5
+    vg = sim.estimate(48, 48).cpu().numpy()
+    over = dict(cdf_pos=sim.sub["cdf_pos"][:sim.cnt[0]].cpu().numpy(),
+                cdf_neg=sim.sub["cdf_neg"][:sim.cnt[1]].cpu().numpy(),
+                cum_pos=sim.sub["totals"][0].item(), cum_neg=sim.sub["totals"][1].item())
+    rsh = O.rsh_of(32, 4)
+    prob = O.Problem(x, y, w, rsh, rsh, cdf_override=over, F=F)
+    vo = prob.velocities(prob.params(MODE_CONSISTENT, 48, 48, hw, seed=0x5EED, step=5))
+    assert rel_l2(vg, vo) < 1e-5, rel_l2(vg, vo)

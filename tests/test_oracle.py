@@ -365,3 +365,58 @@ def test_cg_closed_form_and_dense():
     bj = O.tg_vorticity(xj, yj)
     w, it, rr = O.cg_rbf(xj, yj, bj, hw, tol=1e-12)
     assert it < 0 or rel_l2(A @ w, bj) < 1e-11
+
+
+# --------------------------------------------------------------------- deformed query
+
+def _det1_F(n, rng, smax):
+    a, b = rng.uniform(0, 2 * np.pi, n), rng.uniform(0, 2 * np.pi, n)
+    s = rng.uniform(1.0, smax, n)
+    ca, sa, cb, sb = np.cos(a), np.sin(a), np.cos(b), np.sin(b)
+    return np.stack([ca * s * cb - sa * sb / s, -ca * s * sb - sa * cb / s,
+                     sa * s * cb + ca * sb / s, -sa * s * sb + ca * cb / s], 1).astype(np.float32)
+
+
+def _W64(r, h):
+    q = r / h
+    sig = 40.0 / (7 * np.pi * h * h)
+    return np.where(q <= 0.5, sig * (1 - 6 * q ** 2 + 6 * q ** 3),
+                    np.where(q < 1, 2 * sig * (1 - q) ** 3, 0.0))
+
+
+def test_deformed_query_reduces_to_the_undeformed_one():
+    """the oracle's deformed query (include/vorticity_query.cuh:52-66) with F = I
+    and with rotations == the FP64 min-image brute force of the undeformed sum"""
+    x, y, w, hw = tg_problem(32, 4, seed=61)
+    rsh = O.rsh_of(32, 4)
+    rng = np.random.default_rng(62)
+    q = rng.random((300, 2)).astype(np.float32)
+    d = x[None, :].astype(np.float64) - q[:, :1]
+    e = y[None, :].astype(np.float64) - q[:, 1:]
+    d -= np.round(d)
+    e -= np.round(e)
+    Wm = _W64(np.hypot(d, e), hw)
+    th = rng.uniform(0, 2 * np.pi, x.size)
+    for F in (np.tile(np.array([1, 0, 0, 1], np.float32), (x.size, 1)),
+              np.stack([np.cos(th), -np.sin(th), np.sin(th), np.cos(th)], 1)):
+        prob = O.Problem(x, y, w, rsh, rsh, F=F)
+        for sign, sel in ((1, w > 0), (-1, w < 0)):
+            got = prob.eval(q[:, 0], q[:, 1], sign, hw)
+            exp = Wm[:, sel] @ np.abs(w[sel].astype(np.float64))
+            assert rel_l2(got, exp) < 2e-6, rel_l2(got, exp)
+
+
+def test_deformed_query_single_blob_closed_form():
+    """one particle with F: the field at x_j + F u is |w| W(|u|)"""
+    rng = np.random.default_rng(63)
+    hw = np.float32(0.125)
+    for trial in range(5):
+        F = _det1_F(1, rng, 1.9)
+        xj = rng.random(2).astype(np.float32)
+        prob = O.Problem(xj[:1], xj[1:], np.array([0.8], np.float32), 4, 4, F=F)
+        u = rng.uniform(-1, 1, (400, 2)) * float(hw)
+        Fm = F.reshape(2, 2).astype(np.float64)
+        p = (xj[None, :] + u @ Fm.T) % 1.0
+        got = prob.eval(p[:, 0], p[:, 1], 1, hw)
+        exp = 0.8 * _W64(np.hypot(u[:, 0], u[:, 1]), float(hw))
+        np.testing.assert_allclose(got, exp, rtol=0, atol=2e-5 * exp.max())
