@@ -150,7 +150,8 @@ def copy_probe(nbytes, reps=3):
 
 
 # algorithmic HBM bytes per particle of the HBM-bound stages (hash/field build,
-# |omega| CDF, advection), as the survey of the reference counts them
+# |omega| CDF, advection): positions, strengths and indices read, records and
+# keys written once
 HBM_BYTES_FIELD = 28.0
 HBM_BYTES_CDF = 16.0
 HBM_BYTES_ADVECT = 24.0

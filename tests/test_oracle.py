@@ -333,7 +333,9 @@ def test_eps_direct_sum_brute_force():
 
 # --------------------------------------------------------------------- CG
 
-def test_lambda11_survey_values():
+def test_lambda11_known_values():
+    """lambda_11 h^2 of the Taylor-Green mode on the 100 x 100 and 200 x 200
+    lattices at hwr = 16 (independently computed values)"""
     assert abs(O.tg_lambda11(100, 16) * 1e-4 - 0.92289) < 1e-5
     assert abs(O.tg_lambda11(200, 16) * 2.5e-5 - 0.98020) < 1e-5
 
