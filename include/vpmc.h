@@ -306,6 +306,12 @@ typedef struct vpmc_ensemble_desc {
     float       *vel_xy;              /* device out (float2), or NULL           */
     void        *eval_events[2];      /* optional hipEvent_t pair around the
                                          estimator stages                       */
+    /* device: per-particle deformation F (float4, row-major), or NULL.  With
+       it every step is the CUDA step's (vpmc_step with jac): fields carry the
+       quadratic forms of F, branch-1 jitter is F u, each owner is also
+       estimated at its four aux targets (dx = h_w / 2), and F is replaced by
+       update_jacobian_from5 of the five estimates before the move. */
+    float       *jac;
 } vpmc_ensemble_desc;
 
 size_t vpmc_ensemble_workspace_bytes(const vpmc_ensemble_desc *e);

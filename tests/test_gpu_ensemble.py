@@ -86,3 +86,34 @@ def test_ensemble_graph_replays_match_eager(device):
             assert torch.isfinite(scratch)
         g.close()
     assert g.step_index == 6
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("spl,threads", [(2, 256), (1, 128)])
+def test_deformed_ensemble_equals_plain_path(device, spl, threads):
+    """with kernel deformation (aux targets, F in fields and jitter, F from
+    the five estimates each step) every simulation of the ensemble follows
+    the single-simulation deformed step bit for bit: positions, velocities, F"""
+    from vortexparticlemc_cuda_amd.ensemble import Ensemble
+    from vortexparticlemc_cuda_amd.sim import Simulation
+    _say("deformed ensemble == plain, spl=%d threads=%d" % (spl, threads))
+    N1, N2, steps, seed = 24, 24, 3, 79
+    data = _ensemble_data(sizes=(24, 32, 40))   # 2 h_w (the deformed reach) < L / 2
+    ens = Ensemble([d[1] for d in data], [d[2] for d in data], [d[0].h_w for d in data],
+                   N1=N1, N2=N2, dt=float(data[0][0].dt), seed=seed, cells_per_hw=8, spl=spl,
+                   eval_threads=threads, device=device, deform=True)
+    ens.step(steps)
+    torch.cuda.synchronize()
+    for s, (P, pos, w) in enumerate(data):
+        P.N1, P.N2 = N1, N2
+        sim = Simulation(P, device=device, cells_per_hw=8, seed=seed, sim=s, spl=spl,
+                         eval_threads=threads, deform=True)
+        sim.set_particles(pos, w)
+        for _ in range(steps):
+            v = sim.step()
+        torch.cuda.synchronize()
+        assert np.array_equal(ens.positions(s), sim.pos.cpu().numpy()), s
+        assert np.array_equal(ens.velocities(s), v.cpu().numpy()), s
+        F = ens.jacobians(s)
+        assert np.array_equal(F, sim.jac[:sim.N].cpu().numpy()), s
+        assert not np.allclose(F, [1, 0, 0, 1])

@@ -63,7 +63,7 @@ class EnsembleDesc(C.Structure):
     _fields_ = [("n_sim", ci), ("h_offset", vp), ("h_h_w", vp), ("h_L", vp), ("N1", ci),
                 ("N2", ci), ("mode", ci), ("cells_per_hw", ci), ("spl", ci),
                 ("eval_threads", ci), ("seed", C.c_uint64), ("dt", cf), ("pos_xy", vp), ("omega", vp),
-                ("vel_xy", vp), ("eval_events", vp * 2)]
+                ("vel_xy", vp), ("eval_events", vp * 2), ("jac", vp)]
 
 
 _lib = None

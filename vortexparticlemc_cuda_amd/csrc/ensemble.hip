@@ -69,7 +69,8 @@ __global__ void k_ens_entries(EnsTabs t, int n_sim, int E, const int *__restrict
 __global__ void k_ens_field_keys(EnsTabs t, int E, uint32_t dead, const int *__restrict__ ent_p,
                                  const int *__restrict__ ent_sg, const float2 *__restrict__ pos,
                                  const float *__restrict__ omega, uint32_t *__restrict__ key,
-                                 float4 *__restrict__ rec, float2 *__restrict__ set_xy)
+                                 float4 *__restrict__ rec, float2 *__restrict__ set_xy,
+                                 const float4 *__restrict__ jac, float4 *__restrict__ set_jac)
 {
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= E) return;
@@ -86,8 +87,32 @@ __global__ void k_ens_field_keys(EnsTabs t, int E, uint32_t dead, const int *__r
     const int ix = min(max((int)floorf(x.x * g.inv_cx), 0), g.nx - 1);
     const int iy = min(max((int)floorf(x.y * g.inv_cy), 0), g.ny - 1);
     key[e] = (uint32_t)(t.coff[sg] + ix * g.ny + iy);
-    rec[e] = make_float4(x.x, x.y, fabsf(omega[p]), 0.f);
+    // deformed: the record's 4th word carries its entry (for its quadratic form)
+    rec[e] = make_float4(x.x, x.y, fabsf(omega[p]), jac ? __uint_as_float((uint32_t)e) : 0.f);
     set_xy[e] = x;
+    if (jac) set_jac[e] = jac[p];
+}
+
+// deformed fields: the quadratic form of each sorted record's particle
+__global__ void k_ens_field_gq(int E, const float4 *__restrict__ pts,
+                               const int *__restrict__ ent_p, const float4 *__restrict__ jac,
+                               float4 *__restrict__ gq)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= E) return;
+    const int p = ent_p[__float_as_uint(pts[k].w)];
+    gq[k] = quad_form(p >= 0 ? jac[p] : make_float4(1.f, 0.f, 0.f, 1.f));
+}
+
+// F of every particle from its simulation's five estimates, before the move
+__global__ void k_ens_update_jac(EnsTabs t, int n_sim, int N, float dt, float4 *__restrict__ jac)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    const int s = seg_of(t.poff, n_sim, i);
+    const McD &d = t.sim[s];
+    jac[i] = jacobian_from5(d.n_owner, i - t.poff[s], (const float2 *)d.vel_aux,
+                            0.5f * dt / d.aux_dx);
 }
 
 template <int MODE>
@@ -112,7 +137,7 @@ __global__ void __launch_bounds__(BIN_THREADS)
     bin_gen_one<MODE, true>(d, g, sid, out, (uint32_t)t.toff[s], dk, key, q, r, om);
 }
 
-template <int THREADS, int SPL>
+template <int THREADS, int SPL, bool DEF>
 __global__ void __launch_bounds__(THREADS)
     k_ens_eval(EnsTabs t, int n_sim, int ntile, int per_xcd, const int *__restrict__ start,
                const float4 *__restrict__ srec, float *__restrict__ om)
@@ -123,10 +148,10 @@ __global__ void __launch_bounds__(THREADS)
     if (beg >= end) return;   // uniform over the workgroup
     const int s = seg_of(t.toff, n_sim, tile);
     if (tile - t.toff[s] >= t.geom[s].ntile) return;   // the dead tile: no query
-    __shared__ TileLds<THREADS> L;
+    __shared__ TileLds<THREADS, DEF> L;
     EvalCount ec;
-    bin_eval_tile<THREADS, SPL, false>(t.sim[s], t.geom[s], tile - t.toff[s], beg, end, srec, om,
-                                       L, ec);
+    bin_eval_tile<THREADS, SPL, false, DEF>(t.sim[s], t.geom[s], tile - t.toff[s], beg, end,
+                                            srec, om, L, ec);
 }
 
 template <int MODE>
@@ -169,6 +194,7 @@ __global__ void k_ens_step_add(McD *__restrict__ sims, int n_sim, uint32_t add, 
 struct EnsLayout {
     int n_sim = 0, N = 0, E = 0, C = 0, T = 0, per_xcd = 0;
     int local_bits = 1, nblk = 0;   // the sample sort: bits of a local tile key, blocks
+    bool def = false;               // kernel deformation (desc.jac)
     size_t S = 0;
     std::vector<int4> blk;          // the sample sort's block table (host copy)
     std::vector<uint32_t> blk_base;
@@ -190,6 +216,8 @@ struct EnsLayout {
     float4 *srec = nullptr;
     int *tstart = nullptr;
     int4 *sblk = nullptr;
+    float4 *fgq = nullptr, *set_jac = nullptr;   // deformed fields / sets
+    float2 *vel_aux = nullptr;
     uint32_t *sblk_base = nullptr;
     void *ssort_ws = nullptr, *init_ws = nullptr;
     size_t ssort_bytes = 0, init_bytes = 0;
@@ -213,7 +241,8 @@ static int ens_plan(const vpmc_ensemble_desc *e, EnsLayout *L)
     L->ny.assign(ns, 0);
     L->glog2.assign(ns, 0);
     L->goff.assign(2 * ns + 1, 0);
-    const size_t NS = (size_t)(e->N1 + e->N2);
+    L->def = e->jac != nullptr;
+    const size_t NS = (size_t)samples_per_pair(e->N1, e->N2, L->def ? 1 : 0);
     for (int s = 0; s < ns; ++s) {
         const int n = L->poff[s + 1] - L->poff[s];
         if (n <= 0 || !(e->h_h_w[s] > 0.f) || !(e->h_L[s] > 0.f)) return VPMC_ERR_ARG;
@@ -302,6 +331,9 @@ static size_t ens_carve(EnsLayout &L, void *ws, size_t bytes)
     L.ssort_bytes = radix_seg_workspace_bytes((int)S, L.nblk);
     L.ssort_ws = cv.take<char>(L.ssort_bytes);
     L.vel_sign = cv.take<float2>(2 * N);
+    L.fgq = cv.take<float4>(L.def ? E : 1);
+    L.set_jac = cv.take<float4>(L.def ? E : 1);
+    L.vel_aux = cv.take<float2>(L.def ? 8 * N : 1);
     L.init_bytes = subsets_ws_max(L);
     L.init_ws = cv.take<char>(L.init_bytes);
     return cv.used + 4096;
@@ -353,6 +385,16 @@ static vpmc_mc_desc ens_sim_desc(const vpmc_ensemble_desc *e, const EnsLayout &L
         S.field.pts = (const float *)L.fpts;            // global positions ...
         S.field.cell_start = L.fstart + L.coff[sg];      // ... indexed by global offsets
         S.field.order = nullptr;
+        if (L.def) {   // the CUDA step's F (fields' forms, the sets' jitter)
+            S.field.gq = (const float *)L.fgq;
+            S.field.gq_bound = 0.f;
+            S.set_jac = (const float *)(L.set_jac + L.eoff[sg]);
+        }
+    }
+    if (L.def) {   // the four aux targets at dst -/+ dx, dx = h_w / 2 (SimParam::dx)
+        d.aux = 1;
+        d.aux_dx = 0.5f * e->h_h_w[s];
+        d.vel_aux = (float *)(L.vel_aux + 8 * (size_t)L.poff[s]);
     }
     return d;
 }
@@ -360,13 +402,20 @@ static vpmc_mc_desc ens_sim_desc(const vpmc_ensemble_desc *e, const EnsLayout &L
 static int ens_fields(const vpmc_ensemble_desc *e, EnsLayout &L, hipStream_t st)
 {
     const unsigned gb = (unsigned)((L.E + 255) / 256);
+    const float4 *jac = L.def ? (const float4 *)e->jac : nullptr;
     hipLaunchKernelGGL(k_ens_field_keys, dim3(gb), dim3(256), 0, st, L.t, L.E, (uint32_t)L.C,
                        (const int *)L.ent_p, (const int *)L.ent_sg, (const float2 *)e->pos_xy,
-                       e->omega, L.fkey, L.frec, L.set_xy);
+                       e->omega, L.fkey, L.frec, L.set_xy, jac, L.set_jac);
     VPMC_CHECK_LAUNCH();
-    return radix_sort_to_records(L.fkey, L.E, key_bits_for((uint32_t)L.C), nullptr, L.frec,
-                                 L.fpts, L.fkeys_sorted, L.C, L.fstart, L.fsort_ws, L.fsort_bytes,
-                                 st);
+    VPMC_TRY(radix_sort_to_records(L.fkey, L.E, key_bits_for((uint32_t)L.C), nullptr, L.frec,
+                                   L.fpts, L.fkeys_sorted, L.C, L.fstart, L.fsort_ws,
+                                   L.fsort_bytes, st));
+    if (L.def) {
+        hipLaunchKernelGGL(k_ens_field_gq, dim3(gb), dim3(256), 0, st, L.E,
+                           (const float4 *)L.fpts, (const int *)L.ent_p, jac, L.fgq);
+        VPMC_CHECK_LAUNCH();
+    }
+    return VPMC_OK;
 }
 
 template <int MODE>
@@ -381,19 +430,25 @@ static int ens_estimate(const vpmc_ensemble_desc *e, EnsLayout &L, hipStream_t s
                                   L.skeys_sorted, L.T, L.tstart, L.sblk, L.sblk_base, L.nblk,
                                   L.ssort_ws, L.ssort_bytes, st));
     const int th = e->eval_threads ? e->eval_threads : 256;
-#define VPMC_ENS_EVAL(TH, SP)                                                                 \
-    hipLaunchKernelGGL((k_ens_eval<TH, SP>), dim3((unsigned)(8 * L.per_xcd)), dim3(TH), 0, st,  \
-                       L.t, ns, L.T, L.per_xcd, (const int *)L.tstart, (const float4 *)L.srec, \
-                       L.om)
-#define VPMC_ENS_EVAL_SPL(TH)                        \
-    do {                                             \
-        if (e->spl == 1) VPMC_ENS_EVAL(TH, 1);       \
-        else if (e->spl == 4) VPMC_ENS_EVAL(TH, 4);  \
-        else VPMC_ENS_EVAL(TH, 2);                   \
+#define VPMC_ENS_EVAL(TH, SP, DF)                                                              \
+    hipLaunchKernelGGL((k_ens_eval<TH, SP, DF>), dim3((unsigned)(8 * L.per_xcd)), dim3(TH), 0,  \
+                       st, L.t, ns, L.T, L.per_xcd, (const int *)L.tstart,                    \
+                       (const float4 *)L.srec, L.om)
+#define VPMC_ENS_EVAL_SPL(TH, DF)                        \
+    do {                                                 \
+        if (e->spl == 1) VPMC_ENS_EVAL(TH, 1, DF);       \
+        else if (e->spl == 4) VPMC_ENS_EVAL(TH, 4, DF);  \
+        else VPMC_ENS_EVAL(TH, 2, DF);                   \
     } while (0)
-    if (th == 64) VPMC_ENS_EVAL_SPL(64);
-    else if (th == 128) VPMC_ENS_EVAL_SPL(128);
-    else VPMC_ENS_EVAL_SPL(256);
+#define VPMC_ENS_EVAL_TH(DF)                               \
+    do {                                                   \
+        if (th == 64) VPMC_ENS_EVAL_SPL(64, DF);           \
+        else if (th == 128) VPMC_ENS_EVAL_SPL(128, DF);    \
+        else VPMC_ENS_EVAL_SPL(256, DF);                   \
+    } while (0)
+    if (L.def) VPMC_ENS_EVAL_TH(true);
+    else VPMC_ENS_EVAL_TH(false);
+#undef VPMC_ENS_EVAL_TH
 #undef VPMC_ENS_EVAL_SPL
 #undef VPMC_ENS_EVAL
     VPMC_CHECK_LAUNCH();
@@ -492,6 +547,11 @@ int vpmc_ensemble_step(const vpmc_ensemble_desc *e, void *workspace, size_t work
     if (e->mode == VPMC_MODE_REFERENCE) VPMC_TRY(ens_estimate<VPMC_MODE_REFERENCE>(e, L, st));
     else VPMC_TRY(ens_estimate<VPMC_MODE_CONSISTENT>(e, L, st));
     VPMC_TRY(record_event(e->eval_events[1], st));
+    if (L.def) {   // F from the five estimates, before the move (vpmc_step with jac)
+        hipLaunchKernelGGL(k_ens_update_jac, dim3((unsigned)((L.N + 255) / 256)), dim3(256), 0,
+                           st, L.t, L.n_sim, L.N, e->dt, (float4 *)e->jac);
+        VPMC_CHECK_LAUNCH();
+    }
     hipLaunchKernelGGL(k_ens_advect, dim3((unsigned)((L.N + 255) / 256)), dim3(256), 0, st, L.t,
                        L.n_sim, L.N, e->dt, (float2 *)e->pos_xy, (float2 *)e->vel_xy);
     hipLaunchKernelGGL(k_ens_step_add, dim3((unsigned)((L.n_sim + 63) / 64)), dim3(64), 0, st,

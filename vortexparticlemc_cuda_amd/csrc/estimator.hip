@@ -75,27 +75,12 @@ __global__ void __launch_bounds__(MC_THREADS) k_mc_estimate(McD d)
 
 // update_jacobian_from5 + renorm_det1 (src/cuda/monte_carlo_bs.cu:274-288,
 // include/monte_carlo_bs.cuh:140-144)
-__global__ void k_update_jacobian(int n, const float2 *__restrict__ vs,
-                                  const float2 *__restrict__ va, float c,
+__global__ void k_update_jacobian(int n, const float2 *__restrict__ va, float c,
                                   float4 *__restrict__ F)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    float2 u[4];   // L, R, T, B: v+ - v-
-#pragma unroll
-    for (int t = 0; t < 4; ++t) {
-        const float2 p = va[(size_t)t * n + i], q = va[(size_t)(4 + t) * n + i];
-        u[t] = make_float2(p.x - q.x, p.y - q.y);
-    }
-    float4 f;
-    f.x = 1.f + (u[1].x - u[0].x) * c;
-    f.y = (u[2].x - u[3].x) * c;
-    f.z = (u[1].y - u[0].y) * c;
-    f.w = 1.f + (u[2].y - u[3].y) * c;
-    const float det = f.x * f.w - f.y * f.z;
-    const float sc = 1.f / sqrtf(fmaxf(det, 1e-12f));   // det F -> 1 (incompressible)
-    F[i] = make_float4(f.x * sc, f.y * sc, f.z * sc, f.w * sc);
-    (void)vs;
+    F[i] = jacobian_from5(n, i, va, c);
 }
 
 __global__ void k_advect(int n, float2 *__restrict__ pos, const float2 *__restrict__ vs,
@@ -292,8 +277,7 @@ int vpmc_update_jacobian(int n, const float *vel_sign, const float *vel_aux, flo
     if (n == 0) return VPMC_OK;
     if (!vel_aux || !jac_out) return VPMC_ERR_ARG;
     hipLaunchKernelGGL(k_update_jacobian, dim3((n + 255) / 256), dim3(256), 0, as_stream(stream),
-                       n, (const float2 *)vel_sign, (const float2 *)vel_aux, 0.5f * dt / dx,
-                       (float4 *)jac_out);
+                       n, (const float2 *)vel_aux, 0.5f * dt / dx, (float4 *)jac_out);
     VPMC_CHECK_LAUNCH();
     return VPMC_OK;
 }

@@ -275,6 +275,29 @@ __device__ __forceinline__ void mis_contrib(const McD &d, float om, float rx, fl
     }
 }
 
+// update_jacobian_from5 + renorm_det1 (src/cuda/monte_carlo_bs.cu:274-288,
+// include/monte_carlo_bs.cuh:140-144) for owner j of n: vs/va are the sign
+// estimates ([sign][owner]) and the aux ones ([sign][L, R, T, B][owner]);
+// c = dt / (2 dx)
+__device__ __forceinline__ float4 jacobian_from5(int n, int j, const float2 *__restrict__ va,
+                                                 float c)
+{
+    float2 u[4];   // L, R, T, B: v+ - v-
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        const float2 p = va[(size_t)t * n + j], q = va[(size_t)(4 + t) * n + j];
+        u[t] = make_float2(p.x - q.x, p.y - q.y);
+    }
+    float4 f;   // explicit fmaf: the same rounding in every kernel inlining this
+    f.x = fmaf(u[1].x - u[0].x, c, 1.f);
+    f.y = (u[2].x - u[3].x) * c;
+    f.z = (u[1].y - u[0].y) * c;
+    f.w = fmaf(u[2].y - u[3].y, c, 1.f);
+    const float det = fmaf(f.x, f.w, -(f.y * f.z));
+    const float sc = 1.f / sqrtf(fmaxf(det, 1e-12f));   // det F -> 1 (incompressible)
+    return make_float4(f.x * sc, f.y * sc, f.z * sc, f.w * sc);
+}
+
 // u = v+ - v-, x += u dt, periodic wrap (src/cpu/simulation_step.cpp:46-71)
 __device__ __forceinline__ float2 advect_one(float2 a, float2 b, float2 p, float2 *vel_out,
                                              float dt, float Lx, float Ly, int per)

@@ -106,15 +106,18 @@ struct FieldGeom {           // device-side copy of vpmc_field's geometry
 // |F^-1 d|^2 = G11 dx^2 + 2 G12 dx dy + G22 dy^2, and W(|F^-1 d|) > 0 only
 // for |d| < sigma_max(F) h_w.  sigma_max^2 is the larger eigenvalue of
 // F F^T = (p r; r q): (p + q + sqrt((p - q)^2 + 4 r^2)) / 2, a sum of
-// squares under the root (no cancellation near rotations).
+// squares under the root (no cancellation near rotations).  Every multiply-add
+// is an explicit fmaf, so that each kernel inlining this rounds alike (with
+// implicit contraction the compiler may fuse a different product per kernel).
 __device__ __forceinline__ float4 quad_form(float4 F)
 {
-    const float det = F.x * F.w - F.y * F.z;
+    const float det = fmaf(F.x, F.w, -(F.y * F.z));
     const float i2 = 1.f / (det * det);
-    const float p = F.x * F.x + F.y * F.y, q = F.z * F.z + F.w * F.w;
-    const float r = F.x * F.z + F.y * F.w;
-    const float smax2 = 0.5f * (p + q + sqrtf((p - q) * (p - q) + 4.f * r * r));
-    return make_float4(q * i2, -2.f * r * i2, p * i2, smax2 * (1.f + 1e-5f));
+    const float p = fmaf(F.x, F.x, F.y * F.y), q = fmaf(F.z, F.z, F.w * F.w);
+    const float r = fmaf(F.x, F.z, F.y * F.w);
+    const float pq = p - q;
+    const float smax2 = 0.5f * ((p + q) + sqrtf(fmaf(pq, pq, 4.f * (r * r))));
+    return make_float4(q * i2, -2.f * (r * i2), p * i2, smax2 * (1.f + 1e-5f));
 }
 int field_geom(const vpmc_field *f, float h_w, FieldGeom *g);
 int field_build_impl(int nx, int ny, float Lx, float Ly, int n, const int *n_dev,

@@ -18,9 +18,11 @@ from ._capi import MODE_CONSISTENT, EnsembleDesc, check, lib
 class Ensemble:
     def __init__(self, positions, strengths, h_w, L=1.0, N1=64, N2=64, dt=5e-4,
                  mode=MODE_CONSISTENT, seed=0, cells_per_hw=8, spl=0, eval_threads=0,
-                 device="cuda"):
+                 device="cuda", deform=False):
         """positions[s]: (N_s, 2) array, strengths[s]: (N_s,), h_w: per-simulation
-        kernel radius (scalar or sequence), L: box side (scalar or sequence)"""
+        kernel radius (scalar or sequence), L: box side (scalar or sequence);
+        deform: the CUDA step's kernel deformation (F per particle, self.jac,
+        starting at I; aux targets each step)"""
         self.device = torch.device(device)
         ns = len(positions)
         assert ns == len(strengths) and ns > 0
@@ -50,6 +52,12 @@ class Ensemble:
         d.pos_xy = ops._p(self.pos)
         d.omega = ops._p(self.omega)
         d.vel_xy = ops._p(self.vel)
+        self.jac = None
+        if deform:
+            self.jac = torch.zeros((max(self.N, 1), 4), dtype=torch.float32, device=self.device)
+            self.jac[:, 0] = 1.0
+            self.jac[:, 3] = 1.0
+            d.jac = ops._p(self.jac)
         self.desc = d
         need = lib().vpmc_ensemble_workspace_bytes(C.byref(d))
         if need == 0:
@@ -118,6 +126,11 @@ class Ensemble:
         if s is None:
             return p
         return p[self.offsets[s]:self.offsets[s + 1]]
+
+    def jacobians(self, s=None):
+        """F per particle (N, 4), or of simulation s"""
+        F = self.jac[:self.N].cpu().numpy()
+        return F if s is None else F[self.offsets[s]:self.offsets[s + 1]]
 
     def velocities(self, s=None):
         v = self.vel.cpu().numpy()
