@@ -404,6 +404,21 @@ def ensemble_bench(args, device):
     batched = N * K / min(t_eager, t_graph)
     del ens
     torch.cuda.empty_cache()
+    # the same ensemble with kernel deformation (the CUDA step: four aux
+    # targets per owner, F in the fields and the jitter, F from the five
+    # estimates), same launch shape
+    ensd = Ensemble(pos, w, hw, N1=N1, N2=N2, dt=dt, seed=0, cells_per_hw=cph, spl=spl,
+                    eval_threads=th, device=device, deform=True)
+    ensd.step(1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ensd.step(K)
+    torch.cuda.synchronize()
+    t_def = time.perf_counter() - t0
+    ensd.close()
+    del ensd
+    torch.cuda.empty_cache()
+    log("batched deformed: %.1f ms/step" % (t_def / K * 1e3))
     # plain path: one Simulation per member, stepped one after the other
     sims = []
     for s in range(len(pos)):
@@ -454,6 +469,8 @@ def ensemble_bench(args, device):
             "batched_over_plain_direct": batched / plain_direct,
             "rms_err_vs_direct": err, "cells_per_hw": cph, "spl": spl, "eval_threads": th,
             "autotune_ms": tune_ms,
+            "deformed": {"ms_per_step": t_def / K * 1e3, "particle_steps_per_s": N * K / t_def,
+                         "samples_per_pair": N1 + 5 * N2},
             "cg_init_s": t_cg}
 
 

@@ -61,17 +61,22 @@ def test_ensemble_equals_plain_path(device, spl, threads):
 
 
 @pytest.mark.gpu
-def test_ensemble_graph_replays_match_eager(device):
+@pytest.mark.parametrize("deform", [False, True])
+def test_ensemble_graph_replays_match_eager(device, deform):
+    """replays of a captured 2-step graph equal eager steps bit for bit (with
+    kernel deformation: F, updated inside the graph, as well)"""
     from vortexparticlemc_cuda_amd.ensemble import Ensemble
-    _say("ensemble graph replay")
-    data = _ensemble_data(sizes=(20, 28), seed=9)
+    _say("ensemble graph replay, deform=%d" % deform)
+    data = _ensemble_data(sizes=(20, 28), seed=9)   # 2 h_w < L / 2 for the deformed reach
     args = ([d[1] for d in data], [d[2] for d in data], [d[0].h_w for d in data])
-    kw = dict(N1=16, N2=16, dt=1e-3, seed=3, device=device)
+    kw = dict(N1=16, N2=16, dt=1e-3, seed=3, device=device, deform=deform)
     eager = Ensemble(*args, **kw)
-    ref = []
+    ref, refF = [], []
     for _ in range(3):
         eager.step(2)
         ref.append(eager.positions().copy())
+        if deform:
+            refF.append(eager.jacobians().copy())
     side = torch.cuda.Stream()
     with torch.cuda.stream(side):
         g = Ensemble(*args, **kw)
@@ -80,6 +85,8 @@ def test_ensemble_graph_replays_match_eager(device):
             torch.cuda.synchronize()
             got = g.positions()
             assert np.array_equal(got, ref[k]), (k, rel_l2(got, ref[k]))
+            if deform:
+                assert np.array_equal(g.jacobians(), refF[k]), k
             # eager work of the program between replays
             scratch = torch.randn(4096, device=device).sum()
             torch.cuda.synchronize()

@@ -153,10 +153,12 @@ def _tune_cache_path():
 
 
 def autotune(positions, strengths, h_w, L=1.0, N1=64, N2=64, mode=MODE_CONSISTENT,
-             candidates=TUNE_CANDIDATES, sample_sims=8, steps=2, use_cache=True, device="cuda"):
+             candidates=TUNE_CANDIDATES, sample_sims=8, steps=2, use_cache=True, device="cuda",
+             deform=False):
     """(cells_per_hw, spl, eval_threads) with the shortest step on a subset of the ensemble
     (the first `sample_sims` simulations), timed with HIP events after one
-    warm-up step; results are cached per GPU and workload shape"""
+    warm-up step; results are cached per GPU and workload shape (deform: the
+    deformed step, whose search reaches 2 h_w)"""
     import json
     import os
     sizes = tuple(int(np.asarray(p).shape[0]) for p in positions)
@@ -164,7 +166,7 @@ def autotune(positions, strengths, h_w, L=1.0, N1=64, N2=64, mode=MODE_CONSISTEN
         torch.cuda.get_device_name(device), sizes[:sample_sims],
         tuple(round(float(v), 7) for v in np.broadcast_to(np.asarray(h_w, np.float32),
                                                             (len(sizes),))[:sample_sims]),
-        N1, N2, mode)
+        N1, N2, mode) + ("|deform" if deform else "")
     path = _tune_cache_path()
     cache = {}
     if use_cache and os.path.exists(path):
@@ -181,7 +183,8 @@ def autotune(positions, strengths, h_w, L=1.0, N1=64, N2=64, mode=MODE_CONSISTEN
     for cph, spl, th in candidates:
         try:
             e = Ensemble(positions[:sample_sims], strengths[:sample_sims], hw, L=L, N1=N1, N2=N2,
-                         mode=mode, cells_per_hw=cph, spl=spl, eval_threads=th, device=device)
+                         mode=mode, cells_per_hw=cph, spl=spl, eval_threads=th, device=device,
+                         deform=deform)
         except Exception:   # grid or workgroup invalid for this candidate
             continue
         e.step(1)
