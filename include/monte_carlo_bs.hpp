@@ -1,10 +1,11 @@
 // monte_carlo_bs.hpp -- Monte Carlo Biot-Savart estimator interface.
-// Replaces include/monte_carlo_bs.hpp:7-102 for the GPU path.  MCBSEstimate,
-// MCBSCtx and the inline kernel helpers keep the reference's names and
-// meaning; MCBSCtx points at device buffers.  The per-particle CPU estimator
-// monte_carlo_bs_cpu and its std::mt19937 samplers belong to the reference's
-// CPU backend and are not part of this library; monte_carlo_bs_gpu evaluates
-// the estimator for every particle of a Simulation at once.
+// Replaces include/monte_carlo_bs.hpp:7-102.  MCBSEstimate, MCBSCtx,
+// monte_carlo_bs_cpu, the std::mt19937 samplers and the inline kernel helpers
+// keep the reference's names, signatures and meaning; MCBSCtx points at
+// device buffers.  monte_carlo_bs_cpu consumes the caller's generator exactly
+// as the reference does and evaluates the estimate on the GPU (reference
+// mode, replaying those draws); monte_carlo_bs_gpu evaluates the estimator for
+// every particle of a Simulation at once on Philox streams.
 #pragma once
 
 #include <cmath>
@@ -39,7 +40,39 @@ struct MCBSCtx {
     double cum_neg = 1.0;
     VorticityView pos_view;
     VorticityView neg_view;
+    // subset sizes (no reference counterpart: the reference's CDF vectors are
+    // exactly that long); -1: cdf_pos->size(), cdf_neg->size()
+    int cnt_pos = -1;
+    int cnt_neg = -1;
 };
+
+// src/cpu/monte_carlo_bs.cpp:28-128: the estimate at dst for one sign (+1 / -1)
+// and, with auxiliary != 0, at the four given targets, from N1 branch-1 and N2
+// branch-2 samples per target.  rng advances by exactly the draws the
+// reference takes, in its order: per branch-1 sample the CDF draw and, when
+// periodic, two image draws; per branch-2 sample two draws for the disk at dst
+// and, with auxiliary, two for each of left, right, top, bottom.  The sums run
+// on the GPU (VPMC_MODE_REFERENCE replaying those draws; ctx.pos_view and
+// neg_view are not read).  An empty sign set gives zero (the reference reads
+// sub[-1]).  Blocking: returns after the device work.
+MCBSEstimate monte_carlo_bs_cpu(const float2 &dst, int sign, int auxiliary,
+                                const float2 &dst_left, const float2 &dst_right,
+                                const float2 &dst_top, const float2 &dst_bottom,
+                                const ParticleSet &P, const MCBSCtx &ctx, int N1, int N2,
+                                std::mt19937 &rng);
+
+// src/cpu/monte_carlo_sampling.cpp:9-45, host utilities over host arrays
+// (the reference's CPU samplers; no GPU path calls them):
+// kernel radius by linear interpolation of the L-entry inverse CDF, times h
+float sample_kernel_radius(std::mt19937 &gen, const float *icdf, int L, float h);
+// upper_bound(cdf, cdf + N, u) for one draw u, N - 1 when past the end
+int sample_vorticity_idx_cdf(std::mt19937 &gen, const float *cdf, int N);
+// area-uniform point of the disk (center, R): angle draw first, then radius
+float2 sample_disk_biot_savart(std::mt19937 &gen, const float2 &center, float R);
+
+// the context src/cpu/simulation_step.cpp:13-27 builds from a Simulation
+// (subsets, FP64 CDFs, totals, views; counts set), for monte_carlo_bs_cpu
+MCBSCtx make_mcbs_ctx(const Simulation &sim, bool periodic);
 
 // K(x, y) = (-(x-y)_y, (x-y)_x) / (2 pi (|x-y|^2 + 1e-8))   (:60-66)
 HD inline float2 biot_savart_kernel(const float2 &x, const float2 &y)

@@ -4,9 +4,11 @@
 // Replaces include/vortex_particle_mc.hpp:16-201.  Names, fields and the
 // SimParam construction semantics are the reference's; the implementation
 // (src/cxx_api.cpp in the package's csrc/) calls the HIP kernels through the C
-// ABI of vpmc.h.  The CPU entry points of the reference (step_cpu,
-// build_subsets_cpu, eval_vorticity_cpu) belong to its CPU backend and are not
-// part of this library; step_cpu / build_subsets_cpu throw std::logic_error.
+// ABI of vpmc.h.  The reference's CPU-backend entry points keep their
+// semantics with the arithmetic on the GPU: step_cpu draws the CPU step's
+// std::mt19937 stream (monte_carlo_bs_cpu per particle and sign, in its order),
+// build_subsets_cpu builds the subsets (FP64 CDFs), evalVorticityAbs_cpu
+// queries a view.
 #pragma once
 
 #include <cassert>
@@ -85,6 +87,8 @@ struct VorticityView {
     const int *sub;       // local id -> global id
 };
 
+struct MCBSCtx;
+
 struct SimParam {
     float dt = 0.001f;
     float Lx = 1.f;
@@ -146,12 +150,17 @@ public:
     void do_spatial_hashing();
     void build_global_hash();
 
-    void build_subsets_cpu();   // throws: CPU backend not provided
+    void build_subsets_cpu();   // the same subsets as build_subsets_cuda (FP64 CDFs)
     void build_subsets_cuda();
     void prepare_rhs_b();
 
     std::mt19937 rng;
-    void step_cpu(bool periodic);   // throws: CPU backend not provided
+    // src/cpu/simulation_step.cpp:7-71: monte_carlo_bs_cpu for sign +1 then -1
+    // per particle with the aux targets pos_left .. pos_bottom, on a copy of
+    // rng (the member is not advanced, as in the reference), then x += u dt
+    // for the particle and its four aux points, wrapped when periodic.  One
+    // GPU launch replays the whole step's draws.
+    void step_cpu(bool periodic);
     void step_cuda(bool periodic);
 
     void compute_velocity_naive(bool periodic, float eps_core);
@@ -191,6 +200,7 @@ public:
     int subset_count(int sign) const { return sign > 0 ? cnt_pos_ : cnt_neg_; }
 
     struct Impl;
+    friend MCBSCtx make_mcbs_ctx(const Simulation &sim, bool periodic);
 
 private:
     SpatialHash hash_pos_, hash_neg_, hash_all_;
@@ -210,3 +220,9 @@ private:
 
     Impl *impl_ = nullptr;
 };
+
+// src/cpu/evaluate_vorticity.cpp:8-75: the (signed or |w|) kernel sum at p over
+// the view's 3x3 reference cells; sign filters w > 0 / w < 0 / none (0).  The
+// view holds device pointers; the query runs on the GPU (vpmc_query_view).
+float evalVorticityAbs_cpu(const VorticityView &v, const float2 &p, int sign, bool use_signed,
+                           bool exclude_center, float h_w, bool periodic);

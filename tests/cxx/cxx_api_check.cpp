@@ -2,10 +2,18 @@
 // the device and writes its results for tests/test_gpu_cxx_api.py.
 //   usage: vpmc_cxx_check <input.bin> <outdir>
 // input.bin: int32 Nx, int32 hwr, int32 N, float32 pos[2N], float32 b[N]
+//   usage: vpmc_cxx_check <input.bin> <outdir> mcbs
+// input.bin: int32 Nx, hwr, n, N1, N2, n_own, seed; float32 dx; float32 px[n],
+//   py[n], w[n]; int32 cnt_pos, cnt_neg; float32 cdf_pos[cnt_pos],
+//   cdf_neg[cnt_neg], cum[2]; int32 nq; float32 q[2 nq]
+// (the CPU backend's entry points: monte_carlo_bs_cpu, the samplers,
+// step_cpu, evalVorticityAbs_cpu)
+#include <cstdint>
 #include <cstdio>
 #include <cstdlib>
 #include <fstream>
 #include <iostream>
+#include <random>
 #include <string>
 #include <vector>
 
@@ -21,8 +29,112 @@ template <class T> static void dump(const std::string &path, const std::vector<T
     f.write(reinterpret_cast<const char *>(v.data()), v.size() * sizeof(T));
 }
 
+template <class T> static std::vector<T> take(std::ifstream &in, size_t n)
+{
+    std::vector<T> v(n);
+    in.read(reinterpret_cast<char *>(v.data()), n * sizeof(T));
+    return v;
+}
+
+static int run_mcbs(const char *input, const std::string &out)
+{
+    std::ifstream in(input, std::ios::binary);
+    const std::vector<int> hdr = take<int>(in, 7);
+    const int Nx = hdr[0], hwr = hdr[1], n = hdr[2], N1 = hdr[3], N2 = hdr[4], n_own = hdr[5];
+    const uint32_t seed = (uint32_t)hdr[6];
+    const float dx = take<float>(in, 1)[0];
+    const std::vector<float> px = take<float>(in, n), py = take<float>(in, n),
+                             w = take<float>(in, n);
+    const std::vector<int> cnt = take<int>(in, 2);
+    const std::vector<float> cp = take<float>(in, cnt[0]), cn = take<float>(in, cnt[1]),
+                             cum = take<float>(in, 2);
+    const int nq = take<int>(in, 1)[0];
+    const std::vector<float2> q = take<float2>(in, nq);
+    try {
+        SimParam P(1e-3f, 1.f, 1.f, Nx, (float)hwr);
+        P.N1 = N1;
+        P.N2 = N2;
+        std::vector<float2> pos(n);
+        for (int i = 0; i < n; ++i) pos[i] = make_float2(px[i], py[i]);
+        Simulation sim(P);
+        sim.set_particles(pos, w);
+
+        // monte_carlo_bs_cpu with the reference's FP32 CDFs and totals (so that
+        // both pick the same particles), owners 0 .. n_own-1, +1 then -1, one
+        // generator, aux targets at dst -/+ dx
+        MCBSCtx C = make_mcbs_ctx(sim, true);
+        std::vector<double> cpd(cp.begin(), cp.end()), cnd(cn.begin(), cn.end());
+        dvec<double> dcp(cpd), dcn(cnd);
+        C.cdf_pos = &dcp;
+        C.cdf_neg = &dcn;
+        C.cnt_pos = cnt[0];
+        C.cnt_neg = cnt[1];
+        C.cum_pos = cum[0];
+        C.cum_neg = cum[1];
+        std::mt19937 rng(seed);
+        std::vector<float> est;
+        for (int i = 0; i < n_own; ++i) {
+            const float2 d = pos[i];
+            const float2 dl{d.x - dx, d.y}, dr{d.x + dx, d.y}, dt{d.x, d.y + dx},
+                db{d.x, d.y - dx};
+            for (int sg = 0; sg < 2; ++sg) {
+                const MCBSEstimate e = monte_carlo_bs_cpu(d, sg == 0 ? +1 : -1, 1, dl, dr, dt, db,
+                                                          sim.particles_, C, N1, N2, rng);
+                for (const float2 &v : {e.main, e.left, e.right, e.top, e.bottom}) {
+                    est.push_back(v.x);
+                    est.push_back(v.y);
+                }
+            }
+        }
+        dump(out + "/mcbs_est.bin", est);
+        // the generator advanced by exactly the reference's draws
+        std::mt19937 twin(seed);
+        std::uniform_real_distribution<float> U(0.f, 1.f);
+        const size_t used = (size_t)n_own * 2 * ((size_t)N1 * 3 + (size_t)N2 * 10);
+        for (size_t k = 0; k < used; ++k) (void)U(twin);
+        const std::vector<int> rng_ok = {U(rng) == U(twin) ? 1 : 0};
+        dump(out + "/mcbs_rng.bin", rng_ok);
+
+        // the samplers on mt19937(seed)'s first draws
+        std::mt19937 g(seed);
+        std::vector<float> smp;
+        smp.push_back((float)sample_vorticity_idx_cdf(g, cp.data(), cnt[0]));
+        const float2 disk = sample_disk_biot_savart(g, make_float2(0.3f, 0.6f), 0.7f);
+        smp.push_back(disk.x);
+        smp.push_back(disk.y);
+        const float icdf[5] = {0.f, 0.3f, 0.5f, 0.7f, 1.f};
+        smp.push_back(sample_kernel_radius(g, icdf, 5, 0.25f));
+        dump(out + "/samplers.bin", smp);
+
+        // evalVorticityAbs_cpu on the reference grid's views
+        std::vector<float> qv;
+        for (int sg = 0; sg < 2; ++sg)
+            for (const float2 &p : q)
+                qv.push_back(evalVorticityAbs_cpu(sg == 0 ? sim.makePosView() : sim.makeNegView(),
+                                                  p, sg == 0 ? +1 : -1, false, false, P.h_w,
+                                                  true));
+        dump(out + "/evalq.bin", qv);
+
+        // step_cpu from mt19937(seed), FP64 subsets of the simulation
+        sim.rng = std::mt19937(seed);
+        sim.step_cpu(true);
+        dump(out + "/step_vel.bin", sim.particles_.vel.to_host());
+        dump(out + "/step_pos.bin", sim.particles_.pos.to_host());
+        dump(out + "/step_left.bin", sim.particles_.pos_left.to_host());
+        std::mt19937 fresh(seed);
+        const std::vector<int> member_ok = {sim.rng() == fresh() ? 1 : 0};
+        dump(out + "/step_rng_member.bin", member_ok);
+    } catch (const std::exception &e) {
+        std::cerr << "error: " << e.what() << std::endl;
+        return 1;
+    }
+    std::cout << "cxx mcbs ok" << std::endl;
+    return 0;
+}
+
 int main(int argc, char **argv)
 {
+    if (argc >= 4 && std::string(argv[3]) == "mcbs") return run_mcbs(argv[1], argv[2]);
     if (argc < 3) {
         std::fprintf(stderr, "usage: %s input.bin outdir\n", argv[0]);
         return 2;
@@ -110,13 +222,6 @@ int main(int argc, char **argv)
             threw = true;
         }
         if (!threw) throw std::runtime_error("Backend::CPU accepted");
-        threw = false;
-        try {
-            sim.step_cpu(true);
-        } catch (const std::logic_error &) {
-            threw = true;
-        }
-        if (!threw) throw std::runtime_error("step_cpu accepted");
     } catch (const std::exception &e) {
         std::cerr << "error: " << e.what() << std::endl;
         return 1;

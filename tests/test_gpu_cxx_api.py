@@ -84,3 +84,72 @@ def test_cxx_api_end_to_end(device, tmp_path):
     sd.reinit(tol=1e-6)
     assert np.array_equal(rd("pos_reinit.bin").reshape(-1, 2)[:sd.N], sd.pos.cpu().numpy())
     assert np.array_equal(rd("omega_reinit.bin")[:sd.N], sd.omega.cpu().numpy())
+
+
+def test_cxx_cpu_backend_entry_points_replay_the_reference(device, tmp_path):
+    """monte_carlo_bs_cpu, the std::mt19937 samplers, evalVorticityAbs_cpu and
+    step_cpu through the C++ API: the same generator draws as the reference's
+    src/cpu code, the sums on the GPU.  Compared with the reference's own
+    outputs (tests/golden, written from oracle/_ref: ref_mc_aux,
+    ref_mc_velocities with aux, ref_eval_vorticity) on mt19937(1234)."""
+    print("[case] cxx cpu-backend entry points", flush=True)
+    if not os.path.exists(BIN):
+        pytest.fail("tests/cxx/bin/vpmc_cxx_check not built (run build())")
+    G = np.load(os.path.join(ROOT, "tests", "golden", "ref_cpu_vectors.npz"))
+    x, y, w = G["px"], G["py"], G["w"]
+    N1, N2, n_own = int(G["mc_N1"]), int(G["mc_N2"]), int(G["mc_owners"])
+    Nx, hwr, seed = 32, 4, 1234
+    assert float(G["h_w"]) == hwr / Nx
+    cp, cn, cum = G["cdf_pos_f32"], G["cdf_neg_f32"], G["cum_f32"]
+    q = G["q"]
+    inp = tmp_path / "mcbs.bin"
+    with open(inp, "wb") as f:
+        np.array([Nx, hwr, x.size, N1, N2, n_own, seed], np.int32).tofile(f)
+        np.array([G["mcaux_dx"]], np.float32).tofile(f)
+        for a in (x, y, w):
+            a.astype(np.float32).tofile(f)
+        np.array([cp.size, cn.size], np.int32).tofile(f)
+        for a in (cp, cn, cum):
+            a.astype(np.float32).tofile(f)
+        np.array([len(q)], np.int32).tofile(f)
+        q.astype(np.float32).tofile(f)
+    r = subprocess.run([BIN, str(inp), str(tmp_path), "mcbs"], capture_output=True, text=True,
+                       timeout=120)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0 and "cxx mcbs ok" in r.stdout
+    rd = lambda n, t=np.float32: np.fromfile(tmp_path / n, dtype=t)  # noqa: E731
+    # five estimates per owner and sign == the reference's monte_carlo_bs_cpu
+    est = rd("mcbs_est.bin").reshape(n_own, 2, 5, 2)
+    ref = G["mcaux_est"]
+    for si in range(2):
+        for t in range(5):
+            e = rel_l2(est[:, si, t], ref[:, si, t])
+            assert e < 1e-5, (si, t, e)
+    assert rd("mcbs_rng.bin", np.int32)[0] == 1   # exactly the reference's draws taken
+    # samplers on the generator's first draws (mt19937(1234), as golden mc_aux0_U)
+    U = G["mc_aux0_U"].astype(np.float64)
+    smp = rd("samplers.bin")
+    assert int(smp[0]) == min(int(np.searchsorted(cp, np.float32(U[0]), side="right")),
+                              cp.size - 1)
+    phi, rr = 2 * np.pi * U[1], 0.7 * np.sqrt(U[2])
+    assert np.allclose(smp[1:3], [0.3 + rr * np.cos(phi), 0.6 + rr * np.sin(phi)], atol=2e-6)
+    pos4 = U[3] * 4
+    i0 = int(pos4)
+    tab = [0.0, 0.3, 0.5, 0.7, 1.0]
+    kr = (tab[i0] * (1 - (pos4 - i0)) + tab[min(i0 + 1, 4)] * (pos4 - i0)) * 0.25
+    assert abs(smp[3] - kr) < 1e-6
+    # evalVorticityAbs_cpu == the reference's query at the golden points
+    qv = rd("evalq.bin").reshape(2, -1)
+    assert rel_l2(qv[0], G["query_+1"]) < 1e-5 and rel_l2(qv[1], G["query_-1"]) < 1e-5
+    # step_cpu: particle velocities of the first owners == the reference's step
+    # velocities on mt19937(1234) with aux draws (its FP32 CDFs pick the same
+    # particles as the simulation's FP64 ones for these draws)
+    v = rd("step_vel.bin").reshape(-1, 2)
+    assert rel_l2(v[:n_own], G["mc_aux1_vel"]) < 1e-5, rel_l2(v[:n_own], G["mc_aux1_vel"])
+    p1 = rd("step_pos.bin").reshape(-1, 2)[:x.size]
+    moved = np.stack([x, y], 1).astype(np.float64) + 1e-3 * v[:x.size].astype(np.float64)
+    d = p1 - moved
+    d -= np.rint(d)
+    assert np.abs(d).max() < 1e-6 and (p1 >= 0).all() and (p1 < 1).all()
+    assert np.abs(rd("step_left.bin").reshape(-1, 2)[:x.size]).max() > 0   # aux points moved
+    assert rd("step_rng_member.bin", np.int32)[0] == 1   # the member generator is not advanced

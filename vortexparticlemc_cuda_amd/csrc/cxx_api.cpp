@@ -158,6 +158,137 @@ void cdf_to_icdf(const dvec<float> &cdf, float h_w, dvec<float> &icdf)
 }
 
 // ---------------------------------------------------------------------------
+// monte_carlo_bs_cpu on the GPU (src/cpu/monte_carlo_bs.cpp:28-128)
+//
+// One call of the reference takes from its generator N1 * b1 + N2 * 2 nt
+// draws (b1 = 3 periodic, 1 otherwise; nt = 5 targets with auxiliary, else 1):
+// per branch-1 sample the CDF draw then the two image draws, per branch-2
+// sample two draws for each target's disk in the order dst, left, right,
+// top, bottom.  Branch-1 samples are shared by the targets (the same point and
+// vorticity enter mis_contrib at each), so the call equals nt owners of the
+// reference-mode estimator with the branch-1 draws repeated and each owner's
+// own disk draws: a replay stream with two draws per branch-2 sample.
+
+namespace {
+
+struct ReplaySet {   // one sign set as the estimator reads it
+    int n = 0, nx = 0, ny = 0, glog2 = 0;
+    const double *cdf = nullptr;
+    dvec<int> count;
+    dvec<double> total;
+    dvec<int> guide, cell_start;
+    dvec<float4> pts;
+    dvec<float2> set_xy;
+};
+
+size_t mcbs_call_draws(int N1, int N2, bool periodic, bool aux)
+{
+    return (size_t)N1 * (periodic ? 3 : 1) + (size_t)N2 * 2 * (aux ? 5 : 1);
+}
+
+void mcbs_draw(std::mt19937 &rng, size_t len, float *out)
+{
+    std::uniform_real_distribution<float> U(0.f, 1.f);
+    for (size_t k = 0; k < len; ++k) out[k] = U(rng);
+}
+
+}  // namespace
+
+static void replay_set_build(ReplaySet &S, const MCBSCtx &C, int sign, const ParticleSet &P,
+                             hipStream_t st)
+{
+    const bool pos = sign > 0;
+    const dvec<int> *sub = pos ? C.sub_pos : C.sub_neg;
+    const dvec<double> *cdf = pos ? C.cdf_pos : C.cdf_neg;
+    if (!sub || !cdf)
+        throw std::invalid_argument("monte_carlo_bs_cpu: MCBSCtx needs the sign's sub and cdf");
+    int n = pos ? C.cnt_pos : C.cnt_neg;
+    if (n < 0) n = (int)cdf->size();
+    if ((size_t)n > cdf->size() || (size_t)n > sub->size())
+        throw std::invalid_argument("monte_carlo_bs_cpu: subset count beyond its buffers");
+    if (!(C.h_w > 0.f) || !(C.Lx > 0.f) || !(C.Ly > 0.f))
+        throw std::invalid_argument("monte_carlo_bs_cpu: h_w, Lx, Ly must be positive");
+    vcheck(vpmc_field_grid(C.h_w, C.Lx, C.Ly, 8, &S.nx, &S.ny), "monte_carlo_bs_cpu");
+    S.n = n;
+    S.cdf = raw_ptr(*cdf);
+    const double cum = pos ? C.cum_pos : C.cum_neg;
+    S.count.assign_host(&n, 1);
+    S.total.assign_host(&cum, 1);
+    int gl = 0;
+    while ((1 << gl) < std::max(n, 1)) ++gl;
+    S.glog2 = gl;
+    S.guide.resize((size_t(1) << gl) + 1);
+    S.pts.resize(std::max(n, 1));
+    S.set_xy.resize(std::max(n, 1));
+    S.cell_start.resize((size_t)S.nx * S.ny + 1);
+    const size_t b = vpmc_field_workspace_bytes(n, S.nx, S.ny);
+    vcheck(vpmc_field_build(S.nx, S.ny, C.Lx, C.Ly, n, raw_ptr(*sub), (const float *)P.d_p(),
+                            P.d_omega(), 1, (float *)raw_ptr(S.pts), raw_ptr(S.cell_start),
+                            nullptr, (float *)raw_ptr(S.set_xy), free_fn_scratch().get(b), b, st),
+           "monte_carlo_bs_cpu");
+    if (n > 0)
+        vcheck(vpmc_build_guide(S.cdf, raw_ptr(S.count), gl, raw_ptr(S.guide), st),
+               "monte_carlo_bs_cpu");
+}
+
+// Groups of nt targets (owner g * nt + t); u[s] holds, per group, the draws of
+// the monte_carlo_bs_cpu call for sign slot s (0: set[0], 1: set[1]).  Returns
+// vel_sign: [slot][owner].
+static std::vector<float2> replay_estimate(const std::vector<float2> &targets, int nt,
+                                           const ReplaySet *S, const std::vector<float> *u,
+                                           int N1, int N2, bool periodic, float h_w, float Lx,
+                                           float Ly, hipStream_t st)
+{
+    const int n_owner = (int)targets.size();
+    const size_t G = (size_t)n_owner / nt;
+    const size_t n1 = (size_t)N1 * (periodic ? 3 : 1), per = n1 + 2 * (size_t)N2;
+    const size_t len = n1 + 2 * (size_t)N2 * nt;
+    std::vector<float> U((size_t)n_owner * 2 * per);
+    for (size_t g = 0; g < G; ++g)
+        for (int t = 0; t < nt; ++t)
+            for (int s = 0; s < 2; ++s) {
+                const float *src = u[s].data() + g * len;
+                float *dst = U.data() + ((g * nt + t) * 2 + s) * per;
+                std::copy(src, src + n1, dst);
+                for (int k = 0; k < N2; ++k) {
+                    dst[n1 + 2 * (size_t)k] = src[n1 + (size_t)k * 2 * nt + 2 * t];
+                    dst[n1 + 2 * (size_t)k + 1] = src[n1 + (size_t)k * 2 * nt + 2 * t + 1];
+                }
+            }
+    dvec<float> dU(U);
+    dvec<float2> dT(targets), vs(2 * (size_t)n_owner);
+    vpmc_mc_desc d{};
+    d.mode = VPMC_MODE_REFERENCE;
+    d.N1 = N1;
+    d.N2 = N2;
+    d.h_w = h_w;
+    d.Lx = Lx;
+    d.Ly = Ly;
+    d.periodic = periodic ? 1 : 0;
+    d.uniforms = raw_ptr(dU);
+    d.n_owner = n_owner;
+    d.owner_xy = (const float *)raw_ptr(dT);
+    for (int s = 0; s < 2; ++s) {
+        const ReplaySet &R = S[s];
+        vpmc_sign_set &q = d.set[s];
+        q.n = R.n;
+        q.count_dev = raw_ptr(R.count);
+        q.cdf = R.cdf;
+        q.total_dev = raw_ptr(R.total);
+        q.guide = raw_ptr(R.guide);
+        q.guide_log2 = R.glog2;
+        q.set_xy = (const float *)raw_ptr(R.set_xy);
+        q.field = vpmc_field{R.nx, R.ny, Lx, Ly, R.n, (const float *)raw_ptr(R.pts),
+                             raw_ptr(R.cell_start), nullptr};
+    }
+    d.vel_sign = (float *)raw_ptr(vs);
+    d.algo = VPMC_ALGO_DIRECT;
+    vcheck(vpmc_mc_estimate(&d, st), "monte_carlo_bs_cpu");
+    vpmc_detail::hip_check(hipStreamSynchronize(st), "monte_carlo_bs_cpu");
+    return vs.to_host();
+}
+
+// ---------------------------------------------------------------------------
 // Simulation
 
 struct Simulation::Impl {
@@ -281,7 +412,9 @@ void Simulation::prepare_rhs_b()
 
 void Simulation::build_subsets_cpu()
 {
-    throw std::logic_error("build_subsets_cpu: the CPU backend is not part of libvpmc");
+    // src/cpu/build_particle_sets.cpp:6-42: the same subsets (ascending index),
+    // the CDFs and totals in FP64
+    build_subsets_cuda();
 }
 
 void Simulation::build_subsets_cuda()
@@ -458,9 +591,50 @@ void Simulation::estimate_velocity(bool periodic)
     vpmc_detail::hip_check(hipStreamSynchronize(stream), "estimate_velocity");
 }
 
-void Simulation::step_cpu(bool)
+void Simulation::step_cpu(bool periodic)
 {
-    throw std::logic_error("step_cpu: the CPU backend is not part of libvpmc");
+    const int N = particles_.N_cur;
+    if (N <= 0) return;
+    if (cdf_pos_.size() != (size_t)N) build_subsets_cuda();
+    const MCBSCtx C = make_mcbs_ctx(*this, periodic);
+    ReplaySet S[2];
+    replay_set_build(S[0], C, +1, particles_, stream);
+    replay_set_build(S[1], C, -1, particles_, stream);
+    // every call's draws first (they do not depend on the estimates), in the
+    // reference's order: per particle the +1 call, then the -1 call
+    std::mt19937 g = rng;
+    const size_t len = mcbs_call_draws(P_.N1, P_.N2, periodic, true);
+    std::vector<float> u[2];
+    u[0].resize((size_t)N * len);
+    u[1].resize((size_t)N * len);
+    for (int i = 0; i < N; ++i)
+        for (int s = 0; s < 2; ++s) mcbs_draw(g, len, u[s].data() + (size_t)i * len);
+    // targets: particle i's group is (pos, left, right, top, bottom)
+    dvec<float2> *arr[5] = {&particles_.pos, &particles_.pos_left, &particles_.pos_right,
+                            &particles_.pos_top, &particles_.pos_bottom};
+    std::vector<float2> tg((size_t)5 * N);
+    for (int t = 0; t < 5; ++t) {
+        std::vector<float2> h = arr[t]->to_host();
+        for (int i = 0; i < N; ++i) tg[(size_t)i * 5 + t] = h[i];
+    }
+    const std::vector<float2> v = replay_estimate(tg, 5, S, u, P_.N1, P_.N2, periodic, P_.h_w,
+                                                  P_.Lx, P_.Ly, stream);
+    // x += (v+ - v-) dt per target array, wrapped; vel = v+ - v- of the particles
+    const size_t M = (size_t)5 * N;
+    dvec<float2> vs(2 * (size_t)N), scratch_vel(N);
+    for (int t = 0; t < 5; ++t) {
+        std::vector<float2> h(2 * (size_t)N);
+        for (int i = 0; i < N; ++i) {
+            h[i] = v[(size_t)i * 5 + t];
+            h[(size_t)N + i] = v[M + (size_t)i * 5 + t];
+        }
+        vs.assign_host(h.data(), h.size());
+        vcheck(vpmc_advect(N, (float *)raw_ptr(*arr[t]), (const float *)raw_ptr(vs),
+                           t == 0 ? (float *)particles_.d_v() : (float *)raw_ptr(scratch_vel),
+                           P_.dt, P_.Lx, P_.Ly, periodic ? 1 : 0, stream),
+               "step_cpu");
+        vpmc_detail::hip_check(hipStreamSynchronize(stream), "step_cpu");
+    }
 }
 
 void Simulation::step_cuda(bool periodic)
@@ -637,6 +811,112 @@ void monte_carlo_bs_gpu(Simulation &sim, int N1, int N2, std::vector<MCBSEstimat
     std::vector<float2> v = sim.particles_.vel.to_host();
     est.assign(sim.particles_.N_cur, MCBSEstimate{});
     for (int i = 0; i < sim.particles_.N_cur; ++i) est[i].main = v[i];
+}
+
+MCBSCtx make_mcbs_ctx(const Simulation &sim, bool periodic)
+{
+    MCBSCtx C;
+    C.sub_pos = &sim.sub_pos_;
+    C.sub_neg = &sim.sub_neg_;
+    C.cdf_pos = &sim.cdf_pos_;
+    C.cdf_neg = &sim.cdf_neg_;
+    C.icdf_kernel = &sim.icdf_kernel_;
+    C.kernel_L = sim.icdf_L_;
+    C.h_w = sim.P_.h_w;
+    C.periodic = periodic;
+    C.Lx = sim.P_.Lx;
+    C.Ly = sim.P_.Ly;
+    C.InvLx = 1.f / sim.P_.Lx;
+    C.InvLy = 1.f / sim.P_.Ly;
+    C.cum_pos = sim.cum_pos_;
+    C.cum_neg = sim.cum_neg_;
+    C.pos_view = sim.makePosView();
+    C.neg_view = sim.makeNegView();
+    C.cnt_pos = sim.cnt_pos_;
+    C.cnt_neg = sim.cnt_neg_;
+    return C;
+}
+
+MCBSEstimate monte_carlo_bs_cpu(const float2 &dst, int sign, int auxiliary,
+                                const float2 &dst_left, const float2 &dst_right,
+                                const float2 &dst_top, const float2 &dst_bottom,
+                                const ParticleSet &P, const MCBSCtx &ctx, int N1, int N2,
+                                std::mt19937 &rng)
+{
+    if (N1 < 0 || N2 < 0 || N1 + N2 <= 0)
+        throw std::invalid_argument("monte_carlo_bs_cpu: N1, N2 >= 0 with N1 + N2 > 0");
+    ReplaySet S[2];
+    replay_set_build(S[0], ctx, sign, P, nullptr);
+    const int nt = auxiliary ? 5 : 1;
+    std::vector<float> u[2];
+    u[0].resize(mcbs_call_draws(N1, N2, ctx.periodic, auxiliary != 0));
+    mcbs_draw(rng, u[0].size(), u[0].data());
+    u[1] = u[0];
+    // the set occupies slot 0; slot 1 (the other sign) is given no particles
+    S[1].cdf = S[0].cdf;
+    S[1].nx = S[0].nx;
+    S[1].ny = S[0].ny;
+    S[1].glog2 = 0;
+    const int zero = 0;
+    const double one = 1.0;
+    S[1].count.assign_host(&zero, 1);
+    S[1].total.assign_host(&one, 1);
+    S[1].guide.resize(2);
+    S[1].pts.resize(1);
+    S[1].set_xy.resize(1);
+    S[1].cell_start.resize((size_t)S[1].nx * S[1].ny + 1);
+    const std::vector<float2> tg = {dst, dst_left, dst_right, dst_top, dst_bottom};
+    const std::vector<float2> v =
+        replay_estimate(std::vector<float2>(tg.begin(), tg.begin() + nt), nt, S, u, N1, N2,
+                        ctx.periodic, ctx.h_w, ctx.Lx, ctx.Ly, nullptr);
+    MCBSEstimate e;
+    e.main = v[0];
+    if (auxiliary) {
+        e.left = v[1];
+        e.right = v[2];
+        e.top = v[3];
+        e.bottom = v[4];
+    }
+    return e;
+}
+
+float sample_kernel_radius(std::mt19937 &gen, const float *icdf, int L, float h)
+{
+    std::uniform_real_distribution<float> U(0.f, 1.f);
+    const float pos = U(gen) * float(L - 1);   // position in the table
+    const int lo = int(pos), hi = std::min(lo + 1, L - 1);
+    const float f = pos - float(lo);
+    return (icdf[lo] * (1.f - f) + icdf[hi] * f) * h;
+}
+
+int sample_vorticity_idx_cdf(std::mt19937 &gen, const float *cdf, int N)
+{
+    std::uniform_real_distribution<float> U(0.f, 1.f);
+    const float u = U(gen);
+    const int j = int(std::upper_bound(cdf, cdf + N, u) - cdf);
+    return j < N ? j : N - 1;
+}
+
+float2 sample_disk_biot_savart(std::mt19937 &gen, const float2 &center, float R)
+{
+    std::uniform_real_distribution<float> U(0.f, 1.f);
+    const float phi = 2.f * float(M_PI) * U(gen);
+    const float r = R * std::sqrt(U(gen));
+    return make_float2(center.x + r * std::cos(phi), center.y + r * std::sin(phi));
+}
+
+float evalVorticityAbs_cpu(const VorticityView &v, const float2 &p, int sign, bool use_signed,
+                           bool exclude_center, float h_w, bool periodic)
+{
+    const vpmc_view q{(const float *)v.pos, v.w, v.N, v.vp_sort, v.cell_acc, v.rshx, v.rshy,
+                      v.Lx, v.Ly, v.cnt_view, v.sub};
+    dvec<float2> dp(std::vector<float2>{p});
+    dvec<float> out(1);
+    vcheck(vpmc_query_view(&q, 1, (const float *)raw_ptr(dp), sign, use_signed ? 1 : 0,
+                           exclude_center ? 1 : 0, h_w, periodic ? 1 : 0, raw_ptr(out), nullptr),
+           "evalVorticityAbs_cpu");
+    vpmc_detail::hip_check(hipDeviceSynchronize(), "evalVorticityAbs_cpu");
+    return out.to_host()[0];
 }
 
 // ---------------------------------------------------------------------------
