@@ -2,7 +2,7 @@
 // the device and writes its results for tests/test_gpu_cxx_api.py.
 //   usage: vpmc_cxx_check <input.bin> <outdir>
 // input.bin: int32 Nx, int32 hwr, int32 N, float32 pos[2N], float32 b[N]
-//   usage: vpmc_cxx_check <input.bin> <outdir> mcbs
+//   usage: vpmc_cxx_check <input.bin> <outdir> mcbs | hash
 // input.bin: int32 Nx, hwr, n, N1, N2, n_own, seed; float32 dx; float32 px[n],
 //   py[n], w[n]; int32 cnt_pos, cnt_neg; float32 cdf_pos[cnt_pos],
 //   cdf_neg[cnt_neg], cum[2]; int32 nq; float32 q[2 nq]
@@ -132,9 +132,50 @@ static int run_mcbs(const char *input, const std::string &out)
     return 0;
 }
 
+// hash mode: int32 ncase; per case int32 n, rsh; float32 px[n], py[n].
+// Writes hash_<backend>_<case>.bin = cell_num, cell_acc, vp_cell, vp_map,
+// cell_num after the sort (int32), from the host-span variants.
+static int run_hash(const char *input, const std::string &out)
+{
+    std::ifstream in(input, std::ios::binary);
+    const int ncase = take<int>(in, 1)[0];
+    try {
+        for (int c = 0; c < ncase; ++c) {
+            const std::vector<int> h = take<int>(in, 2);
+            const int n = h[0], rsh = h[1], cells = rsh * rsh;
+            const std::vector<float> px = take<float>(in, n), py = take<float>(in, n);
+            std::vector<int> sub(n);
+            for (int i = 0; i < n; ++i) sub[i] = i;
+            for (int omp = 0; omp < 2; ++omp) {
+                std::vector<int> num(cells), acc(cells), cell(n), map(n);
+                if (omp)
+                    FillCells_omp(rsh, rsh, 1.f, 1.f, n, sub, px, py, num, acc, cell);
+                else
+                    FillCells_cpu(rsh, rsh, 1.f, 1.f, n, sub, px, py, num, acc, cell);
+                std::vector<int> res(num);
+                res.insert(res.end(), acc.begin(), acc.end());
+                res.insert(res.end(), cell.begin(), cell.end());
+                if (omp)
+                    CountingSort_omp(n, num, acc, cell, map);
+                else
+                    CountingSort_cpu(n, num, acc, cell, map);
+                res.insert(res.end(), map.begin(), map.end());
+                res.insert(res.end(), num.begin(), num.end());
+                dump(out + "/hash_" + (omp ? "omp_" : "cpu_") + std::to_string(c) + ".bin", res);
+            }
+        }
+    } catch (const std::exception &e) {
+        std::cerr << "error: " << e.what() << std::endl;
+        return 1;
+    }
+    std::cout << "cxx hash ok" << std::endl;
+    return 0;
+}
+
 int main(int argc, char **argv)
 {
     if (argc >= 4 && std::string(argv[3]) == "mcbs") return run_mcbs(argv[1], argv[2]);
+    if (argc >= 4 && std::string(argv[3]) == "hash") return run_hash(argv[1], argv[2]);
     if (argc < 3) {
         std::fprintf(stderr, "usage: %s input.bin outdir\n", argv[0]);
         return 2;
@@ -215,13 +256,15 @@ int main(int argc, char **argv)
 
         sim.test_vorticity_grid(16);
 
-        bool threw = false;
-        try {
-            SpatialHasher bad(4, 4, Backend::CPU);
-        } catch (const std::invalid_argument &) {
-            threw = true;
+        // Backend::CPU / OMP from host containers: the same hash
+        for (Backend b : {Backend::CPU, Backend::OMP}) {
+            SpatialHasher Hh(rsh, rsh, b);
+            Hh.resize(N);
+            Hh.build(sim.particles_.pos.to_host(), iota, 1.f, 1.f);
+            if (Hh.data().vp_sort.to_host() != H.data().vp_sort.to_host() ||
+                Hh.data().cell_acc.to_host() != H.data().cell_acc.to_host())
+                throw std::runtime_error("host-container SpatialHasher differs");
         }
-        if (!threw) throw std::runtime_error("Backend::CPU accepted");
     } catch (const std::exception &e) {
         std::cerr << "error: " << e.what() << std::endl;
         return 1;

@@ -153,3 +153,39 @@ def test_cxx_cpu_backend_entry_points_replay_the_reference(device, tmp_path):
     assert np.abs(d).max() < 1e-6 and (p1 >= 0).all() and (p1 < 1).all()
     assert np.abs(rd("step_left.bin").reshape(-1, 2)[:x.size]).max() > 0   # aux points moved
     assert rd("step_rng_member.bin", np.int32)[0] == 1   # the member generator is not advanced
+
+
+def test_cxx_host_span_hash_matches_reference(tmp_path):
+    """FillCells_cpu/_omp + CountingSort_cpu/_omp (host spans, run on the GPU)
+    == the reference's CPU hash on its golden cases, bit for bit (the OMP
+    variant in the CPU order: the reference's OMP sort writes from a wrong
+    base, src/omp/hash_omp.cpp:65)"""
+    print("[case] cxx host-span hash", flush=True)
+    if not os.path.exists(BIN):
+        pytest.fail("tests/cxx/bin/vpmc_cxx_check not built (run build())")
+    G = np.load(os.path.join(ROOT, "tests", "golden", "ref_cpu_vectors.npz"))
+    tags = ("a", "b", "c")
+    inp = tmp_path / "hash.bin"
+    with open(inp, "wb") as f:
+        np.array([len(tags)], np.int32).tofile(f)
+        for t in tags:
+            px, py = G["hash_%s_px" % t], G["hash_%s_py" % t]
+            np.array([px.size, int(G["hash_%s_rsh" % t])], np.int32).tofile(f)
+            px.astype(np.float32).tofile(f)
+            py.astype(np.float32).tofile(f)
+    r = subprocess.run([BIN, str(inp), str(tmp_path), "hash"], capture_output=True, text=True,
+                       timeout=120)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0 and "cxx hash ok" in r.stdout
+    for c, t in enumerate(tags):
+        n, cells = G["hash_%s_px" % t].size, int(G["hash_%s_rsh" % t]) ** 2
+        for b in ("cpu", "omp"):
+            res = np.fromfile(tmp_path / ("hash_%s_%d.bin" % (b, c)), dtype=np.int32)
+            num, acc = res[:cells], res[cells:2 * cells]
+            cell, mp = res[2 * cells:2 * cells + n], res[2 * cells + n:2 * cells + 2 * n]
+            after = res[2 * cells + 2 * n:]
+            assert np.array_equal(num, G["hash_%s_num" % t]), (t, b)
+            assert np.array_equal(acc, G["hash_%s_acc" % t]), (t, b)
+            assert np.array_equal(cell, G["hash_%s_cell" % t]), (t, b)
+            assert np.array_equal(mp, G["hash_%s_map" % t]), (t, b)
+            assert (after == 0).all(), (t, b)

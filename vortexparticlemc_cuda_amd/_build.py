@@ -22,7 +22,7 @@ LIB = os.path.join(PKG, "lib", "libvpmc.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 ARCH = os.environ.get("VPMC_OFFLOAD_ARCH", "gfx950")
-COMMON = ["-O3", "-std=c++17", "-fPIC", "-I", INC, "-I", CSRC,
+COMMON = ["-O3", "-std=c++20", "-fPIC", "-I", INC, "-I", CSRC,
           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
           "-Wno-unused-command-line-argument"]
 
@@ -68,7 +68,7 @@ def build_cxx_check():
         return None
     os.makedirs(os.path.dirname(CXX_CHECK_BIN), exist_ok=True)
     libdir = os.path.dirname(LIB)
-    _run([HIPCC, "--offload-arch=" + ARCH, "-O2", "-std=c++17", "-I", INC, CXX_CHECK_SRC,
+    _run([HIPCC, "--offload-arch=" + ARCH, "-O2", "-std=c++20", "-I", INC, CXX_CHECK_SRC,
           "-L", libdir, "-lvpmc", "-Wl,-rpath," + libdir, "-Wl,-rpath,$ORIGIN/../../../"
           "vortexparticlemc_cuda_amd/lib", "-o", CXX_CHECK_BIN])
     return CXX_CHECK_BIN

@@ -121,6 +121,65 @@ void CountingSort_cuda(int N, int *cell_num, const int *cell_acc, const int *vp_
     vpmc_detail::hip_check(hipDeviceSynchronize(), "CountingSort_cuda");
 }
 
+// the host-span variants: the same kernels on copies of the spans
+void FillCells_cpu(int rshx, int rshy, float Lx, float Ly, int N, std::span<const int> sub,
+                   std::span<const float> px, std::span<const float> py,
+                   std::span<int> cell_num, std::span<int> cell_acc, std::span<int> vp_cell)
+{
+    const size_t cells = (size_t)rshx * rshy;
+    if (N < 0 || rshx <= 0 || rshy <= 0 || sub.size() < (size_t)N || px.size() != py.size() ||
+        cell_num.size() < cells || cell_acc.size() < cells || vp_cell.size() < (size_t)N)
+        throw std::invalid_argument("FillCells_cpu: span sizes");
+    for (int i = 0; i < N; ++i)
+        if (sub[i] < 0 || (size_t)sub[i] >= px.size())
+            throw std::invalid_argument("FillCells_cpu: sub index outside the positions");
+    std::vector<float2> xy(px.size());
+    for (size_t j = 0; j < px.size(); ++j) xy[j] = make_float2(px[j], py[j]);
+    dvec<float2> dxy(xy);
+    dvec<int> dsub(std::vector<int>(sub.begin(), sub.begin() + N)), num(cells), acc(cells),
+        key(std::max(N, 1));
+    FillCells_cuda(rshx, rshy, Lx, Ly, N, raw_ptr(dsub), raw_ptr(dxy), raw_ptr(num), raw_ptr(acc),
+                   raw_ptr(key));
+    const std::vector<int> hn = num.to_host(), ha = acc.to_host(), hk = key.to_host();
+    std::copy(hn.begin(), hn.end(), cell_num.begin());
+    std::copy(ha.begin(), ha.end(), cell_acc.begin());
+    std::copy(hk.begin(), hk.begin() + N, vp_cell.begin());
+}
+
+void FillCells_omp(int rshx, int rshy, float Lx, float Ly, int N, std::span<const int> sub,
+                   std::span<const float> px, std::span<const float> py,
+                   std::span<int> cell_num, std::span<int> cell_acc, std::span<int> vp_cell)
+{
+    FillCells_cpu(rshx, rshy, Lx, Ly, N, sub, px, py, cell_num, cell_acc, vp_cell);
+}
+
+void CountingSort_cpu(int N, std::span<int> cell_num, std::span<const int> cell_acc,
+                      std::span<const int> vp_cell, std::span<int> vp_map)
+{
+    const size_t cells = cell_num.size();
+    if (N < 0 || cell_acc.size() < cells || vp_cell.size() < (size_t)N ||
+        vp_map.size() < (size_t)N)
+        throw std::invalid_argument("CountingSort_cpu: span sizes");
+    if (N == 0) return;
+    for (int i = 0; i < N; ++i)
+        if (vp_cell[i] < 0 || (size_t)vp_cell[i] >= cells)
+            throw std::invalid_argument("CountingSort_cpu: cell key outside cell_num");
+    dvec<int> num(std::vector<int>(cell_num.begin(), cell_num.end())),
+        acc(std::vector<int>(cell_acc.begin(), cell_acc.begin() + cells)),
+        key(std::vector<int>(vp_cell.begin(), vp_cell.begin() + N)), map(N);
+    cells_memo() = {raw_ptr(num), (int)cells};
+    CountingSort_cuda(N, raw_ptr(num), raw_ptr(acc), raw_ptr(key), raw_ptr(map));
+    const std::vector<int> hn = num.to_host(), hm = map.to_host();
+    std::copy(hn.begin(), hn.end(), cell_num.begin());
+    std::copy(hm.begin(), hm.end(), vp_map.begin());
+}
+
+void CountingSort_omp(int N, std::span<int> cell_num, std::span<const int> cell_acc,
+                      std::span<const int> vp_cell, std::span<int> vp_map)
+{
+    CountingSort_cpu(N, cell_num, cell_acc, vp_cell, vp_map);
+}
+
 // ---------------------------------------------------------------------------
 // utilities.hpp -- src/utilities.cu:12-96 (host-built tables)
 
