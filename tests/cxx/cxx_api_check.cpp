@@ -21,6 +21,7 @@
 #include "cg_rbf_init.cuh"
 #include "monte_carlo_bs.hpp"
 #include "spatial_hasher.hpp"
+#include "viz_utils.hpp"
 #include "vortex_particle_mc.hpp"
 
 template <class T> static void dump(const std::string &path, const std::vector<T> &v)
@@ -236,6 +237,11 @@ int main(int argc, char **argv)
         sim.step_cuda(true);
         (void)hipDeviceSynchronize();
         dump(out + "/pos2.bin", sim.particles_.pos.to_host());
+        dump(out + "/vel2.bin", sim.particles_.vel.to_host());
+        // the driver's output files (viz_utils.hpp) of this state
+        dump_vorticity_png(sim, out + "/vort_per.png", 48, true);
+        dump_vorticity_png(sim, out + "/vort.png", 48);
+        dump_velocity_svg_centered(sim, out + "/svg/vel.svg", 0.2f, true);
 
         // two steps of the CUDA step with aux targets and kernel deformation
         {

@@ -67,6 +67,23 @@ def test_cxx_api_end_to_end(device, tmp_path):
     d = rd("pos2.bin").reshape(-1, 2) - np.stack([px, py], 1)
     d -= np.rint(d)
     assert np.abs(d).max() < 1e-6
+    # the driver's output files of that state (include/viz_utils.hpp) == the
+    # reference's per-pixel formula in FP64 and its arrow geometry
+    from test_viz import _ref_image
+    from vortexparticlemc_cuda_amd import viz
+    p2 = rd("pos2.bin").reshape(-1, 2)[:N].astype(np.float64)
+    for name, per in (("vort_per.png", True), ("vort.png", False)):
+        img = viz.decode_png((tmp_path / name).read_bytes())
+        ref = _ref_image(p2[:, 0], p2[:, 1], w.astype(np.float64), hw, 48, periodic=per)
+        diff = np.abs(img.astype(int) - ref.astype(int))
+        assert img.shape == (48, 48) and diff.max() <= 1 and (diff > 0).mean() < 0.01, name
+    svg = (tmp_path / "svg" / "vel.svg").read_text().splitlines()
+    ref_svg = viz.velocity_svg(rd("pos2.bin").reshape(-1, 2)[:N],
+                               rd("vel2.bin").reshape(-1, 2)[:N], 0.2, True).splitlines()
+    assert len(svg) == len(ref_svg) and svg[:4] == ref_svg[:4]
+    num = lambda ln: [float(t) for t in ln.split('"')[3:11:2]]  # noqa: E731
+    for a, b in zip(svg[4:-1], ref_svg[4:-1]):
+        assert np.allclose(num(a), num(b), atol=2e-6), (a, b)
     # kernel_deformation: step_cuda with aux targets and F == the Python mirror
     # (same library entry point, same Philox streams) bit for bit
     from vortexparticlemc_cuda_amd._capi import MODE_CONSISTENT

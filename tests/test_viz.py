@@ -34,9 +34,9 @@ def test_svg_arrows_follow_the_reference_geometry():
     assert abs(x1 - 0.85) < 1e-6 and abs(x2 - 0.05) < 1e-6
 
 
-def _ref_image(x, y, w, hw, res):
+def _ref_image(x, y, w, hw, res, periodic=True):
     """viz_utils.hpp:42-80 in FP64: |w|-weighted spline sum, min image on the
-    unit box, r^2 < 1e-12 skipped, 128 + int(value * 127 / max)"""
+    unit box when periodic, r^2 < 1e-12 skipped, 128 + int(value * 127 / max)"""
     sig2 = 40.0 / (7.0 * np.pi * hw * hw)
     c = (np.arange(res) + 0.5) / res
     X, Y = np.meshgrid(c, c, indexing="xy")          # [j, i]: y row, x column
@@ -44,8 +44,9 @@ def _ref_image(x, y, w, hw, res):
     for xi, yi, wi in zip(x, y, np.abs(w)):
         dx = xi - X
         dy = yi - Y
-        dx -= np.rint(dx)
-        dy -= np.rint(dy)
+        if periodic:
+            dx -= np.rint(dx)
+            dy -= np.rint(dy)
         r2 = dx * dx + dy * dy
         q = np.sqrt(r2) / hw
         k = np.where(q <= 0.5, 6 * (q ** 3 - q ** 2) + 1, np.where(q < 1, 2 * (1 - q) ** 3, 0.0))
