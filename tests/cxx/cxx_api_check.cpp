@@ -19,10 +19,34 @@
 
 #include "cg_rbf.hpp"
 #include "cg_rbf_init.cuh"
+#include "monte_carlo_bs.cuh"
 #include "monte_carlo_bs.hpp"
 #include "spatial_hasher.hpp"
 #include "viz_utils.hpp"
 #include "vortex_particle_mc.hpp"
+
+// the device helpers of monte_carlo_bs.cuh on n points: per point the view
+// query, F v, the minimum image of p.x - 0.9, renorm_det1(F), K_eps(p - c)
+__global__ void k_cuh_helpers(VorticityView v, const float2 *p, int n, float h_w, float *out)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float4 F = make_float4(1.2f, 0.3f, -0.1f, 0.9f);
+    const float2 fv = mul_mat2(F, p[i]);
+    const float4 r = renorm_det1(F);
+    const float2 k = bs_kernel_2d(make_float2(p[i].x - 0.5f, p[i].y - 0.5f), 1e-4f);
+    float *o = out + (size_t)i * 10;
+    o[0] = evalVorticityAbs_gpu(v, p[i], +1, false, false, h_w);
+    o[1] = fv.x;
+    o[2] = fv.y;
+    o[3] = wrap_min_image(p[i].x - 0.9f, 1.f, true);
+    o[4] = r.x;
+    o[5] = r.y;
+    o[6] = r.z;
+    o[7] = r.w;
+    o[8] = k.x;
+    o[9] = k.y;
+}
 
 template <class T> static void dump(const std::string &path, const std::vector<T> &v)
 {
@@ -242,6 +266,22 @@ int main(int argc, char **argv)
         dump_vorticity_png(sim, out + "/vort_per.png", 48, true);
         dump_vorticity_png(sim, out + "/vort.png", 48);
         dump_velocity_svg_centered(sim, out + "/svg/vel.svg", 0.2f, true);
+        {   // monte_carlo_bs.cuh's device helpers, and the host query, at 16 points
+            std::vector<float2> hp(16);
+            for (int i = 0; i < 16; ++i)
+                hp[i] = make_float2(0.03f + 0.061f * i, 0.97f - 0.057f * i);
+            dvec<float2> dp(hp);
+            dvec<float> res(160);
+            const VorticityView pv = sim.makePosView();
+            hipLaunchKernelGGL(k_cuh_helpers, dim3(1), dim3(64), 0, 0, pv, raw_ptr(dp), 16,
+                               sim.P_.h_w, raw_ptr(res));
+            (void)hipDeviceSynchronize();
+            std::vector<float> h = res.to_host();
+            for (int i = 0; i < 16; ++i)
+                h.push_back(evalVorticityAbs_cpu(pv, hp[i], +1, false, false, sim.P_.h_w, true));
+            dump(out + "/cuh_helpers.bin", h);
+            dump(out + "/cuh_points.bin", hp);
+        }
 
         // two steps of the CUDA step with aux targets and kernel deformation
         {

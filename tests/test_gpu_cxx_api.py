@@ -77,6 +77,21 @@ def test_cxx_api_end_to_end(device, tmp_path):
         ref = _ref_image(p2[:, 0], p2[:, 1], w.astype(np.float64), hw, 48, periodic=per)
         diff = np.abs(img.astype(int) - ref.astype(int))
         assert img.shape == (48, 48) and diff.max() <= 1 and (diff > 0).mean() < 0.01, name
+    # monte_carlo_bs.cuh's device helpers: the header's query agrees with the
+    # library's (separate FP32 code of one sum: rounding only), the math with numpy
+    hv = rd("cuh_helpers.bin")
+    h, hq = hv[:160].reshape(16, 10), hv[160:]
+    pts = rd("cuh_points.bin").reshape(-1, 2).astype(np.float64)
+    assert np.allclose(h[:, 0], hq, rtol=1e-6, atol=1e-9) and (hq > 0).any()
+    F = np.array([[1.2, 0.3], [-0.1, 0.9]])
+    assert np.allclose(h[:, 1:3], pts @ F.T, atol=1e-6)
+    dmi = pts[:, 0] - 0.9
+    assert np.allclose(h[:, 3], dmi - np.rint(dmi), atol=1e-6)
+    assert np.allclose(h[:, 4:8], np.tile(F.ravel() / np.sqrt(np.linalg.det(F)), (16, 1)),
+                       atol=1e-6)
+    d = pts - 0.5
+    c = 1.0 / (2 * np.pi * ((d ** 2).sum(1) + 1e-4))
+    assert np.allclose(h[:, 8:10], np.stack([-d[:, 1] * c, d[:, 0] * c], 1), rtol=1e-5)
     svg = (tmp_path / "svg" / "vel.svg").read_text().splitlines()
     ref_svg = viz.velocity_svg(rd("pos2.bin").reshape(-1, 2)[:N],
                                rd("vel2.bin").reshape(-1, 2)[:N], 0.2, True).splitlines()
