@@ -34,7 +34,7 @@ struct BinBufs {
     uint32_t *key;            // tile key per sample (ntile: no query)
     float2 *q, *r;
     float *om;
-    float4 *srec;             // cell-sorted samples: (x, y, sample id bits, 0)
+    SRec *srec;               // cell-sorted samples: (x, y, sample id)
     uint32_t *keys_sorted;
     int *start;               // ntile + 1 tile offsets into srec
     void *sort_ws;
@@ -53,7 +53,7 @@ __global__ void __launch_bounds__(BIN_THREADS)
 
 template <int THREADS, int SPL, bool COUNT, bool DEF = false>
 __global__ void __launch_bounds__(THREADS)
-    k_bin_eval(McD m, BinGeom g, const int *__restrict__ start, const float4 *__restrict__ srec,
+    k_bin_eval(McD m, BinGeom g, const int *__restrict__ start, const SRec *__restrict__ srec,
                float *__restrict__ om, unsigned long long *__restrict__ eval_count)
 {
     // workgroups are dealt round-robin to the 8 XCDs: give each XCD a
@@ -88,7 +88,7 @@ __global__ void __launch_bounds__(BIN_THREADS)
 // eval launch for runtime workgroup-size and samples-per-lane choices
 template <int THREADS, bool COUNT, bool DEF>
 static void launch_eval_t(int spl, const McD &m, const BinGeom &g, const int *start,
-                          const float4 *srec, float *om, unsigned long long *cnt, hipStream_t s)
+                          const SRec *srec, float *om, unsigned long long *cnt, hipStream_t s)
 {
     const dim3 grid((unsigned)(8 * g.per_xcd)), block(THREADS);
     if (spl == 1)
@@ -104,7 +104,7 @@ static void launch_eval_t(int spl, const McD &m, const BinGeom &g, const int *st
 
 template <bool COUNT, bool DEF>
 static void launch_eval_d(int threads, int spl, const McD &m, const BinGeom &g,
-                          const int *start, const float4 *srec, float *om,
+                          const int *start, const SRec *srec, float *om,
                           unsigned long long *cnt, hipStream_t s)
 {
     if (threads == 64) launch_eval_t<64, COUNT, DEF>(spl, m, g, start, srec, om, cnt, s);
@@ -115,7 +115,7 @@ static void launch_eval_d(int threads, int spl, const McD &m, const BinGeom &g,
 // deformed sets (field.gq != NULL) take the quadratic-form records
 template <bool COUNT>
 static int launch_eval(int threads, int spl, const McD &m, const BinGeom &g, const int *start,
-                       const float4 *srec, float *om, unsigned long long *cnt, hipStream_t s)
+                       const SRec *srec, float *om, unsigned long long *cnt, hipStream_t s)
 {
     if (threads == 0) threads = 256;
     if (threads != 64 && threads != 128 && threads != 256) return VPMC_ERR_ARG;
@@ -162,7 +162,7 @@ int bin_geom(const McD &m, int N1, int N2, int n_owner, BinGeom *g)
 static size_t bin_ws_bytes(const BinGeom &g)
 {
     size_t S = g.S > 0 ? g.S : 1, T = (size_t)g.ntile;
-    return align256(S * 4) + 2 * align256(S * 8) + align256(S * 4) + align256(S * 16) +
+    return align256(S * 4) + 2 * align256(S * 8) + align256(S * 4) + align256(S * sizeof(SRec)) +
            align256(S * 4) + align256((T + 1) * 4) + radix_workspace_bytes((int)S) + 4096;
 }
 
@@ -174,7 +174,7 @@ static int carve_bin(const BinGeom &g, void *ws, size_t bytes, BinBufs *B)
     B->q = cv.take<float2>(S);
     B->r = cv.take<float2>(S);
     B->om = cv.take<float>(S);
-    B->srec = cv.take<float4>(S);
+    B->srec = cv.take<SRec>(S);
     B->keys_sorted = cv.take<uint32_t>(S);
     B->start = cv.take<int>(T + 1);
     B->sort_bytes = radix_workspace_bytes((int)S);
@@ -234,9 +234,9 @@ int mc_binned_impl(const vpmc_mc_desc *d, void *ws, size_t ws_bytes, hipStream_t
     VPMC_TRY(carve_bin(g, ws, ws_bytes, &B));
     launch_mode(d->mode, d->periodic, m, g, B, false, s);
     VPMC_CHECK_LAUNCH();
-    VPMC_TRY(radix_sort_to_records(B.key, (int)g.S, key_bits_for((uint32_t)g.ntile),
-                                   (const float2 *)B.q, nullptr, B.srec, B.keys_sorted, g.ntile,
-                                   B.start, B.sort_ws, B.sort_bytes, s));
+    VPMC_TRY(radix_sort_samples(B.key, (int)g.S, key_bits_for((uint32_t)g.ntile),
+                                (const float2 *)B.q, B.srec, B.keys_sorted, g.ntile, B.start,
+                                B.sort_ws, B.sort_bytes, s));
     VPMC_TRY(record_event(d->eval_events[0], s));
     if (d->eval_count)
         VPMC_TRY(launch_eval<true>(d->eval_threads, d->spl, m, g, B.start, B.srec, B.om,

@@ -297,7 +297,7 @@ __device__ __forceinline__ void tile_eval_def(const float4 *__restrict__ cand,
 // om[sample id].  Called by the whole workgroup of THREADS threads.
 template <int THREADS, int SPL, bool COUNT, bool DEF = false>
 __device__ __forceinline__ void bin_eval_tile(const McD &m, const BinGeom &g, int tile, int beg,
-                                              int end, const float4 *__restrict__ srec,
+                                              int end, const SRec *__restrict__ srec,
                                               float *__restrict__ om,
                                               TileLds<THREADS, DEF> &L, EvalCount &ec)
 {
@@ -326,8 +326,8 @@ __device__ __forceinline__ void bin_eval_tile(const McD &m, const BinGeom &g, in
         for (int s = 0; s < SPL; ++s) {
             const int idx = bb + (wid * SPL + s) * WAVE + lane;
             live[s] = idx < end;   // tail lanes take the cell centre, results dropped
-            const float4 p = live[s] ? srec[idx] : make_float4(ox, oy, 0.f, 0.f);
-            sid[s] = __float_as_uint(p.z);
+            const SRec p = live[s] ? srec[idx] : SRec{ox, oy, 0u};
+            sid[s] = p.id;
             px[s] = p.x - ox;
             py[s] = p.y - oy;
             pp[s] = fmaf(px[s], px[s], py[s] * py[s]);

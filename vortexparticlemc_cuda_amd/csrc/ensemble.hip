@@ -140,7 +140,7 @@ __global__ void __launch_bounds__(BIN_THREADS)
 template <int THREADS, int SPL, bool DEF>
 __global__ void __launch_bounds__(THREADS)
     k_ens_eval(EnsTabs t, int n_sim, int ntile, int per_xcd, const int *__restrict__ start,
-               const float4 *__restrict__ srec, float *__restrict__ om)
+               const SRec *__restrict__ srec, float *__restrict__ om)
 {
     const int tile = (int)(blockIdx.x & 7) * per_xcd + (int)(blockIdx.x >> 3);
     if (tile >= ntile) return;
@@ -213,7 +213,7 @@ struct EnsLayout {
     uint32_t *skey = nullptr, *skeys_sorted = nullptr;
     float2 *q = nullptr, *r = nullptr, *vel_sign = nullptr;
     float *om = nullptr;
-    float4 *srec = nullptr;
+    SRec *srec = nullptr;
     int *tstart = nullptr;
     int4 *sblk = nullptr;
     float4 *fgq = nullptr, *set_jac = nullptr;   // deformed fields / sets
@@ -323,7 +323,7 @@ static size_t ens_carve(EnsLayout &L, void *ws, size_t bytes)
     L.q = cv.take<float2>(S);
     L.r = cv.take<float2>(S);
     L.om = cv.take<float>(S);
-    L.srec = cv.take<float4>(S);
+    L.srec = cv.take<SRec>(S);
     L.skeys_sorted = cv.take<uint32_t>(S);
     L.tstart = cv.take<int>((size_t)L.T + 1);
     L.sblk = cv.take<int4>((size_t)(L.nblk > 0 ? L.nblk : 1));
@@ -433,7 +433,7 @@ static int ens_estimate(const vpmc_ensemble_desc *e, EnsLayout &L, hipStream_t s
 #define VPMC_ENS_EVAL(TH, SP, DF)                                                              \
     hipLaunchKernelGGL((k_ens_eval<TH, SP, DF>), dim3((unsigned)(8 * L.per_xcd)), dim3(TH), 0,  \
                        st, L.t, ns, L.T, L.per_xcd, (const int *)L.tstart,                    \
-                       (const float4 *)L.srec, L.om)
+                       (const SRec *)L.srec, L.om)
 #define VPMC_ENS_EVAL_SPL(TH, DF)                        \
     do {                                                 \
         if (e->spl == 1) VPMC_ENS_EVAL(TH, 1, DF);       \

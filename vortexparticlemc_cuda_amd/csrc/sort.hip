@@ -11,7 +11,8 @@
 //            ballots give the lanes sharing a digit) and per-wave counts in
 //            LDS, so the pass is stable and the result is the unique stable
 //            order.
-// The 16-byte sample records move with the keys in every pass.
+// The sample records (12 bytes: point and id) move with the keys in every
+// pass; the fields' 16-byte records use the same passes.
 //
 // Segmented form: when the input is already grouped into segments (an
 // ensemble's simulations, each owning the key range [base_s, base_s + 2^b)),
@@ -101,12 +102,25 @@ __device__ __forceinline__ int rs_block_excl_scan(int v, int *wsum, int *total)
 // own input slice.  recs_in == NULL (first pass): the record of input
 // position i is built as (q[i].x, q[i].y, i bits, 0).
 constexpr int RS_WITEMS = RS_TILE / RS_WAVES / WAVE;   // keys per lane
+
+// the first pass's record of input position i from its query point
+template <class R> __device__ __forceinline__ R rs_rec_from(float2 p, uint32_t i);
+template <> __device__ __forceinline__ float4 rs_rec_from<float4>(float2 p, uint32_t i)
+{
+    return make_float4(p.x, p.y, __uint_as_float(i), 0.f);
+}
+template <> __device__ __forceinline__ SRec rs_rec_from<SRec>(float2 p, uint32_t i)
+{
+    return SRec{p.x, p.y, i};
+}
+
+template <class R>
 __global__ void __launch_bounds__(RS_THREADS)
-    k_rs_scatter(const uint32_t *__restrict__ keys_in, const float4 *__restrict__ recs_in,
+    k_rs_scatter(const uint32_t *__restrict__ keys_in, const R *__restrict__ recs_in,
                  const float2 *__restrict__ q, int n, int shift, int nblk,
                  const int4 *__restrict__ blk, const uint32_t *__restrict__ blk_base,
                  const int *__restrict__ offs, uint32_t *__restrict__ keys_out,
-                 float4 *__restrict__ recs_out)
+                 R *__restrict__ recs_out)
 {
     __shared__ uint32_t lk[RS_TILE];
     __shared__ uint16_t li[RS_TILE];        // local input index (< RS_TILE)
@@ -174,7 +188,7 @@ __global__ void __launch_bounds__(RS_THREADS)
     // all RS_ITEMS record loads in flight before the stores
     uint32_t ko[RS_ITEMS];
     int po[RS_ITEMS];
-    float4 rv[RS_ITEMS];
+    R rv[RS_ITEMS];
 #pragma unroll
     for (int j = 0; j < RS_ITEMS; ++j) {
         const int i = j * RS_THREADS + t;
@@ -183,12 +197,7 @@ __global__ void __launch_bounds__(RS_THREADS)
             ko[j] = lk[i];
             const int src = base + li[i];
             po[j] = gofs[((ko[j] - kb) >> shift) & (RS_BINS - 1)] + i;
-            if (recs_in) {
-                rv[j] = recs_in[src];
-            } else {
-                const float2 p = q[src];
-                rv[j] = make_float4(p.x, p.y, __uint_as_float((uint32_t)src), 0.f);
-            }
+            rv[j] = recs_in ? recs_in[src] : rs_rec_from<R>(q[src], (uint32_t)src);
         }
     }
 #pragma unroll
@@ -229,8 +238,9 @@ size_t radix_workspace_bytes(int n)
 // key must lie in [0, nkeys]).  keys and recs0 are left unchanged.  Pass p
 // writes the final buffers when P-1-p is even, the workspace copies
 // otherwise, so the last pass always ends in the outputs.
+template <class R>
 static int radix_sort_impl(const uint32_t *keys, int n, int key_bits, const float2 *q,
-                           const float4 *recs0, float4 *rec, uint32_t *keys_sorted, int nkeys,
+                           const R *recs0, R *rec, uint32_t *keys_sorted, int nkeys,
                            int *start, const int4 *blk, const uint32_t *blk_base, int nblk,
                            void *ws, size_t ws_bytes, hipStream_t s)
 {
@@ -240,7 +250,7 @@ static int radix_sort_impl(const uint32_t *keys, int n, int key_bits, const floa
     if (h >= ((size_t)1 << 31)) return VPMC_ERR_ARG;
     Carve cv(ws, ws_bytes);
     uint32_t *kb = cv.take<uint32_t>((size_t)n);
-    float4 *rb = cv.take<float4>((size_t)n);
+    R *rb = cv.take<R>((size_t)n);
     int *hist = cv.take<int>(h);
     int *offs = cv.take<int>(h);
     const size_t sb = scan_workspace_bytes((int)h, sizeof(int));
@@ -248,16 +258,16 @@ static int radix_sort_impl(const uint32_t *keys, int n, int key_bits, const floa
     if (!kb || !rb || !hist || !offs || !sws) return VPMC_ERR_WORKSPACE;
     const int passes = (key_bits + 7) / 8;
     const uint32_t *kin = keys;
-    const float4 *rin = recs0;
+    const R *rin = recs0;
     for (int p = 0; p < passes; ++p) {
         const bool to_out = ((passes - 1 - p) & 1) == 0;
         uint32_t *kout = to_out ? keys_sorted : kb;
-        float4 *rout = to_out ? rec : rb;
+        R *rout = to_out ? rec : rb;
         hipLaunchKernelGGL(k_rs_hist, dim3(nblk), dim3(RS_THREADS), 0, s, kin, n, 8 * p, nblk,
                            blk, blk_base, hist);
         VPMC_CHECK_LAUNCH();
         VPMC_TRY(scan_i32(hist, offs, (int)h, false, nullptr, sws, sb, s));
-        hipLaunchKernelGGL(k_rs_scatter, dim3(nblk), dim3(RS_THREADS), 0, s, kin, rin, q, n,
+        hipLaunchKernelGGL(k_rs_scatter<R>, dim3(nblk), dim3(RS_THREADS), 0, s, kin, rin, q, n,
                            8 * p, nblk, blk, blk_base, (const int *)offs, kout, rout);
         VPMC_CHECK_LAUNCH();
         kin = kout;
@@ -273,8 +283,16 @@ int radix_sort_to_records(const uint32_t *keys, int n, int key_bits, const float
                           const float4 *recs0, float4 *rec, uint32_t *keys_sorted, int nkeys,
                           int *start, void *ws, size_t ws_bytes, hipStream_t s)
 {
-    return radix_sort_impl(keys, n, key_bits, q, recs0, rec, keys_sorted, nkeys, start, nullptr,
-                           nullptr, rs_nblk(n), ws, ws_bytes, s);
+    return radix_sort_impl<float4>(keys, n, key_bits, q, recs0, rec, keys_sorted, nkeys, start,
+                                   nullptr, nullptr, rs_nblk(n), ws, ws_bytes, s);
+}
+
+int radix_sort_samples(const uint32_t *keys, int n, int key_bits, const float2 *q, SRec *rec,
+                       uint32_t *keys_sorted, int nkeys, int *start, void *ws, size_t ws_bytes,
+                       hipStream_t s)
+{
+    return radix_sort_impl<SRec>(keys, n, key_bits, q, nullptr, rec, keys_sorted, nkeys, start,
+                                 nullptr, nullptr, rs_nblk(n), ws, ws_bytes, s);
 }
 
 int radix_seg_blocks(const std::vector<int> &seg_off, const std::vector<uint32_t> &seg_base,
@@ -309,13 +327,13 @@ size_t radix_seg_workspace_bytes(int n, int nblk)
 }
 
 int radix_sort_segmented(const uint32_t *keys, int n, int local_bits, const float2 *q,
-                         float4 *rec, uint32_t *keys_sorted, int nkeys, int *start,
+                         SRec *rec, uint32_t *keys_sorted, int nkeys, int *start,
                          const int4 *blk, const uint32_t *blk_base, int nblk, void *ws,
                          size_t ws_bytes, hipStream_t s)
 {
     if (!blk || !blk_base || nblk <= 0) return VPMC_ERR_ARG;
-    return radix_sort_impl(keys, n, local_bits, q, nullptr, rec, keys_sorted, nkeys, start, blk,
-                           blk_base, nblk, ws, ws_bytes, s);
+    return radix_sort_impl<SRec>(keys, n, local_bits, q, nullptr, rec, keys_sorted, nkeys, start,
+                                 blk, blk_base, nblk, ws, ws_bytes, s);
 }
 
 }  // namespace vpmc

@@ -53,6 +53,12 @@ struct Carve {
     }
 };
 
+// a binned sample as the sort moves it: query point and sample id (12 bytes)
+struct SRec {
+    float x, y;
+    uint32_t id;
+};
+
 // ---- scans (scan.hip) -------------------------------------------------------
 constexpr int SCAN_THREADS = 256;
 constexpr int SCAN_ITEMS = 8;
@@ -71,6 +77,10 @@ size_t radix_workspace_bytes(int n);
 int radix_sort_to_records(const uint32_t *keys, int n, int key_bits, const float2 *q,
                           const float4 *recs0, float4 *rec, uint32_t *keys_sorted, int nkeys,
                           int *start, void *ws, size_t ws_bytes, hipStream_t s);
+// the same with 12-byte sample records built from q (point, input position)
+int radix_sort_samples(const uint32_t *keys, int n, int key_bits, const float2 *q, SRec *rec,
+                       uint32_t *keys_sorted, int nkeys, int *start, void *ws, size_t ws_bytes,
+                       hipStream_t s);
 // segmented variant: blocks from radix_seg_blocks (segments = contiguous
 // ranges seg_off[g] .. seg_off[g+1] whose keys lie in [seg_base[g],
 // seg_base[g] + 2^local_bits)); keys globally sorted on return
@@ -78,7 +88,7 @@ int radix_seg_blocks(const std::vector<int> &seg_off, const std::vector<uint32_t
                      std::vector<int4> *blk, std::vector<uint32_t> *blk_base);
 size_t radix_seg_workspace_bytes(int n, int nblk);
 int radix_sort_segmented(const uint32_t *keys, int n, int local_bits, const float2 *q,
-                         float4 *rec, uint32_t *keys_sorted, int nkeys, int *start,
+                         SRec *rec, uint32_t *keys_sorted, int nkeys, int *start,
                          const int4 *blk, const uint32_t *blk_base, int nblk, void *ws,
                          size_t ws_bytes, hipStream_t s);
 inline int key_bits_for(uint32_t max_key)
