@@ -245,7 +245,7 @@ int vpmc_mc_estimate(const vpmc_mc_desc *d, void *stream);
 /* Same estimate, binned algorithm (periodic boxes): samples are generated,
    counting-sorted by fine cell and evaluated cell by cell, then reduced per
    owner in sample order.  Needs vpmc_mc_binned_workspace_bytes(d) bytes
-   (about 44 bytes per sample).  Results equal the direct algorithm's up to
+   (about 60 bytes per sample).  Results equal the direct algorithm's up to
    FP32 summation order. */
 size_t vpmc_mc_binned_workspace_bytes(const vpmc_mc_desc *d);
 int vpmc_mc_estimate_binned(const vpmc_mc_desc *d, void *workspace, size_t workspace_bytes,
