@@ -149,6 +149,33 @@ static int run_mcbs(const char *input, const std::string &out)
         std::mt19937 fresh(seed);
         const std::vector<int> member_ok = {sim.rng() == fresh() ? 1 : 0};
         dump(out + "/step_rng_member.bin", member_ok);
+
+        // step_cpu over several replay chunks (N1 = N2 = 2000: 671 particles a
+        // chunk) == monte_carlo_bs_cpu per particle on the same generator
+        {
+            SimParam P2 = P;
+            P2.N1 = P2.N2 = 2000;
+            Simulation s2(P2);
+            s2.set_particles(pos, w);
+            s2.rng = std::mt19937(seed);
+            const MCBSCtx C2 = make_mcbs_ctx(s2, true);
+            const std::vector<float2> x0 = s2.particles_.pos.to_host();
+            std::mt19937 g2(seed);
+            std::vector<float2> loop;
+            const float2 z{0.f, 0.f};   // the aux points start at the origin, as in the reference
+            for (int i = 0; i < 700; ++i) {
+                const MCBSEstimate a = monte_carlo_bs_cpu(x0[i], +1, 1, z, z, z, z, s2.particles_,
+                                                          C2, 2000, 2000, g2);
+                const MCBSEstimate b = monte_carlo_bs_cpu(x0[i], -1, 1, z, z, z, z, s2.particles_,
+                                                          C2, 2000, 2000, g2);
+                loop.push_back(make_float2(a.main.x - b.main.x, a.main.y - b.main.y));
+            }
+            s2.step_cpu(true);
+            std::vector<float2> v2 = s2.particles_.vel.to_host();
+            v2.resize(700);
+            dump(out + "/step_chunks_vel.bin", v2);
+            dump(out + "/step_chunks_loop.bin", loop);
+        }
     } catch (const std::exception &e) {
         std::cerr << "error: " << e.what() << std::endl;
         return 1;

@@ -185,6 +185,10 @@ def test_cxx_cpu_backend_entry_points_replay_the_reference(device, tmp_path):
     assert np.abs(d).max() < 1e-6 and (p1 >= 0).all() and (p1 < 1).all()
     assert np.abs(rd("step_left.bin").reshape(-1, 2)[:x.size]).max() > 0   # aux points moved
     assert rd("step_rng_member.bin", np.int32)[0] == 1   # the member generator is not advanced
+    # step_cpu over two replay chunks == monte_carlo_bs_cpu per particle (particles
+    # 0..699 cross the first chunk's end at 671), bit for bit
+    vc, vl = rd("step_chunks_vel.bin"), rd("step_chunks_loop.bin")
+    assert vc.size == 1400 and np.array_equal(vc, vl) and np.abs(vc).max() > 0
 
 
 def test_cxx_host_span_hash_matches_reference(tmp_path):

@@ -659,33 +659,41 @@ void Simulation::step_cpu(bool periodic)
     ReplaySet S[2];
     replay_set_build(S[0], C, +1, particles_, stream);
     replay_set_build(S[1], C, -1, particles_, stream);
-    // every call's draws first (they do not depend on the estimates), in the
-    // reference's order: per particle the +1 call, then the -1 call
-    std::mt19937 g = rng;
-    const size_t len = mcbs_call_draws(P_.N1, P_.N2, periodic, true);
-    std::vector<float> u[2];
-    u[0].resize((size_t)N * len);
-    u[1].resize((size_t)N * len);
-    for (int i = 0; i < N; ++i)
-        for (int s = 0; s < 2; ++s) mcbs_draw(g, len, u[s].data() + (size_t)i * len);
     // targets: particle i's group is (pos, left, right, top, bottom)
     dvec<float2> *arr[5] = {&particles_.pos, &particles_.pos_left, &particles_.pos_right,
                             &particles_.pos_top, &particles_.pos_bottom};
-    std::vector<float2> tg((size_t)5 * N);
-    for (int t = 0; t < 5; ++t) {
-        std::vector<float2> h = arr[t]->to_host();
-        for (int i = 0; i < N; ++i) tg[(size_t)i * 5 + t] = h[i];
+    std::vector<float2> hpos[5];
+    for (int t = 0; t < 5; ++t) hpos[t] = arr[t]->to_host();
+    // each call's draws (they do not depend on the estimates) in the
+    // reference's order: per particle the +1 call, then the -1 call; replayed
+    // in chunks of particles that keep the replay stream near 64M draws
+    std::mt19937 g = rng;
+    const size_t len = mcbs_call_draws(P_.N1, P_.N2, periodic, true);
+    const size_t per = (size_t)P_.N1 * (periodic ? 3 : 1) + 2 * (size_t)P_.N2;
+    const int chunk = (int)std::max<size_t>(1, ((size_t)1 << 26) / (10 * per));
+    std::vector<float2> vp((size_t)5 * N), vn((size_t)5 * N);   // [particle][target]
+    for (int i0 = 0; i0 < N; i0 += chunk) {
+        const int m = std::min(chunk, N - i0);
+        std::vector<float> u[2];
+        u[0].resize((size_t)m * len);
+        u[1].resize((size_t)m * len);
+        for (int i = 0; i < m; ++i)
+            for (int s = 0; s < 2; ++s) mcbs_draw(g, len, u[s].data() + (size_t)i * len);
+        std::vector<float2> tg((size_t)5 * m);
+        for (int i = 0; i < m; ++i)
+            for (int t = 0; t < 5; ++t) tg[(size_t)i * 5 + t] = hpos[t][i0 + i];
+        const std::vector<float2> v = replay_estimate(tg, 5, S, u, P_.N1, P_.N2, periodic,
+                                                      P_.h_w, P_.Lx, P_.Ly, stream);
+        std::copy(v.begin(), v.begin() + 5 * (size_t)m, vp.begin() + 5 * (size_t)i0);
+        std::copy(v.begin() + 5 * (size_t)m, v.end(), vn.begin() + 5 * (size_t)i0);
     }
-    const std::vector<float2> v = replay_estimate(tg, 5, S, u, P_.N1, P_.N2, periodic, P_.h_w,
-                                                  P_.Lx, P_.Ly, stream);
     // x += (v+ - v-) dt per target array, wrapped; vel = v+ - v- of the particles
-    const size_t M = (size_t)5 * N;
     dvec<float2> vs(2 * (size_t)N), scratch_vel(N);
     for (int t = 0; t < 5; ++t) {
         std::vector<float2> h(2 * (size_t)N);
         for (int i = 0; i < N; ++i) {
-            h[i] = v[(size_t)i * 5 + t];
-            h[(size_t)N + i] = v[M + (size_t)i * 5 + t];
+            h[i] = vp[(size_t)i * 5 + t];
+            h[(size_t)N + i] = vn[(size_t)i * 5 + t];
         }
         vs.assign_host(h.data(), h.size());
         vcheck(vpmc_advect(N, (float *)raw_ptr(*arr[t]), (const float *)raw_ptr(vs),
