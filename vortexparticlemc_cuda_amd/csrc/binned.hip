@@ -163,7 +163,7 @@ static size_t bin_ws_bytes(const BinGeom &g)
 {
     size_t S = g.S > 0 ? g.S : 1, T = (size_t)g.ntile;
     return align256(S * 4) + 2 * align256(S * 8) + align256(S * 4) + align256(S * sizeof(SRec)) +
-           align256(S * 4) + align256((T + 1) * 4) + radix_workspace_bytes((int)S) + 4096;
+           align256(S * 4) + align256((T + 1) * 4) + radix_workspace_bytes((int)S, sizeof(SRec)) + 4096;
 }
 
 static int carve_bin(const BinGeom &g, void *ws, size_t bytes, BinBufs *B)
@@ -177,7 +177,7 @@ static int carve_bin(const BinGeom &g, void *ws, size_t bytes, BinBufs *B)
     B->srec = cv.take<SRec>(S);
     B->keys_sorted = cv.take<uint32_t>(S);
     B->start = cv.take<int>(T + 1);
-    B->sort_bytes = radix_workspace_bytes((int)S);
+    B->sort_bytes = radix_workspace_bytes((int)S, sizeof(SRec));
     B->sort_ws = cv.take<char>(B->sort_bytes);
     if (!B->key || !B->q || !B->r || !B->om || !B->srec || !B->keys_sorted || !B->start ||
         !B->sort_ws)

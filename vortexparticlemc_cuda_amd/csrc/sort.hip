@@ -223,11 +223,11 @@ __global__ void k_rs_bounds(const uint32_t *__restrict__ keys, int n, int nkeys,
 
 static int rs_nblk(int n) { return (n + RS_TILE - 1) / RS_TILE; }
 
-size_t radix_workspace_bytes(int n)
+size_t radix_workspace_bytes(int n, size_t rec_bytes)
 {
     const size_t m = (size_t)(n > 0 ? n : 1);
     const size_t h = (size_t)RS_BINS * rs_nblk(n > 0 ? n : 1);
-    return align256(m * 4) + align256(m * 16) + 2 * align256(h * 4) +
+    return align256(m * 4) + align256(m * rec_bytes) + 2 * align256(h * 4) +
            scan_workspace_bytes((int)h, sizeof(int)) + 1024;
 }
 
@@ -322,7 +322,7 @@ size_t radix_seg_workspace_bytes(int n, int nblk)
 {
     const size_t m = (size_t)(n > 0 ? n : 1);
     const size_t h = (size_t)RS_BINS * (nblk > 0 ? nblk : 1);
-    return align256(m * 4) + align256(m * 16) + 2 * align256(h * 4) +
+    return align256(m * 4) + align256(m * sizeof(SRec)) + 2 * align256(h * 4) +
            scan_workspace_bytes((int)h, sizeof(int)) + 1024;
 }
 

@@ -73,7 +73,7 @@ int scan_f64(const double *in, double *out, int n, bool inclusive, double *total
              size_t ws_bytes, hipStream_t s);
 int fill_i32(int *p, int v, size_t n, hipStream_t s);
 // stable LSD radix sort of tile keys (sort.hip)
-size_t radix_workspace_bytes(int n);
+size_t radix_workspace_bytes(int n, size_t rec_bytes = 16);   // rec_bytes: 16 (float4), 12 (SRec)
 int radix_sort_to_records(const uint32_t *keys, int n, int key_bits, const float2 *q,
                           const float4 *recs0, float4 *rec, uint32_t *keys_sorted, int nkeys,
                           int *start, void *ws, size_t ws_bytes, hipStream_t s);
