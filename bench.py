@@ -391,6 +391,13 @@ def ensemble_bench(args, device):
     ens.step(K)
     torch.cuda.synchronize()
     t_eager = time.perf_counter() - t0
+    # the evaluation kernel alone: HIP events the library records around it
+    s0 = torch.cuda.current_stream()
+    ev_pairs = [_event_pair(s0) for _ in range(K)]
+    for pr in ev_pairs:
+        ens.step(1, events=pr)
+    torch.cuda.synchronize()
+    ens_eval_s = float(np.mean([a.elapsed_time(b) for a, b in ev_pairs])) / 1e3
     side = torch.cuda.Stream()
     with torch.cuda.stream(side):
         ens.capture(K)
@@ -426,6 +433,16 @@ def ensemble_bench(args, device):
                          eval_threads=th)
         sim.set_particles(pos[s], w[s])
         sims.append(sim)
+    # evaluations within h_w and candidates visited by the ensemble's step:
+    # the counting pass of each member at its first step (same Philox streams
+    # and launch shape as the ensemble)
+    n_in = n_vis = 0
+    for sim in sims:
+        sim.eval_count.zero_()
+        sim.estimate(N1, N2, count=True)
+        c = sim.eval_count.cpu().numpy()
+        n_in += int(c[0])
+        n_vis += int(c[1])
     for sim in sims:
         sim.step()
     Kp = max(1, args.ens_plain_steps)
@@ -471,6 +488,12 @@ def ensemble_bench(args, device):
             "autotune_ms": tune_ms,
             "deformed": {"ms_per_step": t_def / K * 1e3, "particle_steps_per_s": N * K / t_def,
                          "samples_per_pair": N1 + 5 * N2},
+            "roofline": {"kernel": "k_ens_eval", "bound": "valu",
+                         "achieved": FLOP_PER_EVAL * n_in / ens_eval_s, "peak": PEAK_FP32,
+                         "unit": "FLOP/s", "frac": FLOP_PER_EVAL * n_in / ens_eval_s / PEAK_FP32,
+                         "avg_launch_ms": ens_eval_s * 1e3, "kernel_evals_per_launch": n_in,
+                         "candidates_per_launch": n_vis,
+                         "frac_of_unpacked": FLOP_PER_EVAL * n_in / ens_eval_s / (PEAK_FP32 / 2)},
             "cg_init_s": t_cg}
 
 

@@ -304,8 +304,9 @@ typedef struct vpmc_ensemble_desc {
     float       *pos_xy;              /* device: all particles (float2)         */
     const float *omega;               /* device: strengths                      */
     float       *vel_xy;              /* device out (float2), or NULL           */
-    void        *eval_events[2];      /* optional hipEvent_t pair around the
-                                         estimator stages                       */
+    void        *eval_events[2];      /* optional hipEvent_t pair, recorded on the
+                                         stream right before and after the
+                                         evaluation kernel (k_ens_eval)         */
     /* device: per-particle deformation F (float4, row-major), or NULL.  With
        it every step is the CUDA step's (vpmc_step with jac): fields carry the
        quadratic forms of F, branch-1 jitter is F u, each owner is also

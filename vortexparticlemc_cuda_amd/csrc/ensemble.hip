@@ -430,6 +430,7 @@ static int ens_estimate(const vpmc_ensemble_desc *e, EnsLayout &L, hipStream_t s
                                   L.skeys_sorted, L.T, L.tstart, L.sblk, L.sblk_base, L.nblk,
                                   L.ssort_ws, L.ssort_bytes, st));
     const int th = e->eval_threads ? e->eval_threads : 256;
+    VPMC_TRY(record_event(e->eval_events[0], st));
 #define VPMC_ENS_EVAL(TH, SP, DF)                                                              \
     hipLaunchKernelGGL((k_ens_eval<TH, SP, DF>), dim3((unsigned)(8 * L.per_xcd)), dim3(TH), 0,  \
                        st, L.t, ns, L.T, L.per_xcd, (const int *)L.tstart,                    \
@@ -452,6 +453,7 @@ static int ens_estimate(const vpmc_ensemble_desc *e, EnsLayout &L, hipStream_t s
 #undef VPMC_ENS_EVAL_SPL
 #undef VPMC_ENS_EVAL
     VPMC_CHECK_LAUNCH();
+    VPMC_TRY(record_event(e->eval_events[1], st));
     const int npair = 2 * L.N;
     hipLaunchKernelGGL((k_ens_final<MODE>), dim3((unsigned)((npair + 3) / 4)), dim3(BIN_THREADS), 0,
                        st, L.t, ns, npair, (const float2 *)L.r, (const float *)L.om);
@@ -543,10 +545,8 @@ int vpmc_ensemble_step(const vpmc_ensemble_desc *e, void *workspace, size_t work
     VPMC_TRY(ens_layout(e, workspace, workspace_bytes, &L));
     hipStream_t st = as_stream(stream);
     VPMC_TRY(ens_fields(e, L, st));
-    VPMC_TRY(record_event(e->eval_events[0], st));
     if (e->mode == VPMC_MODE_REFERENCE) VPMC_TRY(ens_estimate<VPMC_MODE_REFERENCE>(e, L, st));
     else VPMC_TRY(ens_estimate<VPMC_MODE_CONSISTENT>(e, L, st));
-    VPMC_TRY(record_event(e->eval_events[1], st));
     if (L.def) {   // F from the five estimates, before the move (vpmc_step with jac)
         hipLaunchKernelGGL(k_ens_update_jac, dim3((unsigned)((L.N + 255) / 256)), dim3(256), 0,
                            st, L.t, L.n_sim, L.N, e->dt, (float4 *)e->jac);

@@ -79,7 +79,7 @@ class Ensemble:
 
     def step(self, k=1, events=None):
         """k eager steps; events: optional (start, stop) torch.cuda.Event pair
-        recorded by the library around the estimator stages of the last step"""
+        recorded by the library around the evaluation kernel of the last step"""
         for i in range(k):
             if events is not None and i == k - 1:
                 self.desc.eval_events[0] = events[0].cuda_event
