@@ -71,10 +71,10 @@ public:
         n_ = n;
     }
     void clear() { n_ = 0; }
-    size_t size() const { return n_; }
+    HD size_t size() const { return n_; }
     bool empty() const { return n_ == 0; }
-    T *data() { return p_; }
-    const T *data() const { return p_; }
+    HD T *data() { return p_; }
+    HD const T *data() const { return p_; }
     T *begin() { return p_; }
     T *end() { return p_ + n_; }
     const T *begin() const { return p_; }
@@ -113,5 +113,5 @@ private:
     size_t n_ = 0, cap_ = 0;
 };
 
-template <typename Vec> auto raw_ptr(Vec &v) { return v.data(); }
-template <typename Vec> auto raw_ptr(const Vec &v) { return v.data(); }
+template <typename Vec> HD auto raw_ptr(Vec &v) { return v.data(); }
+template <typename Vec> HD auto raw_ptr(const Vec &v) { return v.data(); }

@@ -24,10 +24,11 @@ struct MCBSEstimate {
 };
 
 struct MCBSCtx {
+    // the reference's members and types (include/monte_carlo_bs.hpp:15-35)
     const dvec<int> *sub_pos = nullptr;
     const dvec<int> *sub_neg = nullptr;
-    const dvec<double> *cdf_pos = nullptr;   // FP64 normalised CDF
-    const dvec<double> *cdf_neg = nullptr;
+    const dvec<float> *cdf_pos = nullptr;   // normalised CDF (FP32)
+    const dvec<float> *cdf_neg = nullptr;
     const dvec<float> *icdf_kernel = nullptr;
     int kernel_L = 0;
     float h_w = 1.0f;
@@ -36,12 +37,21 @@ struct MCBSCtx {
     float Ly = 1.0f;
     float InvLx = 1.0f;
     float InvLy = 1.0f;
-    double cum_pos = 1.0;   // sum |w| per sign (FP64)
-    double cum_neg = 1.0;
+    float cum_pos = 1.0f;   // sum |w| per sign
+    float cum_neg = 1.0f;
     VorticityView pos_view;
     VorticityView neg_view;
-    // subset sizes (no reference counterpart: the reference's CDF vectors are
-    // exactly that long); -1: cdf_pos->size(), cdf_neg->size()
+    // Added members (no reference counterpart).  FP64 CDF and total of a
+    // sign: when cdf_pos64 / cdf_neg64 is set it is used, with cum_pos64 /
+    // cum_neg64, in place of the FP32 pair above.  An FP32 CDF is widened to
+    // FP64 exactly, so a caller that fills only the reference's members gets
+    // the reference's particle choices.
+    const dvec<double> *cdf_pos64 = nullptr;
+    const dvec<double> *cdf_neg64 = nullptr;
+    double cum_pos64 = 1.0;
+    double cum_neg64 = 1.0;
+    // subset sizes (the reference's CDF vectors are exactly that long);
+    // -1: the size of the CDF in use
     int cnt_pos = -1;
     int cnt_neg = -1;
 };
@@ -71,7 +81,8 @@ int sample_vorticity_idx_cdf(std::mt19937 &gen, const float *cdf, int N);
 float2 sample_disk_biot_savart(std::mt19937 &gen, const float2 &center, float R);
 
 // the context src/cpu/simulation_step.cpp:13-27 builds from a Simulation
-// (subsets, FP64 CDFs, totals, views; counts set), for monte_carlo_bs_cpu
+// (subsets, the FP32 CDFs and totals of the reference's members and the FP64
+// ones of the added members, views; counts set), for monte_carlo_bs_cpu
 MCBSCtx make_mcbs_ctx(const Simulation &sim, bool periodic);
 
 // K(x, y) = (-(x-y)_y, (x-y)_x) / (2 pi (|x-y|^2 + 1e-8))   (:60-66)

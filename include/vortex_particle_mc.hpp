@@ -54,21 +54,21 @@ struct ParticleSet {
     }
     void set_n_current(const int N_cur_) { N_cur = N_cur_; }
 
-    // device pointers
-    float2 *d_p() { return raw_ptr(pos); }
-    float2 *d_pt() { return raw_ptr(pos_temp); }
-    float2 *d_v() { return raw_ptr(vel); }
-    float4 *d_F() { return raw_ptr(F); }
-    float *d_omega() { return raw_ptr(omega); }
-    float *d_omega_field() { return raw_ptr(omega_field); }
-    float *d_jac() { return raw_ptr(jac); }
-    const float2 *d_p() const { return raw_ptr(pos); }
-    const float2 *d_pt() const { return raw_ptr(pos_temp); }
-    const float2 *d_v() const { return raw_ptr(vel); }
-    const float4 *d_F() const { return raw_ptr(F); }
-    const float *d_omega() const { return raw_ptr(omega); }
-    const float *d_omega_field() const { return raw_ptr(omega_field); }
-    const float *d_jac() const { return raw_ptr(jac); }
+    // device pointers (callable from host and device code, as in the reference)
+    HD float2 *d_p() { return raw_ptr(pos); }
+    HD float2 *d_pt() { return raw_ptr(pos_temp); }
+    HD float2 *d_v() { return raw_ptr(vel); }
+    HD float4 *d_F() { return raw_ptr(F); }
+    HD float *d_omega() { return raw_ptr(omega); }
+    HD float *d_omega_field() { return raw_ptr(omega_field); }
+    HD float *d_jac() { return raw_ptr(jac); }
+    HD const float2 *d_p() const { return raw_ptr(pos); }
+    HD const float2 *d_pt() const { return raw_ptr(pos_temp); }
+    HD const float2 *d_v() const { return raw_ptr(vel); }
+    HD const float4 *d_F() const { return raw_ptr(F); }
+    HD const float *d_omega() const { return raw_ptr(omega); }
+    HD const float *d_omega_field() const { return raw_ptr(omega_field); }
+    HD const float *d_jac() const { return raw_ptr(jac); }
 };
 
 struct VorticityView {
@@ -134,8 +134,11 @@ public:
 
     explicit Simulation(SimParam p);
     ~Simulation();
-    Simulation(const Simulation &) = delete;
-    Simulation &operator=(const Simulation &) = delete;
+    // copies own their device buffers (deep copy, as thrust::device_vector
+    // members copy in the reference); the copy's views point at its own
+    // buffers
+    Simulation(const Simulation &o);
+    Simulation &operator=(const Simulation &o);
 
     // src/vortex_particle_mc.cu:77-120: num_of_p uniform random positions
     // (std::mt19937(42)), strengths fitted by CG to the Taylor-Green field
@@ -165,6 +168,10 @@ public:
 
     void compute_velocity_naive(bool periodic, float eps_core);
     void step_cuda_naive(bool periodic);
+
+    // include/vortex_particle_mc.hpp:160,204-208: w+(p) - w-(p) by the
+    // reference-grid query of each sign's view
+    inline float eval_vorticity_cpu(float2 p, bool periodic) const;
 
     void test_vorticity_grid(int res);
 
@@ -197,6 +204,13 @@ public:
     void reinitialize(int nx_l = 0, int ny_l = 0, float tol = 1e-6f, int max_iter = 1000);
     // velocity estimate at the current positions without advancing
     void estimate_velocity(bool periodic);
+    // One step backwards (the adjoint of step_cuda under common random
+    // numbers, vpmc_step_adjoint): the current positions are the state the
+    // forward step `step` started from; lambda (device float2[N_cur]) holds
+    // dL/dx after that step and becomes dL/dx before it; grad_w (device
+    // float[N_cur]) accumulates dL/dw.  Consistent periodic estimator,
+    // no kernel deformation.
+    void step_adjoint(uint32_t step, float2 *lambda, float *grad_w);
     int subset_count(int sign) const { return sign > 0 ? cnt_pos_ : cnt_neg_; }
 
     struct Impl;
@@ -208,6 +222,7 @@ private:
     dvec<int> sub_all_;
     dvec<int> sub_pos_, sub_neg_;
     dvec<double> cdf_pos_, cdf_neg_;   // FP64 (the reference keeps FP32)
+    dvec<float> cdf_pos_f32_, cdf_neg_f32_;   // the same CDFs rounded to FP32
     int cnt_pos_{}, cnt_neg_{};
     double cum_pos_{}, cum_neg_{};
 
@@ -215,8 +230,8 @@ private:
     dvec<float> icdf_kernel_;
     int icdf_L_ = 0;
 
-    VorticityView pos_view;
-    VorticityView neg_view;
+    VorticityView pos_view{};
+    VorticityView neg_view{};
 
     Impl *impl_ = nullptr;
 };
@@ -226,3 +241,10 @@ private:
 // view holds device pointers; the query runs on the GPU (vpmc_query_view).
 float evalVorticityAbs_cpu(const VorticityView &v, const float2 &p, int sign, bool use_signed,
                            bool exclude_center, float h_w, bool periodic);
+
+inline float Simulation::eval_vorticity_cpu(float2 p, bool periodic) const
+{
+    const float w_pos = evalVorticityAbs_cpu(pos_view, p, 1, false, false, P_.h_w, periodic);
+    const float w_neg = evalVorticityAbs_cpu(neg_view, p, -1, false, false, P_.h_w, periodic);
+    return w_pos - w_neg;
+}

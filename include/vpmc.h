@@ -233,6 +233,17 @@ typedef struct vpmc_mc_desc {
        sets' set_jac and field.gq buffers must be given) and, with aux,
        replaces it by update_jacobian_from5 of this step's estimates. */
     float       *jac;
+    /* Owner ranges (no reference counterpart; the reference launches one
+       block per particle, src/cuda/simulation_step.cu:13,51-58, and so has
+       no sample buffer to bound).  owner_base is the global index of owner
+       0: the Philox counter and the replay stream use owner_base + i for
+       owner i, so an estimate over owners [b, b + n) with owner_base = b
+       equals rows b .. b+n-1 of the estimate over all owners.  The binned
+       algorithm runs its owners in chunks of at most chunk_samples sample
+       slots (0: 2^30; at most 2^31 - 1), so its workspace is bounded by the
+       chunk, not by n_owner; results do not depend on the chunk size. */
+    int          owner_base;
+    uint64_t     chunk_samples;
 } vpmc_mc_desc;
 
 enum { VPMC_ALGO_BINNED = 0, VPMC_ALGO_DIRECT = 1 };
@@ -244,9 +255,12 @@ int vpmc_mc_estimate(const vpmc_mc_desc *d, void *stream);
 
 /* Same estimate, binned algorithm (periodic boxes): samples are generated,
    counting-sorted by fine cell and evaluated cell by cell, then reduced per
-   owner in sample order.  Needs vpmc_mc_binned_workspace_bytes(d) bytes
-   (about 60 bytes per sample).  Results equal the direct algorithm's up to
-   FP32 summation order. */
+   owner in sample order, one owner chunk after the other (chunk_samples).
+   Needs vpmc_mc_binned_workspace_bytes(d) bytes (about 60 bytes per sample
+   slot of one chunk).  Results equal the direct algorithm's up to FP32
+   summation order.  With several chunks, eval_events bracket the
+   evaluations of all chunks (first evaluation start to last evaluation
+   end). */
 size_t vpmc_mc_binned_workspace_bytes(const vpmc_mc_desc *d);
 int vpmc_mc_estimate_binned(const vpmc_mc_desc *d, void *workspace, size_t workspace_bytes,
                             void *stream);
@@ -270,7 +284,11 @@ int vpmc_advect(int n, float *pos_xy, const float *vel_sign, float *vel_xy, floa
  * src/cuda/simulation_step.cu:9-90, with aux = 0): rebuild both sign fields
  * from the current positions, estimate, advect.  Subset, CDF and guide arrays
  * are inputs (w is constant in the inviscid step).  Field buffers are
- * caller-owned in `d->set[s].field` (cast away const) and `set_xy`.
+ * caller-owned in `d->set[s].field` (cast away const) and `set_xy`.  Every
+ * particle is an owner (n_owner == n_total, owner_xy == pos_xy): the step
+ * moves all of them; the binned estimator runs them in owner chunks of
+ * d->chunk_samples sample slots, so any N with aux targets fits a bounded
+ * workspace.
  */
 size_t vpmc_step_workspace_bytes(const vpmc_mc_desc *d);
 int vpmc_step(vpmc_mc_desc *d, int n_total, float *pos_xy, const float *omega,

@@ -11,6 +11,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
 #include <fstream>
 #include <iostream>
 #include <random>
@@ -84,16 +85,18 @@ static int run_mcbs(const char *input, const std::string &out)
         Simulation sim(P);
         sim.set_particles(pos, w);
 
-        // monte_carlo_bs_cpu with the reference's FP32 CDFs and totals (so that
-        // both pick the same particles), owners 0 .. n_own-1, +1 then -1, one
-        // generator, aux targets at dst -/+ dx
+        // monte_carlo_bs_cpu with the reference's FP32 CDFs and totals in the
+        // reference's own MCBSCtx members (as src/cpu/simulation_step.cpp:13-27
+        // fills them), owners 0 .. n_own-1, +1 then -1, one generator, aux
+        // targets at dst -/+ dx
         MCBSCtx C = make_mcbs_ctx(sim, true);
-        std::vector<double> cpd(cp.begin(), cp.end()), cnd(cn.begin(), cn.end());
-        dvec<double> dcp(cpd), dcn(cnd);
+        dvec<float> dcp(cp), dcn(cn);
         C.cdf_pos = &dcp;
         C.cdf_neg = &dcn;
-        C.cnt_pos = cnt[0];
-        C.cnt_neg = cnt[1];
+        C.cdf_pos64 = nullptr;   // no FP64 CDFs: the FP32 members are used
+        C.cdf_neg64 = nullptr;
+        C.cnt_pos = -1;          // the CDF vectors' sizes, as in the reference
+        C.cnt_neg = -1;
         C.cum_pos = cum[0];
         C.cum_neg = cum[1];
         std::mt19937 rng(seed);
@@ -325,6 +328,57 @@ int main(int argc, char **argv)
             sd.reinitialize(0, 0, 1e-6f, 1000);
             dump(out + "/pos_reinit.bin", sd.particles_.pos.to_host());
             dump(out + "/omega_reinit.bin", sd.particles_.omega.to_host());
+        }
+
+        // Simulation::eval_vorticity_cpu (w+ - w- of the reference-grid views,
+        // hashed at the current positions as the reference's main loop does)
+        {
+            sim.do_spatial_hashing();
+            std::vector<float> ev;
+            for (int i = 0; i < 16; ++i)
+                ev.push_back(sim.eval_vorticity_cpu(
+                    make_float2(0.03f + 0.061f * i, 0.97f - 0.057f * i), true));
+            dump(out + "/eval_vort.bin", ev);
+        }
+
+        // a bounded box: estimate_velocity(false), then step_cuda(false), in
+        // both estimator modes (the direct estimator; no wrap)
+        const std::vector<float> omega0 = sim.particles_.omega.to_host();
+        for (int md = 0; md < 2; ++md) {
+            Simulation sn(P);
+            sn.mode = md ? EstimatorMode::Consistent : EstimatorMode::Reference;
+            sn.seed = 7;
+            sn.set_particles(pos, omega0);
+            sn.estimate_velocity(false);
+            dump(out + "/vel_np" + std::to_string(md) + ".bin", sn.particles_.vel.to_host());
+            sn.step_cuda(false);
+            (void)hipDeviceSynchronize();
+            dump(out + "/pos_np" + std::to_string(md) + ".bin", sn.particles_.pos.to_host());
+        }
+
+        // copies own their buffers: a copy-constructed and a copy-assigned
+        // simulation take the same step as the original, bit for bit, and
+        // stepping the original leaves an earlier copy unchanged
+        {
+            Simulation c1 = sim;
+            Simulation c2(SimParam(1e-3f, 1.f, 1.f, 16, 4.f));
+            c2 = sim;
+            const std::vector<float2> before = c1.particles_.pos.to_host();
+            sim.step_cuda(true);
+            const std::vector<float2> untouched = c1.particles_.pos.to_host();
+            c1.step_cuda(true);
+            c2.step_cuda(true);
+            (void)hipDeviceSynchronize();
+            const std::vector<float2> a = sim.particles_.pos.to_host(),
+                                      b = c1.particles_.pos.to_host(),
+                                      c = c2.particles_.pos.to_host();
+            const size_t nb = a.size() * sizeof(float2);
+            const bool same = b.size() == a.size() && c.size() == a.size() &&
+                              std::memcmp(a.data(), b.data(), nb) == 0 &&
+                              std::memcmp(a.data(), c.data(), nb) == 0;
+            const bool kept = std::memcmp(before.data(), untouched.data(), nb) == 0 &&
+                              std::memcmp(before.data(), a.data(), nb) != 0;
+            dump(out + "/copy_ok.bin", std::vector<int>{same ? 1 : 0, kept ? 1 : 0});
         }
 
         sim.test_vorticity_grid(16);

@@ -112,6 +112,26 @@ def test_cxx_api_end_to_end(device, tmp_path):
     F = rd("F_def2.bin").reshape(-1, 4)[:N]
     assert np.array_equal(F, sd.jac[:N].cpu().numpy())
     assert not np.array_equal(F, np.tile([1, 0, 0, 1], (N, 1)).astype(np.float32))
+    # Simulation::eval_vorticity_cpu == the oracle's w+ - w- query on the
+    # reference grid (src/cpu/evaluate_vorticity.cpp:8-75) at the two-step state
+    p2f = rd("pos2.bin").reshape(-1, 2)[:N]
+    pr2 = O.Problem(p2f[:, 0].copy(), p2f[:, 1].copy(), w, rsh, rsh)
+    ev = rd("eval_vort.bin")
+    ep = pr2.eval(pts[:, 0], pts[:, 1], +1, hw) - pr2.eval(pts[:, 0], pts[:, 1], -1, hw)
+    assert rel_l2(ev, ep) < 1e-5 and np.abs(ep).max() > 0
+    # a bounded box (step_cuda(false), estimate_velocity(false): the direct
+    # estimator, no wrap) == the oracle with periodic = 0, both modes
+    for md, mode in ((0, O.MODE_REFERENCE), (1, O.MODE_CONSISTENT)):
+        prn = O.Problem(x, y, w, rsh, rsh)
+        Pn = prn.params(mode, 64, 64, hw, periodic=False, seed=7)
+        vn = prn.velocities(Pn)
+        assert rel_l2(rd("vel_np%d.bin" % md).reshape(-1, 2)[:N], vn) < 1e-5, md
+        assert np.abs(vn).max() > 0
+        pn = rd("pos_np%d.bin" % md).reshape(-1, 2)[:N].astype(np.float64)
+        pe = np.stack([x, y], 1) + vn.astype(np.float64) * 1e-3
+        assert np.abs(pn - pe).max() < 1e-6, md
+    # copy-constructed and copy-assigned simulations step like the original
+    assert list(rd("copy_ok.bin", np.int32)) == [1, 1]
     # reinitialize() == the Python mirror's reinit (same entry point, same state)
     sd.reinit(tol=1e-6)
     assert np.array_equal(rd("pos_reinit.bin").reshape(-1, 2)[:sd.N], sd.pos.cpu().numpy())

@@ -48,14 +48,58 @@ def build(jobs=None, verbose=False):
     srcs = sorted(glob.glob(os.path.join(CSRC, "*.hip")) + glob.glob(os.path.join(CSRC, "*.cpp")))
     jobs = jobs or min(8, os.cpu_count() or 4)
     with cf.ThreadPoolExecutor(jobs) as ex:
+        phash_obj = ex.submit(compile_phash)   # the longest compile: start it first
         objs = list(ex.map(_compile, srcs))
+        phash_obj = phash_obj.result()
     tmp = LIB + ".tmp"
     _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs)
     os.replace(tmp, LIB)
     build_cxx_check()
+    link_phash(phash_obj)
     if verbose:
-        print("built", LIB)
+        print("built", LIB, PHASH)
     return LIB
+
+
+PHASH_SRC = os.path.join(PKG, "bindings", "phash.cpp")
+PHASH = os.path.join(PKG, "lib", "phash.so")
+
+
+def compile_phash():
+    """the bindings module phash (pybind11 + torch C++ API, host code only):
+    compiled to an object in _build/"""
+    import sysconfig
+
+    import torch
+    from torch.utils import cpp_extension as ce
+    inc = []
+    for p in ce.include_paths() + [sysconfig.get_paths()["include"], INC, "/opt/rocm/include"]:
+        inc += ["-isystem" if p.startswith(("/usr", "/opt")) else "-I", p]
+    obj = os.path.join(BUILD, "phash.o")
+    _run(["g++", "-O2", "-std=c++20", "-fPIC", "-fvisibility=hidden", "-c",
+          "-D__HIP_PLATFORM_AMD__=1", "-DTORCH_EXTENSION_NAME=phash",
+          "-DTORCH_API_INCLUDE_EXTENSION_H",
+          "-D_GLIBCXX_USE_CXX11_ABI=%d" % int(torch._C._GLIBCXX_USE_CXX11_ABI)] + inc +
+         [PHASH_SRC, "-o", obj])
+    return obj
+
+
+def link_phash(obj):
+    """lib/phash.so: links libvpmc.so from the same directory and libtorch by rpath"""
+    from torch.utils import cpp_extension as ce
+    libdir = os.path.dirname(LIB)
+    tlib = ce.library_paths()[0]
+    tmp = PHASH + ".tmp"
+    _run(["g++", "-shared", "-fPIC", obj, "-o", tmp, "-L", libdir, "-lvpmc", "-L", tlib,
+          "-lc10", "-ltorch", "-ltorch_cpu", "-ltorch_python", "-L/opt/rocm/lib", "-lamdhip64",
+          "-Wl,-rpath,$ORIGIN", "-Wl,-rpath," + tlib, "-Wl,-rpath,/opt/rocm/lib"])
+    os.replace(tmp, PHASH)
+    return PHASH
+
+
+def build_phash():
+    os.makedirs(BUILD, exist_ok=True)
+    return link_phash(compile_phash())
 
 
 CXX_CHECK_SRC = os.path.join(ROOT, "tests", "cxx", "cxx_api_check.cpp")

@@ -56,7 +56,7 @@ class McDesc(C.Structure):
                 ("owner_xy", vp), ("set", SignSet * 2), ("vel_sign", vp),
                 ("eval_count", vp), ("algo", ci), ("eval_events", vp * 2), ("spl", ci),
                 ("eval_threads", ci), ("aux", ci), ("aux_dx", cf), ("vel_aux", vp),
-                ("jac", vp)]
+                ("jac", vp), ("owner_base", ci), ("chunk_samples", C.c_uint64)]
 
 
 class EnsembleDesc(C.Structure):

@@ -26,6 +26,11 @@
 //   final   one wave per (owner, sign): MIS contributions of its N1+N2
 //           samples in sample order, butterfly sum -> deterministic result.
 // Sample ids are owner-major: sid = (owner*2 + sign)*(N1+N2) + k.
+// The four passes run once per owner chunk (at most chunk_samples sample
+// slots), so the workspace is that of one chunk.  A sample's omega depends
+// only on its point and the field, its draws on the global owner index, and
+// each owner's reduction on its own samples, so the chunk size does not
+// change any result.
 #include "binned_core.hpp"
 
 namespace vpmc {
@@ -159,6 +164,27 @@ int bin_geom(const McD &m, int N1, int N2, int n_owner, BinGeom *g)
     return VPMC_OK;
 }
 
+// Owners per chunk of the binned estimate: balanced chunks of at most
+// chunk_samples (0: 2^30) sample slots, each below the sort's int32 limit.
+constexpr uint64_t BIN_CHUNK_DEFAULT = (uint64_t)1 << 30;
+
+static int bin_chunk_owners(const vpmc_mc_desc *d, int *per_chunk)
+{
+    const uint64_t per_owner = 2 * (uint64_t)samples_per_pair(d->N1, d->N2, d->aux);
+    uint64_t cap = d->chunk_samples ? d->chunk_samples : BIN_CHUNK_DEFAULT;
+    if (cap > ((uint64_t)1 << 31) - 1) cap = ((uint64_t)1 << 31) - 1;
+    if (per_owner > cap) return VPMC_ERR_ARG;   // one owner's samples exceed a chunk
+    const int n = d->n_owner;
+    if (n <= 0) {
+        *per_chunk = 1;
+        return VPMC_OK;
+    }
+    const uint64_t most = cap / per_owner;
+    const uint64_t nchunk = ((uint64_t)n + most - 1) / most;
+    *per_chunk = (int)(((uint64_t)n + nchunk - 1) / nchunk);
+    return VPMC_OK;
+}
+
 static size_t bin_ws_bytes(const BinGeom &g)
 {
     size_t S = g.S > 0 ? g.S : 1, T = (size_t)g.ntile;
@@ -216,10 +242,26 @@ int mc_binned_workspace(const vpmc_mc_desc *d, size_t *bytes)
 {
     McD m;
     VPMC_TRY(make_mcd(d, &m));
+    int per_chunk;
+    VPMC_TRY(bin_chunk_owners(d, &per_chunk));
     BinGeom g;
-    VPMC_TRY(bin_geom(m, d->N1, d->N2, d->n_owner, &g));
+    VPMC_TRY(bin_geom(m, d->N1, d->N2, d->n_owner < per_chunk ? d->n_owner : per_chunk, &g));
     *bytes = bin_ws_bytes(g);
     return VPMC_OK;
+}
+
+// Owners [o0, o0 + cnt) of the call: the chunk's descriptor reads owner
+// o0 + i as its owner i, keys its draws by the global owner index and
+// writes rows o0 + i of vel_sign / vel_aux (stride n_owner of the call).
+static McD chunk_mcd(const McD &m, int o0, int cnt)
+{
+    McD c = m;
+    c.owner = m.owner + o0;
+    c.pbase = m.pbase + o0;
+    c.n_owner = cnt;
+    c.vel_sign = m.vel_sign + o0;
+    c.vel_aux = m.vel_aux ? m.vel_aux + o0 : nullptr;
+    return c;
 }
 
 int mc_binned_impl(const vpmc_mc_desc *d, void *ws, size_t ws_bytes, hipStream_t s)
@@ -228,26 +270,34 @@ int mc_binned_impl(const vpmc_mc_desc *d, void *ws, size_t ws_bytes, hipStream_t
     VPMC_TRY(make_mcd(d, &m));
     if (!d->periodic) return VPMC_ERR_ARG;   // the cell stencil wraps periodically
     if (m.n_owner == 0) return VPMC_OK;
-    BinGeom g;
-    VPMC_TRY(bin_geom(m, d->N1, d->N2, d->n_owner, &g));
+    int per_chunk;
+    VPMC_TRY(bin_chunk_owners(d, &per_chunk));
+    BinGeom g0;
+    VPMC_TRY(bin_geom(m, d->N1, d->N2, per_chunk < m.n_owner ? per_chunk : m.n_owner, &g0));
     BinBufs B;
-    VPMC_TRY(carve_bin(g, ws, ws_bytes, &B));
-    launch_mode(d->mode, d->periodic, m, g, B, false, s);
-    VPMC_CHECK_LAUNCH();
-    VPMC_TRY(radix_sort_samples(B.key, (int)g.S, key_bits_for((uint32_t)g.ntile),
-                                (const float2 *)B.q, B.srec, B.keys_sorted, g.ntile, B.start,
-                                B.sort_ws, B.sort_bytes, s));
-    VPMC_TRY(record_event(d->eval_events[0], s));
-    if (d->eval_count)
-        VPMC_TRY(launch_eval<true>(d->eval_threads, d->spl, m, g, B.start, B.srec, B.om,
-                                   d->eval_count, s));
-    else
-        VPMC_TRY(launch_eval<false>(d->eval_threads, d->spl, m, g, B.start, B.srec, B.om, nullptr,
-                                    s));
-    VPMC_CHECK_LAUNCH();
-    VPMC_TRY(record_event(d->eval_events[1], s));
-    launch_mode(d->mode, d->periodic, m, g, B, true, s);
-    VPMC_CHECK_LAUNCH();
+    VPMC_TRY(carve_bin(g0, ws, ws_bytes, &B));
+    for (int o0 = 0; o0 < m.n_owner; o0 += per_chunk) {
+        const int cnt = per_chunk < m.n_owner - o0 ? per_chunk : m.n_owner - o0;
+        const McD mc = chunk_mcd(m, o0, cnt);
+        BinGeom g = g0;
+        g.S = (size_t)cnt * 2 * g.NS;
+        launch_mode(d->mode, d->periodic, mc, g, B, false, s);
+        VPMC_CHECK_LAUNCH();
+        VPMC_TRY(radix_sort_samples(B.key, (int)g.S, key_bits_for((uint32_t)g.ntile),
+                                    (const float2 *)B.q, B.srec, B.keys_sorted, g.ntile, B.start,
+                                    B.sort_ws, B.sort_bytes, s));
+        if (o0 == 0) VPMC_TRY(record_event(d->eval_events[0], s));
+        if (d->eval_count)
+            VPMC_TRY(launch_eval<true>(d->eval_threads, d->spl, mc, g, B.start, B.srec, B.om,
+                                       d->eval_count, s));
+        else
+            VPMC_TRY(launch_eval<false>(d->eval_threads, d->spl, mc, g, B.start, B.srec, B.om,
+                                        nullptr, s));
+        VPMC_CHECK_LAUNCH();
+        if (o0 + cnt >= m.n_owner) VPMC_TRY(record_event(d->eval_events[1], s));
+        launch_mode(d->mode, d->periodic, mc, g, B, true, s);
+        VPMC_CHECK_LAUNCH();
+    }
     return VPMC_OK;
 }
 

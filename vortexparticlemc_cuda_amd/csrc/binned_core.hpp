@@ -408,8 +408,8 @@ __device__ __forceinline__ void bin_final_pair(const McD &d, const BinGeom &g, i
         const float ax = wave_sum(sx[t]), ay = wave_sum(sy[t]);
         if (lane == 0) {
             const float2 v = make_float2(ax * d.inv_tot, ay * d.inv_tot);
-            if (t == 0) d.vel_sign[(size_t)si * d.n_owner + owner] = v;
-            else d.vel_aux[((size_t)si * 4 + (t - 1)) * d.n_owner + owner] = v;
+            if (t == 0) d.vel_sign[(size_t)si * d.vel_stride + owner] = v;
+            else d.vel_aux[((size_t)si * 4 + (t - 1)) * d.vel_stride + owner] = v;
         }
     }
 }

@@ -233,6 +233,7 @@ namespace {
 struct ReplaySet {   // one sign set as the estimator reads it
     int n = 0, nx = 0, ny = 0, glog2 = 0;
     const double *cdf = nullptr;
+    dvec<double> cdf_wide;   // an FP32 CDF widened to FP64 (exact)
     dvec<int> count;
     dvec<double> total;
     dvec<int> guide, cell_start;
@@ -258,19 +259,32 @@ static void replay_set_build(ReplaySet &S, const MCBSCtx &C, int sign, const Par
 {
     const bool pos = sign > 0;
     const dvec<int> *sub = pos ? C.sub_pos : C.sub_neg;
-    const dvec<double> *cdf = pos ? C.cdf_pos : C.cdf_neg;
-    if (!sub || !cdf)
+    // the FP64 CDF of the added members, else the reference's FP32 one
+    const dvec<double> *cdf64 = pos ? C.cdf_pos64 : C.cdf_neg64;
+    const dvec<float> *cdf32 = pos ? C.cdf_pos : C.cdf_neg;
+    if (!sub || (!cdf64 && !cdf32))
         throw std::invalid_argument("monte_carlo_bs_cpu: MCBSCtx needs the sign's sub and cdf");
+    const size_t cdf_size = cdf64 ? cdf64->size() : cdf32->size();
     int n = pos ? C.cnt_pos : C.cnt_neg;
-    if (n < 0) n = (int)cdf->size();
-    if ((size_t)n > cdf->size() || (size_t)n > sub->size())
+    if (n < 0) n = (int)cdf_size;
+    if ((size_t)n > cdf_size || (size_t)n > sub->size())
         throw std::invalid_argument("monte_carlo_bs_cpu: subset count beyond its buffers");
     if (!(C.h_w > 0.f) || !(C.Lx > 0.f) || !(C.Ly > 0.f))
         throw std::invalid_argument("monte_carlo_bs_cpu: h_w, Lx, Ly must be positive");
     vcheck(vpmc_field_grid(C.h_w, C.Lx, C.Ly, 8, &S.nx, &S.ny), "monte_carlo_bs_cpu");
     S.n = n;
-    S.cdf = raw_ptr(*cdf);
-    const double cum = pos ? C.cum_pos : C.cum_neg;
+    double cum;
+    if (cdf64) {
+        S.cdf = raw_ptr(*cdf64);
+        cum = pos ? C.cum_pos64 : C.cum_neg64;
+    } else {   // every float is a double: the same comparisons as the reference's
+        const std::vector<float> h = cdf32->to_host();
+        const std::vector<double> wide(h.begin(), h.begin() + n);
+        if (n > 0) S.cdf_wide.assign_host(wide.data(), wide.size());
+        else S.cdf_wide.resize(1);
+        S.cdf = raw_ptr(S.cdf_wide);
+        cum = pos ? C.cum_pos : C.cum_neg;
+    }
     S.count.assign_host(&n, 1);
     S.total.assign_host(&cum, 1);
     int gl = 0;
@@ -362,10 +376,66 @@ struct Simulation::Impl {
     dvec<float4> gq[2], set_jac[2];   // kernel deformation: forms and F per set
     dvec<float2> vel_aux;             // [sign][L, R, T, B][owner]
     Scratch ws;
+    Impl() = default;
+    Impl(const Impl &o)   // device buffers deep-copied; scratch is not state
+        : nx(o.nx), ny(o.ny), glog2(o.glog2), counts(o.counts), totals(o.totals),
+          guide{o.guide[0], o.guide[1]}, pts{o.pts[0], o.pts[1]},
+          cell_start{o.cell_start[0], o.cell_start[1]}, order{o.order[0], o.order[1]},
+          set_xy{o.set_xy[0], o.set_xy[1]}, vel_sign(o.vel_sign), gq{o.gq[0], o.gq[1]},
+          set_jac{o.set_jac[0], o.set_jac[1]}, vel_aux(o.vel_aux)
+    {
+    }
 };
 
 Simulation::Simulation(SimParam p) : P_(std::move(p)), impl_(new Impl) {}
 Simulation::~Simulation() { delete impl_; }
+
+Simulation::Simulation(const Simulation &o) : P_(o.P_), impl_(new Impl) { *this = o; }
+
+Simulation &Simulation::operator=(const Simulation &o)
+{
+    if (this == &o) return *this;
+    particles_ = o.particles_;
+    P_ = o.P_;
+    rng = o.rng;
+    mode = o.mode;
+    seed = o.seed;
+    sim_id = o.sim_id;
+    step_index = o.step_index;
+    cells_per_hw = o.cells_per_hw;
+    kernel_deformation = o.kernel_deformation;
+    deformation_bound = o.deformation_bound;
+    stream = o.stream;
+    cg_iterations = o.cg_iterations;
+    cg_rel_residual = o.cg_rel_residual;
+    hash_pos_ = o.hash_pos_;
+    hash_neg_ = o.hash_neg_;
+    hash_all_ = o.hash_all_;
+    sub_all_ = o.sub_all_;
+    sub_pos_ = o.sub_pos_;
+    sub_neg_ = o.sub_neg_;
+    cdf_pos_ = o.cdf_pos_;
+    cdf_neg_ = o.cdf_neg_;
+    cdf_pos_f32_ = o.cdf_pos_f32_;
+    cdf_neg_f32_ = o.cdf_neg_f32_;
+    cnt_pos_ = o.cnt_pos_;
+    cnt_neg_ = o.cnt_neg_;
+    cum_pos_ = o.cum_pos_;
+    cum_neg_ = o.cum_neg_;
+    cdf_kernel = o.cdf_kernel;
+    icdf_kernel = o.icdf_kernel;
+    icdf_kernel_ = o.icdf_kernel_;
+    icdf_L_ = o.icdf_L_;
+    if (impl_ != o.impl_) {
+        Impl *n = new Impl(*o.impl_);
+        delete impl_;
+        impl_ = n;
+    }
+    // views onto this copy's own hash buffers
+    pos_view = hash_pos_.vp_sort.size() ? makePosView() : VorticityView{};
+    neg_view = hash_neg_.vp_sort.size() ? makeNegView() : VorticityView{};
+    return *this;
+}
 
 void Simulation::init(int num_of_p)
 {
@@ -504,6 +574,13 @@ void Simulation::build_subsets_cuda()
     cnt_neg_ = c[1];
     cum_pos_ = t[0];
     cum_neg_ = t[1];
+    // the reference's FP32 CDF vectors (MCBSCtx::cdf_pos / cdf_neg), one entry
+    // per subset member
+    for (int s = 0; s < 2; ++s) {
+        const std::vector<double> h = (s ? cdf_neg_ : cdf_pos_).to_host();
+        const std::vector<float> f(h.begin(), h.begin() + c[s]);
+        (s ? cdf_neg_f32_ : cdf_pos_f32_).assign_host(f.data(), f.size());
+    }
 }
 
 void Simulation::do_spatial_hashing()
@@ -617,7 +694,22 @@ static vpmc_mc_desc make_desc(Simulation &S, int N1, int N2, int cnt_pos, int cn
                              raw_ptr(cell_start[s]), raw_ptr(order[s])};
     }
     d.vel_sign = (float *)vel_sign;
+    // the binned estimator's cell stencil wraps periodically; a bounded box
+    // takes the direct estimator (one workgroup per owner and sign)
+    d.algo = periodic ? VPMC_ALGO_BINNED : VPMC_ALGO_DIRECT;
     return d;
+}
+
+// the estimate of descriptor d with the algorithm make_desc chose
+static void run_estimate(const vpmc_mc_desc &d, Scratch &ws, hipStream_t stream,
+                         const char *what)
+{
+    if (d.algo == VPMC_ALGO_BINNED) {
+        const size_t wb = vpmc_mc_binned_workspace_bytes(&d);
+        vcheck(vpmc_mc_estimate_binned(&d, ws.get(wb), wb, stream), what);
+    } else {
+        vcheck(vpmc_mc_estimate(&d, stream), what);
+    }
 }
 
 // the CUDA step's F in the fields and the jitter (src/cuda/monte_carlo_bs.cu:210-212)
@@ -640,9 +732,13 @@ void Simulation::estimate_velocity(bool periodic)
                                raw_ptr(I.totals), I.guide, I.glog2, raw_ptr(cdf_pos_),
                                raw_ptr(cdf_neg_), I.pts, I.cell_start, I.order, I.set_xy, I.nx,
                                I.ny, raw_ptr(I.vel_sign), periodic);
-    if (kernel_deformation) attach_deformation(d, I, deformation_bound);
-    size_t wb = vpmc_mc_binned_workspace_bytes(&d);
-    vcheck(vpmc_mc_estimate_binned(&d, I.ws.get(wb), wb, stream), "monte_carlo_bs");
+    if (kernel_deformation) {
+        if (!periodic)
+            throw std::invalid_argument(
+                "estimate_velocity: kernel deformation needs a periodic box (binned estimator)");
+        attach_deformation(d, I, deformation_bound);
+    }
+    run_estimate(d, I.ws, stream, "monte_carlo_bs");
     // vel = v+ - v- (dt = 0 leaves the positions unchanged)
     vcheck(vpmc_advect(N, (float *)particles_.d_p(), (const float *)raw_ptr(I.vel_sign),
                        (float *)particles_.d_v(), 0.f, P_.Lx, P_.Ly, periodic ? 1 : 0, stream),
@@ -717,6 +813,9 @@ void Simulation::step_cuda(bool periodic)
                                raw_ptr(cdf_neg_), I.pts, I.cell_start, I.order, I.set_xy, I.nx,
                                I.ny, raw_ptr(I.vel_sign), periodic);
     if (kernel_deformation) {
+        if (!periodic)
+            throw std::invalid_argument(
+                "step_cuda: kernel deformation needs a periodic box (binned estimator)");
         attach_deformation(d, I, deformation_bound);
         // aux = 1: estimates at dst -/+ dx, F = update_jacobian_from5 of them
         // before the move (src/cuda/simulation_step.cu:51-73)
@@ -731,6 +830,31 @@ void Simulation::step_cuda(bool periodic)
                      raw_ptr(sub_neg_), (float *)particles_.d_v(), P_.dt, I.ws.get(b), b, stream),
            "step_cuda");
     ++step_index;
+}
+
+void Simulation::step_adjoint(uint32_t step, float2 *lambda, float *grad_w)
+{
+    // the adjoint of step_cuda(true) at Philox step `step`, from the state the
+    // forward step started from (the current positions)
+    if (kernel_deformation)
+        throw std::invalid_argument("step_adjoint: kernel deformation is not differentiated");
+    Impl &I = *impl_;
+    const int N = particles_.N_cur;
+    if (I.nx == 0) do_spatial_hashing();
+    I.vel_sign.resize(2 * (size_t)N);
+    vpmc_mc_desc d = make_desc(*this, P_.N1, P_.N2, cnt_pos_, cnt_neg_, raw_ptr(I.counts),
+                               raw_ptr(I.totals), I.guide, I.glog2, raw_ptr(cdf_pos_),
+                               raw_ptr(cdf_neg_), I.pts, I.cell_start, I.order, I.set_xy, I.nx,
+                               I.ny, raw_ptr(I.vel_sign), true);
+    d.step = step;
+    const size_t b = vpmc_step_adjoint_workspace_bytes(&d);
+    if (b == 0)
+        throw std::invalid_argument(
+            "step_adjoint: the adjoint differentiates the consistent periodic estimator");
+    vcheck(vpmc_step_adjoint(&d, N, (const float *)particles_.d_p(), particles_.d_omega(),
+                             raw_ptr(sub_pos_), raw_ptr(sub_neg_), P_.dt, (float *)lambda, grad_w,
+                             I.ws.get(b), b, stream),
+           "step_adjoint");
 }
 
 void Simulation::compute_velocity_naive(bool periodic, float eps_core)
@@ -885,8 +1009,12 @@ MCBSCtx make_mcbs_ctx(const Simulation &sim, bool periodic)
     MCBSCtx C;
     C.sub_pos = &sim.sub_pos_;
     C.sub_neg = &sim.sub_neg_;
-    C.cdf_pos = &sim.cdf_pos_;
-    C.cdf_neg = &sim.cdf_neg_;
+    C.cdf_pos = &sim.cdf_pos_f32_;
+    C.cdf_neg = &sim.cdf_neg_f32_;
+    C.cdf_pos64 = &sim.cdf_pos_;
+    C.cdf_neg64 = &sim.cdf_neg_;
+    C.cum_pos64 = sim.cum_pos_;
+    C.cum_neg64 = sim.cum_neg_;
     C.icdf_kernel = &sim.icdf_kernel_;
     C.kernel_L = sim.icdf_L_;
     C.h_w = sim.P_.h_w;
@@ -895,8 +1023,8 @@ MCBSCtx make_mcbs_ctx(const Simulation &sim, bool periodic)
     C.Ly = sim.P_.Ly;
     C.InvLx = 1.f / sim.P_.Lx;
     C.InvLy = 1.f / sim.P_.Ly;
-    C.cum_pos = sim.cum_pos_;
-    C.cum_neg = sim.cum_neg_;
+    C.cum_pos = (float)sim.cum_pos_;
+    C.cum_neg = (float)sim.cum_neg_;
     C.pos_view = sim.makePosView();
     C.neg_view = sim.makeNegView();
     C.cnt_pos = sim.cnt_pos_;

@@ -65,7 +65,7 @@ __global__ void __launch_bounds__(MC_THREADS) k_mc_estimate(McD d)
             ty += red[1][w];
             if (COUNT) { a += redc[0][w]; b += redc[1][w]; }
         }
-        d.vel_sign[(size_t)si * d.n_owner + owner] = make_float2(tx * d.inv_tot, ty * d.inv_tot);
+        d.vel_sign[(size_t)si * d.vel_stride + owner] = make_float2(tx * d.inv_tot, ty * d.inv_tot);
         if (COUNT) {
             atomicAdd(d.eval_count, a);
             atomicAdd(d.eval_count + 1, b);
@@ -156,6 +156,8 @@ int make_mcd(const vpmc_mc_desc *in, McD *d)
     if (in->mode == VPMC_MODE_CONSISTENT && in->periodic && in->Lx != in->Ly)
         return VPMC_ERR_ARG;   // the periodic kernel is written for the square lattice
     if (in->n_owner > 0 && (!in->owner_xy || !in->vel_sign)) return VPMC_ERR_ARG;
+    if (in->owner_base < 0 || (int64_t)in->owner_base + in->n_owner > INT32_MAX)
+        return VPMC_ERR_ARG;
     McD &m = *d;
     m.N1 = in->N1;
     m.N2 = in->N2;
@@ -192,6 +194,8 @@ int make_mcd(const vpmc_mc_desc *in, McD *d)
     m.b2 = in->aux_draws ? 10 : 2;
     m.n_owner = in->n_owner;
     m.owner = (const float2 *)in->owner_xy;
+    m.pbase = in->owner_base;
+    m.vel_stride = in->n_owner;
     for (int s = 0; s < 2; ++s) {
         const vpmc_sign_set &S = in->set[s];
         SetD &D = m.set[s];

@@ -34,6 +34,8 @@ struct McD {
     int b1, b2;
     int n_owner;
     const float2 *owner;
+    int pbase;                // global index of owner 0 (Philox counter, replay stream)
+    int vel_stride;           // owners per sign row of vel_sign / vel_aux
     SetD set[2];
     float2 *vel_sign;
     unsigned long long *eval_count;
@@ -96,9 +98,10 @@ __device__ __forceinline__ float mass_inverse(float u)
 __device__ __forceinline__ void draws(const McD &d, int owner, int si, int branch, int k,
                                       float u[3])
 {
+    const uint32_t og = (uint32_t)(d.pbase + owner);   // global owner index
     if (d.U) {
         size_t per = (size_t)d.N1 * d.b1 + (size_t)d.N2 * d.b2;
-        const float *base = d.U + ((size_t)owner * 2 + si) * per;
+        const float *base = d.U + ((size_t)og * 2 + si) * per;
         if (branch == 1) {
             const float *q = base + (size_t)k * d.b1;
             u[0] = q[0];
@@ -111,7 +114,7 @@ __device__ __forceinline__ void draws(const McD &d, int owner, int si, int branc
             u[2] = 0.f;
         }
     } else {
-        uint4 c = make_uint4((uint32_t)k, (uint32_t)owner, d.step,
+        uint4 c = make_uint4((uint32_t)k, og, d.step,
                              (d.sim << 8) | ((uint32_t)si << 7) | ((uint32_t)branch << 4));
         uint4 x = philox10(c, make_uint2(d.key0, d.key1));
         u[0] = u01(x.x);

@@ -75,6 +75,9 @@ class Simulation:
         self.seed = int(seed)
         self.sim = int(sim)
         self.step_index = 0
+        # sample slots per owner chunk of the binned estimator (0: the
+        # library's default, 2^30); results do not depend on it
+        self.chunk_samples = 0
         self.pos = None
         self.omega = None
 
@@ -171,6 +174,7 @@ class Simulation:
             d.aux_dx = float(P.dx)
             d.vel_aux = ops._p(self.vel_aux)
         d.jac = ops._p(self.jac) if self.deform else None
+        d.chunk_samples = int(self.chunk_samples)
         return d
 
     def _ws_for(self, d):
