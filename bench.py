@@ -1,17 +1,27 @@
-"""Benchmark of the plain GPU path (one MI355X).
+"""Benchmark of the vortex-particle Monte Carlo step on one MI355X.
 
-Workload (default): Taylor-Green vorticity 4 pi sin(2 pi x) sin(2 pi y) on the
-periodic unit box, Nx = Ny = 200 lattice (40,000 particles), hwr = 16
-(h_w = 0.08), N1 = N2 = 128, dt = 1e-3, consistent estimator, Philox seed 0,
-strengths from CG-RBF (FP64 MINRES, tol 1e-6).  A step is one
-pass of the hot path: rebuild both sign fields from the current positions,
-estimate both signs, advect.  Inputs are resident in HBM before timing.
+Headline (--steps K, --warmup W): the batched ensemble.  64 independent
+periodic simulations on [0,1]^2 with Nx drawn from {64, 96, 128, 192, 256}
+(seed 20261203), 8 Gaussian vortices each, hwr = 8, N1 = N2 = 64, dt = 5e-4,
+consistent estimator (FP32 bulk, FP64 |omega| CDF and Omega), RBF strengths.
+A step is one pass of the hot path for every simulation: rebuild both sign
+fields, generate and bin the samples, evaluate, reduce, advect; all
+simulations share one launch per stage and K steps replay from one HIP graph.
+`value` is particle-steps/s over exactly K timed steps with the inputs
+resident in HBM.  Beside it: the RMS velocity error against the FP64 blob
+direct sum, the tuned plain per-simulation path and the reference's launch
+structure on the same ensemble (batched_over_plain,
+batched_over_reference_launch), and the evaluation kernel's roofline.
 
-Prints one JSON line: particle-steps/s over K timed steps, the estimator's
-roofline (algorithmic FLOPs per launch / its event-timed duration), the RMS
-velocity error against the FP64 direct sum, and the reference's CPU path
-(oracle/_ref, compiled from its sources) timed on a bounded sample of the same
-workload on this host.
+Also measured: the plain GPU path at the reference's default size
+(Taylor-Green, Nx = 200, hwr = 16, N1 = N2 = 128) with an error-versus-samples
+sweep and one step at the smallest N1 = N2 that reaches 1e-2 RMS error; the
+CUDA step's semantics (aux targets + F); the large double-shear run
+(1,048,576 particles, N1 = N2 = 256) with the reference step at that size in
+owner chunks; the adjoint workload with its finite-difference check; and the
+reference's own CPU path (oracle/_ref, compiled from its sources) on a bounded
+sample of the headline workload and on the 10,000-particle Taylor-Green case
+in full.
 """
 import argparse
 import ctypes as C
@@ -332,60 +342,123 @@ def deform_bench(args, device, base):
     return out
 
 
-def ensemble_bench(args, device):
-    """The batched ensemble (64 periodic simulations, Nx drawn from
-    {64, 96, 128, 192, 256}, 8 Gaussian vortices each, hwr = 8, N1 = N2 = 64,
-    dt = 5e-4): particle-steps/s of the batched path (one launch per stage,
-    eager and replayed from a HIP graph) and of the plain per-simulation path
-    on the same ensemble.  Both compute the same estimator (bit-identical per
-    simulation), so their RMS velocity errors are equal by construction."""
+def ensemble_data(args, device):
+    """the 64 Gaussian-vortex simulations with their RBF strengths"""
     import torch
 
     from vortexparticlemc_cuda_amd import ops
-    from vortexparticlemc_cuda_amd._capi import ALGO_DIRECT
-    from vortexparticlemc_cuda_amd.ensemble import Ensemble, autotune
-    from vortexparticlemc_cuda_amd.sim import SimParam, Simulation
+    from vortexparticlemc_cuda_amd.sim import SimParam
     from vortexparticlemc_cuda_amd.workloads import gaussian_ensemble
 
     data = gaussian_ensemble(args.ens_sims)
-    hwr, N1, N2, dt = 8, args.ens_N, args.ens_N, 5e-4
     Ps, pos, w = [], [], []
     t0 = time.perf_counter()
     for nx, p, om, _ in data:
-        P = SimParam(dt=dt, resX=nx, hwr=hwr)
-        P.N1, P.N2 = N1, N2
-        h = 1.0 / nx
-        b = (om * 1.0).astype(np.float32)
+        P = SimParam(dt=5e-4, resX=nx, hwr=8)
+        P.N1, P.N2 = args.ens_N, args.ens_N
         # the reference's tolerance (src/cuda/cg_rbf_init.cu:53): tighter fits
         # of these fields amplify the Gram matrix's near-null modes into
         # alternating strengths, which inflate the estimator's variance
-        wt, it, rr, st = ops.cg_rbf(torch.as_tensor(p, device=device), torch.as_tensor(b, device=device),
+        wt, it, rr, st = ops.cg_rbf(torch.as_tensor(p, device=device),
+                                    torch.as_tensor(om.astype(np.float32), device=device),
                                     float(P.h_w), tol=5e-3, max_iter=300, refine=True)
         Ps.append(P)
         pos.append(p)
         w.append(wt.cpu().numpy())
-    t_cg = time.perf_counter() - t0
-    hw = [float(P.h_w) for P in Ps]
-    N = sum(p.shape[0] for p in pos)
-    log("ensemble: %d sims, %d particles, CG init %.1f s" % (len(pos), N, t_cg))
-    cph, spl, th, tune_ms = autotune(pos, w, hw, N1=N1, N2=N2, device=device)
-    log("autotune: cells_per_hw=%d spl=%d eval_threads=%d" % (cph, spl, th))
-    ens = Ensemble(pos, w, hw, N1=N1, N2=N2, dt=dt, seed=0, cells_per_hw=cph, spl=spl,
-                   eval_threads=th, device=device)
-    ens.step(1)
-    v0 = ens.vel.clone()
-    p0 = [torch.as_tensor(p, device=device) for p in pos]
-    # RMS error vs the FP64 blob direct sum on the four smallest simulations
-    small = sorted(range(len(pos)), key=lambda s: pos[s].shape[0])[:4]
+    return Ps, pos, w, time.perf_counter() - t0
+
+
+def ensemble_error(ens, pos, w, hw, probes, device):
+    """RMS |u_MC - u_direct| / RMS |u_direct| of the ensemble's last velocities
+    (estimated at the positions `pos`) over up to `probes` random owners of
+    every simulation, against the FP64 blob direct sum over that simulation"""
+    import torch
+
+    from vortexparticlemc_cuda_amd import ops
+    rng = np.random.default_rng(20261203)
+    v = ens.vel
     num = den = 0.0
-    for s in small:
-        pd = p0[s].double()
-        ud = ops.direct_sum_blob(pd, pd, torch.as_tensor(w[s], device=device).double(), hw[s])
-        um = v0[int(ens.offsets[s]):int(ens.offsets[s + 1])].double()
+    for s in range(len(pos)):
+        n = pos[s].shape[0]
+        idx = np.sort(rng.choice(n, size=min(n, probes), replace=False))
+        pd = torch.as_tensor(pos[s], device=device).double()
+        it = torch.as_tensor(idx, device=device)
+        ud = ops.direct_sum_blob(pd[it].contiguous(), pd,
+                                 torch.as_tensor(w[s], device=device).double(), hw[s])
+        um = v[int(ens.offsets[s]):int(ens.offsets[s + 1])][it].double()
         num += float(((um - ud) ** 2).sum())
         den += float((ud ** 2).sum())
-    err = (num / den) ** 0.5
-    K = args.ens_steps
+    return (num / den) ** 0.5
+
+
+def plain_ensemble_loop(Ps, pos, w, shape, device, seed=0):
+    """one plain Simulation per member (the per-simulation GPU path)"""
+    from vortexparticlemc_cuda_amd.sim import Simulation
+    cph, spl, th = shape
+    sims = []
+    for s in range(len(pos)):
+        sim = Simulation(Ps[s], device=device, cells_per_hw=cph, seed=seed, sim=s, spl=spl,
+                         eval_threads=th)
+        sim.set_particles(pos[s], w[s])
+        sims.append(sim)
+    return sims
+
+
+def time_plain_loop(sims, steps):
+    import torch
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        for sim in sims:
+            sim.step()
+    torch.cuda.synchronize()
+    return time.perf_counter() - t0
+
+
+def ensemble_headline(args, device):
+    """The headline workload: 64 independent periodic simulations, Nx drawn
+    from {64, 96, 128, 192, 256} (seed 20261203), 8 Gaussian vortices each,
+    hwr = 8, N1 = N2 = 64, dt = 5e-4, consistent estimator, RBF strengths.
+    Times exactly `steps` batched steps replayed from one HIP graph after
+    `warmup` eager steps (and one untimed replay that uploads the graph).
+    Beside it: the RMS error of the batched estimate against the FP64 blob
+    direct sum (probe owners of every member), the tuned plain per-simulation
+    path over >= 10 steps and the reference's launch structure (one workgroup
+    per owner and sign) on the same ensemble, each with its own label, the
+    k_ens_eval roofline (HIP events the library records around the kernel),
+    and the same ensemble with kernel deformation."""
+    import torch
+
+    from vortexparticlemc_cuda_amd._capi import ALGO_DIRECT
+    from vortexparticlemc_cuda_amd.ensemble import Ensemble, autotune
+
+    N1 = args.ens_N
+    dt = 5e-4
+    Ps, pos, w, t_cg = ensemble_data(args, device)
+    hw = [float(P.h_w) for P in Ps]
+    N = sum(p.shape[0] for p in pos)
+    sizes = sorted(set(int(P.ResolutionX) for P in Ps))
+    log("ensemble: %d sims (Nx in %s), %d particles, RBF init %.1f s"
+        % (len(pos), sizes, N, t_cg))
+    cph, spl, th, tune_ms = autotune(pos, w, hw, N1=N1, N2=N1, device=device)
+    log("autotune: cells_per_hw=%d spl=%d eval_threads=%d" % (cph, spl, th))
+    ens = Ensemble(pos, w, hw, N1=N1, N2=N1, dt=dt, seed=0, cells_per_hw=cph, spl=spl,
+                   eval_threads=th, device=device)
+    ens.step(1)   # step 0: velocities at the initial positions
+    err = ensemble_error(ens, pos, w, hw, args.ens_probes, device)
+    log("ensemble: RMS error vs FP64 blob direct sum %.4g (%d probe owners per member)"
+        % (err, args.ens_probes))
+    K, W = args.steps, args.warmup
+    ens.step(W)
+    side = torch.cuda.Stream()
+    with torch.cuda.stream(side):
+        ens.capture(K)
+        ens.replay(K)   # the first replay uploads the graph (untimed)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        ens.replay(K)
+        torch.cuda.synchronize()
+        t_graph = time.perf_counter() - t0
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     ens.step(K)
@@ -397,104 +470,110 @@ def ensemble_bench(args, device):
     for pr in ev_pairs:
         ens.step(1, events=pr)
     torch.cuda.synchronize()
-    ens_eval_s = float(np.mean([a.elapsed_time(b) for a, b in ev_pairs])) / 1e3
-    side = torch.cuda.Stream()
-    with torch.cuda.stream(side):
-        ens.capture(K)
-        ens.replay(K)   # first replay uploads the graph
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        ens.replay(K)
-        torch.cuda.synchronize()
-        t_graph = time.perf_counter() - t0
+    eval_s = float(np.mean([a.elapsed_time(b) for a, b in ev_pairs])) / 1e3
     ens.close()
-    batched = N * K / min(t_eager, t_graph)
+    batched = N * K / t_graph
     del ens
     torch.cuda.empty_cache()
+    log("batched: %.3g particle-steps/s (graph %.2f ms/step, eager %.2f ms/step)"
+        % (batched, t_graph / K * 1e3, t_eager / K * 1e3))
     # the same ensemble with kernel deformation (the CUDA step: four aux
-    # targets per owner, F in the fields and the jitter, F from the five
-    # estimates), same launch shape
-    ensd = Ensemble(pos, w, hw, N1=N1, N2=N2, dt=dt, seed=0, cells_per_hw=cph, spl=spl,
+    # targets per owner, F in the fields and the jitter), same launch shape
+    ensd = Ensemble(pos, w, hw, N1=N1, N2=N1, dt=dt, seed=0, cells_per_hw=cph, spl=spl,
                     eval_threads=th, device=device, deform=True)
     ensd.step(1)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    ensd.step(K)
+    ensd.step(args.ens_deform_steps)
     torch.cuda.synchronize()
-    t_def = time.perf_counter() - t0
+    t_def = (time.perf_counter() - t0) / args.ens_deform_steps
     ensd.close()
     del ensd
     torch.cuda.empty_cache()
-    log("batched deformed: %.1f ms/step" % (t_def / K * 1e3))
-    # plain path: one Simulation per member, stepped one after the other
-    sims = []
-    for s in range(len(pos)):
-        sim = Simulation(Ps[s], device=device, cells_per_hw=cph, seed=0, sim=s, spl=spl,
-                         eval_threads=th)
-        sim.set_particles(pos[s], w[s])
-        sims.append(sim)
-    # evaluations within h_w and candidates visited by the ensemble's step:
-    # the counting pass of each member at its first step (same Philox streams
-    # and launch shape as the ensemble)
+    log("batched deformed: %.1f ms/step" % (t_def * 1e3))
+    # evaluations within h_w and candidates visited per step: a counting pass
+    # of each member at step 0 with the ensemble's launch shape (the same
+    # Philox streams and samples as the batched step 0)
+    sims = plain_ensemble_loop(Ps, pos, w, (cph, spl, th), device)
     n_in = n_vis = 0
     for sim in sims:
         sim.eval_count.zero_()
-        sim.estimate(N1, N2, count=True)
+        sim.estimate(N1, N1, count=True)
         c = sim.eval_count.cpu().numpy()
         n_in += int(c[0])
         n_vis += int(c[1])
+    # the tuned plain path: the per-simulation loop with the launch shape
+    # that steps it fastest (one timed step per candidate)
+    plain_tune = {}
+    for shape in ((cph, spl, th),) + tuple(s for s in PLAIN_ENS_TUNE if s != (cph, spl, th)):
+        if shape != (cph, spl, th):
+            del sims
+            torch.cuda.empty_cache()
+            try:
+                sims = plain_ensemble_loop(Ps, pos, w, shape, device)
+            except Exception:
+                sims = None
+                continue
+        for sim in sims:
+            sim.step()
+        plain_tune["%d,%d,%d" % shape] = time_plain_loop(sims, 1) * 1e3
+    best = min(plain_tune, key=plain_tune.get)
+    pshape = tuple(int(v) for v in best.split(","))
+    del sims
+    torch.cuda.empty_cache()
+    sims = plain_ensemble_loop(Ps, pos, w, pshape, device)
     for sim in sims:
         sim.step()
     Kp = max(1, args.ens_plain_steps)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(Kp):
-        for sim in sims:
-            sim.step()
-    torch.cuda.synchronize()
-    t_plain = time.perf_counter() - t0
+    t_plain = time_plain_loop(sims, Kp)
     plain = N * Kp / t_plain
-    # the same per-simulation loop with the reference's launch structure: one
-    # workgroup per (owner, sign) sampling and querying on its own
+    log("plain per-simulation path (shape %s): %.3g particle-steps/s (%.1f ms/step)"
+        % (best, plain, t_plain / Kp * 1e3))
+    # the reference's launch structure per simulation: one workgroup per
+    # (owner, sign) sampling and querying on its own
     # (monte_carlo_bs_kernel_safe, src/cuda/monte_carlo_bs.cu:144-271)
     for sim in sims:
         sim.algo = ALGO_DIRECT
-    for sim in sims:
         sim.step()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for sim in sims:
-        sim.step()
-    torch.cuda.synchronize()
-    t_direct = time.perf_counter() - t0
-    plain_direct = N / t_direct
+    Kd = max(1, args.ens_direct_steps)
+    t_direct = time_plain_loop(sims, Kd)
+    per_launch = N * Kd / t_direct
     del sims
     torch.cuda.empty_cache()
-    log("batched %.3g particle-steps/s (eager %.1f ms/step, graph %.1f ms/step), plain %.3g"
-        % (batched, t_eager / K * 1e3, t_graph / K * 1e3, plain))
-    return {"workload": "64 periodic simulations, Nx from {64,96,128,192,256} (seed 20261203), "
-                        "8 Gaussian vortices each, hwr=8, N1=N2=%d, dt=5e-4, consistent "
-                        "estimator, CG-RBF strengths" % N1,
-            "simulations": len(pos), "particles": N,
+    flop = FLOP_PER_EVAL * n_in
+    return {"workload": "64 periodic simulations on [0,1]^2, Nx from {64,96,128,192,256} "
+                        "(seed 20261203), 8 Gaussian vortices each, hwr=8, N1=N2=%d, dt=5e-4, "
+                        "consistent estimator, FP32 bulk with FP64 CDF/Omega, RBF strengths "
+                        "(tol 5e-3)" % N1,
+            "simulations": len(pos), "particles": N, "lattice_sides": sizes,
+            "samples_per_step": N * 2 * 2 * N1,
             "batched_particle_steps_per_s": batched,
-            "batched_ms_per_step_eager": t_eager / K * 1e3,
             "batched_ms_per_step_graph": t_graph / K * 1e3,
+            "batched_ms_per_step_eager": t_eager / K * 1e3,
+            "steps_timed": K, "warmup": W,
+            "rms_err_vs_direct": err, "error_probe_owners_per_member": args.ens_probes,
             "plain_particle_steps_per_s": plain, "plain_ms_per_step": t_plain / Kp * 1e3,
-            "batched_over_plain": batched / plain, "steps_timed": K, "plain_steps_timed": Kp,
-            "plain_direct_particle_steps_per_s": plain_direct,
-            "plain_direct_ms_per_step": t_direct * 1e3,
-            "batched_over_plain_direct": batched / plain_direct,
-            "rms_err_vs_direct": err, "cells_per_hw": cph, "spl": spl, "eval_threads": th,
-            "autotune_ms": tune_ms,
-            "deformed": {"ms_per_step": t_def / K * 1e3, "particle_steps_per_s": N * K / t_def,
-                         "samples_per_pair": N1 + 5 * N2},
+            "plain_steps_timed": Kp, "plain_launch_shape": best, "plain_tune_ms": plain_tune,
+            "batched_over_plain": batched / plain,
+            "reference_launch_particle_steps_per_s": per_launch,
+            "reference_launch_ms_per_step": t_direct / Kd * 1e3,
+            "batched_over_reference_launch": batched / per_launch,
+            "reference_launch_note": "per-simulation loop with one workgroup per (owner, sign), "
+                                     "the launch structure of monte_carlo_bs_kernel_safe",
+            "target_batched_over_plain": 5.0,
+            "cells_per_hw": cph, "spl": spl, "eval_threads": th, "autotune_ms": tune_ms,
+            "deformed": {"ms_per_step": t_def * 1e3, "particle_steps_per_s": N / t_def,
+                         "samples_per_pair": N1 + 5 * N1},
             "roofline": {"kernel": "k_ens_eval", "bound": "valu",
-                         "achieved": FLOP_PER_EVAL * n_in / ens_eval_s, "peak": PEAK_FP32,
-                         "unit": "FLOP/s", "frac": FLOP_PER_EVAL * n_in / ens_eval_s / PEAK_FP32,
-                         "avg_launch_ms": ens_eval_s * 1e3, "kernel_evals_per_launch": n_in,
+                         "achieved": flop / eval_s, "peak": PEAK_FP32, "unit": "FLOP/s",
+                         "frac": flop / eval_s / PEAK_FP32, "avg_launch_ms": eval_s * 1e3,
+                         "flop_per_launch": flop, "kernel_evals_per_launch": n_in,
                          "candidates_per_launch": n_vis,
-                         "frac_of_unpacked": FLOP_PER_EVAL * n_in / ens_eval_s / (PEAK_FP32 / 2)},
-            "cg_init_s": t_cg}
+                         "frac_of_unpacked": flop / eval_s / (PEAK_FP32 / 2)},
+            "rbf_init_s": t_cg, "_data": (Ps, pos, w)}
+
+
+PLAIN_ENS_TUNE = ((8, 1, 256), (4, 1, 256), (8, 2, 256), (12, 1, 256))
 
 
 def large_bench(args, device):
@@ -600,9 +679,40 @@ def large_bench(args, device):
                         "avg_launch_ms": eval_s * 1e3, "kernel_evals_per_launch": n_in,
                         "candidates_per_launch": n_vis,
                         "samples_per_launch": n * 2 * 2 * N,
-                        "frac_of_unpacked": flop / eval_s / (PEAK_FP32 / 2)}}
+                        "frac_of_unpacked": flop / eval_s / (PEAK_FP32 / 2)},
+           # the whole estimator's algorithmic rate: every kernel of the
+           # estimate (generation, sort, evaluation, reduction) in the time
+           "estimator_frac_of_fp32_peak": flop / est_s / PEAK_FP32}
+    shape = (sim.cells_per_hw, sim.spl, sim.eval_threads)
+    pos_now, w_now = sim.pos.cpu().numpy(), sim.omega.cpu().numpy()
     del sim
     torch.cuda.empty_cache()
+    if args.large_ref_steps > 0:
+        # the reference's step semantics at this size (step_cuda,
+        # src/cuda/simulation_step.cu:51-73: aux targets and F every step):
+        # N1 + 5 N2 samples per owner and sign, 3.2e9 sample slots a step, run
+        # in owner chunks of 2^30 slots
+        simd = Simulation(P, device=device, cells_per_hw=shape[0], seed=0, spl=shape[1],
+                          eval_threads=shape[2], deform=True)
+        simd.set_particles(pos_now, w_now)
+        simd.step(N, N)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.large_ref_steps):
+            simd.step(N, N)
+        torch.cuda.synchronize()
+        t_ref = (time.perf_counter() - t0) / args.large_ref_steps
+        F = simd.jac[:simd.N].double()
+        det = F[:, 0] * F[:, 3] - F[:, 1] * F[:, 2]
+        out["reference_step"] = {
+            "semantics": "aux targets at dst -/+ dx and kernel deformation F every step",
+            "samples_per_step": n * 2 * (N + 5 * N), "chunk_samples": 1 << 30,
+            "ms_per_step": t_ref * 1e3, "particle_steps_per_s": n / t_ref,
+            "steps_timed": args.large_ref_steps,
+            "max_abs_det_minus_1": float((det - 1).abs().max())}
+        log("large: reference step (aux + F) %.1f ms/step" % (t_ref * 1e3))
+        del simd
+        torch.cuda.empty_cache()
     return out
 
 
@@ -750,14 +860,14 @@ def adjoint_bench(args, device):
     return out
 
 
-def read_traffic(kernel, args):
-    """HBM bytes per launch of `kernel` at this workload, from the committed
-    counter summary (FETCH_SIZE + WRITE_SIZE, copy-calibrated); None if absent"""
-    path = os.path.join(ROOT, "profiles", "traffic_%s_tg%d.txt" % (kernel, args.Nx))
+def read_traffic(kernel, workload, want):
+    """HBM bytes per launch of `kernel` on `workload`, from the committed
+    counter summary profiles/traffic_<kernel>_<workload>.txt (FETCH_SIZE +
+    WRITE_SIZE, copy-calibrated) when its workload line carries `want`;
+    None if absent"""
+    path = os.path.join(ROOT, "profiles", "traffic_%s_%s.txt" % (kernel, workload))
     if not os.path.exists(path):
         return None
-    want = "N1=%d N2=%d hwr=%d cells_per_hw=%d spl=%d eval_threads=%d" % (
-        args.N1, args.N2, args.hwr, args.cells_per_hw, args.spl, args.eval_threads)
     ok = False
     for line in open(path):
         if line.startswith("workload:"):
@@ -767,71 +877,234 @@ def read_traffic(kernel, args):
     return None
 
 
-def cpu_baseline(Nx, hwr, N1, N2, target_s=10.0):
-    """the reference's monte_carlo_bs_cpu (its compiled src/cpu sources,
-    oracle/_ref) over a bounded subset of owners of the same workload, aux = 0
-    (the work the GPU path does): single-threaded like its step_cpu, and over
-    the host's cores (OpenMP over owners, one generator per thread)"""
-    import os
+def fixed_error(sim, args, target=1e-2):
+    """RMS error against the FP64 blob direct sum versus N1 = N2 on the plain
+    workload's initial state, the smallest N1 = N2 that the 1/sqrt(N1 + N2)
+    law fitted to those points predicts for `target` (plus 15 %), the error
+    measured there and one timed step there (owners in chunks)"""
+    import torch
 
+    from vortexparticlemc_cuda_amd import ops
+    pos0 = sim.pos.clone()
+    pos64 = pos0.double()
+    ud = ops.direct_sum_blob(pos64, pos64, sim.omega.double(), float(sim.P.h_w))
+    rms = lambda a: float(torch.sqrt((a * a).sum(1).mean()))  # noqa: E731
+    sweep = {}
+    for n in args.error_sweep:
+        sim.pos.copy_(pos0)
+        u = sim.estimate(n, n).double()
+        sweep[n] = rms(u - ud) / rms(ud)
+    ns = np.array(sorted(sweep), np.float64)
+    es = np.array([sweep[int(n)] for n in ns])
+    c = float(np.exp(np.mean(np.log(es) + 0.5 * np.log(2 * ns))))   # err = c / sqrt(N1+N2)
+    n_star = int(np.ceil((c / target) ** 2 / 2 * 1.15))
+    S = sim.N * 2 * 2 * n_star
+    out = {"target": target, "oracle": "FP64 blob direct sum at every particle",
+           "sweep_N1_eq_N2": {str(int(n)): sweep[int(n)] for n in ns},
+           "fit_err_times_sqrt_N1_plus_N2": c, "N1_eq_N2": n_star, "samples_per_step": S}
+    if S > args.fixed_error_max_samples:
+        out["skipped"] = "%.3g samples a step exceed --fixed-error-max-samples" % S
+        return out
+    sim.pos.copy_(pos0)
+    u = sim.estimate(n_star, n_star).double()
+    out["rms_err_vs_direct"] = rms(u - ud) / rms(ud)
+    sim.pos.copy_(pos0)
+    sim.step(n_star, n_star)   # warm-up at this size (workspace, chunks)
+    sim.pos.copy_(pos0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    sim.step(n_star, n_star)
+    torch.cuda.synchronize()
+    t = time.perf_counter() - t0
+    sim.pos.copy_(pos0)
+    sim.ws = None
+    torch.cuda.empty_cache()
+    out.update({"ms_per_step": t * 1e3, "particle_steps_per_s": sim.N / t})
+    log("fixed error: N1=N2=%d, RMS error %.4g, %.1f ms/step" % (n_star, out["rms_err_vs_direct"],
+                                                                t * 1e3))
+    return out
+
+
+def plain_section(args, device):
+    """The plain GPU path at the reference's default size: Taylor-Green,
+    Nx = Ny = 200 (40,000 particles), hwr = 16, N1 = N2 = 128, dt = 1e-3,
+    consistent estimator, Philox seed 0, RBF strengths; tuned launch shape;
+    `plain_steps` timed steps; the k_bin_eval roofline; the HBM-bound stages;
+    the fixed-error measurement"""
+    import torch
+
+    from vortexparticlemc_cuda_amd.sim import Simulation
+    log("plain: setup Nx=%d hwr=%d N1=%d N2=%d" % (args.Nx, args.hwr, args.N1, args.N2))
+    sim, init = setup(args.Nx, args.hwr, args.N1, args.N2, device,
+                      cells_per_hw=args.cells_per_hw, spl=args.spl,
+                      eval_threads=args.eval_threads)
+    sim.algo = 0 if args.algo == "binned" else 1
+    tuned = None
+    if args.algo == "binned" and not args.no_tune:
+        (cph, spl, th), tune_ms = tune_plain(sim)
+        tuned = {"cells_per_hw": cph, "spl": spl, "eval_threads": th, "candidates_ms": tune_ms}
+        log("tuned plain step: cells_per_hw=%d spl=%d eval_threads=%d (%.3f ms)"
+            % (cph, spl, th, tune_ms["%d,%d,%d" % (cph, spl, th)]))
+        s2 = Simulation(sim.P, device=device, cells_per_hw=cph, mode=sim.mode, seed=sim.seed,
+                        spl=spl, eval_threads=th)
+        s2.set_particles(sim.pos.cpu().numpy(), sim.omega.cpu().numpy())
+        sim = s2
+        args.cells_per_hw, args.spl, args.eval_threads = cph, spl, th
+    err = None if args.no_error else velocity_error(sim)
+    fixed = None
+    if not args.no_error and args.error_sweep:
+        fixed = fixed_error(sim, args)
+    run_steps(sim, args.plain_warmup)
+    events = {"est": [], "eval": []}
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    run_steps(sim, args.plain_steps, events)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    est_s = float(np.mean([a.elapsed_time(b) for a, b in events["est"]])) / 1e3
+    eval_s = float(np.mean([a.elapsed_time(b) for a, b in events["eval"]])) / 1e3
+    n_in, n_vis = count_evals(sim)
+    N = sim.N
+    S = N * 2 * (args.N1 + args.N2)
+    kernel = "k_bin_eval" if args.algo == "binned" else "k_mc_estimate"
+    flop = FLOP_PER_EVAL * n_in + (FLOP_PER_SAMPLE * S if args.algo == "direct" else 0)
+    want = "N1=%d N2=%d hwr=%d cells_per_hw=%d spl=%d eval_threads=%d" % (
+        args.N1, args.N2, args.hwr, args.cells_per_hw, args.spl, args.eval_threads)
+    hbm = hbm_stages(sim)
+    log("plain: %.3f ms/step, estimator %.3f ms, evaluation %.3f ms"
+        % (elapsed / args.plain_steps * 1e3, est_s * 1e3, eval_s * 1e3))
+    out = {"workload": "Taylor-Green on periodic [0,1]^2, Nx=Ny=%d lattice (%d particles), "
+                       "hwr=%d, N1=N2=%d, dt=1e-3, consistent estimator, Philox seed 0, RBF "
+                       "strengths" % (args.Nx, N, args.hwr, args.N1),
+           "particles": N, "steps_timed": args.plain_steps,
+           "particle_steps_per_s": N * args.plain_steps / elapsed,
+           "ms_per_step": elapsed / args.plain_steps * 1e3, "estimator_ms": est_s * 1e3,
+           "estimator_fraction_of_step": est_s / (elapsed / args.plain_steps),
+           "tuned": tuned, "error": err, "fixed_error": fixed, "rbf_init": init,
+           "kernel_evals_per_s": n_in / eval_s, "hbm_stages": hbm,
+           "roofline": {"kernel": kernel, "bound": "valu", "achieved": flop / eval_s,
+                        "peak": PEAK_FP32, "unit": "FLOP/s", "frac": flop / eval_s / PEAK_FP32,
+                        "traffic": read_traffic(kernel, "tg%d" % args.Nx, want),
+                        "flop_per_launch": flop, "kernel_evals_per_launch": n_in,
+                        "candidates_per_launch": n_vis, "samples_per_launch": S,
+                        "avg_launch_ms": eval_s * 1e3,
+                        "frac_of_unpacked": flop / eval_s / (PEAK_FP32 / 2)}}
+    return sim, out
+
+
+def _threads():
+    return max(1, min(16, os.cpu_count() or 1))
+
+
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            return next(l.split(":", 1)[1].strip() for l in f if l.startswith("model name"))
+    except (OSError, StopIteration):
+        return "unknown"
+
+
+def cpu_baseline_ensemble(Ps, pos, w, N1, target_s=10.0):
+    """the reference's monte_carlo_bs_cpu (its src/cpu sources compiled into
+    oracle/_ref) over a bounded sample of the ensemble's owners: four members
+    of different sizes, both signs, aux = 0 (the work the GPU step does),
+    OpenMP over owners; particle-steps/s of the sample, which scales to the
+    whole ensemble because every query here visits the same 3 x 3 cells of
+    1152 same-sign candidates (hwr = 8)"""
     import oracle as O
     R = O.ref_lib()
     if R is None:
         return None
+    threads = _threads()
+    picked, seen = [], set()
+    for s, P in enumerate(Ps):
+        if P.ResolutionX not in seen:
+            seen.add(P.ResolutionX)
+            picked.append(s)
+    picked = picked[:4]
+
+    def run(s, m):
+        x = np.ascontiguousarray(pos[s][:, 0], np.float32)
+        y = np.ascontiguousarray(pos[s][:, 1], np.float32)
+        ww = np.ascontiguousarray(w[s], np.float32)
+        rsh = O.rsh_of(Ps[s].ResolutionX, 8)
+        vel = np.zeros(2 * m, np.float32)
+        t0 = time.perf_counter()
+        R.ref_mc_velocities_mt(x.size, x, y, ww, rsh, rsh, 1.0, 1.0, np.float32(Ps[s].h_w), 1,
+                               N1, N1, 5489, 0, m, threads, vel)
+        return time.perf_counter() - t0
+
+    probe = 8 * threads
+    rate = probe / run(picked[0], probe)
+    owners = tsum = 0.0
+    for s in picked:
+        m = int(max(probe, min(pos[s].shape[0], rate * target_s / len(picked))))
+        tsum += run(s, m)
+        owners += m
+    return {"value": owners / tsum, "unit": "particle-steps/s", "cores": threads,
+            "kind": "reference",
+            "sample": "reference monte_carlo_bs_cpu (src/cpu, compiled from source) for %d owners "
+                      "of members %s of the 64-simulation ensemble (Nx %s), both signs, "
+                      "N1=N2=%d, aux=0, OpenMP over owners on %d threads (%s), %.1f s"
+                      % (owners, picked, [Ps[s].ResolutionX for s in picked], N1, threads,
+                         _cpu_model(), tsum)}
+
+
+def cpu_config1_full(steps=10):
+    """the plain CPU-runnable workload in full: Taylor-Green, Nx = Ny = 100
+    (10,000 particles), hwr = 16, N1 = N2 = 64, dt = 1e-3, 10 steps, each the
+    reference's monte_carlo_bs_cpu for every particle and both signs (aux = 0)
+    on all host threads, then x += u dt and the periodic wrap
+    (src/cpu/simulation_step.cpp:46-71); strengths from the lattice's closed
+    form of the RBF solve"""
+    import oracle as O
+    R = O.ref_lib()
+    if R is None:
+        return None
+    threads = _threads()
+    Nx, hwr, N1, dt = 100, 16, 64, np.float32(1e-3)
     x, y = O.lattice(Nx)
     x = x.astype(np.float32)
     y = y.astype(np.float32)
     w = (O.tg_vorticity(x, y) / O.tg_lambda11(Nx, hwr)).astype(np.float32)
     rsh = O.rsh_of(Nx, hwr)
-    hw = np.float32(hwr / Nx)
     n = x.size
-
-    def timed(m, threads):
-        vel = np.zeros(2 * m, np.float32)
-        t0 = time.perf_counter()
-        if threads == 1:
-            R.ref_mc_velocities(n, x, y, w, rsh, rsh, 1.0, 1.0, hw, 1, N1, N2, 0, 5489, 0, m,
-                                vel)
-        else:
-            R.ref_mc_velocities_mt(n, x, y, w, rsh, rsh, 1.0, 1.0, hw, 1, N1, N2, 5489, 0, m,
-                                   threads, vel)
-        return time.perf_counter() - t0
-
-    def run(threads):
-        probe = 8 * threads
-        t1 = timed(probe, threads)
-        m = int(max(probe, min(n, target_s / max(t1 / probe, 1e-9))))
-        t = timed(m, threads)
-        return m, t
-
-    threads = max(1, min(16, os.cpu_count() or 1))
-    m1, t1 = run(1)
-    mt, tt = run(threads)
-    cpu = "unknown"
-    try:
-        with open("/proc/cpuinfo") as f:
-            cpu = next(l.split(":", 1)[1].strip() for l in f if l.startswith("model name"))
-    except (OSError, StopIteration):
-        pass
-    return {"value": mt / tt, "unit": "particle-steps/s", "cores": threads, "kind": "reference",
-            "sample": "reference monte_carlo_bs_cpu (src/cpu, compiled from source) for %d of "
-                      "%d owners, both signs, N1=N2=%d, aux=0, OpenMP over owners on %d threads "
-                      "(%s), %.1f s, scaled per particle" % (mt, n, N1, threads, cpu, tt),
-            "single_thread": {"value": m1 / t1, "cores": 1,
-                              "sample": "%d owners, %.1f s" % (m1, t1)}}
+    vel = np.zeros(2 * n, np.float32)
+    t0 = time.perf_counter()
+    for k in range(steps):
+        R.ref_mc_velocities_mt(n, x, y, w, rsh, rsh, 1.0, 1.0, np.float32(hwr / Nx), 1, N1, N1,
+                               5489 + k, 0, n, threads, vel)
+        x = x + vel[0::2] * dt
+        y = y + vel[1::2] * dt
+        x = (x - np.floor(x)).astype(np.float32)
+        y = (y - np.floor(y)).astype(np.float32)
+    t = time.perf_counter() - t0
+    return {"particle_steps_per_s": n * steps / t, "seconds": t, "steps": steps,
+            "particles": n, "cores": threads, "kind": "reference",
+            "workload": "Taylor-Green, Nx=Ny=100, hwr=16, N1=N2=64, dt=1e-3, 10 steps, aux=0 "
+                        "(the reference's step_cpu runs aux=1, about 3x this work)"}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=100)
-    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=20, help="timed batched steps (headline)")
+    ap.add_argument("--warmup", type=int, default=5, help="untimed batched steps first")
+    ap.add_argument("--ens-sims", type=int, default=64)
+    ap.add_argument("--ens-N", type=int, default=64, help="N1 = N2 of the ensemble")
+    ap.add_argument("--ens-probes", type=int, default=512,
+                    help="probe owners per member for the ensemble's error")
+    ap.add_argument("--ens-plain-steps", type=int, default=10)
+    ap.add_argument("--ens-direct-steps", type=int, default=2)
+    ap.add_argument("--ens-deform-steps", type=int, default=5)
     ap.add_argument("--Nx", type=int, default=200)
     ap.add_argument("--hwr", type=int, default=16)
     ap.add_argument("--N1", type=int, default=128)
     ap.add_argument("--N2", type=int, default=128)
+    ap.add_argument("--plain-steps", type=int, default=20)
+    ap.add_argument("--plain-warmup", type=int, default=5)
     ap.add_argument("--cells-per-hw", type=int, default=8,
-                    help="fine-grid cells per h_w (with --no-tune; else tuned)")
+                    help="fine-grid cells per h_w of the plain path (with --no-tune)")
     ap.add_argument("--spl", type=int, default=0, help="samples per lane (with --no-tune)")
     ap.add_argument("--eval-threads", type=int, default=0,
                     help="evaluation workgroup size (with --no-tune)")
@@ -840,19 +1113,20 @@ def main():
     ap.add_argument("--no-error", action="store_true")
     ap.add_argument("--no-tune", action="store_true",
                     help="skip the launch-shape tuning of the plain step")
+    ap.add_argument("--error-sweep", type=int, nargs="*", default=[32, 128, 512, 2048, 8192],
+                    help="N1 = N2 values of the plain workload's error sweep (none: skip it "
+                         "and the fixed-error step)")
+    ap.add_argument("--fixed-error-max-samples", type=float, default=4e10)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="also run a plain copy of MIB MiB (HBM counter calibration)")
     ap.add_argument("--deform-steps", type=int, default=10,
                     help="steps of the deformed (aux + F) step on the plain workload (0: skip)")
-    ap.add_argument("--ens-sims", type=int, default=64,
-                    help="simulations in the batched-ensemble measurement (0: skip it)")
-    ap.add_argument("--ens-N", type=int, default=64, help="N1 = N2 of the ensemble")
-    ap.add_argument("--ens-steps", type=int, default=10)
-    ap.add_argument("--ens-plain-steps", type=int, default=2)
     ap.add_argument("--large-Nx", type=int, default=1024,
                     help="lattice side of the large double-shear run (0: skip it)")
     ap.add_argument("--large-N", type=int, default=256, help="N1 = N2 of the large run")
     ap.add_argument("--large-steps", type=int, default=10)
+    ap.add_argument("--large-ref-steps", type=int, default=2,
+                    help="timed steps of the reference step semantics (aux + F) at the large size")
     ap.add_argument("--large-probes", type=int, default=4096)
     ap.add_argument("--large-cells-per-hw", type=int, default=12)
     ap.add_argument("--large-spl", type=int, default=1)
@@ -862,7 +1136,7 @@ def main():
     ap.add_argument("--adj-Nx", type=int, default=128,
                     help="lattice side of the adjoint workload (0: skip it)")
     ap.add_argument("--adj-steps", type=int, default=50)
-    ap.add_argument("--adj-fd", type=int, default=32,
+    ap.add_argument("--adj-fd", type=int, default=16,
                     help="coordinates checked by FP64 central differences")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -872,104 +1146,72 @@ def main():
 
     import torch
     device = torch.device("cuda:0")
-    log("setup Nx=%d hwr=%d N1=%d N2=%d" % (args.Nx, args.hwr, args.N1, args.N2))
-    sim, init = setup(args.Nx, args.hwr, args.N1, args.N2, device,
-                      cells_per_hw=args.cells_per_hw, spl=args.spl,
-                      eval_threads=args.eval_threads)
-    sim.algo = 0 if args.algo == "binned" else 1
-    log("cg init", init)
-    tuned = None
-    if args.algo == "binned" and not args.no_tune:
-        (cph, spl, th), tune_ms = tune_plain(sim)
-        tuned = {"cells_per_hw": cph, "spl": spl, "eval_threads": th, "candidates_ms": tune_ms}
-        log("tuned plain step: cells_per_hw=%d spl=%d eval_threads=%d (%.3f ms)"
-            % (cph, spl, th, tune_ms["%d,%d,%d" % (cph, spl, th)]))
-        from vortexparticlemc_cuda_amd.sim import Simulation
-        s2 = Simulation(sim.P, device=device, cells_per_hw=cph, mode=sim.mode, seed=sim.seed,
-                        spl=spl, eval_threads=th)
-        s2.set_particles(sim.pos.cpu().numpy(), sim.omega.cpu().numpy())
-        sim = s2
-        args.cells_per_hw, args.spl, args.eval_threads = cph, spl, th
-    err = None
-    if not args.no_error:
-        err = velocity_error(sim)
-        log("error", err)
-    run_steps(sim, args.warmup)
-    torch.cuda.synchronize()
-    log("warmup done")
-    events = {"est": [], "eval": []}
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    run_steps(sim, args.steps, events)
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
-    est_avg_s = float(np.mean([a.elapsed_time(b) for a, b in events["est"]])) / 1e3
-    eval_avg_s = float(np.mean([a.elapsed_time(b) for a, b in events["eval"]])) / 1e3
-    log("timed %d steps in %.3f s, estimator avg %.3f ms, evaluation kernel avg %.3f ms"
-        % (args.steps, elapsed, est_avg_s * 1e3, eval_avg_s * 1e3))
-    n_in, n_vis = count_evals(sim)
-    N = sim.N
-    S = N * 2 * (args.N1 + args.N2)
-    eval_kernel = "k_bin_eval" if args.algo == "binned" else "k_mc_estimate"
-    # the evaluation kernel's algorithmic work: 15 FLOP per in-support pair
-    # (for the direct kernel, which also samples, 30 FLOP per sample on top)
-    flop = FLOP_PER_EVAL * n_in + (FLOP_PER_SAMPLE * S if args.algo == "direct" else 0)
-    achieved = flop / eval_avg_s
-    traffic = read_traffic(eval_kernel, args)
-    hbm = hbm_stages(sim)
-    log("HBM-bound stages (plain workload)", hbm)
+    head = ensemble_headline(args, device) if args.ens_sims > 0 else None
+    data = head.pop("_data") if head else None
+    sim, plain = plain_section(args, device)
     fma_rate = fma_probe()
     log("measured unpacked FP32 FMA rate %.3e FLOP/s" % fma_rate)
     if args.copy_probe:
         copy_probe(args.copy_probe << 20)
     deform = deform_bench(args, device, sim) if args.deform_steps > 0 else None
-    batched = ensemble_bench(args, device) if args.ens_sims > 0 else None
+    del sim
+    torch.cuda.empty_cache()
     large = large_bench(args, device) if args.large_Nx > 0 else None
     adj = adjoint_bench(args, device) if args.adj_Nx > 0 else None
-    cpu = None if args.no_cpu else cpu_baseline(args.Nx, args.hwr, args.N1, args.N2)
-    value = N * args.steps / elapsed
+    cpu = cpu1 = None
+    if not args.no_cpu:
+        if data:
+            cpu = cpu_baseline_ensemble(*data, args.ens_N)
+        cpu1 = cpu_config1_full()
+    for sec in (plain, head):
+        if sec:
+            sec["roofline"]["measured_unpacked_fma_flops"] = fma_rate
+            sec["roofline"]["frac_of_measured_unpacked"] = sec["roofline"]["achieved"] / fma_rate
+    if head:
+        rf = dict(head["roofline"])
+        rf["traffic"] = read_traffic("k_ens_eval", "ensemble64", "N1=N2=%d cells_per_hw=%d spl=%d "
+                                     "eval_threads=%d" % (args.ens_N, head["cells_per_hw"],
+                                                          head["spl"], head["eval_threads"]))
+        value, K, ms = head["batched_particle_steps_per_s"], args.steps, \
+            head["batched_ms_per_step_graph"]
+        config = {"workload": "batched ensemble: " + head["workload"] + "; one launch per stage "
+                              "for all simulations, step replayed from a HIP graph",
+                  "simulations": head["simulations"], "particles": head["particles"],
+                  "N1": args.ens_N, "N2": args.ens_N, "hwr": 8,
+                  "launch_shape": {"cells_per_hw": head["cells_per_hw"], "spl": head["spl"],
+                                   "eval_threads": head["eval_threads"]},
+                  "parallelism": "single GPU"}
+    else:   # --ens-sims 0: the plain workload carries the line
+        rf = plain["roofline"]
+        value, K, ms = plain["particle_steps_per_s"], args.plain_steps, plain["ms_per_step"]
+        config = {"workload": "plain GPU path: " + plain["workload"], "parallelism": "single GPU"}
     line = {
         "metric": "particle-steps/s at fixed RMS vel. error vs FP64 direct sum; batched vs plain",
         "value": value,
         "unit": "particle-steps/s",
         "n_gpus": 1,
-        "steps": args.steps,
+        "steps": K,
         "warmup": args.warmup,
-        "ms_per_step": elapsed / args.steps * 1e3,
+        "ms_per_step": ms,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32 (FP64 CDF/Omega)",
         "data": "synthetic",
-        "config": {"workload": "plain GPU path: Taylor-Green on periodic [0,1]^2, "
-                               "Nx=Ny=%d lattice (%d particles), hwr=%d, N1=N2=%d, dt=1e-3, "
-                               "consistent estimator, Philox seed 0, CG-RBF strengths"
-                               % (args.Nx, N, args.hwr, args.N1),
-                   "particles": N, "N1": args.N1, "N2": args.N2, "hwr": args.hwr,
-                   "cells_per_hw": args.cells_per_hw, "estimator_algo": args.algo,
-                   "tuned": tuned,
-                   "parallelism": "single GPU"},
-        "roofline": {"kernel": eval_kernel, "bound": "valu", "achieved": achieved,
-                     "peak": PEAK_FP32, "unit": "FLOP/s", "frac": achieved / PEAK_FP32,
-                     "traffic": traffic,
-                     "flop_per_launch": flop, "kernel_evals_per_launch": n_in,
-                     "candidates_per_launch": n_vis, "samples_per_launch": S,
-                     "avg_launch_ms": eval_avg_s * 1e3,
-                     "unpacked_fp32_issue_peak": PEAK_FP32 / 2,
-                     "frac_of_unpacked": achieved / (PEAK_FP32 / 2),
-                     "measured_unpacked_fma_flops": fma_rate,
-                     "frac_of_measured_unpacked": achieved / fma_rate},
-        "kernel_evals_per_s": n_in / eval_avg_s,
-        "hbm_stages": hbm,
-        "estimator_ms": est_avg_s * 1e3,
-        "estimator_fraction_of_step": est_avg_s / (elapsed / args.steps),
-        "error": err,
-        "cg_init": init,
+        "config": config,
+        "rms_err_vs_direct": head["rms_err_vs_direct"] if head else
+        (plain["error"] or {}).get("rms_err_vs_direct"),
+        "batched_over_plain": head["batched_over_plain"] if head else None,
+        "batched_over_reference_launch": head["batched_over_reference_launch"] if head else None,
+        "fixed_error": plain["fixed_error"],
+        "roofline": rf,
+        "batched": head,
+        "plain": plain,
         "deform": deform,
-        "batched": batched,
         "large": large,
         "adjoint": adj,
         "cpu_baseline": cpu,
+        "cpu_config1_full": cpu1,
     }
     print(json.dumps(line), flush=True)
 

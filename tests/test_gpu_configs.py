@@ -64,8 +64,18 @@ def test_config1_strengths_and_reference_replay(device):
     _say("config 1: strengths + reference-mode replay, 10,000 owners")
     Nx, hwr, P, pos, b, w, rr, st, sim = _config1(device, O.MODE_REFERENCE)
     assert st == 0 and rr < 1e-8
+    # the FP64 Gram operator of the oracle (diagonal included,
+    # src/cuda/cg_rbf_init.cu:11-31) at the FP32 positions: the FP32 strengths
+    # leave a residual of their own rounding only (~3e-8 relative)
+    x64, y64 = pos[:, 0].astype(np.float64), pos[:, 1].astype(np.float64)
+    res = O.rbf_matvec(x64, y64, w.astype(np.float64), float(P.h_w)) - b
+    assert np.linalg.norm(res) / np.linalg.norm(b) < 1e-7
+    # the exact lattice's closed form w = b / lambda_11: the FP32 positions
+    # perturb the circulant Gram matrix, and a solve to 1e-8 resolves that
+    # perturbation in its near-null modes (a 1e-6 solve agrees to 2e-7,
+    # test_cg_taylor_green_closed_form)
     w_closed = b / O.tg_lambda11(Nx, hwr)
-    assert rel_l2(w, w_closed) < 2e-7
+    assert rel_l2(w, w_closed) < 1e-5
     N, N1, N2 = pos.shape[0], 64, 64
     rng = np.random.default_rng(11)
     U = rng.random(N * 2 * (N1 * 3 + N2 * 2), dtype=np.float32)
@@ -243,24 +253,38 @@ def test_config4_adjoint_fp64_trajectory_vs_central_differences(device):
     assert abs(L0 - float(loss.detach())) < 1e-6 * abs(L0)
     wn = w0.abs().cpu().numpy()
     pick = rng.choice(np.nonzero(wn > 0.2 * wn.max())[0], size=8, replace=False)
-    eps_x, eps_w = 1e-4, 1e-4 * float(w0.abs().max())
-    ex, ew = [], []
+    eps_w = 1e-4 * float(w0.abs().max())
+
+    def dpos(i, k, e):
+        a, b = p64.clone(), p64.clone()
+        a[i, k] += e
+        b[i, k] -= e
+        return (L(a, w64) - L(b, w64)) / (2 * e)
+
+    # Positions: a branch-1 sample at distance r from another owner puts a
+    # 1/r Biot-Savart term in the frozen-sample velocity, whose central
+    # difference at eps carries a truncation error ~ (eps / r)^2 of itself;
+    # one Richardson step from eps = 1e-4 and 5e-5 cancels that term, and
+    # eps = 1e-6 is compared as well.
+    ex_r, ex_6, ex_raw, ew = [], [], [], []
     for c, i in enumerate(pick):
         if c % 2 == 0:
             k = c // 2 % 2
-            a, b = p64.clone(), p64.clone()
-            a[i, k] += eps_x
-            b[i, k] -= eps_x
-            fd = (L(a, w64) - L(b, w64)) / (2 * eps_x)
-            ex.append(abs(fd - gp[i, k]) / np.abs(gp).max())
+            d4, d45 = dpos(i, k, 1e-4), dpos(i, k, 5e-5)
+            rich = (4.0 * d45 - d4) / 3.0
+            gm = np.abs(gp).max()
+            ex_raw.append(abs(d4 - gp[i, k]) / gm)
+            ex_r.append(abs(rich - gp[i, k]) / gm)
+            ex_6.append(abs(dpos(i, k, 1e-6) - gp[i, k]) / gm)
         else:
             a, b = w64.clone(), w64.clone()
             a[i] += eps_w
             b[i] -= eps_w
             fd = (L(p64, a) - L(p64, b)) / (2 * eps_w)
             ew.append(abs(fd - gw[i]) / np.abs(gw).max())
-    _say("config 4: max |g - fd| / max|g|: positions %.3g, strengths %.3g" % (max(ex), max(ew)))
-    assert max(ex) < 1e-4 and max(ew) < 1e-4
+    _say("config 4: max |g - fd| / max|g|: positions eps=1e-4 %.3g, Richardson(1e-4) %.3g, "
+         "eps=1e-6 %.3g; strengths eps=1e-4 %.3g" % (max(ex_raw), max(ex_r), max(ex_6), max(ew)))
+    assert max(ex_r) < 1e-4 and max(ex_6) < 1e-5 and max(ew) < 1e-4
 
 
 # --------------------------------------------------------------------------- grid field
