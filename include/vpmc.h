@@ -295,6 +295,20 @@ int vpmc_step(vpmc_mc_desc *d, int n_total, float *pos_xy, const float *omega,
               const int *sub_pos, const int *sub_neg, float *vel_xy, float dt,
               void *workspace, size_t workspace_bytes, void *stream);
 
+/* No counterpart in the reference: vpmc_step for callers that cannot
+   allocate (code under HIP graph capture, serving loops that forbid
+   hipMalloc).  It takes no workspace and allocates nothing: its scratch is
+   a static device array of VPMC_NOALLOC_SCRATCH_BYTES per device, held by
+   every GPU the library runs on until the process ends, and the binned
+   estimator's owner chunks are sized to fit it (at most d->chunk_samples
+   slots), which leaves every result bit-identical to vpmc_step's.  Returns
+   VPMC_ERR_WORKSPACE when a field build or one owner's samples need more than
+   the array.  Calls that overlap on one device race on the array. */
+#define VPMC_NOALLOC_SCRATCH_BYTES (256u << 20)
+int vpmc_step_noalloc(vpmc_mc_desc *d, int n_total, float *pos_xy, const float *omega,
+                      const int *sub_pos, const int *sub_neg, float *vel_xy, float dt,
+                      void *stream);
+
 /* ---------------------------------------------------------------------------
  * Ensemble of independent periodic simulations (no reference counterpart: the
  * reference runs one Simulation per process).  All simulations advance in

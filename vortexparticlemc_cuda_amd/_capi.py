@@ -117,6 +117,7 @@ def lib():
     L.vpmc_step_workspace_bytes.restype = csz
     L.vpmc_step_workspace_bytes.argtypes = [C.POINTER(McDesc)]
     L.vpmc_step.argtypes = [C.POINTER(McDesc), ci, vp, vp, vp, vp, vp, cf, vp, csz, vp]
+    L.vpmc_step_noalloc.argtypes = [C.POINTER(McDesc), ci, vp, vp, vp, vp, vp, cf, vp]
     L.vpmc_rbf_workspace_bytes.restype = csz
     L.vpmc_rbf_workspace_bytes.argtypes = [ci, ci, ci]
     L.vpmc_rbf_matvec.argtypes = [C.POINTER(Field), cf, vp, vp, vp]
@@ -172,7 +173,7 @@ EXPORTS = [
     "vpmc_query_view", "vpmc_field_grid", "vpmc_field_workspace_bytes", "vpmc_field_build",
     "vpmc_field_build_jac", "vpmc_field_query", "vpmc_mc_estimate", "vpmc_mc_binned_workspace_bytes",
     "vpmc_mc_estimate_binned", "vpmc_advect", "vpmc_update_jacobian", "vpmc_step_workspace_bytes",
-    "vpmc_step", "vpmc_rbf_workspace_bytes", "vpmc_rbf_matvec", "vpmc_rbf_matvec_f64", "vpmc_rbf_matvec_particles",
+    "vpmc_step", "vpmc_step_noalloc", "vpmc_rbf_workspace_bytes", "vpmc_rbf_matvec", "vpmc_rbf_matvec_f64", "vpmc_rbf_matvec_particles",
     "vpmc_cg_rbf", "vpmc_cg_rbf_f64", "vpmc_cg_workspace_bytes", "vpmc_cg_lattice_workspace_bytes",
     "vpmc_cg_rbf_lattice", "vpmc_reinit_workspace_bytes",
     "vpmc_reinit", "vpmc_direct_sum_eps", "vpmc_direct_sum_blob",
