@@ -40,15 +40,26 @@ struct RsBlk {
     int base, cnt, h0, hstride;
     uint32_t kbase;
 };
-__device__ __forceinline__ RsBlk rs_block(const int4 *__restrict__ blk,
+// Workgroups are dealt round-robin to the 8 XCDs.  Block b of the sort is
+// workgroup (b % per) * 8 + b / per, so that each XCD runs a contiguous range
+// of blocks: block b's digit runs end in the cache lines where block b+1's
+// runs of the same digits begin, and both then fill those lines in one L2.
+__device__ __forceinline__ int rs_logical_block(int nblk)
+{
+    const int per = (nblk + 7) >> 3;
+    return (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
+}
+inline unsigned rs_grid(int nblk) { return 8u * (unsigned)((nblk + 7) >> 3); }
+
+__device__ __forceinline__ RsBlk rs_block(int b, const int4 *__restrict__ blk,
                                           const uint32_t *__restrict__ blk_base, int n, int nblk)
 {
     if (blk) {
-        const int4 v = blk[blockIdx.x];
-        return RsBlk{v.x, v.y, v.z, v.w, blk_base[blockIdx.x]};
+        const int4 v = blk[b];
+        return RsBlk{v.x, v.y, v.z, v.w, blk_base[b]};
     }
-    const int base = (int)blockIdx.x * RS_TILE;
-    return RsBlk{base, min(RS_TILE, n - base), (int)blockIdx.x, nblk, 0u};
+    const int base = b * RS_TILE;
+    return RsBlk{base, min(RS_TILE, n - base), b, nblk, 0u};
 }
 
 __global__ void __launch_bounds__(RS_THREADS)
@@ -56,10 +67,12 @@ __global__ void __launch_bounds__(RS_THREADS)
               const int4 *__restrict__ blk, const uint32_t *__restrict__ blk_base,
               int *__restrict__ hist)
 {
+    const int lb = rs_logical_block(nblk);
+    if (lb >= nblk) return;   // uniform over the workgroup
     __shared__ int h[RS_BINS];
     h[threadIdx.x] = 0;
     __syncthreads();
-    const RsBlk B = rs_block(blk, blk_base, n, nblk);
+    const RsBlk B = rs_block(lb, blk, blk_base, n, nblk);
     for (int r = 0; r < RS_ITEMS; ++r) {
         const int e = r * RS_THREADS + threadIdx.x;
         if (e < B.cnt) atomicAdd(&h[((keys[B.base + e] - B.kbase) >> shift) & (RS_BINS - 1)], 1);
@@ -122,13 +135,15 @@ __global__ void __launch_bounds__(RS_THREADS)
                  const int *__restrict__ offs, uint32_t *__restrict__ keys_out,
                  R *__restrict__ recs_out)
 {
+    const int lb = rs_logical_block(nblk);
+    if (lb >= nblk) return;   // uniform over the workgroup
     __shared__ uint32_t lk[RS_TILE];
     __shared__ uint16_t li[RS_TILE];        // local input index (< RS_TILE)
     __shared__ int wh[RS_WAVES][RS_BINS];   // per-wave digit counts, then offsets
     __shared__ int gofs[RS_BINS];           // global position - local position
     __shared__ int wsum[RS_WAVES];
     const int t = threadIdx.x, lane = t & (WAVE - 1), wid = t / WAVE;
-    const RsBlk B = rs_block(blk, blk_base, n, nblk);
+    const RsBlk B = rs_block(lb, blk, blk_base, n, nblk);
     const int base = B.base, cnt = B.cnt;
     const uint32_t kb = B.kbase;
 #pragma unroll
@@ -221,6 +236,34 @@ __global__ void k_rs_bounds(const uint32_t *__restrict__ keys, int n, int nkeys,
     for (int k = prev + 1; k <= cur; ++k) start[k] = i;
 }
 
+// the same for positions 4t .. 4t+3 (up to n) per thread, keys read as one
+// 16-byte load (keys 16-byte aligned)
+__global__ void k_rs_bounds4(const uint32_t *__restrict__ keys, int n, int nkeys,
+                             int *__restrict__ start)
+{
+    const int i0 = 4 * (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i0 > n) return;
+    int prev = i0 == 0 ? -1 : (int)keys[i0 - 1];
+    int cur[4];
+    if (i0 + 4 <= n) {
+        const uint4 v = *reinterpret_cast<const uint4 *>(keys + i0);
+        cur[0] = (int)v.x;
+        cur[1] = (int)v.y;
+        cur[2] = (int)v.z;
+        cur[3] = (int)v.w;
+    } else {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) cur[j] = i0 + j < n ? (int)keys[i0 + j] : nkeys;
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int i = i0 + j;
+        if (i > n) break;
+        for (int k = prev + 1; k <= cur[j]; ++k) start[k] = i;
+        prev = cur[j];
+    }
+}
+
 static int rs_nblk(int n) { return (n + RS_TILE - 1) / RS_TILE; }
 
 size_t radix_workspace_bytes(int n, size_t rec_bytes)
@@ -263,18 +306,22 @@ static int radix_sort_impl(const uint32_t *keys, int n, int key_bits, const floa
         const bool to_out = ((passes - 1 - p) & 1) == 0;
         uint32_t *kout = to_out ? keys_sorted : kb;
         R *rout = to_out ? rec : rb;
-        hipLaunchKernelGGL(k_rs_hist, dim3(nblk), dim3(RS_THREADS), 0, s, kin, n, 8 * p, nblk,
+        hipLaunchKernelGGL(k_rs_hist, dim3(rs_grid(nblk)), dim3(RS_THREADS), 0, s, kin, n, 8 * p, nblk,
                            blk, blk_base, hist);
         VPMC_CHECK_LAUNCH();
         VPMC_TRY(scan_i32(hist, offs, (int)h, false, nullptr, sws, sb, s));
-        hipLaunchKernelGGL(k_rs_scatter<R>, dim3(nblk), dim3(RS_THREADS), 0, s, kin, rin, q, n,
+        hipLaunchKernelGGL(k_rs_scatter<R>, dim3(rs_grid(nblk)), dim3(RS_THREADS), 0, s, kin, rin, q, n,
                            8 * p, nblk, blk, blk_base, (const int *)offs, kout, rout);
         VPMC_CHECK_LAUNCH();
         kin = kout;
         rin = rout;
     }
-    hipLaunchKernelGGL(k_rs_bounds, dim3((n + 1 + 255) / 256), dim3(256), 0, s,
-                       (const uint32_t *)keys_sorted, n, nkeys, start);
+    if ((reinterpret_cast<uintptr_t>(keys_sorted) & 15) == 0)
+        hipLaunchKernelGGL(k_rs_bounds4, dim3(((n >> 2) + 1 + 255) / 256), dim3(256), 0, s,
+                           (const uint32_t *)keys_sorted, n, nkeys, start);
+    else
+        hipLaunchKernelGGL(k_rs_bounds, dim3((n + 1 + 255) / 256), dim3(256), 0, s,
+                           (const uint32_t *)keys_sorted, n, nkeys, start);
     VPMC_CHECK_LAUNCH();
     return VPMC_OK;
 }
