@@ -186,7 +186,14 @@ def hbm_stages(sim, reps=5):
             ops.field_build(sim.fields[k], sim.pos, sim.omega, sub=sim.sub[key], abs_w=True,
                             ws=ws, jac=sim.jac if sim.deform else None)
 
+    rshx, rshy = sim.P.rsh
+
+    def ref_hash():   # FillCells + CountingSort over every particle on the reference grid
+        num, acc, cell = ops.fill_cells(rshx, rshy, float(sim.P.Lx), float(sim.P.Ly), sim.pos)
+        ops.counting_sort(num, acc, cell)
+
     stages = (("field_build", fields, HBM_BYTES_FIELD),
+              ("reference_grid_hash", ref_hash, HBM_BYTES_FIELD),
               ("subsets_cdf", lambda: ops.build_subsets(sim.omega), HBM_BYTES_CDF),
               ("advect", lambda: ops.advect(pos, vel_sign, 0.0), HBM_BYTES_ADVECT))
     out = {}

@@ -116,15 +116,26 @@ __device__ __forceinline__ int rs_block_excl_scan(int v, int *wsum, int *total)
 // position i is built as (q[i].x, q[i].y, i bits, 0).
 constexpr int RS_WITEMS = RS_TILE / RS_WAVES / WAVE;   // keys per lane
 
-// the first pass's record of input position i from its query point
-template <class R> __device__ __forceinline__ R rs_rec_from(float2 p, uint32_t i);
-template <> __device__ __forceinline__ float4 rs_rec_from<float4>(float2 p, uint32_t i)
+// the first pass's record of input position i: from its query point q[i]
+// (float4, SRec), or the position itself (uint32_t: the hash's local index)
+template <class R>
+__device__ __forceinline__ R rs_rec_from(const float2 *__restrict__ q, uint32_t i);
+template <>
+__device__ __forceinline__ float4 rs_rec_from<float4>(const float2 *__restrict__ q, uint32_t i)
 {
+    const float2 p = q[i];
     return make_float4(p.x, p.y, __uint_as_float(i), 0.f);
 }
-template <> __device__ __forceinline__ SRec rs_rec_from<SRec>(float2 p, uint32_t i)
+template <>
+__device__ __forceinline__ SRec rs_rec_from<SRec>(const float2 *__restrict__ q, uint32_t i)
 {
+    const float2 p = q[i];
     return SRec{p.x, p.y, i};
+}
+template <>
+__device__ __forceinline__ uint32_t rs_rec_from<uint32_t>(const float2 *__restrict__, uint32_t i)
+{
+    return i;
 }
 
 template <class R>
@@ -212,7 +223,7 @@ __global__ void __launch_bounds__(RS_THREADS)
             ko[j] = lk[i];
             const int src = base + li[i];
             po[j] = gofs[((ko[j] - kb) >> shift) & (RS_BINS - 1)] + i;
-            rv[j] = recs_in ? recs_in[src] : rs_rec_from<R>(q[src], (uint32_t)src);
+            rv[j] = recs_in ? recs_in[src] : rs_rec_from<R>(q, (uint32_t)src);
         }
     }
 #pragma unroll
@@ -340,6 +351,14 @@ int radix_sort_samples(const uint32_t *keys, int n, int key_bits, const float2 *
 {
     return radix_sort_impl<SRec>(keys, n, key_bits, q, nullptr, rec, keys_sorted, nkeys, start,
                                  nullptr, nullptr, rs_nblk(n), ws, ws_bytes, s);
+}
+
+int radix_sort_indices(const uint32_t *keys, int n, int key_bits, uint32_t *idx,
+                       uint32_t *keys_sorted, int nkeys, int *start, void *ws, size_t ws_bytes,
+                       hipStream_t s)
+{
+    return radix_sort_impl<uint32_t>(keys, n, key_bits, nullptr, nullptr, idx, keys_sorted, nkeys,
+                                     start, nullptr, nullptr, rs_nblk(n), ws, ws_bytes, s);
 }
 
 int radix_seg_blocks(const std::vector<int> &seg_off, const std::vector<uint32_t> &seg_base,

@@ -81,6 +81,11 @@ int radix_sort_to_records(const uint32_t *keys, int n, int key_bits, const float
 int radix_sort_samples(const uint32_t *keys, int n, int key_bits, const float2 *q, SRec *rec,
                        uint32_t *keys_sorted, int nkeys, int *start, void *ws, size_t ws_bytes,
                        hipStream_t s);
+// the same carrying only the input positions: idx = the stable argsort of keys
+// (workspace radix_workspace_bytes(n, 4))
+int radix_sort_indices(const uint32_t *keys, int n, int key_bits, uint32_t *idx,
+                       uint32_t *keys_sorted, int nkeys, int *start, void *ws, size_t ws_bytes,
+                       hipStream_t s);
 // segmented variant: blocks from radix_seg_blocks (segments = contiguous
 // ranges seg_off[g] .. seg_off[g+1] whose keys lie in [seg_base[g],
 // seg_base[g] + 2^local_bits)); keys globally sorted on return
