@@ -457,16 +457,21 @@ def ensemble_headline(args, device):
         % (err, args.ens_probes))
     K, W = args.steps, args.warmup
     ens.step(W)
+    torch.cuda.synchronize()
+    log("ensemble: %d warmup steps done" % W)
     side = torch.cuda.Stream()
     with torch.cuda.stream(side):
         ens.capture(K)
+        log("ensemble: graph of %d steps captured" % K)
         ens.replay(K)   # the first replay uploads the graph (untimed)
         torch.cuda.synchronize()
+        log("ensemble: first replay done")
         t0 = time.perf_counter()
         ens.replay(K)
         torch.cuda.synchronize()
         t_graph = time.perf_counter() - t0
     torch.cuda.synchronize()
+    log("ensemble: timed replay %.2f ms/step" % (t_graph / K * 1e3))
     t0 = time.perf_counter()
     ens.step(K)
     torch.cuda.synchronize()

@@ -96,6 +96,30 @@ def test_ensemble_graph_replays_match_eager(device, deform):
 
 
 @pytest.mark.gpu
+def test_ensemble_orders_work_across_streams(device):
+    """eager steps queued on the default stream, then a graph replayed on a
+    side stream and positions read back on the default stream, with no
+    synchronisation by the caller: the result equals four eager steps"""
+    from vortexparticlemc_cuda_amd.ensemble import Ensemble
+    _say("ensemble work ordered across streams")
+    data = _ensemble_data(sizes=(20, 28), seed=9)
+    args = ([d[1] for d in data], [d[2] for d in data], [d[0].h_w for d in data])
+    kw = dict(N1=16, N2=16, dt=1e-3, seed=3, device=device)
+    ref = Ensemble(*args, **kw)
+    ref.step(4)
+    want = ref.positions()
+    e = Ensemble(*args, **kw)
+    e.step(2)
+    side = torch.cuda.Stream()
+    with torch.cuda.stream(side):
+        e.capture(2)
+        e.replay(2)
+    got = e.positions()
+    e.close()
+    assert np.array_equal(got, want), rel_l2(got, want)
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("spl,threads", [(2, 256), (1, 128)])
 def test_deformed_ensemble_equals_plain_path(device, spl, threads):
     """with kernel deformation (aux targets, F in fields and jitter, F from

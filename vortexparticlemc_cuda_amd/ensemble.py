@@ -5,6 +5,12 @@ library's ensemble entry points (vpmc_ensemble_*): every stage of a step is
 one launch for all simulations, and a sequence of steps can be captured into
 a HIP graph and replayed.  Simulation s evolves exactly as a plain
 Simulation(sim=s) with the same seed, cells_per_hw and spl would.
+
+Every step, replay and read runs on torch's current stream.  All of them
+share one workspace and the particle state, so work issued on one stream
+waits for what the ensemble issued earlier on another (a graph replayed on a
+side stream after eager steps on the default stream, say): two steps that
+overlapped would sort into the same buffers.
 """
 import ctypes as C
 
@@ -65,14 +71,28 @@ class Ensemble:
         self.ws = ops.workspace(need, self.device)
         self.step_index = 0
         self._graphs = {}
+        self._stream = None
+        self._order()
         check(lib().vpmc_ensemble_init(C.byref(d), ops._p(self.ws), self.ws.numel(),
                                        ops._stream()), "vpmc_ensemble_init")
+
+    def _order(self):
+        """orders the current stream after the ensemble's work on another one"""
+        cur = torch.cuda.current_stream(self.device)
+        if self._stream is not None and self._stream != cur:
+            cur.wait_stream(self._stream)
+            # the caching allocator must not hand these out while cur uses them
+            for t in (self.ws, self.pos, self.omega, self.vel, self.jac):
+                if t is not None:
+                    t.record_stream(cur)
+        self._stream = cur
 
     @property
     def workspace_bytes(self):
         return self.ws.numel()
 
     def set_step(self, step):
+        self._order()
         check(lib().vpmc_ensemble_set_step(C.byref(self.desc), ops._p(self.ws), self.ws.numel(),
                                            int(step), ops._stream()), "vpmc_ensemble_set_step")
         self.step_index = int(step)
@@ -80,6 +100,7 @@ class Ensemble:
     def step(self, k=1, events=None):
         """k eager steps; events: optional (start, stop) torch.cuda.Event pair
         recorded by the library around the evaluation kernel of the last step"""
+        self._order()
         for i in range(k):
             if events is not None and i == k - 1:
                 self.desc.eval_events[0] = events[0].cuda_event
@@ -107,6 +128,7 @@ class Ensemble:
     def replay(self, steps):
         """advance by `steps` steps through the captured graph"""
         h = self.capture(steps)
+        self._order()
         check(lib().vpmc_graph_launch(h, ops._stream()), "vpmc_graph_launch")
         self.step_index += steps
 
@@ -122,6 +144,7 @@ class Ensemble:
             pass
 
     def positions(self, s=None):
+        self._order()
         p = self.pos.cpu().numpy()
         if s is None:
             return p
@@ -129,10 +152,12 @@ class Ensemble:
 
     def jacobians(self, s=None):
         """F per particle (N, 4), or of simulation s"""
+        self._order()
         F = self.jac[:self.N].cpu().numpy()
         return F if s is None else F[self.offsets[s]:self.offsets[s + 1]]
 
     def velocities(self, s=None):
+        self._order()
         v = self.vel.cpu().numpy()
         if s is None:
             return v
