@@ -238,8 +238,8 @@ __global__ void __launch_bounds__(BIN_THREADS)
 {
     const size_t sid = (size_t)blockIdx.x * BIN_THREADS + threadIdx.x;
     if (sid >= g.S) return;
-    const uint32_t pair = (uint32_t)(sid / g.NS);
-    const int k = (int)(sid - (size_t)pair * g.NS);
+    const uint32_t pair = div_ns(g, (uint32_t)sid);   // sid < 2^31 (bin_geom)
+    const int k = (int)((uint32_t)sid - pair * g.NS);
     const int owner = (int)(pair >> 1), si = (int)(pair & 1);
     const SetD &S = d.set[si];
     const int n_s = S.count_dev ? *S.count_dev : S.n;
@@ -654,8 +654,8 @@ __global__ void __launch_bounds__(ADJ_THREADS) k_f64_sample(McD d, BinGeom g, F6
 {
     const size_t sid = (size_t)blockIdx.x * ADJ_THREADS + threadIdx.x;
     if (sid >= g.S) return;
-    const uint32_t pair = (uint32_t)(sid / g.NS);
-    const int k = (int)(sid - (size_t)pair * g.NS);
+    const uint32_t pair = div_ns(g, (uint32_t)sid);   // sid < 2^31 (bin_geom)
+    const int k = (int)((uint32_t)sid - pair * g.NS);
     const int owner = (int)(pair >> 1), si = (int)(pair & 1);
     const SetD &S = d.set[si];
     const int n_s = S.count_dev ? *S.count_dev : S.n;

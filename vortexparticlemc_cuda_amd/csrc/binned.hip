@@ -161,6 +161,7 @@ int bin_geom(const McD &m, int N1, int N2, int n_owner, BinGeom *g)
     g->NS = (uint32_t)samples_per_pair(N1, N2, m.aux);
     g->S = (size_t)n_owner * 2 * g->NS;
     if (g->S >= (size_t)1 << 31) return VPMC_ERR_ARG;   // int32 sample slots
+    set_ns_division(g);
     return VPMC_OK;
 }
 

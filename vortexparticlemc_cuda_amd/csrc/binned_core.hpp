@@ -19,7 +19,33 @@ struct BinGeom {
     int ext[MAX_COL_EXT];     // rows each side for column offset ox (index ox+cols), -1: none
     uint32_t NS;              // N1 + N2
     size_t S;                 // samples
+    // sample id -> (owner, sign) pair without a 64-bit division: for ids
+    // below 2^31, id / NS = umulhi(id, ns_mul) >> (ns_l - 1), ns_mul =
+    // ceil(2^(31 + ns_l) / NS), ns_l = ceil(log2 NS) (ns_mul = 0: NS = 1)
+    uint32_t ns_mul;
+    int ns_l;
 };
+
+// id / NS for 0 <= id < 2^31 (exact: the rounding error of ns_mul times id
+// stays below 2^(31 + ns_l) / NS)
+__device__ __forceinline__ uint32_t div_ns(const BinGeom &g, uint32_t id)
+{
+    return g.ns_mul == 0u ? id : (__umulhi(id, g.ns_mul) >> (g.ns_l - 1));
+}
+
+inline void set_ns_division(BinGeom *g)
+{
+    const uint32_t d = g->NS;
+    if (d <= 1u) {
+        g->ns_mul = 0u;
+        g->ns_l = 0;
+        return;
+    }
+    int l = 0;
+    while ((1ull << l) < d) ++l;   // ceil(log2 d)
+    g->ns_l = l;
+    g->ns_mul = (uint32_t)((((unsigned __int128)1 << (31 + l)) + d - 1) / d);
+}
 
 // grid of the tiles and stencil for descriptor geometry (host)
 int bin_geom(const McD &m, int N1, int N2, int n_owner, BinGeom *g);
@@ -33,8 +59,8 @@ __device__ __forceinline__ void bin_gen_one(const McD &d, const BinGeom &g, size
                                             uint32_t *__restrict__ key, float2 *__restrict__ q,
                                             float2 *__restrict__ r, float *__restrict__ om)
 {
-    const uint32_t pair = (uint32_t)(sid / g.NS);
-    const int k = (int)(sid - (size_t)pair * g.NS);
+    const uint32_t pair = div_ns(g, (uint32_t)sid);   // sid < 2^31 (bin_geom)
+    const int k = (int)((uint32_t)sid - pair * g.NS);
     const int owner = (int)(pair >> 1), si = (int)(pair & 1);
     const SetD &S = d.set[si];
     const int n_s = S.count_dev ? *S.count_dev : S.n;
