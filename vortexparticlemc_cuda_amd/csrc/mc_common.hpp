@@ -66,33 +66,37 @@ int make_mcd(const vpmc_mc_desc *in, McD *d);
 // outer piece, in t = 1-q, t^4/2 - 0.4 t^5 = 7(1-u)/80 on [0, 1/2] from
 // (2b)^(1/4).  Both functions are increasing there, and the dropped
 // higher-order terms are negative, so the start lies at or right of the
-// root.  Agrees with the oracle's FP64 bracketed Newton to 2.3e-7 relative.
+// root.  Both pieces are Newton on c2 y^2 + c4 y^4 + c5 y^5 = rhs with the
+// piece's coefficients chosen per lane, so a wave whose draws fall in both
+// pieces runs one sequence of steps instead of both.  Agrees with the
+// oracle's FP64 bracketed Newton to 2.3e-7 relative.
 __device__ __forceinline__ float mass_inverse(float u)
 {
-    if (!(u > 0.f)) return 0.f;
-    if (u >= 1.f) return 1.f;
-    if (u <= 11.f / 14.f) {
-        const float a = (7.f / 80.f) * u;
-        float x = fminf(__builtin_amdgcn_sqrtf(2.f * a), 0.5f);
-#pragma unroll
-        for (int it = 0; it < 4; ++it) {
-            float x2 = x * x;
-            float g = fmaf(x2, fmaf(x2, fmaf(1.2f, x, -1.5f), 0.5f), -a);
-            float d = x * fmaf(x2, fmaf(6.f, x, -6.f), 1.f);
-            if (d > 0.f) x = fminf(fmaxf(x - g * __builtin_amdgcn_rcpf(d), 0.f), 0.5f);
-        }
-        return x;
-    }
-    const float b = (7.f / 80.f) * (1.f - u);
-    float t = fminf(__builtin_amdgcn_sqrtf(__builtin_amdgcn_sqrtf(2.f * b)), 0.5f);
+    const bool inner = u <= 11.f / 14.f;
+    const float rhs = (7.f / 80.f) * (inner ? u : 1.f - u);
+    const float c2 = inner ? 0.5f : 0.f, c4 = inner ? -1.5f : 0.5f, c5 = inner ? 1.2f : -0.4f;
+    const float e2 = inner ? 1.f : 0.f, e4 = inner ? -6.f : 2.f, e5 = inner ? 6.f : -2.f;
+    const float s = __builtin_amdgcn_sqrtf(2.f * rhs);
+    float y = fminf(inner ? s : __builtin_amdgcn_sqrtf(s), 0.5f);
 #pragma unroll
     for (int it = 0; it < 4; ++it) {
-        float t2 = t * t;
-        float h = fmaf(t2 * t2, fmaf(-0.4f, t, 0.5f), -b);
-        float d = 2.f * t2 * t * (1.f - t);
-        if (d > 0.f) t = fminf(fmaxf(t - h * __builtin_amdgcn_rcpf(d), 0.f), 0.5f);
+        const float y2 = y * y;
+        const float g = fmaf(y2, fmaf(y2, fmaf(c5, y, c4), c2), -rhs);
+        const float d = y * fmaf(y2, fmaf(e5, y, e4), e2);   // dg/dy
+        const float yn = fminf(fmaxf(y - g * __builtin_amdgcn_rcpf(d), 0.f), 0.5f);
+        y = d > 0.f ? yn : y;
     }
-    return 1.f - t;
+    const float x = inner ? y : 1.f - y;
+    return u > 0.f ? (u >= 1.f ? 1.f : x) : 0.f;
+}
+
+// sin and cos of 2 pi u for a draw u in [0, 1]: v_sin_f32 and v_cos_f32 take
+// their argument in revolutions (max abs error 2e-7 over every 24-bit draw
+// against FP64, as sincosf(2 pi u))
+__device__ __forceinline__ void sincos_rev(float u, float &s, float &c)
+{
+    s = __builtin_amdgcn_sinf(u);
+    c = __builtin_amdgcn_cosf(u);
 }
 
 __device__ __forceinline__ void draws(const McD &d, int owner, int si, int branch, int k,
@@ -202,7 +206,7 @@ __device__ __forceinline__ bool gen_sample(const McD &d, const SetD &S, int n_s,
             float2 xj = S.set_xy[j];
             float rho = mass_inverse(u[1]) * d.h_w;
             float s, c;
-            sincosf(2.f * PI_F * u[2], &s, &c);
+            sincos_rev(u[2], s, c);
             float jx = rho * c, jy = rho * s;
             if (S.set_jac) {   // the deformed blob: F_j u (src/cuda/monte_carlo_bs.cu:210-212)
                 const float4 F = S.set_jac[j];
@@ -222,7 +226,7 @@ __device__ __forceinline__ bool gen_sample(const McD &d, const SetD &S, int n_s,
             if (src) *src = -1;
             float rho = d.Rc * u[1];
             float s, c;
-            sincosf(2.f * PI_F * u[0], &s, &c);
+            sincos_rev(u[0], s, c);
             float cx = rho * c, cy = rho * s;
             spx = dst.x + cx;
             spy = dst.y + cy;
@@ -233,8 +237,13 @@ __device__ __forceinline__ bool gen_sample(const McD &d, const SetD &S, int n_s,
     }
     if (!(isfinite(spx) && isfinite(spy))) return false;
     if (PER) {
-        qx = spx - floorf(spx / d.Lx) * d.Lx;
-        qy = spy - floorf(spy / d.Ly) * d.Ly;
+        if (d.Lx == 1.f && d.Ly == 1.f) {   // the unit box: no division
+            qx = spx - floorf(spx);
+            qy = spy - floorf(spy);
+        } else {
+            qx = spx - floorf(spx / d.Lx) * d.Lx;
+            qy = spy - floorf(spy / d.Ly) * d.Ly;
+        }
     } else {
         if (spx < 0.f || spx > d.Lx || spy < 0.f || spy > d.Ly) return false;
         qx = spx;

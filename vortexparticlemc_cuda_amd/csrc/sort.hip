@@ -3,14 +3,19 @@
 // Random-key binning with one global atomic per element is bound by the
 // device's atomic throughput (about 2e10/s on MI355X, 1 ms for 2e7 keys).
 // Here every pass over an 8-bit digit is:
-//   hist     one block per RS_TILE contiguous keys: digit histogram in LDS,
+//   hist     one block per RS_SUPER contiguous keys: digit histogram in LDS,
 //            written digit-major (hist[d * nblk + b]);
 //   scan     exclusive prefix of hist: the output offset of (digit, block);
-//   scatter  the block re-reads its keys in order; the rank of a key among
-//            equal digits of its block comes from wave ballots (eight
-//            ballots give the lanes sharing a digit) and per-wave counts in
-//            LDS, so the pass is stable and the result is the unique stable
-//            order.
+//   scatter  the block re-reads its keys in order, RS_TILE keys per round;
+//            the rank of a key among equal digits of its round comes from
+//            wave ballots (eight ballots give the lanes sharing a digit) and
+//            per-wave counts in LDS, and a running offset per digit carries
+//            the block's position from round to round, so the pass is stable
+//            and the result is the unique stable order.
+// A block covers RS_ROUNDS rounds so that the digit-major histogram, whose
+// 256 entries per block are written and read at a stride of nblk (one cache
+// line each), holds a quarter of the entries that one block per round would
+// need; the LDS of the scatter stays that of one round.
 // The sample records (12 bytes: point and id) move with the keys in every
 // pass; the fields' 16-byte records use the same passes.
 //
@@ -29,7 +34,9 @@ namespace vpmc {
 
 constexpr int RS_THREADS = 256;
 constexpr int RS_ITEMS = 16;
-constexpr int RS_TILE = RS_THREADS * RS_ITEMS;   // keys per block
+constexpr int RS_TILE = RS_THREADS * RS_ITEMS;   // keys per round
+constexpr int RS_ROUNDS = 4;                     // rounds per block
+constexpr int RS_SUPER = RS_TILE * RS_ROUNDS;    // keys per block
 constexpr int RS_BINS = 256;                     // 8-bit digits
 constexpr int RS_WAVES = RS_THREADS / WAVE;
 static_assert(RS_THREADS == RS_BINS, "one thread per digit");
@@ -58,8 +65,18 @@ __device__ __forceinline__ RsBlk rs_block(int b, const int4 *__restrict__ blk,
         const int4 v = blk[b];
         return RsBlk{v.x, v.y, v.z, v.w, blk_base[b]};
     }
-    const int base = b * RS_TILE;
-    return RsBlk{base, min(RS_TILE, n - base), b, nblk, 0u};
+    const int base = b * RS_SUPER;
+    return RsBlk{base, min(RS_SUPER, n - base), b, nblk, 0u};
+}
+
+// LDS copies of the digit histogram: lane l counts into copy l % RS_HCOPY,
+// so that lanes of one wave whose keys share a digit (frequent: samples
+// cluster where the vorticity is) rarely add to the same LDS word
+constexpr int RS_HCOPY = 16;
+
+__device__ __forceinline__ void rs_hist_add(int *h, uint32_t key, uint32_t kbase, int shift, int c)
+{
+    atomicAdd(&h[(((key - kbase) >> shift) & (RS_BINS - 1)) * RS_HCOPY + c], 1);
 }
 
 __global__ void __launch_bounds__(RS_THREADS)
@@ -69,16 +86,42 @@ __global__ void __launch_bounds__(RS_THREADS)
 {
     const int lb = rs_logical_block(nblk);
     if (lb >= nblk) return;   // uniform over the workgroup
-    __shared__ int h[RS_BINS];
-    h[threadIdx.x] = 0;
+    __shared__ __align__(16) int h[RS_BINS * RS_HCOPY];
+    for (int i = threadIdx.x; i < RS_BINS * RS_HCOPY; i += RS_THREADS) h[i] = 0;
     __syncthreads();
     const RsBlk B = rs_block(lb, blk, blk_base, n, nblk);
-    for (int r = 0; r < RS_ITEMS; ++r) {
-        const int e = r * RS_THREADS + threadIdx.x;
-        if (e < B.cnt) atomicAdd(&h[((keys[B.base + e] - B.kbase) >> shift) & (RS_BINS - 1)], 1);
+    const uint32_t *kp = keys + B.base;
+    const int c = (int)threadIdx.x & (RS_HCOPY - 1);
+    if (((reinterpret_cast<uintptr_t>(kp)) & 15) == 0) {
+        const uint4 *k4 = reinterpret_cast<const uint4 *>(kp);
+        const int n4 = B.cnt >> 2;
+        for (int r = 0; r < RS_SUPER / 4 / RS_THREADS; ++r) {
+            const int e = r * RS_THREADS + threadIdx.x;
+            if (e < n4) {
+                const uint4 v = k4[e];
+                rs_hist_add(h, v.x, B.kbase, shift, c);
+                rs_hist_add(h, v.y, B.kbase, shift, c);
+                rs_hist_add(h, v.z, B.kbase, shift, c);
+                rs_hist_add(h, v.w, B.kbase, shift, c);
+            }
+        }
+        const int e = 4 * n4 + (int)threadIdx.x;
+        if (threadIdx.x < 4 && e < B.cnt) rs_hist_add(h, kp[e], B.kbase, shift, c);
+    } else {
+        for (int r = 0; r < RS_SUPER / RS_THREADS; ++r) {
+            const int e = r * RS_THREADS + threadIdx.x;
+            if (e < B.cnt) rs_hist_add(h, kp[e], B.kbase, shift, c);
+        }
     }
     __syncthreads();
-    hist[(size_t)B.h0 + (size_t)threadIdx.x * B.hstride] = h[threadIdx.x];
+    const int4 *row = reinterpret_cast<const int4 *>(h + threadIdx.x * RS_HCOPY);
+    int sum = 0;
+#pragma unroll
+    for (int k = 0; k < RS_HCOPY / 4; ++k) {
+        const int4 v = row[k];
+        sum += v.x + v.y + v.z + v.w;
+    }
+    hist[(size_t)B.h0 + (size_t)threadIdx.x * B.hstride] = sum;
 }
 
 // exclusive prefix over the block of one value per thread; *total = sum
@@ -103,14 +146,16 @@ __device__ __forceinline__ int rs_block_excl_scan(int v, int *wsum, int *total)
     return base + x - v;
 }
 
-// One block = RS_TILE consecutive input positions; wave w owns the
-// contiguous quarter [w * RS_TILE/4, (w+1) * RS_TILE/4) and ranks its keys in
-// element order against a wave-private digit histogram in LDS (no barrier
-// inside the wave's loop: one wave's LDS operations complete in order).  One
-// barrier later, per-digit offsets across waves give every key its stable
-// position inside the block; keys and local input indices are placed in LDS
-// in digit order and written out in that order, so that a digit's elements
-// of this block go to one contiguous range starting at offs[d * nblk + b].
+// One round = RS_TILE consecutive input positions of the block's range; wave
+// w owns the contiguous quarter [w * RS_TILE/4, (w+1) * RS_TILE/4) and ranks
+// its keys in element order against a wave-private digit histogram in LDS
+// (no barrier inside the wave's loop: one wave's LDS operations complete in
+// order).  One barrier later, per-digit offsets across waves give every key
+// its stable position inside the round; keys and local input indices are
+// placed in LDS in digit order and written out in that order, so that a
+// digit's elements of this round go to one contiguous range, which starts at
+// offs[d * nblk + b] in the first round and right after the previous round's
+// range of digit d in the next.
 // The 16-byte records travel with the keys: each is read from the block's
 // own input slice.  recs_in == NULL (first pass): the record of input
 // position i is built as (q[i].x, q[i].y, i bits, 0).
@@ -151,87 +196,100 @@ __global__ void __launch_bounds__(RS_THREADS)
     __shared__ uint32_t lk[RS_TILE];
     __shared__ uint16_t li[RS_TILE];        // local input index (< RS_TILE)
     __shared__ int wh[RS_WAVES][RS_BINS];   // per-wave digit counts, then offsets
-    __shared__ int gofs[RS_BINS];           // global position - local position
+    __shared__ int gofs[RS_BINS];           // global position - local position (this round)
     __shared__ int wsum[RS_WAVES];
     const int t = threadIdx.x, lane = t & (WAVE - 1), wid = t / WAVE;
     const RsBlk B = rs_block(lb, blk, blk_base, n, nblk);
-    const int base = B.base, cnt = B.cnt;
     const uint32_t kb = B.kbase;
-#pragma unroll
-    for (int w = 0; w < RS_WAVES; ++w) wh[w][t] = 0;
-    __syncthreads();
+    // running output position of digit t for this block's next round
+    int run = offs[(size_t)B.h0 + (size_t)t * B.hstride];
     const unsigned long long lt = (1ull << lane) - 1ull;
-    uint32_t kr[RS_WITEMS];
-    int rk[RS_WITEMS];
+    for (int round = 0; round < RS_ROUNDS; ++round) {
+        const int base = B.base + round * RS_TILE;
+        const int cnt = min(RS_TILE, B.cnt - round * RS_TILE);
+        if (cnt <= 0) break;   // uniform over the workgroup
 #pragma unroll
-    for (int r = 0; r < RS_WITEMS; ++r) {
-        const int e = (wid * RS_WITEMS + r) * WAVE + lane;
-        const bool valid = e < cnt;
-        kr[r] = valid ? keys_in[base + e] : kb;
-        const int d = (int)(((kr[r] - kb) >> shift) & (RS_BINS - 1));
-        unsigned long long peers = __ballot(valid);
+        for (int w = 0; w < RS_WAVES; ++w) wh[w][t] = 0;
+        __syncthreads();
+        uint32_t kr[RS_WITEMS];
+        int rk[RS_WITEMS];
 #pragma unroll
-        for (int b = 0; b < 8; ++b) {
-            const bool bit = (d >> b) & 1;
-            const unsigned long long m = __ballot(bit);
-            peers &= bit ? m : ~m;
+        for (int r = 0; r < RS_WITEMS; ++r) {
+            const int e = (wid * RS_WITEMS + r) * WAVE + lane;
+            const bool valid = e < cnt;
+            kr[r] = valid ? keys_in[base + e] : kb;
         }
-        const int before = __popcll(peers & lt);
-        const int prior = valid ? wh[wid][d] : 0;
-        __builtin_amdgcn_wave_barrier();   // every lane's read precedes the leader's write
-        rk[r] = prior + before;
-        if (valid && before == 0) wh[wid][d] = prior + __popcll(peers);
-        __builtin_amdgcn_wave_barrier();
-    }
-    __syncthreads();
-    // digit t: exclusive offsets of the waves inside the block
-    int c[RS_WAVES], sum = 0;
 #pragma unroll
-    for (int w = 0; w < RS_WAVES; ++w) {
-        c[w] = wh[w][t];
-        sum += c[w];
-    }
-    int tot;
-    const int ls = rs_block_excl_scan(sum, wsum, &tot);
-    gofs[t] = offs[(size_t)B.h0 + (size_t)t * B.hstride] - ls;
-    int acc = ls;
+        for (int r = 0; r < RS_WITEMS; ++r) {
+            const int e = (wid * RS_WITEMS + r) * WAVE + lane;
+            const bool valid = e < cnt;
+            const int d = (int)(((kr[r] - kb) >> shift) & (RS_BINS - 1));
+            unsigned long long peers = __ballot(valid);
 #pragma unroll
-    for (int w = 0; w < RS_WAVES; ++w) {
-        wh[w][t] = acc;   // column t: read above by this thread only
-        acc += c[w];
-    }
-    __syncthreads();
-#pragma unroll
-    for (int r = 0; r < RS_WITEMS; ++r) {
-        const int e = (wid * RS_WITEMS + r) * WAVE + lane;
-        if (e < cnt) {
-            const int pos = wh[wid][((kr[r] - kb) >> shift) & (RS_BINS - 1)] + rk[r];
-            lk[pos] = kr[r];
-            li[pos] = (uint16_t)e;
+            for (int b = 0; b < 8; ++b) {
+                const bool bit = (d >> b) & 1;
+                const unsigned long long m = __ballot(bit);
+                peers &= bit ? m : ~m;
+            }
+            const int before = __popcll(peers & lt);
+            const int prior = valid ? wh[wid][d] : 0;
+            __builtin_amdgcn_wave_barrier();   // every lane's read precedes the leader's write
+            rk[r] = prior + before;
+            if (valid && before == 0) wh[wid][d] = prior + __popcll(peers);
+            __builtin_amdgcn_wave_barrier();
         }
-    }
-    __syncthreads();
-    // all RS_ITEMS record loads in flight before the stores
-    uint32_t ko[RS_ITEMS];
-    int po[RS_ITEMS];
-    R rv[RS_ITEMS];
+        __syncthreads();
+        // digit t: exclusive offsets of the waves inside the round
+        int c[RS_WAVES], sum = 0;
 #pragma unroll
-    for (int j = 0; j < RS_ITEMS; ++j) {
-        const int i = j * RS_THREADS + t;
-        po[j] = -1;
-        if (i < cnt) {
-            ko[j] = lk[i];
-            const int src = base + li[i];
-            po[j] = gofs[((ko[j] - kb) >> shift) & (RS_BINS - 1)] + i;
-            rv[j] = recs_in ? recs_in[src] : rs_rec_from<R>(q, (uint32_t)src);
+        for (int w = 0; w < RS_WAVES; ++w) {
+            c[w] = wh[w][t];
+            sum += c[w];
         }
-    }
+        int tot;
+        const int ls = rs_block_excl_scan(sum, wsum, &tot);
+        gofs[t] = run - ls;
+        run += sum;
+        int acc = ls;
 #pragma unroll
-    for (int j = 0; j < RS_ITEMS; ++j) {
-        if (po[j] >= 0) {
-            keys_out[po[j]] = ko[j];
-            recs_out[po[j]] = rv[j];
+        for (int w = 0; w < RS_WAVES; ++w) {
+            wh[w][t] = acc;   // column t: read above by this thread only
+            acc += c[w];
         }
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < RS_WITEMS; ++r) {
+            const int e = (wid * RS_WITEMS + r) * WAVE + lane;
+            if (e < cnt) {
+                const int pos = wh[wid][((kr[r] - kb) >> shift) & (RS_BINS - 1)] + rk[r];
+                lk[pos] = kr[r];
+                li[pos] = (uint16_t)e;
+            }
+        }
+        __syncthreads();
+        // all RS_ITEMS record loads in flight before the stores
+        uint32_t ko[RS_ITEMS];
+        int po[RS_ITEMS];
+        R rv[RS_ITEMS];
+#pragma unroll
+        for (int j = 0; j < RS_ITEMS; ++j) {
+            const int i = j * RS_THREADS + t;
+            po[j] = -1;
+            if (i < cnt) {
+                ko[j] = lk[i];
+                const int src = base + li[i];
+                po[j] = gofs[((ko[j] - kb) >> shift) & (RS_BINS - 1)] + i;
+                rv[j] = recs_in ? recs_in[src] : rs_rec_from<R>(q, (uint32_t)src);
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < RS_ITEMS; ++j) {
+            if (po[j] >= 0) {
+                keys_out[po[j]] = ko[j];
+                recs_out[po[j]] = rv[j];
+            }
+        }
+        __syncthreads();   // lk, li, gofs and wh are rewritten by the next round
     }
 }
 
@@ -275,7 +333,7 @@ __global__ void k_rs_bounds4(const uint32_t *__restrict__ keys, int n, int nkeys
     }
 }
 
-static int rs_nblk(int n) { return (n + RS_TILE - 1) / RS_TILE; }
+static int rs_nblk(int n) { return (n + RS_SUPER - 1) / RS_SUPER; }
 
 size_t radix_workspace_bytes(int n, size_t rec_bytes)
 {
@@ -372,10 +430,10 @@ int radix_seg_blocks(const std::vector<int> &seg_off, const std::vector<uint32_t
     for (int g = 0; g < ns; ++g) {
         const int b0 = seg_off[g], b1 = seg_off[g + 1];
         if (b1 < b0) return VPMC_ERR_ARG;
-        const int nb = (b1 - b0 + RS_TILE - 1) / RS_TILE;
+        const int nb = (b1 - b0 + RS_SUPER - 1) / RS_SUPER;
         for (int k = 0; k < nb; ++k) {
-            const int beg = b0 + k * RS_TILE;
-            blk->push_back(make_int4(beg, std::min(RS_TILE, b1 - beg), (int)(hbase + k), nb));
+            const int beg = b0 + k * RS_SUPER;
+            blk->push_back(make_int4(beg, std::min(RS_SUPER, b1 - beg), (int)(hbase + k), nb));
             blk_base->push_back(seg_base[g]);
         }
         hbase += (long long)RS_BINS * nb;
