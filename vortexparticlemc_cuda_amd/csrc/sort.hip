@@ -156,9 +156,15 @@ __device__ __forceinline__ int rs_block_excl_scan(int v, int *wsum, int *total)
 // digit's elements of this round go to one contiguous range, which starts at
 // offs[d * nblk + b] in the first round and right after the previous round's
 // range of digit d in the next.
-// The 16-byte records travel with the keys: each is read from the block's
-// own input slice.  recs_in == NULL (first pass): the record of input
-// position i is built as (q[i].x, q[i].y, i bits, 0).
+// The records travel with the keys.  STG = false: after the ranking, each
+// thread gathers the records of its output positions from the round's input
+// slice.  STG = true (sample records and indices after the first pass): each
+// thread loads the records of its own input positions with its keys, in
+// coalesced loads, and places them in LDS at their ranked position, from
+// where they are stored in order; this replaces the gather (one cache line
+// per record) with LDS traffic at twice the LDS footprint.  The first pass
+// (recs_in == NULL) builds the record of input position i as
+// (q[i].x, q[i].y, i) and keeps the gather, which measures faster there.
 constexpr int RS_WITEMS = RS_TILE / RS_WAVES / WAVE;   // keys per lane
 
 // the first pass's record of input position i: from its query point q[i]
@@ -183,7 +189,7 @@ __device__ __forceinline__ uint32_t rs_rec_from<uint32_t>(const float2 *__restri
     return i;
 }
 
-template <class R>
+template <class R, bool STG>
 __global__ void __launch_bounds__(RS_THREADS)
     k_rs_scatter(const uint32_t *__restrict__ keys_in, const R *__restrict__ recs_in,
                  const float2 *__restrict__ q, int n, int shift, int nblk,
@@ -194,7 +200,10 @@ __global__ void __launch_bounds__(RS_THREADS)
     const int lb = rs_logical_block(nblk);
     if (lb >= nblk) return;   // uniform over the workgroup
     __shared__ uint32_t lk[RS_TILE];
-    __shared__ uint16_t li[RS_TILE];        // local input index (< RS_TILE)
+    __shared__ uint16_t li[STG ? 1 : RS_TILE];   // local input index (< RS_TILE)
+    // STG: the records in the round's digit order, as 32-bit words
+    constexpr int RW = (int)(sizeof(R) / 4);
+    __shared__ uint32_t lr[STG ? RS_TILE * RW : 1];
     __shared__ int wh[RS_WAVES][RS_BINS];   // per-wave digit counts, then offsets
     __shared__ int gofs[RS_BINS];           // global position - local position (this round)
     __shared__ int wsum[RS_WAVES];
@@ -213,11 +222,18 @@ __global__ void __launch_bounds__(RS_THREADS)
         __syncthreads();
         uint32_t kr[RS_WITEMS];
         int rk[RS_WITEMS];
+        uint32_t rw[STG ? RS_WITEMS : 1][RW];
 #pragma unroll
         for (int r = 0; r < RS_WITEMS; ++r) {
             const int e = (wid * RS_WITEMS + r) * WAVE + lane;
             const bool valid = e < cnt;
             kr[r] = valid ? keys_in[base + e] : kb;
+            if constexpr (STG) {
+                // a lane past the round's end reads position base (cnt >= 1)
+                const uint32_t *src = reinterpret_cast<const uint32_t *>(recs_in + base + (valid ? e : 0));
+#pragma unroll
+                for (int w = 0; w < RW; ++w) rw[r][w] = valid ? src[w] : 0u;
+            }
         }
 #pragma unroll
         for (int r = 0; r < RS_WITEMS; ++r) {
@@ -263,33 +279,53 @@ __global__ void __launch_bounds__(RS_THREADS)
             if (e < cnt) {
                 const int pos = wh[wid][((kr[r] - kb) >> shift) & (RS_BINS - 1)] + rk[r];
                 lk[pos] = kr[r];
-                li[pos] = (uint16_t)e;
+                if constexpr (STG) {
+#pragma unroll
+                    for (int w = 0; w < RW; ++w) lr[pos * RW + w] = rw[r][w];
+                } else {
+                    li[pos] = (uint16_t)e;
+                }
             }
         }
         __syncthreads();
-        // all RS_ITEMS record loads in flight before the stores
-        uint32_t ko[RS_ITEMS];
-        int po[RS_ITEMS];
-        R rv[RS_ITEMS];
+        if constexpr (STG) {
 #pragma unroll
-        for (int j = 0; j < RS_ITEMS; ++j) {
-            const int i = j * RS_THREADS + t;
-            po[j] = -1;
-            if (i < cnt) {
-                ko[j] = lk[i];
-                const int src = base + li[i];
-                po[j] = gofs[((ko[j] - kb) >> shift) & (RS_BINS - 1)] + i;
-                rv[j] = recs_in ? recs_in[src] : rs_rec_from<R>(q, (uint32_t)src);
+            for (int j = 0; j < RS_ITEMS; ++j) {
+                const int i = j * RS_THREADS + t;
+                if (i < cnt) {
+                    const uint32_t k = lk[i];
+                    const int p = gofs[((k - kb) >> shift) & (RS_BINS - 1)] + i;
+                    keys_out[p] = k;
+                    uint32_t *dst = reinterpret_cast<uint32_t *>(recs_out + p);
+#pragma unroll
+                    for (int w = 0; w < RW; ++w) dst[w] = lr[i * RW + w];
+                }
+            }
+        } else {
+            // all RS_ITEMS record loads in flight before the stores
+            uint32_t ko[RS_ITEMS];
+            int po[RS_ITEMS];
+            R rv[RS_ITEMS];
+#pragma unroll
+            for (int j = 0; j < RS_ITEMS; ++j) {
+                const int i = j * RS_THREADS + t;
+                po[j] = -1;
+                if (i < cnt) {
+                    ko[j] = lk[i];
+                    const int src = base + li[i];
+                    po[j] = gofs[((ko[j] - kb) >> shift) & (RS_BINS - 1)] + i;
+                    rv[j] = recs_in ? recs_in[src] : rs_rec_from<R>(q, (uint32_t)src);
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < RS_ITEMS; ++j) {
+                if (po[j] >= 0) {
+                    keys_out[po[j]] = ko[j];
+                    recs_out[po[j]] = rv[j];
+                }
             }
         }
-#pragma unroll
-        for (int j = 0; j < RS_ITEMS; ++j) {
-            if (po[j] >= 0) {
-                keys_out[po[j]] = ko[j];
-                recs_out[po[j]] = rv[j];
-            }
-        }
-        __syncthreads();   // lk, li, gofs and wh are rewritten by the next round
+        __syncthreads();   // lk, li/lr, gofs and wh are rewritten by the next round
     }
 }
 
@@ -379,8 +415,16 @@ static int radix_sort_impl(const uint32_t *keys, int n, int key_bits, const floa
                            blk, blk_base, hist);
         VPMC_CHECK_LAUNCH();
         VPMC_TRY(scan_i32(hist, offs, (int)h, false, nullptr, sws, sb, s));
-        hipLaunchKernelGGL(k_rs_scatter<R>, dim3(rs_grid(nblk)), dim3(RS_THREADS), 0, s, kin, rin, q, n,
-                           8 * p, nblk, blk, blk_base, (const int *)offs, kout, rout);
+        // records staged through LDS after the first pass (16-byte field
+        // records keep the gather: their staging would leave one block per CU)
+        if (rin && sizeof(R) <= sizeof(SRec))
+            hipLaunchKernelGGL((k_rs_scatter<R, true>), dim3(rs_grid(nblk)), dim3(RS_THREADS), 0, s,
+                               kin, rin, q, n, 8 * p, nblk, blk, blk_base, (const int *)offs, kout,
+                               rout);
+        else
+            hipLaunchKernelGGL((k_rs_scatter<R, false>), dim3(rs_grid(nblk)), dim3(RS_THREADS), 0, s,
+                               kin, rin, q, n, 8 * p, nblk, blk, blk_base, (const int *)offs, kout,
+                               rout);
         VPMC_CHECK_LAUNCH();
         kin = kout;
         rin = rout;
