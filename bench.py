@@ -585,7 +585,7 @@ def ensemble_headline(args, device):
             "rbf_init_s": t_cg, "_data": (Ps, pos, w)}
 
 
-PLAIN_ENS_TUNE = ((8, 1, 256), (4, 1, 256), (8, 2, 256), (12, 1, 256))
+PLAIN_ENS_TUNE = ((8, 1, 256), (6, 1, 256), (5, 1, 256), (4, 1, 256), (8, 2, 256), (12, 1, 256))
 
 
 def large_bench(args, device):

@@ -37,7 +37,8 @@ namespace vpmc {
 
 struct BinBufs {
     uint32_t *key;            // tile key per sample (ntile: no query)
-    float2 *q, *r;
+    SRec *rec0;               // samples in sample order: (query point, sample id)
+    float2 *r;
     float *om;
     SRec *srec;               // cell-sorted samples: (x, y, sample id)
     uint32_t *keys_sorted;
@@ -48,12 +49,12 @@ struct BinBufs {
 
 template <int MODE, bool PER>
 __global__ void __launch_bounds__(BIN_THREADS)
-    k_bin_gen(McD d, BinGeom g, uint32_t *__restrict__ key, float2 *__restrict__ q,
+    k_bin_gen(McD d, BinGeom g, uint32_t *__restrict__ key, SRec *__restrict__ rec0,
               float2 *__restrict__ r, float *__restrict__ om)
 {
     size_t sid = (size_t)blockIdx.x * BIN_THREADS + threadIdx.x;
     if (sid >= g.S) return;
-    bin_gen_one<MODE, PER>(d, g, sid, sid, 0u, (uint32_t)g.ntile, key, q, r, om);
+    bin_gen_one<MODE, PER>(d, g, sid, sid, 0u, (uint32_t)g.ntile, key, rec0, r, om);
 }
 
 template <int THREADS, int SPL, bool COUNT, bool DEF = false>
@@ -189,7 +190,8 @@ static int bin_chunk_owners(const vpmc_mc_desc *d, int *per_chunk)
 static size_t bin_ws_bytes(const BinGeom &g)
 {
     size_t S = g.S > 0 ? g.S : 1, T = (size_t)g.ntile;
-    return align256(S * 4) + 2 * align256(S * 8) + align256(S * 4) + align256(S * sizeof(SRec)) +
+    return align256(S * 4) + align256(S * sizeof(SRec)) + align256(S * 8) + align256(S * 4) +
+           align256(S * sizeof(SRec)) +
            align256(S * 4) + align256((T + 1) * 4) + radix_workspace_bytes((int)S, sizeof(SRec)) + 4096;
 }
 
@@ -198,7 +200,7 @@ static int carve_bin(const BinGeom &g, void *ws, size_t bytes, BinBufs *B)
     Carve cv(ws, bytes);
     size_t S = g.S > 0 ? g.S : 1, T = (size_t)g.ntile;
     B->key = cv.take<uint32_t>(S);
-    B->q = cv.take<float2>(S);
+    B->rec0 = cv.take<SRec>(S);
     B->r = cv.take<float2>(S);
     B->om = cv.take<float>(S);
     B->srec = cv.take<SRec>(S);
@@ -206,7 +208,7 @@ static int carve_bin(const BinGeom &g, void *ws, size_t bytes, BinBufs *B)
     B->start = cv.take<int>(T + 1);
     B->sort_bytes = radix_workspace_bytes((int)S, sizeof(SRec));
     B->sort_ws = cv.take<char>(B->sort_bytes);
-    if (!B->key || !B->q || !B->r || !B->om || !B->srec || !B->keys_sorted || !B->start ||
+    if (!B->key || !B->rec0 || !B->r || !B->om || !B->srec || !B->keys_sorted || !B->start ||
         !B->sort_ws)
         return VPMC_ERR_WORKSPACE;
     return VPMC_OK;
@@ -219,7 +221,7 @@ static void launch_gen_final(const McD &m, const BinGeom &g, const BinBufs &B, b
     if (!final) {
         unsigned gb = (unsigned)((g.S + BIN_THREADS - 1) / BIN_THREADS);
         hipLaunchKernelGGL((k_bin_gen<MODE, PER>), dim3(gb), dim3(BIN_THREADS), 0, s, m, g, B.key,
-                           B.q, B.r, B.om);
+                           B.rec0, B.r, B.om);
     } else {
         unsigned gb = (unsigned)((2 * (size_t)m.n_owner + 3) / 4);
         hipLaunchKernelGGL((k_bin_final<MODE, PER>), dim3(gb), dim3(BIN_THREADS), 0, s, m, g,
@@ -285,7 +287,7 @@ int mc_binned_impl(const vpmc_mc_desc *d, void *ws, size_t ws_bytes, hipStream_t
         launch_mode(d->mode, d->periodic, mc, g, B, false, s);
         VPMC_CHECK_LAUNCH();
         VPMC_TRY(radix_sort_samples(B.key, (int)g.S, key_bits_for((uint32_t)g.ntile),
-                                    (const float2 *)B.q, B.srec, B.keys_sorted, g.ntile, B.start,
+                                    B.rec0, B.srec, B.keys_sorted, g.ntile, B.start,
                                     B.sort_ws, B.sort_bytes, s));
         if (o0 == 0) VPMC_TRY(record_event(d->eval_events[0], s));
         if (d->eval_count)

@@ -51,12 +51,13 @@ inline void set_ns_division(BinGeom *g)
 int bin_geom(const McD &m, int N1, int N2, int n_owner, BinGeom *g);
 
 // One sample: draws, query point, MIS displacement and tile key.  Writes
-// key[out] (tile_base + local tile, or dead_key without a query), q[out],
-// r[out] and om[out] = 0.
+// key[out] (tile_base + local tile, or dead_key without a query),
+// rec0[out] = (query point, out) (the record the sort moves), r[out] and
+// om[out] = 0.
 template <int MODE, bool PER>
 __device__ __forceinline__ void bin_gen_one(const McD &d, const BinGeom &g, size_t sid,
                                             size_t out, uint32_t tile_base, uint32_t dead_key,
-                                            uint32_t *__restrict__ key, float2 *__restrict__ q,
+                                            uint32_t *__restrict__ key, SRec *__restrict__ rec0,
                                             float2 *__restrict__ r, float *__restrict__ om)
 {
     const uint32_t pair = div_ns(g, (uint32_t)sid);   // sid < 2^31 (bin_geom)
@@ -69,7 +70,7 @@ __device__ __forceinline__ void bin_gen_one(const McD &d, const BinGeom &g, size
     bool live = n_s > 0 &&
                 gen_sample<MODE, PER>(d, S, n_s, owner, si, k, d.owner[owner], qx, qy, rx, ry);
     r[out] = make_float2(rx, ry);
-    q[out] = make_float2(qx, qy);
+    rec0[out] = SRec{qx, qy, (uint32_t)out};
     if (!live) {
         key[out] = dead_key;
         return;

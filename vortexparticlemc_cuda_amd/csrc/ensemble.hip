@@ -118,7 +118,7 @@ __global__ void k_ens_update_jac(EnsTabs t, int n_sim, int N, float dt, float4 *
 template <int MODE>
 __global__ void __launch_bounds__(BIN_THREADS)
     k_ens_gen(EnsTabs t, int n_sim, uint32_t *__restrict__ key,
-              float2 *__restrict__ q, float2 *__restrict__ r, float *__restrict__ om)
+              SRec *__restrict__ rec0, float2 *__restrict__ r, float *__restrict__ om)
 {
     const int first = blockIdx.x * BIN_THREADS;
     const int s = seg_of(t.soff, n_sim, first);   // block-uniform
@@ -129,12 +129,12 @@ __global__ void __launch_bounds__(BIN_THREADS)
     const uint32_t dk = (uint32_t)(t.toff[s] + g.ntile);   // the simulation's dead tile
     if (sid >= g.S) {   // padding of the simulation's sample range
         key[out] = dk;
-        q[out] = make_float2(0.f, 0.f);
+        rec0[out] = SRec{0.f, 0.f, (uint32_t)out};
         r[out] = make_float2(0.f, 0.f);
         om[out] = 0.f;
         return;
     }
-    bin_gen_one<MODE, true>(d, g, sid, out, (uint32_t)t.toff[s], dk, key, q, r, om);
+    bin_gen_one<MODE, true>(d, g, sid, out, (uint32_t)t.toff[s], dk, key, rec0, r, om);
 }
 
 template <int THREADS, int SPL, bool DEF>
@@ -211,7 +211,8 @@ struct EnsLayout {
     void *fsort_ws = nullptr;
     size_t fsort_bytes = 0;
     uint32_t *skey = nullptr, *skeys_sorted = nullptr;
-    float2 *q = nullptr, *r = nullptr, *vel_sign = nullptr;
+    float2 *r = nullptr, *vel_sign = nullptr;
+    SRec *rec0 = nullptr;   // samples in sample order (query point, id)
     float *om = nullptr;
     SRec *srec = nullptr;
     int *tstart = nullptr;
@@ -320,7 +321,7 @@ static size_t ens_carve(EnsLayout &L, void *ws, size_t bytes)
     L.fsort_bytes = radix_workspace_bytes(L.E);
     L.fsort_ws = cv.take<char>(L.fsort_bytes);
     L.skey = cv.take<uint32_t>(S);
-    L.q = cv.take<float2>(S);
+    L.rec0 = cv.take<SRec>(S);
     L.r = cv.take<float2>(S);
     L.om = cv.take<float>(S);
     L.srec = cv.take<SRec>(S);
@@ -423,10 +424,10 @@ static int ens_estimate(const vpmc_ensemble_desc *e, EnsLayout &L, hipStream_t s
 {
     const int ns = L.n_sim;
     hipLaunchKernelGGL((k_ens_gen<MODE>), dim3((unsigned)(L.S / BIN_THREADS)), dim3(BIN_THREADS), 0,
-                       st, L.t, ns, L.skey, L.q, L.r, L.om);
+                       st, L.t, ns, L.skey, L.rec0, L.r, L.om);
     VPMC_CHECK_LAUNCH();
     // segmented by simulation: samples are grouped by simulation already
-    VPMC_TRY(radix_sort_segmented(L.skey, (int)L.S, L.local_bits, (const float2 *)L.q, L.srec,
+    VPMC_TRY(radix_sort_segmented(L.skey, (int)L.S, L.local_bits, L.rec0, L.srec,
                                   L.skeys_sorted, L.T, L.tstart, L.sblk, L.sblk_base, L.nblk,
                                   L.ssort_ws, L.ssort_bytes, st));
     const int th = e->eval_threads ? e->eval_threads : 256;

@@ -162,9 +162,10 @@ __device__ __forceinline__ int rs_block_excl_scan(int v, int *wsum, int *total)
 // thread loads the records of its own input positions with its keys, in
 // coalesced loads, and places them in LDS at their ranked position, from
 // where they are stored in order; this replaces the gather (one cache line
-// per record) with LDS traffic at twice the LDS footprint.  The first pass
-// (recs_in == NULL) builds the record of input position i as
-// (q[i].x, q[i].y, i) and keeps the gather, which measures faster there.
+// per record) with LDS traffic at twice the LDS footprint.  Without input
+// records (recs_in == NULL: the first pass of the fields' sort and of the
+// hash) the record of input position i is built from it, (q[i].x, q[i].y,
+// i, 0) or i, and gathered.
 constexpr int RS_WITEMS = RS_TILE / RS_WAVES / WAVE;   // keys per lane
 
 // the first pass's record of input position i: from its query point q[i]
@@ -447,11 +448,11 @@ int radix_sort_to_records(const uint32_t *keys, int n, int key_bits, const float
                                    nullptr, nullptr, rs_nblk(n), ws, ws_bytes, s);
 }
 
-int radix_sort_samples(const uint32_t *keys, int n, int key_bits, const float2 *q, SRec *rec,
+int radix_sort_samples(const uint32_t *keys, int n, int key_bits, const SRec *recs0, SRec *rec,
                        uint32_t *keys_sorted, int nkeys, int *start, void *ws, size_t ws_bytes,
                        hipStream_t s)
 {
-    return radix_sort_impl<SRec>(keys, n, key_bits, q, nullptr, rec, keys_sorted, nkeys, start,
+    return radix_sort_impl<SRec>(keys, n, key_bits, nullptr, recs0, rec, keys_sorted, nkeys, start,
                                  nullptr, nullptr, rs_nblk(n), ws, ws_bytes, s);
 }
 
@@ -494,14 +495,14 @@ size_t radix_seg_workspace_bytes(int n, int nblk)
            scan_workspace_bytes((int)h, sizeof(int)) + 1024;
 }
 
-int radix_sort_segmented(const uint32_t *keys, int n, int local_bits, const float2 *q,
+int radix_sort_segmented(const uint32_t *keys, int n, int local_bits, const SRec *recs0,
                          SRec *rec, uint32_t *keys_sorted, int nkeys, int *start,
                          const int4 *blk, const uint32_t *blk_base, int nblk, void *ws,
                          size_t ws_bytes, hipStream_t s)
 {
     if (!blk || !blk_base || nblk <= 0) return VPMC_ERR_ARG;
-    return radix_sort_impl<SRec>(keys, n, local_bits, q, nullptr, rec, keys_sorted, nkeys, start,
-                                 blk, blk_base, nblk, ws, ws_bytes, s);
+    return radix_sort_impl<SRec>(keys, n, local_bits, nullptr, recs0, rec, keys_sorted, nkeys,
+                                 start, blk, blk_base, nblk, ws, ws_bytes, s);
 }
 
 }  // namespace vpmc

@@ -77,8 +77,8 @@ size_t radix_workspace_bytes(int n, size_t rec_bytes = 16);   // rec_bytes: 16 (
 int radix_sort_to_records(const uint32_t *keys, int n, int key_bits, const float2 *q,
                           const float4 *recs0, float4 *rec, uint32_t *keys_sorted, int nkeys,
                           int *start, void *ws, size_t ws_bytes, hipStream_t s);
-// the same with 12-byte sample records built from q (point, input position)
-int radix_sort_samples(const uint32_t *keys, int n, int key_bits, const float2 *q, SRec *rec,
+// the same with 12-byte sample records (query point, sample id)
+int radix_sort_samples(const uint32_t *keys, int n, int key_bits, const SRec *recs0, SRec *rec,
                        uint32_t *keys_sorted, int nkeys, int *start, void *ws, size_t ws_bytes,
                        hipStream_t s);
 // the same carrying only the input positions: idx = the stable argsort of keys
@@ -92,7 +92,7 @@ int radix_sort_indices(const uint32_t *keys, int n, int key_bits, uint32_t *idx,
 int radix_seg_blocks(const std::vector<int> &seg_off, const std::vector<uint32_t> &seg_base,
                      std::vector<int4> *blk, std::vector<uint32_t> *blk_base);
 size_t radix_seg_workspace_bytes(int n, int nblk);
-int radix_sort_segmented(const uint32_t *keys, int n, int local_bits, const float2 *q,
+int radix_sort_segmented(const uint32_t *keys, int n, int local_bits, const SRec *recs0,
                          SRec *rec, uint32_t *keys_sorted, int nkeys, int *start,
                          const int4 *blk, const uint32_t *blk_base, int nblk, void *ws,
                          size_t ws_bytes, hipStream_t s);

@@ -167,7 +167,10 @@ class Ensemble:
 # ---- autotuner ------------------------------------------------------------------
 
 # (cells_per_hw, samples per lane, evaluation workgroup size)
-TUNE_CANDIDATES = tuple((c, s, t) for c in (8, 4) for s in (1, 2, 4) for t in (64, 128, 256))
+# cells_per_hw sets the visited pairs per in-support pair, (pi + 4/c + 1/c^2)/pi
+# (1.34 at 4, 1.27 at 5, 1.22 at 6, 1.17 at 8), against more, smaller tiles
+# and, past 2^16 tiles per simulation, a third sort pass
+TUNE_CANDIDATES = tuple((c, s, t) for c in (8, 6, 5, 4) for s in (1, 2) for t in (128, 256))
 
 
 def _tune_cache_path():
@@ -181,17 +184,25 @@ def autotune(positions, strengths, h_w, L=1.0, N1=64, N2=64, mode=MODE_CONSISTEN
              candidates=TUNE_CANDIDATES, sample_sims=8, steps=2, use_cache=True, device="cuda",
              deform=False):
     """(cells_per_hw, spl, eval_threads) with the shortest step on a subset of the ensemble
-    (the first `sample_sims` simulations), timed with HIP events after one
-    warm-up step; results are cached per GPU and workload shape (deform: the
-    deformed step, whose search reaches 2 h_w)"""
+    (`sample_sims` simulations spread over its sizes, the largest included:
+    the largest simulation sets the sort's passes), timed with HIP events
+    after one warm-up step; results are cached per GPU and workload shape
+    (deform: the deformed step, whose search reaches 2 h_w)"""
     import json
     import os
-    sizes = tuple(int(np.asarray(p).shape[0]) for p in positions)
+    all_sizes = [int(np.asarray(p).shape[0]) for p in positions]
+    order = np.argsort(all_sizes, kind="stable")
+    pick = sorted(set(int(order[int(round(v))])
+                      for v in np.linspace(0, len(order) - 1, min(sample_sims, len(order)))))
+    positions = [positions[i] for i in pick]
+    strengths = [strengths[i] for i in pick]
+    hw_all = np.broadcast_to(np.asarray(h_w, np.float32), (len(all_sizes),))
+    L_all = np.broadcast_to(np.asarray(L, np.float32), (len(all_sizes),))
+    h_w, L = hw_all[pick], L_all[pick]
+    sizes = tuple(all_sizes[i] for i in pick)
     key = "%s|sizes=%s|hw=%s|N1=%d|N2=%d|mode=%d" % (
-        torch.cuda.get_device_name(device), sizes[:sample_sims],
-        tuple(round(float(v), 7) for v in np.broadcast_to(np.asarray(h_w, np.float32),
-                                                            (len(sizes),))[:sample_sims]),
-        N1, N2, mode) + ("|deform" if deform else "")
+        torch.cuda.get_device_name(device), sizes,
+        tuple(round(float(v), 7) for v in h_w), N1, N2, mode) + ("|deform" if deform else "")
     path = _tune_cache_path()
     cache = {}
     if use_cache and os.path.exists(path):
@@ -203,11 +214,10 @@ def autotune(positions, strengths, h_w, L=1.0, N1=64, N2=64, mode=MODE_CONSISTEN
         c = cache[key]
         return (int(c["cells_per_hw"]), int(c["spl"]), int(c["eval_threads"]),
                 c.get("times_ms", {}))
-    hw = np.broadcast_to(np.asarray(h_w, np.float32), (len(sizes),))[:sample_sims]
     times = {}
     for cph, spl, th in candidates:
         try:
-            e = Ensemble(positions[:sample_sims], strengths[:sample_sims], hw, L=L, N1=N1, N2=N2,
+            e = Ensemble(positions, strengths, h_w, L=L, N1=N1, N2=N2,
                          mode=mode, cells_per_hw=cph, spl=spl, eval_threads=th, device=device,
                          deform=deform)
         except Exception:   # grid or workgroup invalid for this candidate
