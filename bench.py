@@ -765,13 +765,13 @@ def adjoint_bench(args, device):
     sim = Simulation(P, device=device, cells_per_hw=8, seed=0)
     sim.set_particles(pos, w_tg.cpu().numpy())
     with torch.no_grad():
-        X_ref = A.rollout(sim, pos_t, w_tg, T, checkpoint=K)
+        X_ref = A.rollout(sim, pos_t, w_tg, T, checkpoint=K, trajectory="f32")
         ref = A.grid_vorticity(X_ref, w_tg, h, G, G, f64=True)
     p = pos_t.clone().requires_grad_(True)
     wt = w0.clone().requires_grad_(True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    XT = A.rollout(sim, p, wt, T, checkpoint=K)
+    XT = A.rollout(sim, p, wt, T, checkpoint=K, trajectory="f32")
     om = A.grid_vorticity(XT, wt, h, G, G, f64=True)
     loss = ((om - ref) ** 2).mean()
     torch.cuda.synchronize()

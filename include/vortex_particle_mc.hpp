@@ -211,6 +211,12 @@ public:
     // float[N_cur]) accumulates dL/dw.  Consistent periodic estimator,
     // no kernel deformation.
     void step_adjoint(uint32_t step, float2 *lambda, float *grad_w);
+    // The FP64 forward of the same frozen-sample step (vpmc_step_f64): pos
+    // (device double2[N_cur]) advances in place by the velocity at pos with
+    // strengths w (device double[N_cur]); the draws, the sampling tables and
+    // the MIS density are those of the current strengths.  No counterpart in
+    // the reference: the accurate trajectory for step_adjoint's gradients.
+    void step_f64(uint32_t step, double2 *pos, const double *w);
     int subset_count(int sign) const { return sign > 0 ? cnt_pos_ : cnt_neg_; }
 
     struct Impl;

@@ -233,8 +233,8 @@ def test_config4_adjoint_fp64_trajectory_vs_central_differences(device):
     sim = Simulation(P, device=device, cells_per_hw=8, seed=0)
     sim.set_particles(pos, w_tg.cpu().numpy())
     with torch.no_grad():
-        ref = A.grid_vorticity(A.rollout(sim, pos_t, w_tg, T, checkpoint=K), w_tg, h, G, G,
-                               f64=True)
+        ref = A.grid_vorticity(A.rollout(sim, pos_t, w_tg, T, checkpoint=K, trajectory="f32"),
+                               w_tg, h, G, G, f64=True)
     p = pos_t.clone().requires_grad_(True)
     wt = w0.clone().requires_grad_(True)
     XT = A.rollout(sim, p, wt, T, checkpoint=K, trajectory="f64")

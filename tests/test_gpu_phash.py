@@ -62,14 +62,15 @@ def test_phash_simulation_steps_like_the_mirror(device):
     assert not np.array_equal(c.positions(), s.positions())
 
 
-def test_phash_rollout_gradient_matches_python_autograd(device):
+@pytest.mark.parametrize("traj", ["f64", "f32"])
+def test_phash_rollout_gradient_matches_python_autograd(device, traj):
     import torch
 
     import vortexparticlemc_cuda_amd as v
     from vortexparticlemc_cuda_amd import adjoint as A
     from vortexparticlemc_cuda_amd._capi import MODE_CONSISTENT
     from vortexparticlemc_cuda_amd.sim import SimParam, Simulation
-    _say("phash.rollout backward == adjoint.rollout backward")
+    _say("phash.rollout backward == adjoint.rollout backward, %s trajectory" % traj)
     phash = v.phash()
     pos, w = _problem()
     T, K = 5, 2
@@ -89,7 +90,12 @@ def test_phash_rollout_gradient_matches_python_autograd(device):
     for run in ("phash", "python"):
         p = torch.as_tensor(pos, device=device).requires_grad_(True)
         ww = torch.as_tensor(w, device=device).requires_grad_(True)
-        X = phash.rollout(s, p, ww, T, K) if run == "phash" else A.rollout(m, p, ww, T, K)
+        if run == "phash":
+            X = phash.rollout(s, p, ww, T, K) if traj == "f64" else phash.rollout(s, p, ww, T, K, "f32")
+        else:
+            X = A.rollout(m, p, ww, T, K) if traj == "f64" else A.rollout(m, p, ww, T, K, "f32")
+        # the default trajectory is FP64 and returns FP64 positions
+        assert X.dtype == (torch.float64 if traj == "f64" else torch.float32)
         loss = (X * target).sum()
         loss.backward()
         grads.append((X.detach().cpu().numpy(), p.grad.cpu().numpy(), ww.grad.cpu().numpy()))

@@ -174,17 +174,17 @@ class _Rollout(torch.autograd.Function):
         return lam.to(ctx.in_dtype), gw, None, None, None, None
 
 
-def rollout(sim, pos0, w, steps, checkpoint=10, trajectory="f32"):
+def rollout(sim, pos0, w, steps, checkpoint=10, trajectory="f64"):
     """positions after `steps` plain steps (Philox steps 0 .. steps-1) from
     pos0 with strengths w; differentiable in pos0 and w.
 
-    trajectory="f32": the FP32 step (vpmc_step); the positions carry FP32
-    rounding accumulated over the steps, which bounds how well the gradient
-    matches the exact derivative (about 1e-4 of its largest entry on a loss
-    that compares nearly equal fields).  trajectory="f64": the positions
-    advance by the FP64 forward of the same frozen-sample step
-    (vpmc_step_f64) and are returned in FP64; the step adjoints then
-    linearise at those states rounded to FP32 (about 1e-5)."""
+    trajectory="f64" (default): the positions advance by the FP64 forward of
+    the same frozen-sample step (vpmc_step_f64) and are returned in FP64; the
+    step adjoints linearise at those states rounded to FP32, and the gradient
+    matches the exact derivative to about 1e-5 of its largest entry.
+    trajectory="f32": the FP32 step (vpmc_step), about 8x cheaper; the
+    positions carry FP32 rounding accumulated over the steps, which bounds
+    the match at about 1e-4 on a loss that compares nearly equal fields."""
     if trajectory not in ("f32", "f64"):
         raise ValueError("trajectory must be 'f32' or 'f64'")
     return _Rollout.apply(pos0, w, sim, int(steps), int(checkpoint), trajectory == "f64")

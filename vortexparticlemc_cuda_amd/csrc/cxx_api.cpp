@@ -857,6 +857,27 @@ void Simulation::step_adjoint(uint32_t step, float2 *lambda, float *grad_w)
            "step_adjoint");
 }
 
+void Simulation::step_f64(uint32_t step, double2 *pos, const double *w)
+{
+    if (kernel_deformation)
+        throw std::invalid_argument("step_f64: kernel deformation is not supported");
+    Impl &I = *impl_;
+    const int N = particles_.N_cur;
+    if (I.nx == 0) do_spatial_hashing();
+    I.vel_sign.resize(2 * (size_t)N);
+    vpmc_mc_desc d = make_desc(*this, P_.N1, P_.N2, cnt_pos_, cnt_neg_, raw_ptr(I.counts),
+                               raw_ptr(I.totals), I.guide, I.glog2, raw_ptr(cdf_pos_),
+                               raw_ptr(cdf_neg_), I.pts, I.cell_start, I.order, I.set_xy, I.nx,
+                               I.ny, raw_ptr(I.vel_sign), true);
+    d.step = step;
+    const size_t b = vpmc_step_f64_workspace_bytes(&d);
+    if (b == 0)
+        throw std::invalid_argument("step_f64: the consistent periodic estimator only");
+    vcheck(vpmc_step_f64(&d, N, (double *)pos, w, particles_.d_omega(), raw_ptr(sub_pos_),
+                         raw_ptr(sub_neg_), (double)P_.dt, I.ws.get(b), b, stream),
+           "step_f64");
+}
+
 void Simulation::compute_velocity_naive(bool periodic, float eps_core)
 {
     // src/cuda/simulation_step.cu:92-108 on the FP64 direct-sum kernel
