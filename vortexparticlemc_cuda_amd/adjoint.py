@@ -182,7 +182,7 @@ def rollout(sim, pos0, w, steps, checkpoint=10, trajectory="f64"):
     the same frozen-sample step (vpmc_step_f64) and are returned in FP64; the
     step adjoints linearise at those states rounded to FP32, and the gradient
     matches the exact derivative to about 1e-5 of its largest entry.
-    trajectory="f32": the FP32 step (vpmc_step), about 8x cheaper; the
+    trajectory="f32": the FP32 step (vpmc_step), about 3x cheaper; the
     positions carry FP32 rounding accumulated over the steps, which bounds
     the match at about 1e-4 on a loss that compares nearly equal fields."""
     if trajectory not in ("f32", "f64"):

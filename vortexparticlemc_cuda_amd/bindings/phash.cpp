@@ -317,7 +317,7 @@ PYBIND11_MODULE(phash, m)
         "(N)); a torch.autograd.Function: backward runs the hand-written step adjoint "
         "(vpmc_step_adjoint) over checkpointed recomputation.  trajectory='f64' (default): "
         "the FP64 forward of the frozen-sample step, positions returned in FP64, gradients "
-        "to ~1e-5 of the exact derivative; 'f32': step_cuda(true), ~8x cheaper, the FP32 "
+        "to ~1e-5 of the exact derivative; 'f32': step_cuda(true), ~3x cheaper, the FP32 "
         "rounding of the positions limits the gradient to ~1e-4.  The simulation must "
         "outlive the backward pass.");
 }
