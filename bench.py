@@ -447,8 +447,13 @@ def ensemble_headline(args, device):
     sizes = sorted(set(int(P.ResolutionX) for P in Ps))
     log("ensemble: %d sims (Nx in %s), %d particles, RBF init %.1f s"
         % (len(pos), sizes, N, t_cg))
-    cph, spl, th, tune_ms = autotune(pos, w, hw, N1=N1, N2=N1, device=device)
-    log("autotune: cells_per_hw=%d spl=%d eval_threads=%d" % (cph, spl, th))
+    if args.ens_shape:   # a fixed launch shape (profiling), no tuning
+        cph, spl, th = (int(v) for v in args.ens_shape.split(","))
+        tune_ms = {}
+        log("launch shape (given): cells_per_hw=%d spl=%d eval_threads=%d" % (cph, spl, th))
+    else:
+        cph, spl, th, tune_ms = autotune(pos, w, hw, N1=N1, N2=N1, device=device)
+        log("autotune: cells_per_hw=%d spl=%d eval_threads=%d" % (cph, spl, th))
     ens = Ensemble(pos, w, hw, N1=N1, N2=N1, dt=dt, seed=0, cells_per_hw=cph, spl=spl,
                    eval_threads=th, device=device)
     ens.step(1)   # step 0: velocities at the initial positions
@@ -1106,6 +1111,8 @@ def main():
     ap.add_argument("--ens-N", type=int, default=64, help="N1 = N2 of the ensemble")
     ap.add_argument("--ens-probes", type=int, default=512,
                     help="probe owners per member for the ensemble's error")
+    ap.add_argument("--ens-shape", default="",
+                    help="cells_per_hw,spl,eval_threads of the ensemble instead of the tuner's")
     ap.add_argument("--ens-plain-steps", type=int, default=10)
     ap.add_argument("--ens-direct-steps", type=int, default=2)
     ap.add_argument("--ens-deform-steps", type=int, default=5)

@@ -283,7 +283,8 @@ __global__ void __launch_bounds__(ADJ_THREADS)
     const int npieces = 3 * (2 * g.cols + 1);
     const int npass = (total + CAP - 1) / CAP;
     int n = 0;
-    if (npass == 1) n = tile_stage<ADJ_THREADS>(g, pts, npieces, 0, total, ox, oy, L);
+    if (npass == 1)
+        n = tile_stage<ADJ_THREADS>(g, pts, npieces, 0, total, cx, ox, oy, m.Lx, m.Ly, L);
     const int wid = (int)threadIdx.x / WAVE;
     for (int bb = beg; bb < end; bb += ADJ_THREADS) {   // uniform trip count
         if (npass == 1 && bb + wid * WAVE >= end) break;   // no sample left for this wave
@@ -295,7 +296,8 @@ __global__ void __launch_bounds__(ADJ_THREADS)
         for (int pass = 0; pass < npass; ++pass) {
             if (npass > 1)
                 n = tile_stage<ADJ_THREADS>(g, pts, npieces, pass * CAP,
-                                            min(total, (pass + 1) * CAP), ox, oy, L);
+                                            min(total, (pass + 1) * CAP), cx, ox, oy, m.Lx, m.Ly,
+                                            L);
             for (int k = 0; k < n; ++k) {
                 // staged record (-2 c', |c'|^2, w), c' = particle - cell centre
                 const float4 r = L.cand[k];
@@ -439,7 +441,8 @@ __global__ void __launch_bounds__(ADJ_THREADS)
         for (int i = threadIdx.x; i < total; i += ADJ_THREADS) {
             while (lo + 1 < npieces && L.p_off[lo + 1] <= i) ++lo;
             const float4 sm = rec2[L.p_beg[lo] + (i - L.p_off[lo])];
-            const float sx = sm.x + L.p_sx[lo], sy = sm.y + L.p_sy[lo];
+            const float2 sh = piece_shift(g, cx, lo, d.Lx, d.Ly);
+            const float sx = sm.x + sh.x, sy = sm.y + sh.y;
 #pragma unroll
             for (int j = 0; j < ADJ_PCHUNK; ++j) {
                 if (j >= np) break;   // uniform

@@ -125,9 +125,21 @@ template <int THREADS, bool DEF = false> struct TileLds {
     float4 cand_g[DEF ? Cfg::CAP : 1];   // deformed records: quadratic forms
     int p_beg[Cfg::MAXP];
     int p_off[Cfg::MAXP + 1];
-    float p_sx[Cfg::MAXP], p_sy[Cfg::MAXP];
     int wsum[Cfg::WAVES];
 };
+
+// Periodic image shift of stencil piece `lo` (column lo / 3 of the tile at
+// column cx; kind lo % 3: rows wrapped below, rows wrapped above, rows in
+// range), recomputed rather than kept in LDS so that a 256-thread tile's LDS
+// stays under 20 KB (8 workgroups per CU)
+__device__ __forceinline__ float2 piece_shift(const BinGeom &g, int cx, int lo, float Lx, float Ly)
+{
+    const int j = lo / 3, k = lo - 3 * j;
+    const int colw = cx + j - g.cols;
+    const float sx = colw < 0 ? -Lx : (colw >= g.nx ? Lx : 0.f);
+    const float sy = k == 0 ? -Ly : (k == 1 ? Ly : 0.f);
+    return make_float2(sx, sy);
+}
 
 // 2 cols + 1 <= TileCfg<THREADS>::MAXCOL (checked on the host)
 template <int THREADS, bool DEF = false>
@@ -138,13 +150,12 @@ __device__ __forceinline__ int tile_pieces(const BinGeom &g, const int *__restri
     const int ncol = 2 * g.cols + 1;
     const int j = threadIdx.x;
     int len[3] = {0, 0, 0}, pb[3] = {0, 0, 0};
-    float sx = 0.f, sy[3] = {-Ly, Ly, 0.f};
     if (j < ncol) {
         const int e = g.ext[j];
         if (e >= 0) {
-            int colw = cx + j - g.cols;
-            if (colw < 0) { colw += g.nx; sx = -Lx; }
-            else if (colw >= g.nx) { colw -= g.nx; sx = Lx; }
+            int colw = cx + j - g.cols;   // image shifts: piece_shift
+            if (colw < 0) colw += g.nx;
+            else if (colw >= g.nx) colw -= g.nx;
             const int *csc = cell_start + (size_t)colw * g.ny;
             int r0 = cy - e, r1 = cy + e;
             if (r0 < 0) {   // rows ny+r0 .. ny-1: particle image y - Ly
@@ -168,8 +179,6 @@ __device__ __forceinline__ int tile_pieces(const BinGeom &g, const int *__restri
         for (int k = 0; k < 3; ++k) {
             L.p_beg[3 * j + k] = pb[k];
             L.p_off[3 * j + k] = off;
-            L.p_sx[3 * j + k] = sx;
-            L.p_sy[3 * j + k] = sy[k];
             off += len[k];
         }
     }
@@ -184,8 +193,8 @@ __device__ __forceinline__ int tile_pieces(const BinGeom &g, const int *__restri
 // DEF: records are (c'x, c'y, w, 0) with the quadratic form in cand_g (gq).
 template <int THREADS, bool DEF = false>
 __device__ __forceinline__ int tile_stage(const BinGeom &g, const float4 *__restrict__ pts,
-                                          int npieces, int r0, int r1, float ox, float oy,
-                                          TileLds<THREADS, DEF> &L,
+                                          int npieces, int r0, int r1, int cx, float ox, float oy,
+                                          float Lx, float Ly, TileLds<THREADS, DEF> &L,
                                           const float4 *__restrict__ gq = nullptr)
 {
     const int lane = threadIdx.x & (WAVE - 1), wid = threadIdx.x / WAVE;
@@ -204,7 +213,8 @@ __device__ __forceinline__ int tile_stage(const BinGeom &g, const float4 *__rest
             }
             const int src = L.p_beg[lo] + (i - L.p_off[lo]);
             const float4 c = pts[src];
-            const float x = (c.x + L.p_sx[lo]) - ox, y = (c.y + L.p_sy[lo]) - oy;
+            const float2 sh = piece_shift(g, cx, lo, Lx, Ly);
+            const float x = (c.x + sh.x) - ox, y = (c.y + sh.y) - oy;
             const float gx = fmaxf(fabsf(x) - hx, 0.f), gy = fmaxf(fabsf(y) - hy, 0.f);
             keep = fmaf(gx, gx, gy * gy) <= g.rs2;
             if (DEF) {   // r^2 = p'^T G p' - 2 (G c')^T p' + c'^T G c'
@@ -340,7 +350,8 @@ __device__ __forceinline__ void bin_eval_tile(const McD &m, const BinGeom &g, in
     const int npieces = 3 * (2 * g.cols + 1);
     const int npass = (total + CAP - 1) / CAP;
     int n = 0;
-    if (npass == 1) n = tile_stage<THREADS, DEF>(g, pts, npieces, 0, total, ox, oy, L, gq);
+    if (npass == 1)
+        n = tile_stage<THREADS, DEF>(g, pts, npieces, 0, total, cx, ox, oy, m.Lx, m.Ly, L, gq);
     const int lane = threadIdx.x & (WAVE - 1), wid = threadIdx.x / WAVE;
     for (int bb = beg; bb < end; bb += THREADS * SPL) {   // uniform trip count
         // a wave past the tile's last sample stops (no barrier follows when
@@ -363,7 +374,8 @@ __device__ __forceinline__ void bin_eval_tile(const McD &m, const BinGeom &g, in
         for (int pass = 0; pass < npass; ++pass) {
             if (npass > 1)
                 n = tile_stage<THREADS, DEF>(g, pts, npieces, pass * CAP,
-                                             min(total, (pass + 1) * CAP), ox, oy, L, gq);
+                                             min(total, (pass + 1) * CAP), cx, ox, oy, m.Lx,
+                                             m.Ly, L, gq);
             if constexpr (DEF)
                 tile_eval_def<SPL, COUNT>(L.cand, L.cand_g, n, px, py, m.inv_hw, acc, ec, live);
             else
