@@ -262,7 +262,8 @@ __global__ void __launch_bounds__(BIN_THREADS)
 
 // om(y) and grad om(y) for every live sample: one workgroup per tile (sign,
 // fine cell) of samples, the cell's particle stencil staged in LDS as in the
-// forward's evaluation (tile_stage), one sample per thread.
+// forward's evaluation (tile_stage, in world units: scale 1), one sample per
+// thread.
 __global__ void __launch_bounds__(ADJ_THREADS)
     k_adj_eval(McD m, BinGeom g, const int *__restrict__ start, const float4 *__restrict__ srec,
                float *__restrict__ om, float2 *__restrict__ G)
@@ -284,7 +285,7 @@ __global__ void __launch_bounds__(ADJ_THREADS)
     const int npass = (total + CAP - 1) / CAP;
     int n = 0;
     if (npass == 1)
-        n = tile_stage<ADJ_THREADS>(g, pts, npieces, 0, total, cx, ox, oy, m.Lx, m.Ly, L);
+        n = tile_stage<ADJ_THREADS>(g, pts, npieces, 0, total, cx, ox, oy, m.Lx, m.Ly, 1.f, L);
     const int wid = (int)threadIdx.x / WAVE;
     for (int bb = beg; bb < end; bb += ADJ_THREADS) {   // uniform trip count
         if (npass == 1 && bb + wid * WAVE >= end) break;   // no sample left for this wave
@@ -297,7 +298,7 @@ __global__ void __launch_bounds__(ADJ_THREADS)
             if (npass > 1)
                 n = tile_stage<ADJ_THREADS>(g, pts, npieces, pass * CAP,
                                             min(total, (pass + 1) * CAP), cx, ox, oy, m.Lx, m.Ly,
-                                            L);
+                                            1.f, L);
             for (int k = 0; k < n; ++k) {
                 // staged record (-2 c', |c'|^2, w), c' = particle - cell centre
                 const float4 r = L.cand[k];

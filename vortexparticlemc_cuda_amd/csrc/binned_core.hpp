@@ -80,7 +80,15 @@ __device__ __forceinline__ void bin_gen_one(const McD &d, const BinGeom &g, size
     key[out] = tile_base + (uint32_t)(si * g.ncell + cx * g.ny + cy);
 }
 
-constexpr float CBRT4 = 1.5874010519681994f;    // 4^(1/3)
+// The pair loop evaluates the spline in units where r'^2 = 3 q^2 (q = r / h_w,
+// coordinates scaled by sqrt(3) / h_w): with t = max(1 - q, 0),
+//   W(q) / (2 sig2) = t^3 - 4 max(t - 1/2, 0)^3 = min(t^3, 1/2 - 3 q^2 t),
+// since the difference t^3 - (1/2 - 3 q^2 t) is 4 (t - 1/2)^3, negative for
+// q > 1/2 and positive below, and both pieces vanish at t = 0 except the
+// second, which the min drops.  Per pair: r'^2 (3), sqrt, t (clamped FMA),
+// 1/2 - r'^2 t, t^3 (2), min, accumulate: 10 VALU instructions.
+constexpr float SQRT3 = 1.7320508075688772f;
+constexpr float INV_SQRT3 = 0.5773502691896258f;
 
 // Tile evaluation comes in workgroup sizes THREADS = 64, 128, 256 (chosen by
 // the host for the samples per tile); a pass stages up to CAP = 4 THREADS
@@ -190,11 +198,13 @@ __device__ __forceinline__ int tile_pieces(const BinGeom &g, const int *__restri
 // stages the candidates with raw index in [r0, r1) (r1 - r0 <= CAP) that
 // can reach the cell: cand[0 .. n) in raw order, padded with zero weights to
 // a multiple of 4; returns n (padded).  Starts and ends with a barrier.
-// DEF: records are (c'x, c'y, w, 0) with the quadratic form in cand_g (gq).
+// Records hold c' = ksc (c - o) (o the cell centre, ksc = sqrt(3) / h_w):
+// (-2 c'x, -2 c'y, |c'|^2, w).  DEF: the quadratic form of the deformed
+// blob in the same units, split over cand and cand_g (tile_eval_def).
 template <int THREADS, bool DEF = false>
 __device__ __forceinline__ int tile_stage(const BinGeom &g, const float4 *__restrict__ pts,
                                           int npieces, int r0, int r1, int cx, float ox, float oy,
-                                          float Lx, float Ly, TileLds<THREADS, DEF> &L,
+                                          float Lx, float Ly, float ksc, TileLds<THREADS, DEF> &L,
                                           const float4 *__restrict__ gq = nullptr)
 {
     const int lane = threadIdx.x & (WAVE - 1), wid = threadIdx.x / WAVE;
@@ -217,15 +227,18 @@ __device__ __forceinline__ int tile_stage(const BinGeom &g, const float4 *__rest
             const float x = (c.x + sh.x) - ox, y = (c.y + sh.y) - oy;
             const float gx = fmaxf(fabsf(x) - hx, 0.f), gy = fmaxf(fabsf(y) - hy, 0.f);
             keep = fmaf(gx, gx, gy * gy) <= g.rs2;
-            if (DEF) {   // r^2 = p'^T G p' - 2 (G c')^T p' + c'^T G c'
+            if (DEF) {   // r'^2 = p^T G' p - 2 (G' c)^T p + c^T G' c, G' = ksc^2 G
                 const float4 G = gq[src];
                 keep = keep && fmaf(gx, gx, gy * gy) <= G.w * g.rs2_1;   // own reach
-                const float g12 = 0.5f * G.y;
-                const float gcx = fmaf(G.x, x, g12 * y), gcy = fmaf(g12, x, G.z * y);
-                rec = make_float4(G.x, G.y, G.z, -2.f * gcx);
+                const float k2 = ksc * ksc;
+                const float g11 = k2 * G.x, g12x2 = k2 * G.y, g22 = k2 * G.z;
+                const float g12 = 0.5f * g12x2;
+                const float gcx = fmaf(g11, x, g12 * y), gcy = fmaf(g12, x, g22 * y);
+                rec = make_float4(g11, g12x2, g22, -2.f * gcx);
                 recg = make_float4(-2.f * gcy, fmaf(x, gcx, y * gcy), c.z, 0.f);
             } else {
-                rec = make_float4(-2.f * x, -2.f * y, fmaf(x, x, y * y), c.z);
+                const float xs = ksc * x, ys = ksc * y;
+                rec = make_float4(-2.f * xs, -2.f * ys, fmaf(xs, xs, ys * ys), c.z);
             }
         }
         const unsigned long long m = __ballot(keep);
@@ -255,29 +268,48 @@ __device__ __forceinline__ int tile_stage(const BinGeom &g, const float4 *__rest
     return npad;
 }
 
+// the spline term of one (sample, candidate) pair from r'^2 = 3 q^2 (see SQRT3)
+__device__ __forceinline__ float spline_term(float r2)
+{
+    const float s = __builtin_amdgcn_sqrtf(fabsf(r2));   // |.|: rounding below 0
+    const float t = __builtin_amdgcn_fmed3f(fmaf(s, -INV_SQRT3, 1.f), 0.f, 1.f);
+    const float in = fmaf(-r2, t, 0.5f);
+    return fminf(t * t * t, in);
+}
+
 template <int SPL, bool COUNT>
 __device__ __forceinline__ void tile_eval(const float4 *__restrict__ cand, int n,
                                           const float (&px)[SPL], const float (&py)[SPL],
-                                          const float (&pp)[SPL], float inv_hw,
-                                          float (&acc)[SPL], EvalCount &ec,
-                                          const bool (&live)[SPL])
+                                          const float (&pp)[SPL], float (&acc)[SPL],
+                                          EvalCount &ec, const bool (&live)[SPL])
 {
     auto one = [&](const float4 &c) {
 #pragma unroll
         for (int s = 0; s < SPL; ++s) {
-            float r2 = fmaf(c.x, px[s], fmaf(c.y, py[s], c.z + pp[s]));
-            float q = __builtin_amdgcn_sqrtf(fabsf(r2));   // |.|: rounding below 0
-            float t = __builtin_amdgcn_fmed3f(fmaf(-inv_hw, q, 1.f), 0.f, 1.f);
-            float v = __builtin_amdgcn_fmed3f(fmaf(t, CBRT4, -0.5f * CBRT4), 0.f, 1.f);
-            float t2 = t * t, v3 = v * v * v;
-            acc[s] = fmaf(c.w, fmaf(t2, t, -v3), acc[s]);
+            const float r2 = fmaf(c.x, px[s], fmaf(c.y, py[s], c.z + pp[s]));
+            acc[s] = fmaf(c.w, spline_term(r2), acc[s]);
             if (COUNT && live[s] && c.w > 0.f) {   // padding records have w = 0
-                ec.in_support += (t > 0.f) ? 1u : 0u;
+                ec.in_support += (r2 < 3.f) ? 1u : 0u;
                 ec.visited += 1u;
             }
         }
     };
-    for (int k = 0; k < n; k += 4) {   // n is a multiple of 4 (zero-weight padding)
+    // n is a multiple of 4 (zero-weight padding); eight records a trip keep
+    // the LDS address update to one instruction per eight pairs
+    int k = 0;
+    for (; k + 8 <= n; k += 8) {
+        const float4 c0 = cand[k], c1 = cand[k + 1], c2 = cand[k + 2], c3 = cand[k + 3];
+        const float4 c4 = cand[k + 4], c5 = cand[k + 5], c6 = cand[k + 6], c7 = cand[k + 7];
+        one(c0);
+        one(c1);
+        one(c2);
+        one(c3);
+        one(c4);
+        one(c5);
+        one(c6);
+        one(c7);
+    }
+    if (k < n) {
         const float4 c0 = cand[k], c1 = cand[k + 1], c2 = cand[k + 2], c3 = cand[k + 3];
         one(c0);
         one(c1);
@@ -286,14 +318,15 @@ __device__ __forceinline__ void tile_eval(const float4 *__restrict__ cand, int n
     }
 }
 
-// deformed records: r^2 = |F^-1 (p' - c')|^2 as five FMAs on the sample's
-// monomials (px^2, px py, py^2, px, py); records (G11, 2 G12, G22, -2 (Gc')x)
-// and (-2 (Gc')y, c'^T G c', w, 0), then the same spline
+// deformed records: r'^2 = 3 |F^-1 (p - c)|^2 / h_w^2 as five FMAs on the
+// sample's monomials (px^2, px py, py^2, px, py); records (G11, 2 G12, G22,
+// -2 (Gc)x) and (-2 (Gc)y, c^T G c, w, 0) of G = (3 / h_w^2) F^-T F^-1,
+// then the same spline
 template <int SPL, bool COUNT>
 __device__ __forceinline__ void tile_eval_def(const float4 *__restrict__ cand,
                                               const float4 *__restrict__ cand_g, int n,
                                               const float (&px)[SPL], const float (&py)[SPL],
-                                              float inv_hw, float (&acc)[SPL], EvalCount &ec,
+                                              float (&acc)[SPL], EvalCount &ec,
                                               const bool (&live)[SPL])
 {
     float xx[SPL], xy[SPL], yy[SPL];
@@ -308,13 +341,9 @@ __device__ __forceinline__ void tile_eval_def(const float4 *__restrict__ cand,
         for (int s = 0; s < SPL; ++s) {
             const float r2 = fmaf(c.x, xx[s], fmaf(c.y, xy[s], fmaf(c.z, yy[s],
                                   fmaf(c.w, px[s], fmaf(e.x, py[s], e.y)))));
-            float q = __builtin_amdgcn_sqrtf(fabsf(r2));
-            float t = __builtin_amdgcn_fmed3f(fmaf(-inv_hw, q, 1.f), 0.f, 1.f);
-            float v = __builtin_amdgcn_fmed3f(fmaf(t, CBRT4, -0.5f * CBRT4), 0.f, 1.f);
-            float t2 = t * t, v3 = v * v * v;
-            acc[s] = fmaf(e.z, fmaf(t2, t, -v3), acc[s]);
+            acc[s] = fmaf(e.z, spline_term(r2), acc[s]);
             if (COUNT && live[s] && e.z > 0.f) {
-                ec.in_support += (t > 0.f) ? 1u : 0u;
+                ec.in_support += (r2 < 3.f) ? 1u : 0u;
                 ec.visited += 1u;
             }
         }
@@ -349,9 +378,11 @@ __device__ __forceinline__ void bin_eval_tile(const McD &m, const BinGeom &g, in
                                                 m.Ly, L);
     const int npieces = 3 * (2 * g.cols + 1);
     const int npass = (total + CAP - 1) / CAP;
+    const float ksc = SQRT3 * m.inv_hw;   // pair-loop units: r'^2 = 3 q^2
     int n = 0;
     if (npass == 1)
-        n = tile_stage<THREADS, DEF>(g, pts, npieces, 0, total, cx, ox, oy, m.Lx, m.Ly, L, gq);
+        n = tile_stage<THREADS, DEF>(g, pts, npieces, 0, total, cx, ox, oy, m.Lx, m.Ly, ksc, L,
+                                     gq);
     const int lane = threadIdx.x & (WAVE - 1), wid = threadIdx.x / WAVE;
     for (int bb = beg; bb < end; bb += THREADS * SPL) {   // uniform trip count
         // a wave past the tile's last sample stops (no barrier follows when
@@ -366,8 +397,12 @@ __device__ __forceinline__ void bin_eval_tile(const McD &m, const BinGeom &g, in
             live[s] = idx < end;   // tail lanes take the cell centre, results dropped
             const SRec p = live[s] ? srec[idx] : SRec{ox, oy, 0u};
             sid[s] = p.id;
-            px[s] = p.x - ox;
+            px[s] = p.x - ox;   // deformed: the quadratic form carries ksc^2
             py[s] = p.y - oy;
+            if (!DEF) {
+                px[s] *= ksc;
+                py[s] *= ksc;
+            }
             pp[s] = fmaf(px[s], px[s], py[s] * py[s]);
             acc[s] = 0.f;
         }
@@ -375,11 +410,11 @@ __device__ __forceinline__ void bin_eval_tile(const McD &m, const BinGeom &g, in
             if (npass > 1)
                 n = tile_stage<THREADS, DEF>(g, pts, npieces, pass * CAP,
                                              min(total, (pass + 1) * CAP), cx, ox, oy, m.Lx,
-                                             m.Ly, L, gq);
+                                             m.Ly, ksc, L, gq);
             if constexpr (DEF)
-                tile_eval_def<SPL, COUNT>(L.cand, L.cand_g, n, px, py, m.inv_hw, acc, ec, live);
+                tile_eval_def<SPL, COUNT>(L.cand, L.cand_g, n, px, py, acc, ec, live);
             else
-                tile_eval<SPL, COUNT>(L.cand, n, px, py, pp, m.inv_hw, acc, ec, live);
+                tile_eval<SPL, COUNT>(L.cand, n, px, py, pp, acc, ec, live);
         }
 #pragma unroll
         for (int s = 0; s < SPL; ++s)
