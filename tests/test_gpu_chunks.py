@@ -117,6 +117,35 @@ def test_chunked_deformed_steps_are_bit_identical(device):
         assert np.array_equal(a, b)
 
 
+@pytest.mark.parametrize("threads", [64, 128, 256])
+def test_split_tail_waves_change_no_bit(device, threads):
+    """A tile's last wave with at most 32 samples spreads each sample over 2
+    or 4 lanes; the counting pass evaluates every sample on one lane.  The
+    four-class summation order makes both give the same sums: equal
+    velocities bit for bit, for every workgroup size, and the same again
+    with the owners in chunks (other tiles' tails).  N1, N2 are chosen so
+    that tiles end at many different fill levels of their last wave."""
+    _say("split tail waves, %d threads" % threads)
+    N1, N2 = 23, 13
+    sim = _sim(device, 1, Nx=56)
+    sim.eval_threads = threads
+    sim.spl = 1
+    sim.rebuild_fields()
+    sim.launch_estimate(sim.desc(N1, N2))
+    v1 = sim.vel_sign.cpu().numpy().copy()
+    sim.eval_count.zero_()
+    sim.vel_sign.zero_()
+    sim.launch_estimate(sim.desc(N1, N2, count=True))
+    assert np.array_equal(sim.vel_sign.cpu().numpy(), v1)
+    n_in, n_vis = (int(v) for v in sim.eval_count.cpu())
+    assert 0 < n_in <= n_vis
+    sim.chunk_samples = 2 * (N1 + N2) * 211
+    sim.vel_sign.zero_()
+    sim.launch_estimate(sim.desc(N1, N2))
+    assert np.array_equal(sim.vel_sign.cpu().numpy(), v1)
+    sim.chunk_samples = 0
+
+
 def test_reference_step_at_large_size_runs_in_chunks(device):
     """the CUDA step's semantics (aux targets + F) on the large run: double
     shear layer, 1,048,576 particles, hwr = 8, N1 = N2 = 256: 3.2e9 sample

@@ -58,7 +58,7 @@ __global__ void __launch_bounds__(BIN_THREADS)
 }
 
 template <int THREADS, int SPL, bool COUNT, bool DEF = false>
-__global__ void __launch_bounds__(THREADS)
+__global__ void __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(EVAL_WAVES_PER_EU<THREADS, SPL, DEF>)))
     k_bin_eval(McD m, BinGeom g, const int *__restrict__ start, const SRec *__restrict__ srec,
                float *__restrict__ om, unsigned long long *__restrict__ eval_count)
 {

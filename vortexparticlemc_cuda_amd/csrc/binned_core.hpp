@@ -277,45 +277,103 @@ __device__ __forceinline__ float spline_term(float r2)
     return fminf(t * t * t, in);
 }
 
+// Summation order of a sample's terms.  Record j of a staged pass adds to
+// class j mod 4 (a pass holds a multiple of four records: zero-weight
+// padding), each class in record order, and the sample's sum is
+// (c0 + c2) + (c1 + c3).  The same sum comes out whether one lane
+// accumulates all four classes, two lanes split them ({0, 2}, {1, 3}) or
+// four lanes take one each, since every split adds the same partial sums in
+// the same tree; a tile's last, partly filled wave therefore spreads its
+// samples over 2 or 4 lanes each (tile_eval_split) without changing a bit.
+struct Acc4 {
+    float c[4];
+};
+
+__device__ __forceinline__ float acc4_sum(const Acc4 &a)
+{
+    return (a.c[0] + a.c[2]) + (a.c[1] + a.c[3]);
+}
+
 template <int SPL, bool COUNT>
 __device__ __forceinline__ void tile_eval(const float4 *__restrict__ cand, int n,
                                           const float (&px)[SPL], const float (&py)[SPL],
-                                          const float (&pp)[SPL], float (&acc)[SPL],
+                                          const float (&pp)[SPL], Acc4 (&acc)[SPL],
                                           EvalCount &ec, const bool (&live)[SPL])
 {
-    auto one = [&](const float4 &c) {
+    auto one = [&](const float4 &c, int cls) {
 #pragma unroll
         for (int s = 0; s < SPL; ++s) {
             const float r2 = fmaf(c.x, px[s], fmaf(c.y, py[s], c.z + pp[s]));
-            acc[s] = fmaf(c.w, spline_term(r2), acc[s]);
+            acc[s].c[cls] = fmaf(c.w, spline_term(r2), acc[s].c[cls]);
             if (COUNT && live[s] && c.w > 0.f) {   // padding records have w = 0
                 ec.in_support += (r2 < 3.f) ? 1u : 0u;
                 ec.visited += 1u;
             }
         }
     };
-    // n is a multiple of 4 (zero-weight padding); eight records a trip keep
-    // the LDS address update to one instruction per eight pairs
+    // n is a multiple of 4; eight records a trip keep the LDS address update
+    // to one instruction per eight pairs
     int k = 0;
     for (; k + 8 <= n; k += 8) {
         const float4 c0 = cand[k], c1 = cand[k + 1], c2 = cand[k + 2], c3 = cand[k + 3];
         const float4 c4 = cand[k + 4], c5 = cand[k + 5], c6 = cand[k + 6], c7 = cand[k + 7];
-        one(c0);
-        one(c1);
-        one(c2);
-        one(c3);
-        one(c4);
-        one(c5);
-        one(c6);
-        one(c7);
+        one(c0, 0);
+        one(c1, 1);
+        one(c2, 2);
+        one(c3, 3);
+        one(c4, 0);
+        one(c5, 1);
+        one(c6, 2);
+        one(c7, 3);
     }
     if (k < n) {
         const float4 c0 = cand[k], c1 = cand[k + 1], c2 = cand[k + 2], c3 = cand[k + 3];
-        one(c0);
-        one(c1);
-        one(c2);
-        one(c3);
+        one(c0, 0);
+        one(c1, 1);
+        one(c2, 2);
+        one(c3, 3);
     }
+}
+
+// One sample on 2^ksh lanes (ksh = 1, 2): lane part r = lane mod 2^ksh takes
+// the records of classes r (and r + 2 when ksh = 1); the partial sums are
+// combined by tail_combine.
+__device__ __forceinline__ void tile_eval_split(const float4 *__restrict__ cand, int n, int ksh,
+                                                int r, float px, float py, float pp, Acc4 &acc)
+{
+    auto one = [&](const float4 &c, float &a) {
+        const float r2 = fmaf(c.x, px, fmaf(c.y, py, c.z + pp));
+        a = fmaf(c.w, spline_term(r2), a);
+    };
+    if (ksh == 1) {   // classes r, r + 2
+        for (int k = 0; k < n; k += 4) {
+            const float4 c0 = cand[k + r], c1 = cand[k + r + 2];
+            one(c0, acc.c[0]);
+            one(c1, acc.c[2]);
+        }
+    } else {          // class r
+        int k = 0;
+        for (; k + 8 <= n; k += 8) {
+            const float4 c0 = cand[k + r], c1 = cand[k + 4 + r];
+            one(c0, acc.c[0]);
+            one(c1, acc.c[0]);
+        }
+        if (k < n) one(cand[k + r], acc.c[0]);
+    }
+}
+
+// the sample's sum on lane part 0 of its group (every lane of the wave calls it)
+__device__ __forceinline__ float tail_combine(const Acc4 &acc, int ksh)
+{
+    if (ksh == 0) return acc4_sum(acc);
+    float e;
+    if (ksh == 1) {
+        e = acc.c[0] + acc.c[2];   // part r: classes r, r + 2
+    } else {
+        const float a = acc.c[0];   // part r: class r; pair r with r + 2
+        e = a + __shfl_xor(a, 2, WAVE);
+    }
+    return e + __shfl_xor(e, 1, WAVE);
 }
 
 // deformed records: r'^2 = 3 |F^-1 (p - c)|^2 / h_w^2 as five FMAs on the
@@ -326,7 +384,7 @@ template <int SPL, bool COUNT>
 __device__ __forceinline__ void tile_eval_def(const float4 *__restrict__ cand,
                                               const float4 *__restrict__ cand_g, int n,
                                               const float (&px)[SPL], const float (&py)[SPL],
-                                              float (&acc)[SPL], EvalCount &ec,
+                                              Acc4 (&acc)[SPL], EvalCount &ec,
                                               const bool (&live)[SPL])
 {
     float xx[SPL], xy[SPL], yy[SPL];
@@ -336,12 +394,12 @@ __device__ __forceinline__ void tile_eval_def(const float4 *__restrict__ cand,
         xy[s] = px[s] * py[s];
         yy[s] = py[s] * py[s];
     }
-    auto one = [&](const float4 &c, const float4 &e) {
+    auto one = [&](const float4 &c, const float4 &e, int cls) {
 #pragma unroll
         for (int s = 0; s < SPL; ++s) {
             const float r2 = fmaf(c.x, xx[s], fmaf(c.y, xy[s], fmaf(c.z, yy[s],
                                   fmaf(c.w, px[s], fmaf(e.x, py[s], e.y)))));
-            acc[s] = fmaf(e.z, spline_term(r2), acc[s]);
+            acc[s].c[cls] = fmaf(e.z, spline_term(r2), acc[s].c[cls]);
             if (COUNT && live[s] && e.z > 0.f) {
                 ec.in_support += (r2 < 3.f) ? 1u : 0u;
                 ec.visited += 1u;
@@ -351,10 +409,10 @@ __device__ __forceinline__ void tile_eval_def(const float4 *__restrict__ cand,
     for (int k = 0; k < n; k += 4) {   // n is a multiple of 4 (zero-weight padding)
         const float4 c0 = cand[k], e0 = cand_g[k], c1 = cand[k + 1], e1 = cand_g[k + 1];
         const float4 c2 = cand[k + 2], e2 = cand_g[k + 2], c3 = cand[k + 3], e3 = cand_g[k + 3];
-        one(c0, e0);
-        one(c1, e1);
-        one(c2, e2);
-        one(c3, e3);
+        one(c0, e0, 0);
+        one(c1, e1, 1);
+        one(c2, e2, 2);
+        one(c3, e3, 3);
     }
 }
 
@@ -388,12 +446,19 @@ __device__ __forceinline__ void bin_eval_tile(const McD &m, const BinGeom &g, in
         // a wave past the tile's last sample stops (no barrier follows when
         // the stencil is staged once), freeing its SIMD slot for other waves
         if (npass == 1 && bb + wid * SPL * WAVE >= end) break;
-        float px[SPL], py[SPL], pp[SPL], acc[SPL];
+        // the tile's last wave with at most 32 samples (one sample per lane)
+        // gives each sample 2 or 4 lanes (same sums, see Acc4); not in the
+        // counting pass or with deformed records
+        const int mw = end - (bb + wid * WAVE);
+        const int ksh = (SPL == 1 && !COUNT && !DEF && mw > 0 && mw <= 32) ? (mw <= 16 ? 2 : 1) : 0;
+        const int part = lane & ((1 << ksh) - 1);
+        float px[SPL], py[SPL], pp[SPL];
+        Acc4 acc[SPL];
         uint32_t sid[SPL];
         bool live[SPL];
 #pragma unroll
         for (int s = 0; s < SPL; ++s) {
-            const int idx = bb + (wid * SPL + s) * WAVE + lane;
+            const int idx = bb + (wid * SPL + s) * WAVE + (lane >> ksh);
             live[s] = idx < end;   // tail lanes take the cell centre, results dropped
             const SRec p = live[s] ? srec[idx] : SRec{ox, oy, 0u};
             sid[s] = p.id;
@@ -404,7 +469,8 @@ __device__ __forceinline__ void bin_eval_tile(const McD &m, const BinGeom &g, in
                 py[s] *= ksc;
             }
             pp[s] = fmaf(px[s], px[s], py[s] * py[s]);
-            acc[s] = 0.f;
+#pragma unroll
+            for (int c4 = 0; c4 < 4; ++c4) acc[s].c[c4] = 0.f;
         }
         for (int pass = 0; pass < npass; ++pass) {
             if (npass > 1)
@@ -413,14 +479,25 @@ __device__ __forceinline__ void bin_eval_tile(const McD &m, const BinGeom &g, in
                                              m.Ly, ksc, L, gq);
             if constexpr (DEF)
                 tile_eval_def<SPL, COUNT>(L.cand, L.cand_g, n, px, py, acc, ec, live);
-            else
+            else if (ksh == 0)
                 tile_eval<SPL, COUNT>(L.cand, n, px, py, pp, acc, ec, live);
+            else
+                tile_eval_split(L.cand, n, ksh, part, px[0], py[0], pp[0], acc[0]);
         }
 #pragma unroll
-        for (int s = 0; s < SPL; ++s)
-            if (live[s]) om[sid[s]] = 2.f * m.sig2 * acc[s];
+        for (int s = 0; s < SPL; ++s) {
+            const float sum = tail_combine(acc[s], ksh);
+            if (live[s] && part == 0) om[sid[s]] = 2.f * m.sig2 * sum;
+        }
     }
 }
+
+// Waves per SIMD the evaluation kernels are compiled for: 256-thread tiles
+// with one sample per lane fit 64 VGPRs, eight waves per SIMD (the LDS of
+// eight such tiles per CU); smaller tiles are held to seven by their LDS,
+// and more samples per lane take the registers they need.
+template <int THREADS, int SPL, bool DEF>
+constexpr int EVAL_WAVES_PER_EU = (THREADS == 256 && SPL == 1 && !DEF) ? 8 : 1;
 
 // host check: the stencil's columns fit a workgroup of `threads`
 inline bool tile_threads_ok(const BinGeom &g, int threads)

@@ -138,7 +138,7 @@ __global__ void __launch_bounds__(BIN_THREADS)
 }
 
 template <int THREADS, int SPL, bool DEF>
-__global__ void __launch_bounds__(THREADS)
+__global__ void __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(EVAL_WAVES_PER_EU<THREADS, SPL, DEF>)))
     k_ens_eval(EnsTabs t, int n_sim, int ntile, int per_xcd, const int *__restrict__ start,
                const SRec *__restrict__ srec, float *__restrict__ om)
 {
