@@ -50,11 +50,11 @@ struct BinBufs {
 template <int MODE, bool PER>
 __global__ void __launch_bounds__(BIN_THREADS)
     k_bin_gen(McD d, BinGeom g, uint32_t *__restrict__ key, SRec *__restrict__ rec0,
-              float2 *__restrict__ r, float *__restrict__ om)
+              float2 *__restrict__ r)
 {
     size_t sid = (size_t)blockIdx.x * BIN_THREADS + threadIdx.x;
     if (sid >= g.S) return;
-    bin_gen_one<MODE, PER>(d, g, sid, sid, 0u, (uint32_t)g.ntile, key, rec0, r, om);
+    bin_gen_one<MODE, PER>(d, g, sid, sid, 0u, (uint32_t)g.ntile, key, rec0, r);
 }
 
 template <int THREADS, int SPL, bool COUNT, bool DEF = false>
@@ -221,7 +221,7 @@ static void launch_gen_final(const McD &m, const BinGeom &g, const BinBufs &B, b
     if (!final) {
         unsigned gb = (unsigned)((g.S + BIN_THREADS - 1) / BIN_THREADS);
         hipLaunchKernelGGL((k_bin_gen<MODE, PER>), dim3(gb), dim3(BIN_THREADS), 0, s, m, g, B.key,
-                           B.rec0, B.r, B.om);
+                           B.rec0, B.r);
     } else {
         unsigned gb = (unsigned)((2 * (size_t)m.n_owner + 3) / 4);
         hipLaunchKernelGGL((k_bin_final<MODE, PER>), dim3(gb), dim3(BIN_THREADS), 0, s, m, g,

@@ -52,24 +52,25 @@ int bin_geom(const McD &m, int N1, int N2, int n_owner, BinGeom *g);
 
 // One sample: draws, query point, MIS displacement and tile key.  Writes
 // key[out] (tile_base + local tile, or dead_key without a query),
-// rec0[out] = (query point, out) (the record the sort moves), r[out] and
-// om[out] = 0.
+// rec0[out] = (query point, out) (the record the sort moves) and r[out],
+// NaN for a sample without a query: the final pass reads no vorticity for
+// it, so om needs no clearing (om[out] is written by the evaluation for
+// every live sample).
 template <int MODE, bool PER>
 __device__ __forceinline__ void bin_gen_one(const McD &d, const BinGeom &g, size_t sid,
                                             size_t out, uint32_t tile_base, uint32_t dead_key,
                                             uint32_t *__restrict__ key, SRec *__restrict__ rec0,
-                                            float2 *__restrict__ r, float *__restrict__ om)
+                                            float2 *__restrict__ r)
 {
     const uint32_t pair = div_ns(g, (uint32_t)sid);   // sid < 2^31 (bin_geom)
     const int k = (int)((uint32_t)sid - pair * g.NS);
     const int owner = (int)(pair >> 1), si = (int)(pair & 1);
     const SetD &S = d.set[si];
     const int n_s = S.count_dev ? *S.count_dev : S.n;
-    om[out] = 0.f;
     float qx = 0.f, qy = 0.f, rx = 0.f, ry = 0.f;
     bool live = n_s > 0 &&
                 gen_sample<MODE, PER>(d, S, n_s, owner, si, k, d.owner[owner], qx, qy, rx, ry);
-    r[out] = make_float2(rx, ry);
+    r[out] = live ? make_float2(rx, ry) : make_float2(__builtin_nanf(""), __builtin_nanf(""));
     rec0[out] = SRec{qx, qy, (uint32_t)out};
     if (!live) {
         key[out] = dead_key;
@@ -528,7 +529,8 @@ __device__ __forceinline__ void bin_final_pair(const McD &d, const BinGeom &g, i
         const size_t base = (size_t)pair * g.NS;
         for (uint32_t k = lane; k < g.NS; k += WAVE) {
             const float2 rr = r[base + k];
-            const float o = om[base + k];
+            const float ol = om[base + k];
+            const float o = rr.x == rr.x ? ol : 0.f;   // NaN: no query (om not written)
             if ((int)k < d.N1) {
                 mis_contrib<MODE, PER>(d, o, rr.x, rr.y, Om, cum, sx[0], sy[0]);
                 if (d.aux) {

@@ -118,7 +118,7 @@ __global__ void k_ens_update_jac(EnsTabs t, int n_sim, int N, float dt, float4 *
 template <int MODE>
 __global__ void __launch_bounds__(BIN_THREADS)
     k_ens_gen(EnsTabs t, int n_sim, uint32_t *__restrict__ key,
-              SRec *__restrict__ rec0, float2 *__restrict__ r, float *__restrict__ om)
+              SRec *__restrict__ rec0, float2 *__restrict__ r)
 {
     const int first = blockIdx.x * BIN_THREADS;
     const int s = seg_of(t.soff, n_sim, first);   // block-uniform
@@ -130,11 +130,10 @@ __global__ void __launch_bounds__(BIN_THREADS)
     if (sid >= g.S) {   // padding of the simulation's sample range
         key[out] = dk;
         rec0[out] = SRec{0.f, 0.f, (uint32_t)out};
-        r[out] = make_float2(0.f, 0.f);
-        om[out] = 0.f;
+        r[out] = make_float2(__builtin_nanf(""), __builtin_nanf(""));   // no query
         return;
     }
-    bin_gen_one<MODE, true>(d, g, sid, out, (uint32_t)t.toff[s], dk, key, rec0, r, om);
+    bin_gen_one<MODE, true>(d, g, sid, out, (uint32_t)t.toff[s], dk, key, rec0, r);
 }
 
 template <int THREADS, int SPL, bool DEF>
@@ -424,7 +423,7 @@ static int ens_estimate(const vpmc_ensemble_desc *e, EnsLayout &L, hipStream_t s
 {
     const int ns = L.n_sim;
     hipLaunchKernelGGL((k_ens_gen<MODE>), dim3((unsigned)(L.S / BIN_THREADS)), dim3(BIN_THREADS), 0,
-                       st, L.t, ns, L.skey, L.rec0, L.r, L.om);
+                       st, L.t, ns, L.skey, L.rec0, L.r);
     VPMC_CHECK_LAUNCH();
     // segmented by simulation: samples are grouped by simulation already
     VPMC_TRY(radix_sort_segmented(L.skey, (int)L.S, L.local_bits, L.rec0, L.srec,
