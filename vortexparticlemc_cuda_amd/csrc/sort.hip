@@ -330,44 +330,22 @@ __global__ void __launch_bounds__(RS_THREADS)
     }
 }
 
-// start[k] = first position with key >= k, for k in [0, nkeys]; keys sorted,
-// every key in [0, nkeys]
-__global__ void k_rs_bounds(const uint32_t *__restrict__ keys, int n, int nkeys,
-                            int *__restrict__ start)
+// start[k] = lower_bound(keys, keys + n, k) for k in [0, nkeys], one thread
+// per key: runs of empty keys cost nothing extra (a position-parallel form,
+// where the thread at a run's start fills the gap before it, writes a sign
+// set's empty region of a field, thousands of cells, from one thread)
+__global__ void k_rs_bounds_search(const uint32_t *__restrict__ keys, int n, int nkeys,
+                                   int *__restrict__ start)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i > n) return;
-    const int prev = i == 0 ? -1 : (int)keys[i - 1];
-    const int cur = i == n ? nkeys : (int)keys[i];
-    for (int k = prev + 1; k <= cur; ++k) start[k] = i;
-}
-
-// the same for positions 4t .. 4t+3 (up to n) per thread, keys read as one
-// 16-byte load (keys 16-byte aligned)
-__global__ void k_rs_bounds4(const uint32_t *__restrict__ keys, int n, int nkeys,
-                             int *__restrict__ start)
-{
-    const int i0 = 4 * (int)(blockIdx.x * blockDim.x + threadIdx.x);
-    if (i0 > n) return;
-    int prev = i0 == 0 ? -1 : (int)keys[i0 - 1];
-    int cur[4];
-    if (i0 + 4 <= n) {
-        const uint4 v = *reinterpret_cast<const uint4 *>(keys + i0);
-        cur[0] = (int)v.x;
-        cur[1] = (int)v.y;
-        cur[2] = (int)v.z;
-        cur[3] = (int)v.w;
-    } else {
-#pragma unroll
-        for (int j = 0; j < 4; ++j) cur[j] = i0 + j < n ? (int)keys[i0 + j] : nkeys;
+    const int k = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (k > nkeys) return;
+    uint32_t lo = 0, hi = (uint32_t)n;
+    while (lo < hi) {
+        const uint32_t mid = lo + ((hi - lo) >> 1);
+        if (keys[mid] < (uint32_t)k) lo = mid + 1;
+        else hi = mid;
     }
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-        const int i = i0 + j;
-        if (i > n) break;
-        for (int k = prev + 1; k <= cur[j]; ++k) start[k] = i;
-        prev = cur[j];
-    }
+    start[k] = (int)lo;
 }
 
 static int rs_nblk(int n) { return (n + RS_SUPER - 1) / RS_SUPER; }
@@ -430,12 +408,8 @@ static int radix_sort_impl(const uint32_t *keys, int n, int key_bits, const floa
         kin = kout;
         rin = rout;
     }
-    if ((reinterpret_cast<uintptr_t>(keys_sorted) & 15) == 0)
-        hipLaunchKernelGGL(k_rs_bounds4, dim3(((n >> 2) + 1 + 255) / 256), dim3(256), 0, s,
-                           (const uint32_t *)keys_sorted, n, nkeys, start);
-    else
-        hipLaunchKernelGGL(k_rs_bounds, dim3((n + 1 + 255) / 256), dim3(256), 0, s,
-                           (const uint32_t *)keys_sorted, n, nkeys, start);
+    hipLaunchKernelGGL(k_rs_bounds_search, dim3((unsigned)(((size_t)nkeys + 1 + 255) / 256)),
+                       dim3(256), 0, s, (const uint32_t *)keys_sorted, n, nkeys, start);
     VPMC_CHECK_LAUNCH();
     return VPMC_OK;
 }
