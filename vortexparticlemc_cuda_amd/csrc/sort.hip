@@ -230,10 +230,11 @@ __global__ void __launch_bounds__(RS_THREADS)
             const bool valid = e < cnt;
             kr[r] = valid ? keys_in[base + e] : kb;
             if constexpr (STG) {
-                // a lane past the round's end reads position base (cnt >= 1)
-                const uint32_t *src = reinterpret_cast<const uint32_t *>(recs_in + base + (valid ? e : 0));
-#pragma unroll
-                for (int w = 0; w < RW; ++w) rw[r][w] = valid ? src[w] : 0u;
+                // one load of the whole record (dwordx3 for 12 bytes); a lane
+                // past the round's end reads position base (cnt >= 1) and its
+                // words are never placed
+                const R v = recs_in[base + (valid ? e : 0)];
+                __builtin_memcpy(rw[r], &v, sizeof(R));
             }
         }
 #pragma unroll
