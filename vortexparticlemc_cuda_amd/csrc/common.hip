@@ -6,12 +6,19 @@ static thread_local int g_last_hip_error = 0;
 void set_hip_error(hipError_t e) { g_last_hip_error = (int)e; }
 
 // FP32 FMA issue-rate probe: 16 independent dependency chains per lane (more
-// than the VALU latency), iters rounds of 16 v_fma_f32; the result is stored
-// only when it equals an impossible value, which keeps every chain live.
+// than the VALU latency), iters rounds of 16 FMAs; the result is stored only
+// when it equals an impossible value, which keeps every chain live.  Both
+// multiplicands live in VGPRs (a lane-dependent term the compiler cannot
+// fold): with a scalar operand the FMA issues at half the rate on gfx950,
+// while the pair loops of the evaluation kernels take VGPR operands.
 constexpr int PROBE_CHAINS = 16;
 __global__ void __launch_bounds__(256) k_fma_probe(int iters, float a, float b,
                                                    float *__restrict__ out)
 {
+    const float tiny = (float)threadIdx.x * 1e-30f;
+    const float av = a + tiny * tiny, bv = b + tiny;   // per-lane registers
+    a = av;
+    b = bv;
     float x[PROBE_CHAINS];
 #pragma unroll
     for (int k = 0; k < PROBE_CHAINS; ++k) x[k] = (float)threadIdx.x * 1e-3f + (float)k;
