@@ -169,11 +169,15 @@ def _brute_periodic(qx, qy, px, py, w, hw):
     return out
 
 
-@pytest.mark.parametrize("Nx,hwr,cph", [(32, 4, 8), (64, 8, 4), (50, 16, 8), (30, 4, 2)])
+@pytest.mark.parametrize("Nx,hwr,cph", [(32, 4, 8), (64, 8, 4), (50, 16, 8), (30, 4, 2),
+                                         (64, 16, 2)])
 def test_field_query_matches_cpu(device, Nx, hwr, cph):
     """fine-grid field sum == the FP64 brute-force periodic sum, and == the
     CPU query on the reference grid (same in-support set, other summation
-    order) where that grid has >= 3 cells a side.  With fewer, the
+    order) where that grid has >= 3 cells a side; the field's cell order is
+    stable for both of the build's sorts (64 x 64, hwr 16 on an 8 x 8 grid:
+    32 particles a cell, the radix-sort build; the others, the atomic one).
+    With fewer than 3 reference cells a side, the
     reference's 3 x 3 stencil visits a cell more than once
     (src/cpu/evaluate_vorticity.cpp:22-40): at Nx=50, hwr=16 the grid is 1 x 1
     and its sum counts particles several times (9 times at the box centre);
