@@ -40,7 +40,7 @@ struct BinBufs {
     SRec *rec0;               // samples in sample order: (query point, sample id)
     float2 *r;
     float *om;
-    SRec *srec;               // cell-sorted samples: (x, y, sample id)
+    SRec *srec;               // cell-sorted samples: (offsets in the cell, sample id)
     uint32_t *keys_sorted;
     int *start;               // ntile + 1 tile offsets into srec
     void *sort_ws;

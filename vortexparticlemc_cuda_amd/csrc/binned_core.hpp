@@ -52,7 +52,8 @@ int bin_geom(const McD &m, int N1, int N2, int n_owner, BinGeom *g);
 
 // One sample: draws, query point, MIS displacement and tile key.  Writes
 // key[out] (tile_base + local tile, or dead_key without a query),
-// rec0[out] = (query point, out) (the record the sort moves) and r[out],
+// rec0[out] = (the point's offsets in its cell, out) (the record the sort
+// moves) and r[out],
 // NaN for a sample without a query: the final pass reads no vorticity for
 // it, so om needs no clearing (om[out] is written by the evaluation for
 // every live sample).
@@ -71,13 +72,14 @@ __device__ __forceinline__ void bin_gen_one(const McD &d, const BinGeom &g, size
     bool live = n_s > 0 &&
                 gen_sample<MODE, PER>(d, S, n_s, owner, si, k, d.owner[owner], qx, qy, rx, ry);
     r[out] = live ? make_float2(rx, ry) : make_float2(__builtin_nanf(""), __builtin_nanf(""));
-    rec0[out] = SRec{qx, qy, (uint32_t)out};
     if (!live) {
+        rec0[out] = SRec{0u, (uint32_t)out};
         key[out] = dead_key;
         return;
     }
     int cx = min(max((int)(qx * g.inv_cx), 0), g.nx - 1);
     int cy = min(max((int)(qy * g.inv_cy), 0), g.ny - 1);
+    rec0[out] = SRec{srec_pack(qx, qy, cx, cy, g.cw, g.ch), (uint32_t)out};
     key[out] = tile_base + (uint32_t)(si * g.ncell + cx * g.ny + cy);
 }
 
@@ -461,10 +463,10 @@ __device__ __forceinline__ void bin_eval_tile(const McD &m, const BinGeom &g, in
         for (int s = 0; s < SPL; ++s) {
             const int idx = bb + (wid * SPL + s) * WAVE + (lane >> ksh);
             live[s] = idx < end;   // tail lanes take the cell centre, results dropped
-            const SRec p = live[s] ? srec[idx] : SRec{ox, oy, 0u};
+            const SRec p = live[s] ? srec[idx] : SRec{0x80008000u, 0u};
             sid[s] = p.id;
-            px[s] = p.x - ox;   // deformed: the quadratic form carries ksc^2
-            py[s] = p.y - oy;
+            srec_offset(p.xy, g.cw, g.ch, px[s], py[s]);   // relative to the cell centre
+            // deformed: the quadratic form carries ksc^2
             if (!DEF) {
                 px[s] *= ksc;
                 py[s] *= ksc;

@@ -129,7 +129,7 @@ __global__ void __launch_bounds__(BIN_THREADS)
     const uint32_t dk = (uint32_t)(t.toff[s] + g.ntile);   // the simulation's dead tile
     if (sid >= g.S) {   // padding of the simulation's sample range
         key[out] = dk;
-        rec0[out] = SRec{0.f, 0.f, (uint32_t)out};
+        rec0[out] = SRec{0u, (uint32_t)out};
         r[out] = make_float2(__builtin_nanf(""), __builtin_nanf(""));   // no query
         return;
     }

@@ -16,7 +16,7 @@
 // 256 entries per block are written and read at a stride of nblk (one cache
 // line each), holds a quarter of the entries that one block per round would
 // need; the LDS of the scatter stays that of one round.
-// The sample records (12 bytes: point and id) move with the keys in every
+// The sample records (8 bytes: point offsets and id) move with the keys in every
 // pass; the fields' 16-byte records use the same passes.
 //
 // Segmented form: when the input is already grouped into segments (an
@@ -179,10 +179,9 @@ __device__ __forceinline__ float4 rs_rec_from<float4>(const float2 *__restrict__
     return make_float4(p.x, p.y, __uint_as_float(i), 0.f);
 }
 template <>
-__device__ __forceinline__ SRec rs_rec_from<SRec>(const float2 *__restrict__ q, uint32_t i)
+__device__ __forceinline__ SRec rs_rec_from<SRec>(const float2 *__restrict__, uint32_t i)
 {
-    const float2 p = q[i];
-    return SRec{p.x, p.y, i};
+    return SRec{0u, i};   // sample sorts always pass their records
 }
 template <>
 __device__ __forceinline__ uint32_t rs_rec_from<uint32_t>(const float2 *__restrict__, uint32_t i)
@@ -230,7 +229,7 @@ __global__ void __launch_bounds__(RS_THREADS)
             const bool valid = e < cnt;
             kr[r] = valid ? keys_in[base + e] : kb;
             if constexpr (STG) {
-                // one load of the whole record (dwordx3 for 12 bytes); a lane
+                // one load of the whole record (dwordx2 for 8 bytes); a lane
                 // past the round's end reads position base (cnt >= 1) and its
                 // words are never placed
                 const R v = recs_in[base + (valid ? e : 0)];

@@ -53,11 +53,36 @@ struct Carve {
     }
 };
 
-// a binned sample as the sort moves it: query point and sample id (12 bytes)
+// A binned sample as the sort moves it (8 bytes): the query point as 16-bit
+// offsets inside its fine cell (x in the high half) and the sample id.  The
+// cell is the sample's tile, known from its key, so the offsets carry the
+// point to c / 2^17 (c the cell side): below the FP32 spacing of coordinates
+// near 1 for cells up to 2^-7, a few ulp on the coarsest grids used.
 struct SRec {
-    float x, y;
+    uint32_t xy;
     uint32_t id;
 };
+
+constexpr float SREC_Q = 65536.f;
+
+// offsets of point (qx, qy) in cell (cx, cy) of side (cw, ch), clamped into
+// the cell (the key's cell comes from qx * inv_cx, a rounding apart)
+__host__ __device__ inline uint32_t srec_pack(float qx, float qy, int cx, int cy, float cw,
+                                              float ch)
+{
+    const float ux = (qx - (float)cx * cw) * (SREC_Q / cw);
+    const float uy = (qy - (float)cy * ch) * (SREC_Q / ch);
+    const int ix = ux < 0.f ? 0 : (ux > 65535.f ? 65535 : (int)ux);
+    const int iy = uy < 0.f ? 0 : (uy > 65535.f ? 65535 : (int)uy);
+    return ((uint32_t)ix << 16) | (uint32_t)iy;
+}
+
+// the point relative to its cell's centre: the midpoint of the offset's bin
+__host__ __device__ inline void srec_offset(uint32_t xy, float cw, float ch, float &dx, float &dy)
+{
+    dx = ((float)(xy >> 16) + (0.5f - 0.5f * SREC_Q)) * (cw * (1.f / SREC_Q));
+    dy = ((float)(xy & 0xffffu) + (0.5f - 0.5f * SREC_Q)) * (ch * (1.f / SREC_Q));
+}
 
 // ---- scans (scan.hip) -------------------------------------------------------
 constexpr int SCAN_THREADS = 256;
@@ -73,11 +98,11 @@ int scan_f64(const double *in, double *out, int n, bool inclusive, double *total
              size_t ws_bytes, hipStream_t s);
 int fill_i32(int *p, int v, size_t n, hipStream_t s);
 // stable LSD radix sort of tile keys (sort.hip)
-size_t radix_workspace_bytes(int n, size_t rec_bytes = 16);   // rec_bytes: 16 (float4), 12 (SRec)
+size_t radix_workspace_bytes(int n, size_t rec_bytes = 16);   // rec_bytes: 16 (float4), 8 (SRec), 4 (indices)
 int radix_sort_to_records(const uint32_t *keys, int n, int key_bits, const float2 *q,
                           const float4 *recs0, float4 *rec, uint32_t *keys_sorted, int nkeys,
                           int *start, void *ws, size_t ws_bytes, hipStream_t s);
-// the same with 12-byte sample records (query point, sample id)
+// the same with 8-byte sample records (point offsets, sample id)
 int radix_sort_samples(const uint32_t *keys, int n, int key_bits, const SRec *recs0, SRec *rec,
                        uint32_t *keys_sorted, int nkeys, int *start, void *ws, size_t ws_bytes,
                        hipStream_t s);
