@@ -14,10 +14,14 @@
 //            and the result is the unique stable order.
 // A block covers RS_ROUNDS rounds so that the digit-major histogram, whose
 // 256 entries per block are written and read at a stride of nblk (one cache
-// line each), holds a quarter of the entries that one block per round would
-// need; the LDS of the scatter stays that of one round.
-// The sample records (8 bytes: point offsets and id) move with the keys in every
-// pass; the fields' 16-byte records use the same passes.
+// line each), holds a third of the entries that one block per round would
+// need; the LDS of the scatter stays that of one round.  A round of 6144
+// keys (24 a thread) writes digit runs of 24 keys on average: longer runs
+// fill more of each cache line they touch (with 8-byte records, 4096-key
+// rounds took 7% longer a pass; 78 KB of LDS keeps two blocks a CU).
+// The sample records (8 bytes: point offsets and id) move with the keys in
+// every pass, staged through LDS; the fields' 16-byte records use the same
+// passes with a gather.
 //
 // Segmented form: when the input is already grouped into segments (an
 // ensemble's simulations, each owning the key range [base_s, base_s + 2^b)),
@@ -33,9 +37,9 @@
 namespace vpmc {
 
 constexpr int RS_THREADS = 256;
-constexpr int RS_ITEMS = 16;
+constexpr int RS_ITEMS = 24;
 constexpr int RS_TILE = RS_THREADS * RS_ITEMS;   // keys per round
-constexpr int RS_ROUNDS = 4;                     // rounds per block
+constexpr int RS_ROUNDS = 3;                     // rounds per block
 constexpr int RS_SUPER = RS_TILE * RS_ROUNDS;    // keys per block
 constexpr int RS_BINS = 256;                     // 8-bit digits
 constexpr int RS_WAVES = RS_THREADS / WAVE;
@@ -396,11 +400,17 @@ static int radix_sort_impl(const uint32_t *keys, int n, int key_bits, const floa
         VPMC_TRY(scan_i32(hist, offs, (int)h, false, nullptr, sws, sb, s));
         // records staged through LDS after the first pass (16-byte field
         // records keep the gather: their staging would leave one block per CU)
-        if (rin && sizeof(R) <= sizeof(SRec))
-            hipLaunchKernelGGL((k_rs_scatter<R, true>), dim3(rs_grid(nblk)), dim3(RS_THREADS), 0, s,
-                               kin, rin, q, n, 8 * p, nblk, blk, blk_base, (const int *)offs, kout,
-                               rout);
-        else
+        if constexpr (sizeof(R) <= sizeof(SRec)) {
+            if (rin) {
+                hipLaunchKernelGGL((k_rs_scatter<R, true>), dim3(rs_grid(nblk)), dim3(RS_THREADS), 0,
+                                   s, kin, rin, q, n, 8 * p, nblk, blk, blk_base,
+                                   (const int *)offs, kout, rout);
+                VPMC_CHECK_LAUNCH();
+                kin = kout;
+                rin = rout;
+                continue;
+            }
+        }
             hipLaunchKernelGGL((k_rs_scatter<R, false>), dim3(rs_grid(nblk)), dim3(RS_THREADS), 0, s,
                                kin, rin, q, n, 8 * p, nblk, blk, blk_base, (const int *)offs, kout,
                                rout);
