@@ -256,7 +256,7 @@ int vpmc_mc_estimate(const vpmc_mc_desc *d, void *stream);
 /* Same estimate, binned algorithm (periodic boxes): samples are generated,
    counting-sorted by fine cell and evaluated cell by cell, then reduced per
    owner in sample order, one owner chunk after the other (chunk_samples).
-   Needs vpmc_mc_binned_workspace_bytes(d) bytes (about 60 bytes per sample
+   Needs vpmc_mc_binned_workspace_bytes(d) bytes (about 48 bytes per sample
    slot of one chunk).  Results equal the direct algorithm's up to FP32
    summation order.  With several chunks, eval_events bracket the
    evaluations of all chunks (first evaluation start to last evaluation
