@@ -137,6 +137,38 @@ def test_rollout_adjoint_matches_fp64_finite_differences(device, traj):
     assert max(errs_x) < tol and max(errs_w) < tol
 
 
+@pytest.mark.parametrize("hwr,N1,N2", [(4, 16, 16), (6, 24, 8)])
+def test_f64_step_matches_fp32_step(device, hwr, N1, N2):
+    """one FP64 frozen-sample step (samples evaluated in cell order) against
+    the FP32 step on the same Philox stream and the same sampling tables: the
+    velocities agree to FP32 rounding, sample by sample the same estimate (a
+    sample evaluated for another sample's owner or sign would move them by
+    the velocity's own size); a second call on the same inputs repeats every
+    bit"""
+    import torch
+
+    from vortexparticlemc_cuda_amd import adjoint as A
+    sim, pos, w = _sim(device, Nx=32, hwr=hwr, N1=N1, N2=N2)
+    dt = float(sim.P.dt)
+    sim.step_index = 3
+    u32 = sim.estimate(N1, N2).double().cpu().numpy()
+    w64 = torch.tensor(w, device=device, dtype=torch.float64)
+    out = []
+    for _ in range(2):
+        X = torch.tensor(pos, device=device, dtype=torch.float64)
+        A._step_f64(sim, X, w64, 3, None)
+        torch.cuda.synchronize()
+        out.append(X.cpu().numpy())
+    assert np.array_equal(out[0], out[1])
+    d = out[0] - pos.astype(np.float64)
+    d -= np.round(d)   # the wrap into [0, 1)
+    u64 = d / dt
+    err = np.abs(u64 - u32).max() / np.abs(u32).max()
+    print("FP64 step vs FP32 step: max |du| / max |u| = %.3g" % err)
+    assert np.abs(u32).max() > 0
+    assert err < 1e-3
+
+
 def test_adjoint_is_deterministic(device):
     import torch
 

@@ -637,6 +637,7 @@ struct F64Args {
     FieldGeom geom;
     double h_w, inv_hw, sig2, L, Rc, alpha, w1, w2, inv_tot;
     double2 *contrib;           // per sample
+    double2 *y, *r;             // per sample: query point, owner - point
     Kper64 kc;
 };
 
@@ -654,7 +655,12 @@ __global__ void k_f64_recs(int n_s, const int *__restrict__ count_dev,
     set64[t] = pos[sub[t]];
 }
 
-__global__ void __launch_bounds__(ADJ_THREADS) k_f64_sample(McD d, BinGeom g, F64Args a)
+// Per sample: the draws, the point y (wrapped into the box) and the min-image
+// displacement r = x_i - y, in FP64; key = the FP32 field cell of y in its
+// sign's block of cells, or 2 nx ny for a sample without a query (empty set,
+// branch-2 offset outside the box).
+__global__ void __launch_bounds__(ADJ_THREADS) k_f64_gen(McD d, BinGeom g, F64Args a,
+                                                         uint32_t *__restrict__ key)
 {
     const size_t sid = (size_t)blockIdx.x * ADJ_THREADS + threadIdx.x;
     if (sid >= g.S) return;
@@ -663,14 +669,15 @@ __global__ void __launch_bounds__(ADJ_THREADS) k_f64_sample(McD d, BinGeom g, F6
     const int owner = (int)(pair >> 1), si = (int)(pair & 1);
     const SetD &S = d.set[si];
     const int n_s = S.count_dev ? *S.count_dev : S.n;
-    double2 out = make_double2(0.0, 0.0);
+    const FieldGeom &fg = a.geom;
+    uint32_t kk = 2u * (uint32_t)fg.nx * (uint32_t)fg.ny;
+    double yx = 0.0, yy = 0.0, rx = 0.0, ry = 0.0;
     if (n_s > 0) {
         const bool b1 = k < d.N1;
         float u[3];
         draws(d, owner, si, b1 ? 1 : 2, b1 ? k : k - d.N1, u);
         const double2 xi = a.pos[owner];
         const double L = a.L;
-        double yx, yy, rx, ry;
         bool live = true;
         if (b1) {
             const int j = guided_upper_bound(u[0], S.cdf, S.guide, S.glog2, n_s);
@@ -697,26 +704,53 @@ __global__ void __launch_bounds__(ADJ_THREADS) k_f64_sample(McD d, BinGeom g, F6
         if (live) {
             yx -= floor(yx / L) * L;
             yy -= floor(yy / L) * L;
-            double om = 0.0, om0 = 0.0;
-            auto fn = [&](const double4 &c, double dx, double dy) {
-                double w, gr;
-                spline_wg64(dx * dx + dy * dy, a.inv_hw, w, gr);
-                om += c.z * w;
-                om0 += c.w * w;
-            };
-            field_visit64(a.geom, a.cell_start[si], a.rec[si], yx, yy, fn);
-            om *= a.sig2;
-            om0 *= a.sig2;
-            const double r2 = rx * rx + ry * ry, rn = sqrt(r2);
-            if (om0 > 0.0 && rn > 0.0) {
-                const double Om0 = *S.total_dev;
-                const double D = a.w1 * om0 / Om0 + a.w2 * a.alpha / rn;
-                double ksx, ksy;
-                kper_smooth_f64(a.kc.c, KPER_TERMS_ADJ64, L, rx, ry, ksx, ksy);
-                const double f = 1.0 / (2.0 * PI_D * r2);
-                const double gk = om / D;
-                out = make_double2((-ry * f + ksx) * gk, (rx * f + ksy) * gk);
-            }
+            // the cell field_visit64 starts from
+            const int cx = min((int)((float)yx * fg.inv_cx), fg.nx - 1);
+            const int cy = min((int)((float)yy * fg.inv_cy), fg.ny - 1);
+            kk = ((uint32_t)si * (uint32_t)fg.nx + (uint32_t)cx) * (uint32_t)fg.ny + (uint32_t)cy;
+        }
+    }
+    key[sid] = kk;
+    a.y[sid] = make_double2(yx, yy);
+    a.r[sid] = make_double2(rx, ry);
+}
+
+// One thread per sample in cell order (order[t] = sample id, keys[t] its
+// key): the lanes of a wave query points of one cell and walk the same
+// records.  Each sample's sum runs in the field's cell order, so the
+// contribution does not depend on which thread computes it.
+__global__ void __launch_bounds__(ADJ_THREADS) k_f64_eval(McD d, BinGeom g, F64Args a,
+                                                          const uint32_t *__restrict__ order,
+                                                          const uint32_t *__restrict__ keys)
+{
+    const size_t t = (size_t)blockIdx.x * ADJ_THREADS + threadIdx.x;
+    if (t >= g.S) return;
+    const uint32_t sid = order[t];
+    const uint32_t cells = (uint32_t)a.geom.nx * (uint32_t)a.geom.ny;
+    const uint32_t kk = keys[t];
+    double2 out = make_double2(0.0, 0.0);
+    if (kk < 2u * cells) {
+        const int si = kk >= cells ? 1 : 0;
+        const double2 y = a.y[sid], r = a.r[sid];
+        double om = 0.0, om0 = 0.0;
+        auto fn = [&](const double4 &c, double dx, double dy) {
+            double w, gr;
+            spline_wg64(dx * dx + dy * dy, a.inv_hw, w, gr);
+            om += c.z * w;
+            om0 += c.w * w;
+        };
+        field_visit64(a.geom, a.cell_start[si], a.rec[si], y.x, y.y, fn);
+        om *= a.sig2;
+        om0 *= a.sig2;
+        const double r2 = r.x * r.x + r.y * r.y, rn = sqrt(r2);
+        if (om0 > 0.0 && rn > 0.0) {
+            const double Om0 = *d.set[si].total_dev;
+            const double D = a.w1 * om0 / Om0 + a.w2 * a.alpha / rn;
+            double ksx, ksy;
+            kper_smooth_f64(a.kc.c, KPER_TERMS_ADJ64, a.L, r.x, r.y, ksx, ksy);
+            const double f = 1.0 / (2.0 * PI_D * r2);
+            const double gk = om / D;
+            out = make_double2((-r.y * f + ksx) * gk, (r.x * f + ksy) * gk);
         }
     }
     a.contrib[sid] = out;
@@ -912,9 +946,13 @@ size_t vpmc_step_f64_workspace_bytes(const vpmc_mc_desc *d)
     const size_t n = (size_t)(d->n_owner > 0 ? d->n_owner : 1);
     const int mx = d->set[0].n > d->set[1].n ? d->set[0].n : d->set[1].n;
     const size_t nm = (size_t)(mx > 0 ? mx : 1);
+    const size_t S = (size_t)(g.S > 0 ? g.S : 1);
+    const size_t cells = (size_t)d->set[0].field.nx * (size_t)d->set[0].field.ny;
+    const size_t fb = vpmc_field_workspace_bytes(mx, d->set[0].field.nx, d->set[0].field.ny);
+    const size_t rb = radix_workspace_bytes((int)S, 4);
     return align256(n * 8) + 2 * align256(nm * 32) + 2 * align256(nm * 16) +
-           align256((g.S > 0 ? g.S : 1) * 16) + align256(2 * n * 16) +
-           vpmc_field_workspace_bytes(mx, d->set[0].field.nx, d->set[0].field.ny) + 4096;
+           3 * align256(S * 16) + 3 * align256(S * 4) + align256((2 * cells + 2) * 4) +
+           align256(2 * n * 16) + (fb > rb ? fb : rb) + 4096;
 }
 
 int vpmc_step_f64(vpmc_mc_desc *d, int n_total, double *pos_xy, const double *w,
@@ -938,12 +976,24 @@ int vpmc_step_f64(vpmc_mc_desc *d, int n_total, double *pos_xy, const double *w,
     float2 *pos32 = cv.take<float2>((size_t)n_total);
     double4 *rec[2] = {cv.take<double4>(nm), cv.take<double4>(nm)};
     double2 *set64[2] = {cv.take<double2>(nm), cv.take<double2>(nm)};
-    double2 *contrib = cv.take<double2>(g.S > 0 ? g.S : 1);
+    const size_t S = (size_t)(g.S > 0 ? g.S : 1);
+    double2 *contrib = cv.take<double2>(S);
+    double2 *ys = cv.take<double2>(S);
+    double2 *rs = cv.take<double2>(S);
+    uint32_t *key = cv.take<uint32_t>(S);
+    uint32_t *order = cv.take<uint32_t>(S);
+    uint32_t *key_sorted = cv.take<uint32_t>(S);
+    const int fnx = d->set[0].field.nx, fny = d->set[0].field.ny;
+    const size_t cells = (size_t)fnx * (size_t)fny;
+    int *cstart = cv.take<int>(2 * cells + 2);
     double2 *vel = cv.take<double2>(2 * (size_t)n_total);
-    const size_t fb = vpmc_field_workspace_bytes(mx, d->set[0].field.nx, d->set[0].field.ny);
-    void *fws = cv.take<char>(fb);
-    if (!pos32 || !rec[0] || !rec[1] || !set64[0] || !set64[1] || !contrib || !vel || !fws)
+    const size_t fb = vpmc_field_workspace_bytes(mx, fnx, fny);
+    const size_t rb = radix_workspace_bytes((int)S, 4);
+    void *fws = cv.take<char>(fb > rb ? fb : rb);   // the field builds, then the sort
+    if (!pos32 || !rec[0] || !rec[1] || !set64[0] || !set64[1] || !contrib || !ys || !rs ||
+        !key || !order || !key_sorted || !cstart || !vel || !fws)
         return VPMC_ERR_WORKSPACE;
+    if (2 * cells + 1 > 0x7fffffffu) return VPMC_ERR_ARG;
     const unsigned gn = (unsigned)((n_total + 255) / 256);
     hipLaunchKernelGGL(k_d2f_xy, dim3(gn), dim3(256), 0, s, n_total, (const double2 *)pos_xy,
                        pos32);
@@ -982,10 +1032,19 @@ int vpmc_step_f64(vpmc_mc_desc *d, int n_total, double *pos_xy, const double *w,
     a.w2 = (double)d->N2 / (double)(d->N1 + d->N2);
     a.inv_tot = 1.0 / (double)(d->N1 + d->N2);
     a.contrib = contrib;
+    a.y = ys;
+    a.r = rs;
     const double *kc = kper_coeffs_f64_host(KPER_TERMS_ADJ64);
     for (int i = 0; i < KPER_TERMS_ADJ64; ++i) a.kc.c[i] = kc[i];
     const unsigned gs = (unsigned)((g.S + ADJ_THREADS - 1) / ADJ_THREADS);
-    hipLaunchKernelGGL(k_f64_sample, dim3(gs), dim3(ADJ_THREADS), 0, s, m, g, a);
+    hipLaunchKernelGGL(k_f64_gen, dim3(gs), dim3(ADJ_THREADS), 0, s, m, g, a, key);
+    VPMC_CHECK_LAUNCH();
+    const uint32_t dead = 2u * (uint32_t)cells;
+    VPMC_TRY(radix_sort_indices(key, (int)g.S, key_bits_for(dead), order, key_sorted,
+                                (int)dead, cstart, fws, fb > rb ? fb : rb, s));
+    hipLaunchKernelGGL(k_f64_eval, dim3(gs), dim3(ADJ_THREADS), 0, s, m, g, a,
+                       (const uint32_t *)order, (const uint32_t *)key_sorted);
+    VPMC_CHECK_LAUNCH();
     hipLaunchKernelGGL(k_f64_reduce, dim3((unsigned)((2 * (size_t)n_total + 255) / 256)),
                        dim3(256), 0, s, n_total, g.NS, a.inv_tot, (const double2 *)contrib, vel);
     hipLaunchKernelGGL(k_f64_advect, dim3(gn), dim3(256), 0, s, n_total, (double2 *)pos_xy,

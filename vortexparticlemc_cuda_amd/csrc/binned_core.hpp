@@ -529,9 +529,20 @@ __device__ __forceinline__ void bin_final_pair(const McD &d, const BinGeom &g, i
         float cum = Om;
         if (MODE == VPMC_MODE_REFERENCE && PER) cum *= 9.f;
         const size_t base = (size_t)pair * g.NS;
+        // the next sample's loads are issued before this sample's arithmetic
+        float2 rn = make_float2(0.f, 0.f);
+        float on = 0.f;
+        if ((uint32_t)lane < g.NS) {
+            rn = r[base + lane];
+            on = om[base + lane];
+        }
         for (uint32_t k = lane; k < g.NS; k += WAVE) {
-            const float2 rr = r[base + k];
-            const float ol = om[base + k];
+            const float2 rr = rn;
+            const float ol = on;
+            if (k + WAVE < g.NS) {
+                rn = r[base + k + WAVE];
+                on = om[base + k + WAVE];
+            }
             const float o = rr.x == rr.x ? ol : 0.f;   // NaN: no query (om not written)
             if ((int)k < d.N1) {
                 mis_contrib<MODE, PER>(d, o, rr.x, rr.y, Om, cum, sx[0], sy[0]);
