@@ -404,6 +404,30 @@ size_t vpmc_step_f64_workspace_bytes(const vpmc_mc_desc *d);
 int vpmc_step_f64(vpmc_mc_desc *d, int n_total, double *pos_xy, const double *w,
                   const float *omega0, const int *sub_pos, const int *sub_neg, double dt,
                   void *workspace, size_t workspace_bytes, void *stream);
+/* The deformed step (step_cuda with kernel deformation: the four auxiliary
+   targets and F, src/cuda/simulation_step.cu:51-73, src/cuda/monte_carlo_bs.cu:
+   166-169, 210-212, 274-288; include/vorticity_query.cuh:52-66) backwards:
+   d->jac holds the F the forward step started from (read only), d->aux = 1
+   with d->vel_aux, and the sets carry field.gq / set_jac buffers (rebuilt
+   here).  The derivative includes F's: a particle's blob is W(|F^-1 d|) in
+   every query and its branch-1 jitter is F u (u frozen), and the step's
+   F' = renorm(I + dt/(2 dx) [u_R - u_L | u_T - u_B]) follows the four aux
+   estimates.  lambda: dL/dx' -> dL/dx; jac_bar (n float4): dL/dF' -> dL/dF;
+   grad_w accumulates dL/dw.  Runs the step's forward estimate first (into
+   d->vel_sign, d->vel_aux). */
+size_t vpmc_step_adjoint_deformed_workspace_bytes(const vpmc_mc_desc *d);
+int vpmc_step_adjoint_deformed(vpmc_mc_desc *d, int n_total, const float *pos_xy,
+                               const float *omega, const int *sub_pos, const int *sub_neg,
+                               float dt, float *lambda, float *jac_bar, float *grad_w,
+                               void *workspace, size_t workspace_bytes, void *stream);
+/* FP64 forward of the deformed frozen-sample step (checks): jac64 (n double4,
+   F per particle) is replaced by this step's F' before the move; d->aux = 1
+   and the sets' field.gq pointers set (they give the fields' 2 h_w search
+   radius; their contents are not read).  Workspace: vpmc_step_f64_workspace_bytes. */
+int vpmc_step_f64_deformed(vpmc_mc_desc *d, int n_total, double *pos_xy, const double *w,
+                           const float *omega0, const int *sub_pos, const int *sub_neg,
+                           double dt, double *jac64, void *workspace, size_t workspace_bytes,
+                           void *stream);
 
 /* ---------------------------------------------------------------------------
  * Vorticity sampled by the B-spline kernel on a grid (no single reference

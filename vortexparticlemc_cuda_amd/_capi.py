@@ -150,6 +150,11 @@ def lib():
     L.vpmc_step_f64_workspace_bytes.restype = csz
     L.vpmc_step_f64_workspace_bytes.argtypes = [mp]
     L.vpmc_step_f64.argtypes = [mp, ci, vp, vp, vp, vp, vp, cd, vp, csz, vp]
+    L.vpmc_step_adjoint_deformed_workspace_bytes.restype = csz
+    L.vpmc_step_adjoint_deformed_workspace_bytes.argtypes = [mp]
+    L.vpmc_step_adjoint_deformed.argtypes = [mp, ci, vp, vp, vp, vp, cf, vp, vp, vp, vp, csz,
+                                             vp]
+    L.vpmc_step_f64_deformed.argtypes = [mp, ci, vp, vp, vp, vp, vp, cd, vp, vp, csz, vp]
     L.vpmc_grid_vorticity_workspace_bytes.restype = csz
     L.vpmc_grid_vorticity_workspace_bytes.argtypes = [ci, ci, ci]
     L.vpmc_grid_vorticity.argtypes = [ci, vp, vp, cf, cf, cf, ci, ci, ci, vp, vp, csz, vp]
@@ -181,6 +186,7 @@ EXPORTS = [
     "vpmc_ensemble_set_step", "vpmc_ensemble_graph_create", "vpmc_graph_launch",
     "vpmc_graph_destroy", "vpmc_mc_adjoint_workspace_bytes", "vpmc_mc_adjoint",
     "vpmc_step_adjoint_workspace_bytes", "vpmc_step_adjoint", "vpmc_step_f64_workspace_bytes",
-    "vpmc_step_f64", "vpmc_grid_vorticity_workspace_bytes", "vpmc_grid_vorticity",
+    "vpmc_step_f64", "vpmc_step_adjoint_deformed_workspace_bytes", "vpmc_step_adjoint_deformed",
+    "vpmc_step_f64_deformed", "vpmc_grid_vorticity_workspace_bytes", "vpmc_grid_vorticity",
     "vpmc_grid_vorticity_adjoint", "vpmc_grid_vorticity_f64",
 ]

@@ -20,6 +20,10 @@ checkpoint states, not with steps.
 
 rollout_f64 / grid_vorticity_f64 evaluate the same frozen-sample functions in
 FP64 (vpmc_step_f64, vpmc_grid_vorticity_f64), for finite-difference checks.
+
+A Simulation(deform=True) differentiates the deformed step (aux targets and
+F, vpmc_step_adjoint_deformed): F is part of the state, the rollout starts
+from sim's current F, and the checkpoints keep it with the positions.
 """
 import ctypes as C
 
@@ -41,8 +45,10 @@ def _set_nominal(sim, w):
         sim._alloc()
 
 
-def _adj_ws(sim, d):
-    need = lib().vpmc_step_adjoint_workspace_bytes(C.byref(d))
+def _adj_ws(sim, d, deformed=False):
+    fn = (lib().vpmc_step_adjoint_deformed_workspace_bytes if deformed
+          else lib().vpmc_step_adjoint_workspace_bytes)
+    need = fn(C.byref(d))
     if need == 0:
         raise ValueError("the adjoint needs the consistent, periodic, Philox-driven estimator")
     if getattr(sim, "adj_ws", None) is None or sim.adj_ws.numel() < need:
@@ -86,39 +92,105 @@ def mc_velocity(sim, pos, w):
     return _McVelocity.apply(pos, w, sim)
 
 
+def _state(sim):
+    """the FP32 state a step starts from: positions, and F when deformed"""
+    return (sim.pos.clone(), sim.jac.clone() if sim.deform else None)
+
+
+def _set_state(sim, st):
+    sim.pos.copy_(st[0])
+    if sim.deform:
+        sim.jac.copy_(st[1])
+
+
 def _run(sim, t0, t1, keep=None):
-    """advance sim.pos from step t0 to t1 (forward steps); keep: list that
+    """advance sim's state from step t0 to t1 (forward steps); keep: list that
     receives the state before each step"""
     for t in range(t0, t1):
         if keep is not None:
-            keep.append(sim.pos.clone())
+            keep.append(_state(sim))
         sim.step_index = t
         sim.step()
     sim.step_index = t1
 
 
-def _step_f64(sim, X, w64, t, ws):
-    """one frozen-sample step of the FP64 positions X in place (vpmc_step_f64)"""
+def _step_f64(sim, X, w64, t, ws, F64=None):
+    """one frozen-sample step of the FP64 positions X in place (vpmc_step_f64);
+    a deformed simulation also advances its FP64 F (F64, (N, 4)) in place
+    (vpmc_step_f64_deformed)"""
     P = sim.P
-    d = sim.desc(P.N1, P.N2)
+    d = sim.desc(P.N1, P.N2, aux=sim.deform)
     d.step = t
     need = lib().vpmc_step_f64_workspace_bytes(C.byref(d))
     if ws is None or ws.numel() < need:
         ws = ops.workspace(need, sim.device)
-    check(lib().vpmc_step_f64(C.byref(d), sim.N, ops._p(X), ops._p(w64), ops._p(sim.omega),
-                              ops._p(sim.sub["sub_pos"]), ops._p(sim.sub["sub_neg"]),
-                              float(P.dt), ops._p(ws), ws.numel(), ops._stream()),
-          "vpmc_step_f64")
+    args = (C.byref(d), sim.N, ops._p(X), ops._p(w64), ops._p(sim.omega),
+            ops._p(sim.sub["sub_pos"]), ops._p(sim.sub["sub_neg"]), float(P.dt))
+    if sim.deform:
+        check(lib().vpmc_step_f64_deformed(*args, ops._p(F64), ops._p(ws), ws.numel(),
+                                           ops._stream()), "vpmc_step_f64_deformed")
+    else:
+        check(lib().vpmc_step_f64(*args, ops._p(ws), ws.numel(), ops._stream()),
+              "vpmc_step_f64")
     return ws
 
 
-def _run_f64(sim, X, w64, t0, t1, keep=None):
+def _run_f64(sim, X, w64, t0, t1, keep=None, F64=None):
     ws = None
     for t in range(t0, t1):
         if keep is not None:
-            keep.append(X.clone())
-        ws = _step_f64(sim, X, w64, t, ws)
+            keep.append((X.clone(), F64.clone() if F64 is not None else None))
+        ws = _step_f64(sim, X, w64, t, ws, F64)
     sim.step_index = t1
+
+
+def _step_adjoint(sim, st, t, lam, gw, jbar=None):
+    """one step backwards at state st = (positions, F) of step t: lam (dL/dx')
+    becomes dL/dx in place, gw accumulates dL/dw; a deformed simulation also
+    turns jbar (dL/dF', (N, 4)) into dL/dF in place"""
+    P = sim.P
+    _set_state(sim, st)
+    d = sim.desc(P.N1, P.N2, aux=sim.deform)
+    d.step = t
+    ws = _adj_ws(sim, d, sim.deform)
+    args = (C.byref(d), sim.N, ops._p(sim.pos), ops._p(sim.omega), ops._p(sim.sub["sub_pos"]),
+            ops._p(sim.sub["sub_neg"]), float(P.dt), ops._p(lam))
+    if sim.deform:
+        check(lib().vpmc_step_adjoint_deformed(*args, ops._p(jbar), ops._p(gw), ops._p(ws),
+                                               ws.numel(), ops._stream()),
+              "vpmc_step_adjoint_deformed")
+    else:
+        check(lib().vpmc_step_adjoint(*args, ops._p(gw), ops._p(ws), ws.numel(),
+                                      ops._stream()), "vpmc_step_adjoint")
+
+
+def step_vjp(sim, pos, F, t, lam, jbar=None):
+    """the vector-Jacobian product of step t at positions pos (float32 (N, 2))
+    and, for a deformed simulation, F (float32 (N, 4)), with sim's strengths
+    as the nominal ones: returns (dL/dx, dL/dw, dL/dF) for the cotangents
+    lam = dL/dx' and jbar = dL/dF' (dL/dF None when undeformed)"""
+    lam = lam.to(torch.float32).contiguous().clone()
+    gw = torch.zeros(sim.N, dtype=torch.float32, device=lam.device)
+    if sim.deform:
+        jbar = (torch.zeros((sim.N, 4), dtype=torch.float32, device=lam.device) if jbar is None
+                else jbar.to(torch.float32).contiguous().clone())
+        st = (pos.to(torch.float32), F.to(torch.float32))
+    else:
+        jbar, st = None, (pos.to(torch.float32), None)
+    _step_adjoint(sim, st, int(t), lam, gw, jbar)
+    return lam, gw, jbar
+
+
+def step_f64(sim, X, w64, t, F64=None):
+    """FP64 frozen-sample step t from positions X (float64 (N, 2)) with
+    strengths w64: returns the new positions and, for a deformed simulation,
+    the new F from F64 (float64 (N, 4)); the sampling tables and MIS density
+    stay those of sim's nominal strengths"""
+    X = X.to(torch.float64).contiguous().clone()
+    w64 = w64.to(torch.float64).contiguous()
+    F = F64.to(torch.float64).contiguous().clone() if sim.deform else None
+    _step_f64(sim, X, w64, int(t), None, F)
+    return X, F
 
 
 class _Rollout(torch.autograd.Function):
@@ -127,18 +199,22 @@ class _Rollout(torch.autograd.Function):
         _set_nominal(sim, w)
         K = max(1, int(checkpoint))
         ckpt = {}
+        F0 = sim.jac[:sim.N].clone() if sim.deform else None   # F the rollout starts from
         if traj64:
             X = pos0.detach().to(torch.float64).contiguous().clone()
             w64 = w.detach().to(torch.float64).contiguous()
+            F64 = F0.to(torch.float64).contiguous() if sim.deform else None
             for t0 in range(0, steps, K):
-                ckpt[t0] = X.clone()
-                _run_f64(sim, X, w64, t0, min(steps, t0 + K))
+                ckpt[t0] = (X.clone(), F64.clone() if F64 is not None else None)
+                _run_f64(sim, X, w64, t0, min(steps, t0 + K), F64=F64)
             ctx.w64 = w64
             out = X
+            if sim.deform:
+                sim.jac[:sim.N].copy_(F64)
         else:
             sim.pos.copy_(pos0.detach())
             for t0 in range(0, steps, K):
-                ckpt[t0] = sim.pos.clone()
+                ckpt[t0] = _state(sim)
                 _run(sim, t0, min(steps, t0 + K))
             out = sim.pos.clone()
         ctx.sim, ctx.steps, ctx.K, ctx.ckpt, ctx.traj64 = sim, steps, K, ckpt, traj64
@@ -150,26 +226,21 @@ class _Rollout(torch.autograd.Function):
         sim, steps, K = ctx.sim, ctx.steps, ctx.K
         lam = gX.to(torch.float32).contiguous().clone()
         gw = torch.zeros(sim.N, dtype=torch.float32, device=lam.device)
-        P = sim.P
+        jbar = (torch.zeros((sim.N, 4), dtype=torch.float32, device=lam.device)
+                if sim.deform else None)   # the final F does not enter the loss
         for t0 in reversed(range(0, steps, K)):
             t1 = min(steps, t0 + K)
             states = []
             if ctx.traj64:   # recompute in FP64, linearise at the rounded states
-                X = ctx.ckpt[t0].clone()
-                _run_f64(sim, X, ctx.w64, t0, t1, keep=states)
+                X, F64 = (v.clone() if v is not None else None for v in ctx.ckpt[t0])
+                _run_f64(sim, X, ctx.w64, t0, t1, keep=states, F64=F64)
+                states = [(x.to(torch.float32), f.to(torch.float32) if f is not None else None)
+                          for x, f in states]
             else:
-                sim.pos.copy_(ctx.ckpt[t0])
+                _set_state(sim, ctx.ckpt[t0])
                 _run(sim, t0, t1, keep=states)
             for t in reversed(range(t0, t1)):
-                sim.pos.copy_(states[t - t0])
-                d = sim.desc(P.N1, P.N2)
-                d.step = t
-                ws = _adj_ws(sim, d)
-                check(lib().vpmc_step_adjoint(C.byref(d), sim.N, ops._p(sim.pos),
-                                              ops._p(sim.omega), ops._p(sim.sub["sub_pos"]),
-                                              ops._p(sim.sub["sub_neg"]), float(P.dt),
-                                              ops._p(lam), ops._p(gw), ops._p(ws), ws.numel(),
-                                              ops._stream()), "vpmc_step_adjoint")
+                _step_adjoint(sim, states[t - t0], t, lam, gw, jbar)
         sim.step_index = steps
         return lam.to(ctx.in_dtype), gw, None, None, None, None
 
@@ -236,24 +307,19 @@ def grid_vorticity(pos, w, h_w, gnx, gny, L=1.0, cells_per_hw=8, f64=False):
 
 # ---- FP64 forwards of the same functions (finite-difference checks) ------------
 
-def rollout_f64(sim, pos0, w, steps):
+def rollout_f64(sim, pos0, w, steps, F0=None):
     """FP64 positions after `steps` frozen-sample steps from pos0 (float64,
     (N, 2)) with strengths w (float64); the sampling tables and the MIS
-    density stay those of sim's nominal strengths"""
+    density stay those of sim's nominal strengths.  A deformed simulation
+    starts from F0 ((N, 4); default: sim's current F) and advances F in FP64."""
     X = pos0.to(torch.float64).contiguous().clone()
     w = w.to(torch.float64).contiguous()
-    P = sim.P
+    F64 = None
+    if sim.deform:
+        F64 = (sim.jac[:sim.N] if F0 is None else F0).to(torch.float64).contiguous().clone()
     ws = None
     for t in range(steps):
-        d = sim.desc(P.N1, P.N2)
-        d.step = t
-        need = lib().vpmc_step_f64_workspace_bytes(C.byref(d))
-        if ws is None or ws.numel() < need:
-            ws = ops.workspace(need, sim.device)
-        check(lib().vpmc_step_f64(C.byref(d), sim.N, ops._p(X), ops._p(w), ops._p(sim.omega),
-                                  ops._p(sim.sub["sub_pos"]), ops._p(sim.sub["sub_neg"]),
-                                  float(P.dt), ops._p(ws), ws.numel(), ops._stream()),
-              "vpmc_step_f64")
+        ws = _step_f64(sim, X, w, t, ws, F64)
     return X
 
 

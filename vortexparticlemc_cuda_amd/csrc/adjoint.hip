@@ -83,8 +83,8 @@ __device__ __forceinline__ void spline_wg64(double r2, double inv_hw, double &w,
 }
 
 // Visits every record of field f within the search radius of p (periodic
-// images by column / row wrap, as field_sum): fn(record, dx, dy) with
-// (dx, dy) = record position - p in the image frame.
+// images by column / row wrap, as field_sum): fn(record, dx, dy, k) with
+// (dx, dy) = record position - p in the image frame and k the record's index.
 template <class F>
 __device__ __forceinline__ void field_visit(const FieldD &f, float px, float py, F &fn)
 {
@@ -109,7 +109,7 @@ __device__ __forceinline__ void field_visit(const FieldD &f, float px, float py,
         auto run = [&](int b, int e, float sx, float sy) {
             for (int k = b; k < e; ++k) {
                 const float4 c = f.pts[k];
-                fn(c, c.x - sx, c.y - sy);
+                fn(c, c.x - sx, c.y - sy, k);
             }
         };
         if (r0 < 0) { run(cs[g.ny + r0], cs[g.ny], qx, py + g.Ly); r0 = 0; }
@@ -146,7 +146,7 @@ __device__ __forceinline__ void field_visit64(const FieldGeom &g, const int *__r
         auto run = [&](int b, int e, double sx, double sy) {
             for (int k = b; k < e; ++k) {
                 const double4 c = recs[k];
-                fn(c, c.x - sx, c.y - sy);
+                fn(c, c.x - sx, c.y - sy, k);
             }
         };
         if (r0 < 0) { run(cs[g.ny + r0], cs[g.ny], qx, py + (double)g.Ly); r0 = 0; }
@@ -234,7 +234,7 @@ __device__ __forceinline__ double mass_inverse_f64(double u)
 __global__ void __launch_bounds__(BIN_THREADS)
     k_adj_gen(McD d, BinGeom g, const int *__restrict__ sub0, const int *__restrict__ sub1,
               uint32_t *__restrict__ key, float2 *__restrict__ q, float2 *__restrict__ r,
-              int *__restrict__ src)
+              int *__restrict__ src, float2 *__restrict__ jit)
 {
     const size_t sid = (size_t)blockIdx.x * BIN_THREADS + threadIdx.x;
     if (sid >= g.S) return;
@@ -245,8 +245,11 @@ __global__ void __launch_bounds__(BIN_THREADS)
     const int n_s = S.count_dev ? *S.count_dev : S.n;
     float qx = 0.f, qy = 0.f, rx = 0.f, ry = 0.f;
     int j = -1;
+    float2 ju = make_float2(0.f, 0.f);
     const bool live = n_s > 0 && gen_sample<VPMC_MODE_CONSISTENT, true>(
-                                     d, S, n_s, owner, si, k, d.owner[owner], qx, qy, rx, ry, &j);
+                                     d, S, n_s, owner, si, k, d.owner[owner], qx, qy, rx, ry, &j,
+                                     jit ? &ju : nullptr);
+    if (jit) jit[sid] = ju;
     r[sid] = make_float2(rx, ry);
     q[sid] = make_float2(qx, qy);
     const int *sub = si ? sub1 : sub0;
@@ -319,11 +322,80 @@ __global__ void __launch_bounds__(ADJ_THREADS)
     }
 }
 
-// one wave per (owner, sign)
+// Deformed fields: om(y) and grad om(y) per live sample, one thread per
+// sample in tile order (the lanes of a wave query points of one cell and
+// walk the same records); record k's kernel is W(|F^-1 d|) through its
+// quadratic form G (field.gq), so grad_y W = W'(q)/(q h_w^2) G (y - x).
+__global__ void __launch_bounds__(ADJ_THREADS)
+    k_adj_eval_def(McD m, BinGeom g, const int *__restrict__ start,
+                   const float4 *__restrict__ srec, const uint32_t *__restrict__ keys_sorted,
+                   float *__restrict__ om, float2 *__restrict__ G)
+{
+    const int p = blockIdx.x * ADJ_THREADS + threadIdx.x;
+    if (p >= start[g.ntile]) return;   // dead samples sort last
+    const float4 sr = srec[p];
+    const int si = keys_sorted[p] >= (uint32_t)g.ncell ? 1 : 0;
+    const FieldD &f = m.set[si].field;
+    float acc = 0.f, gx = 0.f, gy = 0.f;
+    auto fn = [&](const float4 &c, float dx, float dy, int k) {   // (dx, dy) = x - y
+        const float4 Q = f.gq[k];   // (G11, 2 G12, G22, sigma_max^2)
+        const float r2 = fmaxf(fmaf(dx, fmaf(Q.y, dy, Q.x * dx), Q.z * dy * dy), 0.f);
+        float w, gr;
+        spline_wg(r2, m.inv_hw, w, gr);
+        acc = fmaf(c.z, w, acc);
+        const float hx = fmaf(0.5f * Q.y, dy, Q.x * dx), hy = fmaf(0.5f * Q.y, dx, Q.z * dy);
+        gx = fmaf(-c.z * gr, hx, gx);   // G (y - x) = -(hx, hy)
+        gy = fmaf(-c.z * gr, hy, gy);
+    };
+    field_visit(f, sr.x, sr.y, fn);
+    const uint32_t sid = __float_as_uint(sr.z);
+    om[sid] = m.sig2 * acc;
+    G[sid] = make_float2(m.sig2 * gx, m.sig2 * gy);
+}
+
+// One (sample, target) term: (ax, ay) the target's adjoint weight (sign and
+// 1/(N1+N2) applied), o = om(y), (rx, ry) = target - y.  Adds ds/d om to b
+// and ds/d om + ds/d om0 to bt, and returns ds/dr in (drx, dry); a term the
+// forward does not add (om <= 0, r = 0) adds nothing.
+__device__ __forceinline__ void adj_term(const McD &d, float ax, float ay, float o, float Om,
+                                         float rx, float ry, float &b, float &bt, float &drx,
+                                         float &dry)
+{
+    drx = dry = 0.f;
+    const float r2 = rx * rx + ry * ry, rn = sqrtf(r2);
+    if (!(o > 0.f && rn > 0.f)) return;
+    const float f = 1.f / (2.f * PI_F * r2);
+    float ksx, ksy;
+    kper_smooth(d, rx, ry, ksx, ksy);
+    const float Kx = -ry * f + ksx, Ky = rx * f + ksy;
+    const float aK = ax * Kx + ay * Ky;
+    const float D = d.w1 * o / Om + d.w2 * d.alpha_c / rn;
+    const float iD = 1.f / D;
+    const float bb = aK * iD;
+    b += bb;
+    bt += bb - aK * o * d.w1 * iD * iD / Om;
+    // grad_r (a . K): free part, then the smooth part
+    const float P = -ax * ry + ay * rx;
+    const float gfx = (ay - 2.f * P * rx / r2) * f;
+    const float gfy = (-ax - 2.f * P * ry / r2) * f;
+    float gsx, gsy;
+    kper_smooth_grad(d, rx, ry, ax, ay, gsx, gsy);
+    const float c1 = o * iD, c2 = aK * o * d.w2 * d.alpha_c * iD * iD / (r2 * rn);
+    drx = c1 * (gfx + gsx) + c2 * rx;
+    dry = c1 * (gfy + gsy) + c2 * ry;
+}
+
+// one wave per (owner, sign).  With auxiliary targets (d.aux), a branch-1
+// sample adds its term at each of the five targets (displacement r + offset,
+// min image, as the forward's final pass) with that target's adjoint weight
+// adj_aux[(t-1) n + owner], and a disk sample of target t at t only; the
+// targets move with the owner.  jit (deformed sets): the sample's jitter
+// before F, carried to the source record for dL/dF of the source.
 __global__ void __launch_bounds__(BIN_THREADS)
     k_adj_final(McD d, BinGeom g, const uint32_t *__restrict__ key, const float2 *__restrict__ r,
                 const float *__restrict__ om, const float2 *__restrict__ G,
                 const int *__restrict__ src, const float2 *__restrict__ adj_vel,
+                const float2 *__restrict__ adj_aux, const float2 *__restrict__ jit,
                 float2 *__restrict__ own_grad, float2 *__restrict__ beta,
                 float4 *__restrict__ srcrec, uint32_t *__restrict__ srckey, uint32_t dead_src)
 {
@@ -333,56 +405,72 @@ __global__ void __launch_bounds__(BIN_THREADS)
     const int owner = pair >> 1, si = pair & 1;
     const SetD &S = d.set[si];
     const int n_s = S.count_dev ? *S.count_dev : S.n;
-    const float2 av = adj_vel[owner];
-    const float sg = si ? -d.inv_tot : d.inv_tot;   // u = v+ - v-, v = (1/NS) sum
-    const float ax = sg * av.x, ay = sg * av.y;
+    const float sg = si ? -d.inv_tot : d.inv_tot;   // u = v+ - v-, v = sum / (N1 + N2)
+    float ax[5], ay[5];
+    {
+        const float2 av = adj_vel[owner];
+        ax[0] = sg * av.x;
+        ay[0] = sg * av.y;
+    }
+#pragma unroll
+    for (int t = 1; t < 5; ++t) {
+        const float2 av = d.aux ? adj_aux[(size_t)(t - 1) * d.n_owner + owner]
+                                : make_float2(0.f, 0.f);
+        ax[t] = sg * av.x;
+        ay[t] = sg * av.y;
+    }
     const float Om = n_s > 0 ? (float)(*S.total_dev) : 1.f;
     float gxs = 0.f, gys = 0.f;
     const size_t base = (size_t)pair * g.NS;
     for (uint32_t k = lane; k < g.NS; k += WAVE) {
         const size_t sid = base + k;
         float b = 0.f, bt = 0.f, ex = 0.f, ey = 0.f;
+        float2 ju = make_float2(0.f, 0.f);
         uint32_t sk = dead_src;
         if (n_s > 0 && key[sid] < (uint32_t)g.ntile) {
             const float o = om[sid];
             const float2 rr = r[sid];
-            const float r2 = rr.x * rr.x + rr.y * rr.y, rn = sqrtf(r2);
-            if (o > 0.f && rn > 0.f) {
-                const float f = 1.f / (2.f * PI_F * r2);
-                float ksx, ksy;
-                kper_smooth(d, rr.x, rr.y, ksx, ksy);
-                const float Kx = -rr.y * f + ksx, Ky = rr.x * f + ksy;
-                const float aK = ax * Kx + ay * Ky;
-                const float D = d.w1 * o / Om + d.w2 * d.alpha_c / rn;
-                const float iD = 1.f / D;
-                b = aK * iD;
-                bt = b - aK * o * d.w1 * iD * iD / Om;
-                // grad_r (a . K): free part, then the smooth part
-                const float P = -ax * rr.y + ay * rr.x;
-                float gfx = (ay - 2.f * P * rr.x / r2) * f;
-                float gfy = (-ax - 2.f * P * rr.y / r2) * f;
-                float gsx, gsy;
-                kper_smooth_grad(d, rr.x, rr.y, ax, ay, gsx, gsy);
-                const float c1 = o * iD, c2 = aK * o * d.w2 * d.alpha_c * iD * iD / (r2 * rn);
-                const float drx = c1 * (gfx + gsx) + c2 * rr.x;
-                const float dry = c1 * (gfy + gsy) + c2 * rr.y;
-                const float2 Gk = G[sid];
-                const float dyx = bt * Gk.x, dyy = bt * Gk.y;
-                const int j = src[sid];
-                if (j >= 0) {   // branch 1: r = x_i - (x_j + jitter)
-                    gxs += drx;
-                    gys += dry;
-                    ex = dyx - drx;
-                    ey = dyy - dry;
-                    sk = (uint32_t)j;
-                } else {   // branch 2: y = x_i + offset, r fixed
-                    gxs += dyx;
-                    gys += dyy;
+            const float2 Gk = G[sid];
+            const int j = src[sid];
+            if (j >= 0) {   // branch 1: r = target - (x_j + jitter)
+                float drx, dry;
+                adj_term(d, ax[0], ay[0], o, Om, rr.x, rr.y, b, bt, drx, dry);
+                float sx = drx, sy = dry;
+                if (d.aux) {
+#pragma unroll
+                    for (int t = 1; t < 5; ++t) {
+                        const float2 of = aux_offset(t, d.aux_dx);
+                        float rx = rr.x + of.x, ry = rr.y + of.y;
+                        if (rx > 0.5f * d.Lx) rx -= d.Lx; else if (rx < -0.5f * d.Lx) rx += d.Lx;
+                        if (ry > 0.5f * d.Ly) ry -= d.Ly; else if (ry < -0.5f * d.Ly) ry += d.Ly;
+                        adj_term(d, ax[t], ay[t], o, Om, rx, ry, b, bt, drx, dry);
+                        sx += drx;
+                        sy += dry;
+                    }
                 }
+                gxs += sx;
+                gys += sy;
+                ex = bt * Gk.x - sx;
+                ey = bt * Gk.y - sy;
+                sk = (uint32_t)j;
+                if (jit) ju = jit[sid];
+            } else {   // branch 2: y = target + offset, r fixed
+                const int kb = (int)k - d.N1;
+                const int t = kb < d.N2 ? 0 : 1 + (kb - d.N2) / d.N2;
+                float drx, dry, bx = ax[0], by = ay[0];
+#pragma unroll
+                for (int tt = 1; tt < 5; ++tt)
+                    if (tt == t) {
+                        bx = ax[tt];
+                        by = ay[tt];
+                    }
+                adj_term(d, bx, by, o, Om, rr.x, rr.y, b, bt, drx, dry);
+                gxs += bt * Gk.x;
+                gys += bt * Gk.y;
             }
         }
         beta[sid] = make_float2(b, bt);
-        srcrec[sid] = make_float4(ex, ey, 0.f, 0.f);
+        srcrec[sid] = make_float4(ex, ey, ju.x, ju.y);
         srckey[sid] = sk;
     }
     gxs = wave_sum(gxs);
@@ -400,18 +488,41 @@ __global__ void k_adj_recs(const int *__restrict__ start, int ntile, const float
     rec2[p] = make_float4(s.x, s.y, b.x, b.y);
 }
 
+// dL/dF of F = (a b; c d) from dL/d(G11, 2 G12, G22) of its quadratic form
+// G = F^-T F^-1 = (q, -r; -r, p) / det^2 with p = a^2 + b^2, q = c^2 + d^2,
+// r = ac + bd, det = ad - bc (quad_form's first three outputs)
+__device__ __forceinline__ float4 quad_form_vjp(float4 F, float g1, float g2, float g3)
+{
+    const float a = F.x, b = F.y, c = F.z, d = F.w;
+    const float det = a * d - b * c;
+    const float i2 = 1.f / (det * det), gi = -2.f * i2 / det;   // d(det^-2)/d det
+    const float p = a * a + b * b, q = c * c + d * d, r = a * c + b * d;
+    const float S = (g1 * q - 2.f * g2 * r + g3 * p) * gi;   // coefficient of d det
+    return make_float4(i2 * (2.f * g3 * a - 2.f * g2 * c) + S * d,
+                       i2 * (2.f * g3 * b - 2.f * g2 * d) - S * c,
+                       i2 * (2.f * g1 * c - 2.f * g2 * a) - S * b,
+                       i2 * (2.f * g1 * d - 2.f * g2 * b) + S * a);
+}
+
 // One workgroup per (sign, fine cell): the particles of the cell, up to
 // ADJ_PCHUNK at a time, against every sample of the same sign in the cell's
 // stencil (the forward's tile stencil over the sample array), threads
 // striding over the samples; fixed-order block reduction per particle.
+// DEF: the particle's kernel is W(|F^-1 d|) (its quadratic form gq in field
+// order, F = jac by global index): grad_y W = W'(q)/(q h_w^2) G d, and
+// dW/dG = W'(q)/(2 q h_w^2) (dx^2, dx dy, dy^2) for (G11, 2 G12, G22), which
+// quad_form_vjp turns into dL/dF (grad_jac).
 constexpr int ADJ_PCHUNK = 8;
 
+template <bool DEF>
 __global__ void __launch_bounds__(ADJ_THREADS)
     k_adj_field(McD d, BinGeom g, const int *__restrict__ start, const float4 *__restrict__ rec2,
                 const int *__restrict__ order0, const int *__restrict__ order1,
                 const int *__restrict__ sub0, const int *__restrict__ sub1,
-                float2 *__restrict__ grad_pos, float *__restrict__ grad_w)
+                const float4 *__restrict__ jac, float2 *__restrict__ grad_pos,
+                float *__restrict__ grad_w, float4 *__restrict__ grad_jac)
 {
+    constexpr int NR = DEF ? 6 : 3;   // reduced values per particle
     const int tile = (int)blockIdx.x;
     const int si = tile >= g.ncell ? 1 : 0;
     const int c = tile - si * g.ncell;
@@ -420,7 +531,7 @@ __global__ void __launch_bounds__(ADJ_THREADS)
     if (pb >= pe) return;   // uniform: no particle of this sign in the cell
     const int cx = c / g.ny, cy = c - cx * g.ny;
     __shared__ TileLds<ADJ_THREADS> L;
-    __shared__ float red[ADJ_THREADS / WAVE][3 * ADJ_PCHUNK];
+    __shared__ float red[ADJ_THREADS / WAVE][NR * ADJ_PCHUNK];
     const int total = tile_pieces<ADJ_THREADS>(g, start + (size_t)si * g.ncell, cx, cy, d.Lx,
                                                 d.Ly, L);
     const int npieces = 3 * (2 * g.cols + 1);
@@ -430,13 +541,16 @@ __global__ void __launch_bounds__(ADJ_THREADS)
     const float sg = si ? -1.f : 1.f;   // d|w|/dw on the set
     for (int p0 = pb; p0 < pe; p0 += ADJ_PCHUNK) {
         const int np = min(ADJ_PCHUNK, pe - p0);
-        float px[ADJ_PCHUNK], py[ADJ_PCHUNK], A[ADJ_PCHUNK], Bx[ADJ_PCHUNK], By[ADJ_PCHUNK];
+        float px[ADJ_PCHUNK], py[ADJ_PCHUNK], acc[NR][ADJ_PCHUNK];
+        float4 Q[DEF ? ADJ_PCHUNK : 1];
 #pragma unroll
         for (int j = 0; j < ADJ_PCHUNK; ++j) {
             const float4 q = S.field.pts[p0 + min(j, np - 1)];
             px[j] = q.x;
             py[j] = q.y;
-            A[j] = Bx[j] = By[j] = 0.f;
+            if constexpr (DEF) Q[j] = S.field.gq[p0 + min(j, np - 1)];
+#pragma unroll
+            for (int v = 0; v < NR; ++v) acc[v][j] = 0.f;
         }
         int lo = 0;   // last piece with p_off <= i (i only grows)
         for (int i = threadIdx.x; i < total; i += ADJ_THREADS) {
@@ -450,62 +564,139 @@ __global__ void __launch_bounds__(ADJ_THREADS)
                 // (dx, dy) = y - x_m; grad_y W = W'(r)/r (y - x_m)
                 const float dx = sx - px[j], dy = sy - py[j];
                 float w, gr;
-                spline_wg(fmaf(dx, dx, dy * dy), d.inv_hw, w, gr);
-                A[j] = fmaf(sm.z, w, A[j]);
-                Bx[j] = fmaf(sm.w * gr, dx, Bx[j]);
-                By[j] = fmaf(sm.w * gr, dy, By[j]);
+                if constexpr (DEF) {
+                    const float4 G = Q[j];
+                    const float r2 =
+                        fmaxf(fmaf(dx, fmaf(G.y, dy, G.x * dx), G.z * dy * dy), 0.f);
+                    spline_wg(r2, d.inv_hw, w, gr);
+                    const float sg2 = sm.w * gr;
+                    acc[0][j] = fmaf(sm.z, w, acc[0][j]);
+                    acc[1][j] = fmaf(sg2, fmaf(0.5f * G.y, dy, G.x * dx), acc[1][j]);
+                    acc[2][j] = fmaf(sg2, fmaf(0.5f * G.y, dx, G.z * dy), acc[2][j]);
+                    acc[3][j] = fmaf(sg2 * dx, dx, acc[3][j]);
+                    acc[4][j] = fmaf(sg2 * dx, dy, acc[4][j]);
+                    acc[5][j] = fmaf(sg2 * dy, dy, acc[5][j]);
+                } else {
+                    spline_wg(fmaf(dx, dx, dy * dy), d.inv_hw, w, gr);
+                    acc[0][j] = fmaf(sm.z, w, acc[0][j]);
+                    acc[1][j] = fmaf(sm.w * gr, dx, acc[1][j]);
+                    acc[2][j] = fmaf(sm.w * gr, dy, acc[2][j]);
+                }
             }
         }
 #pragma unroll
         for (int j = 0; j < ADJ_PCHUNK; ++j) {
-            const float a = wave_sum(A[j]), bx = wave_sum(Bx[j]), by = wave_sum(By[j]);
-            if (lane == 0) {
-                red[wid][3 * j] = a;
-                red[wid][3 * j + 1] = bx;
-                red[wid][3 * j + 2] = by;
+#pragma unroll
+            for (int v = 0; v < NR; ++v) {
+                const float a = wave_sum(acc[v][j]);
+                if (lane == 0) red[wid][NR * j + v] = a;
             }
         }
         __syncthreads();
         if ((int)threadIdx.x < np) {
             const int j = threadIdx.x;
-            float a = 0.f, bx = 0.f, by = 0.f;
+            float tot[NR];
 #pragma unroll
-            for (int w = 0; w < ADJ_THREADS / WAVE; ++w) {
-                a += red[w][3 * j];
-                bx += red[w][3 * j + 1];
-                by += red[w][3 * j + 2];
-            }
+            for (int v = 0; v < NR; ++v) tot[v] = 0.f;
+#pragma unroll
+            for (int w = 0; w < ADJ_THREADS / WAVE; ++w)
+#pragma unroll
+                for (int v = 0; v < NR; ++v) tot[v] += red[w][NR * j + v];
             const int t = p0 + j;
             const float aw = S.field.pts[t].z;
             const int gm = sub[order[t]];
-            grad_w[gm] += sg * d.sig2 * a;
+            grad_w[gm] += sg * d.sig2 * tot[0];
             float2 gp = grad_pos[gm];
-            gp.x -= aw * d.sig2 * bx;
-            gp.y -= aw * d.sig2 * by;
+            gp.x -= aw * d.sig2 * tot[1];
+            gp.y -= aw * d.sig2 * tot[2];
             grad_pos[gm] = gp;
+            if constexpr (DEF) {
+                const float h = 0.5f * aw * d.sig2;
+                const float4 fb = quad_form_vjp(jac[gm], h * tot[3], h * tot[4], h * tot[5]);
+                float4 gj = grad_jac[gm];
+                gj.x += fb.x;
+                gj.y += fb.y;
+                gj.z += fb.z;
+                gj.w += fb.w;
+                grad_jac[gm] = gj;
+            }
         }
         __syncthreads();   // red is rewritten by the next chunk
     }
 }
 
-// owner terms (both signs) and the branch-1 source terms, per particle
+// owner terms (both signs) and the branch-1 source terms, per particle; the
+// source records are (dL/dy, jitter before F): y = x_j + F_j u gives
+// dL/dF_j += dL/dy u^T (grad_jac, deformed sets)
 __global__ void k_adj_sources(int n, const int *__restrict__ src_start,
                               const float4 *__restrict__ srcrec_sorted,
-                              const float2 *__restrict__ own_grad, float2 *__restrict__ grad_pos)
+                              const float2 *__restrict__ own_grad, float2 *__restrict__ grad_pos,
+                              float4 *__restrict__ grad_jac)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const float2 a = own_grad[2 * (size_t)i], b = own_grad[2 * (size_t)i + 1];
     float sx = a.x + b.x, sy = a.y + b.y;
+    float fa = 0.f, fb = 0.f, fc = 0.f, fd = 0.f;
     for (int p = src_start[i]; p < src_start[i + 1]; ++p) {
         const float4 e = srcrec_sorted[p];
         sx += e.x;
         sy += e.y;
+        fa = fmaf(e.x, e.z, fa);
+        fb = fmaf(e.x, e.w, fb);
+        fc = fmaf(e.y, e.z, fc);
+        fd = fmaf(e.y, e.w, fd);
     }
     float2 gp = grad_pos[i];
     gp.x += sx;
     gp.y += sy;
     grad_pos[i] = gp;
+    if (grad_jac) {
+        float4 gj = grad_jac[i];
+        gj.x += fa;
+        gj.y += fb;
+        gj.z += fc;
+        gj.w += fd;
+        grad_jac[i] = gj;
+    }
+}
+
+// The F update's adjoint, per owner: F' = A / sqrt(det A) with
+// A = I + c [u_R - u_L | u_T - u_B] (jacobian_from5; c = dt / (2 dx), u = v+ - v-
+// of this step's aux estimates va[sign][L, R, T, B][owner]).  jbar holds
+// dL/dF' on entry; the adjoint weights of the four targets go to
+// adj_aux[t][owner] (L, R, T, B) and jbar is cleared for the step's dL/dF.
+__global__ void k_jac_adj(int n, const float2 *__restrict__ va, float c,
+                          float4 *__restrict__ jbar, float2 *__restrict__ adj_aux)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    float2 u[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        const float2 p = va[(size_t)t * n + j], q = va[(size_t)(4 + t) * n + j];
+        u[t] = make_float2(p.x - q.x, p.y - q.y);
+    }
+    float4 A;
+    A.x = fmaf(u[1].x - u[0].x, c, 1.f);
+    A.y = (u[2].x - u[3].x) * c;
+    A.z = (u[1].y - u[0].y) * c;
+    A.w = fmaf(u[2].y - u[3].y, c, 1.f);
+    const float det = fmaf(A.x, A.w, -(A.y * A.z));
+    const bool clamp = !(det > 1e-12f);
+    const float sc = 1.f / sqrtf(clamp ? 1e-12f : det);
+    const float4 Fb = jbar[j];
+    // dsc/dA = -sc^3 / 2 d det/dA, d det/dA = (A.w, -A.z, -A.y, A.x)
+    const float k = clamp ? 0.f : -0.5f * sc * sc * sc *
+                                      (Fb.x * A.x + Fb.y * A.y + Fb.z * A.z + Fb.w * A.w);
+    const float4 Ab = make_float4(Fb.x * sc + k * A.w, Fb.y * sc - k * A.z, Fb.z * sc - k * A.y,
+                                  Fb.w * sc + k * A.x);
+    const float2 aR = make_float2(c * Ab.x, c * Ab.z), aT = make_float2(c * Ab.y, c * Ab.w);
+    adj_aux[j] = make_float2(-aR.x, -aR.y);                  // L
+    adj_aux[(size_t)n + j] = aR;                             // R
+    adj_aux[2 * (size_t)n + j] = aT;                         // T
+    adj_aux[3 * (size_t)n + j] = make_float2(-aT.x, -aT.y);  // B
+    jbar[j] = make_float4(0.f, 0.f, 0.f, 0.f);
 }
 
 __global__ void k_scale_xy(int n, const float2 *__restrict__ x, float a, float2 *__restrict__ y)
@@ -516,7 +707,7 @@ __global__ void k_scale_xy(int n, const float2 *__restrict__ x, float a, float2 
 
 struct AdjBufs {
     uint32_t *key, *keys_sorted, *srckey, *srckey_sorted;
-    float2 *q, *r, *G, *beta, *own;
+    float2 *q, *r, *G, *beta, *own, *jit;
     float *om;
     int *src, *start, *src_start;
     float4 *srec, *rec2, *srcrec, *srcrec_sorted;
@@ -524,15 +715,16 @@ struct AdjBufs {
     size_t sort_bytes;
 };
 
-static size_t adj_ws_bytes(const BinGeom &g, int n)
+// def: deformed sets (the branch-1 jitter per sample is kept)
+static size_t adj_ws_bytes(const BinGeom &g, int n, bool def)
 {
     const size_t S = g.S > 0 ? g.S : 1, T = (size_t)g.ntile, N = (size_t)(n > 0 ? n : 1);
-    return 4 * align256(S * 4) + 5 * align256(S * 8) + align256(S * 4) + align256(S * 4) +
-           align256((T + 1) * 4) + align256((N + 1) * 4) + 4 * align256(S * 16) +
-           align256(2 * N * 8) + radix_workspace_bytes((int)S) + 8192;
+    return 4 * align256(S * 4) + (def ? 6 : 5) * align256(S * 8) + align256(S * 4) +
+           align256(S * 4) + align256((T + 1) * 4) + align256((N + 1) * 4) +
+           4 * align256(S * 16) + align256(2 * N * 8) + radix_workspace_bytes((int)S) + 8192;
 }
 
-static int carve_adj(const BinGeom &g, int n, void *ws, size_t bytes, AdjBufs *B)
+static int carve_adj(const BinGeom &g, int n, bool def, void *ws, size_t bytes, AdjBufs *B)
 {
     Carve cv(ws, bytes);
     const size_t S = g.S > 0 ? g.S : 1, T = (size_t)g.ntile, N = (size_t)(n > 0 ? n : 1);
@@ -544,6 +736,7 @@ static int carve_adj(const BinGeom &g, int n, void *ws, size_t bytes, AdjBufs *B
     B->r = cv.take<float2>(S);
     B->G = cv.take<float2>(S);
     B->beta = cv.take<float2>(S);
+    B->jit = def ? cv.take<float2>(S) : nullptr;
     B->own = cv.take<float2>(2 * N);
     B->om = cv.take<float>(S);
     B->src = cv.take<int>(S);
@@ -556,18 +749,25 @@ static int carve_adj(const BinGeom &g, int n, void *ws, size_t bytes, AdjBufs *B
     B->sort_bytes = radix_workspace_bytes((int)S);
     B->sort_ws = cv.take<char>(B->sort_bytes);
     if (!B->key || !B->keys_sorted || !B->srckey || !B->srckey_sorted || !B->q || !B->r ||
-        !B->G || !B->beta || !B->own || !B->om || !B->src || !B->start || !B->src_start ||
-        !B->srec || !B->rec2 || !B->srcrec || !B->srcrec_sorted || !B->sort_ws)
+        !B->G || !B->beta || (def && !B->jit) || !B->own || !B->om || !B->src || !B->start ||
+        !B->src_start || !B->srec || !B->rec2 || !B->srcrec || !B->srcrec_sorted || !B->sort_ws)
         return VPMC_ERR_WORKSPACE;
     return VPMC_OK;
 }
 
-static int adj_check(const vpmc_mc_desc *d, McD *m, BinGeom *g)
+// def = false: undeformed kernels only.  def = true: the deformed step (the
+// sets' F and quadratic forms, the four auxiliary targets).
+static int adj_check(const vpmc_mc_desc *d, bool def, McD *m, BinGeom *g)
 {
     if (!d) return VPMC_ERR_ARG;
-    if (d->mode != VPMC_MODE_CONSISTENT || !d->periodic || d->uniforms || d->aux || d->jac ||
-        d->set[0].field.gq || d->set[1].field.gq)
-        return VPMC_ERR_ARG;   // the adjoint covers undeformed kernels only
+    if (d->mode != VPMC_MODE_CONSISTENT || !d->periodic || d->uniforms) return VPMC_ERR_ARG;
+    if (def) {
+        if (!d->jac || !d->aux || !d->vel_aux || !(d->aux_dx > 0.f)) return VPMC_ERR_ARG;
+        for (int s = 0; s < 2; ++s)
+            if (!d->set[s].field.gq || !d->set[s].set_jac) return VPMC_ERR_ARG;
+    } else if (d->aux || d->jac || d->set[0].field.gq || d->set[1].field.gq) {
+        return VPMC_ERR_ARG;   // deformed steps: vpmc_step_adjoint_deformed
+    }
     VPMC_TRY(make_mcd(d, m));
     VPMC_TRY(bin_geom(*m, d->N1, d->N2, d->n_owner, g));
     if (!tile_threads_ok(*g, ADJ_THREADS)) return VPMC_ERR_GRID;
@@ -577,46 +777,65 @@ static int adj_check(const vpmc_mc_desc *d, McD *m, BinGeom *g)
     return VPMC_OK;
 }
 
-int mc_adjoint_impl(const vpmc_mc_desc *d, const int *sub_pos, const int *sub_neg,
-                    const float *adj_vel, float *grad_pos, float *grad_w, void *ws,
-                    size_t ws_bytes, hipStream_t s)
+// adj_aux (4 n float2, targets L, R, T, B), jac (F by global index) and
+// grad_jac (dL/dF, accumulated) are given for a deformed descriptor only
+int mc_adjoint_impl(const vpmc_mc_desc *d, bool def, const int *sub_pos, const int *sub_neg,
+                    const float *adj_vel, const float *adj_aux, float *grad_pos, float *grad_w,
+                    float *grad_jac, void *ws, size_t ws_bytes, hipStream_t s)
 {
     McD m;
     BinGeom g;
-    VPMC_TRY(adj_check(d, &m, &g));
+    VPMC_TRY(adj_check(d, def, &m, &g));
     const int n = d->n_owner;
     if (n == 0) return VPMC_OK;
     if (!adj_vel || !grad_pos || !grad_w || !sub_pos || !sub_neg) return VPMC_ERR_ARG;
+    if (def && (!adj_aux || !grad_jac)) return VPMC_ERR_ARG;
     AdjBufs B;
-    VPMC_TRY(carve_adj(g, n, ws, ws_bytes, &B));
+    VPMC_TRY(carve_adj(g, n, def, ws, ws_bytes, &B));
     const unsigned gs = (unsigned)((g.S + BIN_THREADS - 1) / BIN_THREADS);
     hipLaunchKernelGGL(k_adj_gen, dim3(gs), dim3(BIN_THREADS), 0, s, m, g, sub_pos, sub_neg,
-                       B.key, B.q, B.r, B.src);
+                       B.key, B.q, B.r, B.src, B.jit);
     VPMC_CHECK_LAUNCH();
     VPMC_TRY(radix_sort_to_records(B.key, (int)g.S, key_bits_for((uint32_t)g.ntile),
                                    (const float2 *)B.q, nullptr, B.srec, B.keys_sorted, g.ntile,
                                    B.start, B.sort_ws, B.sort_bytes, s));
     const unsigned ga = (unsigned)((g.S + ADJ_THREADS - 1) / ADJ_THREADS);
-    hipLaunchKernelGGL(k_adj_eval, dim3((unsigned)(8 * g.per_xcd)), dim3(ADJ_THREADS), 0, s, m, g,
-                       (const int *)B.start, (const float4 *)B.srec, B.om, B.G);
+    if (def)
+        hipLaunchKernelGGL(k_adj_eval_def, dim3(ga), dim3(ADJ_THREADS), 0, s, m, g,
+                           (const int *)B.start, (const float4 *)B.srec,
+                           (const uint32_t *)B.keys_sorted, B.om, B.G);
+    else
+        hipLaunchKernelGGL(k_adj_eval, dim3((unsigned)(8 * g.per_xcd)), dim3(ADJ_THREADS), 0, s,
+                           m, g, (const int *)B.start, (const float4 *)B.srec, B.om, B.G);
     const unsigned gf = (unsigned)((2 * (size_t)n + 3) / 4);
     hipLaunchKernelGGL(k_adj_final, dim3(gf), dim3(BIN_THREADS), 0, s, m, g,
                        (const uint32_t *)B.key, (const float2 *)B.r, (const float *)B.om,
-                       (const float2 *)B.G, (const int *)B.src, (const float2 *)adj_vel, B.own,
-                       B.beta, B.srcrec, B.srckey, (uint32_t)n);
+                       (const float2 *)B.G, (const int *)B.src, (const float2 *)adj_vel,
+                       (const float2 *)adj_aux, (const float2 *)B.jit, B.own, B.beta, B.srcrec,
+                       B.srckey, (uint32_t)n);
     hipLaunchKernelGGL(k_adj_recs, dim3(ga), dim3(ADJ_THREADS), 0, s, (const int *)B.start,
                        g.ntile, (const float4 *)B.srec, (const float2 *)B.beta, B.rec2);
     VPMC_CHECK_LAUNCH();
-    hipLaunchKernelGGL(k_adj_field, dim3((unsigned)g.ntile), dim3(ADJ_THREADS), 0, s, m, g,
-                       (const int *)B.start, (const float4 *)B.rec2, d->set[0].field.order,
-                       d->set[1].field.order, sub_pos, sub_neg, (float2 *)grad_pos, grad_w);
+    if (def)
+        hipLaunchKernelGGL(k_adj_field<true>, dim3((unsigned)g.ntile), dim3(ADJ_THREADS), 0, s,
+                           m, g, (const int *)B.start, (const float4 *)B.rec2,
+                           d->set[0].field.order, d->set[1].field.order, sub_pos, sub_neg,
+                           (const float4 *)d->jac, (float2 *)grad_pos, grad_w,
+                           (float4 *)grad_jac);
+    else
+        hipLaunchKernelGGL(k_adj_field<false>, dim3((unsigned)g.ntile), dim3(ADJ_THREADS), 0, s,
+                           m, g, (const int *)B.start, (const float4 *)B.rec2,
+                           d->set[0].field.order, d->set[1].field.order, sub_pos, sub_neg,
+                           (const float4 *)nullptr, (float2 *)grad_pos, grad_w,
+                           (float4 *)nullptr);
     VPMC_CHECK_LAUNCH();
     VPMC_TRY(radix_sort_to_records(B.srckey, (int)g.S, key_bits_for((uint32_t)n), nullptr,
                                    (const float4 *)B.srcrec, B.srcrec_sorted, B.srckey_sorted, n,
                                    B.src_start, B.sort_ws, B.sort_bytes, s));
     hipLaunchKernelGGL(k_adj_sources, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n,
                        (const int *)B.src_start, (const float4 *)B.srcrec_sorted,
-                       (const float2 *)B.own, (float2 *)grad_pos);
+                       (const float2 *)B.own, (float2 *)grad_pos,
+                       def ? (float4 *)grad_jac : (float4 *)nullptr);
     VPMC_CHECK_LAUNCH();
     return VPMC_OK;
 }
@@ -639,13 +858,33 @@ struct F64Args {
     double2 *contrib;           // per sample
     double2 *y, *r;             // per sample: query point, owner - point
     Kper64 kc;
+    // deformed step (all NULL / 0 otherwise): per set, cell order, the
+    // records' quadratic forms (G11, 2 G12, G22, 0) of F = jac64; per set,
+    // local order, F; per sample (om, om0); the aux estimates
+    // [sign][L, R, T, B][owner]
+    const double4 *G64[2];
+    const double4 *setjac64[2];
+    double2 *om2;
+    double2 *vel_aux;
+    double aux_dx;
 };
+
+// G = F^-T F^-1 of F = (a b; c d) as (G11, 2 G12, G22, 0), FP64
+__device__ __forceinline__ double4 quad_form_f64(double4 F)
+{
+    const double det = F.x * F.w - F.y * F.z;
+    const double i2 = 1.0 / (det * det);
+    const double p = F.x * F.x + F.y * F.y, q = F.z * F.z + F.w * F.w;
+    const double r = F.x * F.z + F.y * F.w;
+    return make_double4(q * i2, -2.0 * r * i2, p * i2, 0.0);
+}
 
 __global__ void k_f64_recs(int n_s, const int *__restrict__ count_dev,
                            const int *__restrict__ order, const int *__restrict__ sub,
                            const double2 *__restrict__ pos, const double *__restrict__ w,
                            const float *__restrict__ w0, double sg, double4 *__restrict__ rec,
-                           double2 *__restrict__ set64)
+                           double2 *__restrict__ set64, const double4 *__restrict__ jac64,
+                           double4 *__restrict__ G64, double4 *__restrict__ setjac64)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     const int n = count_dev ? *count_dev : n_s;
@@ -653,6 +892,10 @@ __global__ void k_f64_recs(int n_s, const int *__restrict__ count_dev,
     const int gt = sub[order[t]];
     rec[t] = make_double4(pos[gt].x, pos[gt].y, sg * w[gt], fabs((double)w0[gt]));
     set64[t] = pos[sub[t]];
+    if (jac64) {
+        G64[t] = quad_form_f64(jac64[gt]);
+        setjac64[t] = jac64[sub[t]];
+    }
 }
 
 // Per sample: the draws, the point y (wrapped into the box) and the min-image
@@ -674,9 +917,19 @@ __global__ void __launch_bounds__(ADJ_THREADS) k_f64_gen(McD d, BinGeom g, F64Ar
     double yx = 0.0, yy = 0.0, rx = 0.0, ry = 0.0;
     if (n_s > 0) {
         const bool b1 = k < d.N1;
+        int t = 0, kb = b1 ? k : k - d.N1;   // t: target of a disk sample (0 = owner)
+        if (!b1 && kb >= d.N2) {
+            t = 1 + (kb - d.N2) / d.N2;
+            kb = (kb - d.N2) - (t - 1) * d.N2;
+        }
         float u[3];
-        draws(d, owner, si, b1 ? 1 : 2, b1 ? k : k - d.N1, u);
-        const double2 xi = a.pos[owner];
+        draws(d, owner, si, b1 ? 1 : 2 + t, kb, u);
+        double2 xi = a.pos[owner];
+        if (t) {   // the auxiliary target
+            const float2 o = aux_offset(t, d.aux_dx);
+            xi.x += (double)o.x;
+            xi.y += (double)o.y;
+        }
         const double L = a.L;
         bool live = true;
         if (b1) {
@@ -685,8 +938,15 @@ __global__ void __launch_bounds__(ADJ_THREADS) k_f64_gen(McD d, BinGeom g, F64Ar
             const double rho = mass_inverse_f64((double)u[1]) * a.h_w;
             double sn, cs;
             sincos(2.0 * PI_D * (double)u[2], &sn, &cs);
-            yx = xj.x + rho * cs;
-            yy = xj.y + rho * sn;
+            double jx = rho * cs, jy = rho * sn;
+            if (a.setjac64[si]) {   // the deformed blob: F_j u
+                const double4 F = a.setjac64[si][j];
+                const double ex = F.x * jx + F.y * jy, ey = F.z * jx + F.w * jy;
+                jx = ex;
+                jy = ey;
+            }
+            yx = xj.x + jx;
+            yy = xj.y + jy;
             rx = xi.x - yx;
             ry = xi.y - yy;
             if (rx > 0.5 * L) rx -= L; else if (rx < -0.5 * L) rx += L;
@@ -733,15 +993,25 @@ __global__ void __launch_bounds__(ADJ_THREADS) k_f64_eval(McD d, BinGeom g, F64A
         const int si = kk >= cells ? 1 : 0;
         const double2 y = a.y[sid], r = a.r[sid];
         double om = 0.0, om0 = 0.0;
-        auto fn = [&](const double4 &c, double dx, double dy) {
+        const double4 *G64 = a.G64[si];
+        auto fn = [&](const double4 &c, double dx, double dy, int k) {
             double w, gr;
-            spline_wg64(dx * dx + dy * dy, a.inv_hw, w, gr);
+            double r2 = dx * dx + dy * dy;
+            if (G64) {   // deformed record: |F^-1 d|^2
+                const double4 G = G64[k];
+                r2 = fmax(G.x * dx * dx + G.y * dx * dy + G.z * dy * dy, 0.0);
+            }
+            spline_wg64(r2, a.inv_hw, w, gr);
             om += c.z * w;
             om0 += c.w * w;
         };
         field_visit64(a.geom, a.cell_start[si], a.rec[si], y.x, y.y, fn);
         om *= a.sig2;
         om0 *= a.sig2;
+        if (a.om2) {   // the targets' contributions follow in k_f64_reduce5
+            a.om2[sid] = make_double2(om, om0);
+            return;
+        }
         const double r2 = r.x * r.x + r.y * r.y, rn = sqrt(r2);
         if (om0 > 0.0 && rn > 0.0) {
             const double Om0 = *d.set[si].total_dev;
@@ -753,7 +1023,88 @@ __global__ void __launch_bounds__(ADJ_THREADS) k_f64_eval(McD d, BinGeom g, F64A
             out = make_double2((-r.y * f + ksx) * gk, (r.x * f + ksy) * gk);
         }
     }
-    a.contrib[sid] = out;
+    if (a.om2) a.om2[sid] = make_double2(0.0, 0.0);
+    else a.contrib[sid] = out;
+}
+
+// one MIS contribution in FP64 (the terms of k_f64_eval)
+__device__ __forceinline__ double2 f64_contrib(const F64Args &a, double om, double om0,
+                                               double Om0, double rx, double ry)
+{
+    const double r2 = rx * rx + ry * ry, rn = sqrt(r2);
+    if (!(om0 > 0.0 && rn > 0.0)) return make_double2(0.0, 0.0);
+    const double D = a.w1 * om0 / Om0 + a.w2 * a.alpha / rn;
+    double ksx, ksy;
+    kper_smooth_f64(a.kc.c, KPER_TERMS_ADJ64, a.L, rx, ry, ksx, ksy);
+    const double f = 1.0 / (2.0 * PI_D * r2);
+    const double gk = om / D;
+    return make_double2((-ry * f + ksx) * gk, (rx * f + ksy) * gk);
+}
+
+// per (owner, sign) with the auxiliary targets: sums in sample order at the
+// five targets (a branch-1 sample at every target with r + offset, min
+// image; a disk sample at its own); vel[si * n + owner] and
+// vel_aux[(si * 4 + t - 1) * n + owner]
+__global__ void k_f64_reduce5(McD d, BinGeom g, F64Args a, double2 *__restrict__ vel)
+{
+    const int pair = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n = d.n_owner;
+    if (pair >= 2 * n) return;
+    const int owner = pair >> 1, si = pair & 1;
+    double sx[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, sy[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    const SetD &S = d.set[si];
+    const int n_s = S.count_dev ? *S.count_dev : S.n;
+    const double L = a.L;
+    if (n_s > 0) {
+        const double Om0 = *S.total_dev;
+        const size_t base = (size_t)pair * g.NS;
+        for (uint32_t k = 0; k < g.NS; ++k) {
+            const double2 o = a.om2[base + k], r = a.r[base + k];
+            if ((int)k < d.N1) {
+                for (int t = 0; t < 5; ++t) {
+                    double rx = r.x, ry = r.y;
+                    if (t) {
+                        const float2 of = aux_offset(t, d.aux_dx);
+                        rx += (double)of.x;
+                        ry += (double)of.y;
+                        if (rx > 0.5 * L) rx -= L; else if (rx < -0.5 * L) rx += L;
+                        if (ry > 0.5 * L) ry -= L; else if (ry < -0.5 * L) ry += L;
+                    }
+                    const double2 c = f64_contrib(a, o.x, o.y, Om0, rx, ry);
+                    sx[t] += c.x;
+                    sy[t] += c.y;
+                }
+            } else {
+                const int kb = (int)k - d.N1;
+                const int t = kb < d.N2 ? 0 : 1 + (kb - d.N2) / d.N2;
+                const double2 c = f64_contrib(a, o.x, o.y, Om0, r.x, r.y);
+                sx[t] += c.x;
+                sy[t] += c.y;
+            }
+        }
+    }
+    vel[(size_t)si * n + owner] = make_double2(sx[0] * a.inv_tot, sy[0] * a.inv_tot);
+    for (int t = 1; t < 5; ++t)
+        a.vel_aux[((size_t)si * 4 + (t - 1)) * n + owner] =
+            make_double2(sx[t] * a.inv_tot, sy[t] * a.inv_tot);
+}
+
+// update_jacobian_from5 + renorm_det1 in FP64 (jacobian_from5's formula)
+__global__ void k_f64_jac(int n, const double2 *__restrict__ va, double c,
+                          double4 *__restrict__ jac64)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    double2 u[4];
+    for (int t = 0; t < 4; ++t) {
+        const double2 p = va[(size_t)t * n + j], q = va[(size_t)(4 + t) * n + j];
+        u[t] = make_double2(p.x - q.x, p.y - q.y);
+    }
+    const double fx = 1.0 + (u[1].x - u[0].x) * c, fy = (u[2].x - u[3].x) * c;
+    const double fz = (u[1].y - u[0].y) * c, fw = 1.0 + (u[2].y - u[3].y) * c;
+    const double det = fx * fw - fy * fz;
+    const double sc = 1.0 / sqrt(fmax(det, 1e-12));
+    jac64[j] = make_double4(fx * sc, fy * sc, fz * sc, fw * sc);
 }
 
 // per (owner, sign): sum in sample order; vel[si * n + owner]
@@ -805,7 +1156,7 @@ __global__ void k_grid_query(FieldD f, float inv_hw, float sig2, int gnx, int gn
     const float zx = ((float)a + 0.5f) * (f.g.Lx / (float)gnx);
     const float zy = ((float)b + 0.5f) * (f.g.Ly / (float)gny);
     float acc = 0.f;
-    auto fn = [&](const float4 &c, float dx, float dy) {
+    auto fn = [&](const float4 &c, float dx, float dy, int) {
         float w, gr;
         spline_wg(fmaf(dx, dx, dy * dy), inv_hw, w, gr);
         acc = fmaf(c.z, w, acc);
@@ -885,16 +1236,16 @@ size_t vpmc_mc_adjoint_workspace_bytes(const vpmc_mc_desc *d)
 {
     McD m;
     BinGeom g;
-    if (adj_check(d, &m, &g) != VPMC_OK) return 0;
-    return adj_ws_bytes(g, d->n_owner);
+    if (adj_check(d, false, &m, &g) != VPMC_OK) return 0;
+    return adj_ws_bytes(g, d->n_owner, false);
 }
 
 int vpmc_mc_adjoint(const vpmc_mc_desc *d, const int *sub_pos, const int *sub_neg,
                     const float *adj_vel, float *grad_pos, float *grad_w, void *workspace,
                     size_t workspace_bytes, void *stream)
 {
-    return mc_adjoint_impl(d, sub_pos, sub_neg, adj_vel, grad_pos, grad_w, workspace,
-                           workspace_bytes, as_stream(stream));
+    return mc_adjoint_impl(d, false, sub_pos, sub_neg, adj_vel, nullptr, grad_pos, grad_w,
+                           nullptr, workspace, workspace_bytes, as_stream(stream));
 }
 
 size_t vpmc_step_adjoint_workspace_bytes(const vpmc_mc_desc *d)
@@ -933,8 +1284,65 @@ int vpmc_step_adjoint(vpmc_mc_desc *d, int n_total, const float *pos_xy, const f
     hipLaunchKernelGGL(k_scale_xy, dim3((unsigned)((n_total + 255) / 256)), dim3(256), 0, s,
                        n_total, (const float2 *)lambda, dt, av);
     VPMC_CHECK_LAUNCH();
-    return mc_adjoint_impl(d, sub_pos, sub_neg, (const float *)av, lambda, grad_w, rest,
-                           rest_bytes, s);
+    return mc_adjoint_impl(d, false, sub_pos, sub_neg, (const float *)av, nullptr, lambda,
+                           grad_w, nullptr, rest, rest_bytes, s);
+}
+
+size_t vpmc_step_adjoint_deformed_workspace_bytes(const vpmc_mc_desc *d)
+{
+    McD m;
+    BinGeom g;
+    if (adj_check(d, true, &m, &g) != VPMC_OK) return 0;
+    const size_t a = adj_ws_bytes(g, d->n_owner, true);
+    const size_t e = vpmc_mc_binned_workspace_bytes(d);
+    if (e == 0) return 0;
+    const int mx = d->set[0].n > d->set[1].n ? d->set[0].n : d->set[1].n;
+    const size_t f = vpmc_field_workspace_bytes(mx, d->set[0].field.nx, d->set[0].field.ny);
+    const size_t n = (size_t)(d->n_owner > 0 ? d->n_owner : 1);
+    size_t rest = a > e ? a : e;
+    if (f > rest) rest = f;
+    return align256(n * 8) + align256(4 * n * 8) + rest;
+}
+
+int vpmc_step_adjoint_deformed(vpmc_mc_desc *d, int n_total, const float *pos_xy,
+                               const float *omega, const int *sub_pos, const int *sub_neg,
+                               float dt, float *lambda, float *jac_bar, float *grad_w,
+                               void *workspace, size_t workspace_bytes, void *stream)
+{
+    if (!d || n_total < 0 || d->n_owner != n_total || d->owner_xy != pos_xy) return VPMC_ERR_ARG;
+    McD m;
+    BinGeom g;
+    VPMC_TRY(adj_check(d, true, &m, &g));
+    if (n_total == 0) return VPMC_OK;
+    if (!lambda || !jac_bar || !grad_w || !omega || !d->vel_sign) return VPMC_ERR_ARG;
+    hipStream_t s = as_stream(stream);
+    Carve cv(workspace, workspace_bytes);
+    float2 *av = cv.take<float2>((size_t)n_total);
+    float2 *aa = cv.take<float2>(4 * (size_t)n_total);
+    if (!av || !aa) return VPMC_ERR_WORKSPACE;
+    char *rest = cv.base + cv.used;
+    const size_t rest_bytes = cv.cap - cv.used;
+    const int *subs[2] = {sub_pos, sub_neg};
+    for (int k = 0; k < 2; ++k) {   // the fields with this step's F, as vpmc_step
+        vpmc_sign_set &S = d->set[k];
+        VPMC_TRY(field_build_impl(S.field.nx, S.field.ny, S.field.Lx, S.field.Ly, S.n, nullptr,
+                                  subs[k], pos_xy, omega, 1, const_cast<float *>(S.field.pts),
+                                  const_cast<int *>(S.field.cell_start),
+                                  const_cast<int *>(S.field.order),
+                                  const_cast<float *>(S.set_xy), rest, rest_bytes, s, d->jac,
+                                  const_cast<float *>(S.field.gq),
+                                  const_cast<float *>(S.set_jac)));
+    }
+    // this step's five estimates (the aux ones set F'), then F''s adjoint
+    VPMC_TRY(vpmc_mc_estimate_binned(d, rest, rest_bytes, stream));
+    hipLaunchKernelGGL(k_jac_adj, dim3((unsigned)((n_total + 255) / 256)), dim3(256), 0, s,
+                       n_total, (const float2 *)d->vel_aux, 0.5f * dt / d->aux_dx,
+                       (float4 *)jac_bar, aa);
+    hipLaunchKernelGGL(k_scale_xy, dim3((unsigned)((n_total + 255) / 256)), dim3(256), 0, s,
+                       n_total, (const float2 *)lambda, dt, av);
+    VPMC_CHECK_LAUNCH();
+    return mc_adjoint_impl(d, true, sub_pos, sub_neg, (const float *)av, (const float *)aa,
+                           lambda, grad_w, jac_bar, rest, rest_bytes, s);
 }
 
 size_t vpmc_step_f64_workspace_bytes(const vpmc_mc_desc *d)
@@ -950,22 +1358,28 @@ size_t vpmc_step_f64_workspace_bytes(const vpmc_mc_desc *d)
     const size_t cells = (size_t)d->set[0].field.nx * (size_t)d->set[0].field.ny;
     const size_t fb = vpmc_field_workspace_bytes(mx, d->set[0].field.nx, d->set[0].field.ny);
     const size_t rb = radix_workspace_bytes((int)S, 4);
+    // the deformed step's quadratic forms, F per set and aux estimates
+    const size_t def = d->aux ? 4 * align256(nm * 32) + align256(8 * n * 16) : 0;
     return align256(n * 8) + 2 * align256(nm * 32) + 2 * align256(nm * 16) +
            3 * align256(S * 16) + 3 * align256(S * 4) + align256((2 * cells + 2) * 4) +
-           align256(2 * n * 16) + (fb > rb ? fb : rb) + 4096;
+           align256(2 * n * 16) + (fb > rb ? fb : rb) + def + 4096;
 }
 
-int vpmc_step_f64(vpmc_mc_desc *d, int n_total, double *pos_xy, const double *w,
-                  const float *omega0, const int *sub_pos, const int *sub_neg, double dt,
-                  void *workspace, size_t workspace_bytes, void *stream)
+// jac64 (deformed step, aux targets): F per particle in FP64, replaced by
+// the F' of this step's five estimates before the move, as vpmc_step does
+static int step_f64_impl(vpmc_mc_desc *d, int n_total, double *pos_xy, const double *w,
+                         const float *omega0, const int *sub_pos, const int *sub_neg, double dt,
+                         double *jac64, void *workspace, size_t workspace_bytes, hipStream_t s)
 {
     if (!d || n_total < 0 || d->n_owner != n_total) return VPMC_ERR_ARG;
-    if (d->mode != VPMC_MODE_CONSISTENT || !d->periodic || d->uniforms || d->Lx != d->Ly ||
-        d->aux)
+    if (d->mode != VPMC_MODE_CONSISTENT || !d->periodic || d->uniforms || d->Lx != d->Ly)
         return VPMC_ERR_ARG;
+    const bool def = jac64 != nullptr;
+    if (def != (d->aux != 0)) return VPMC_ERR_ARG;   // aux targets exactly with F
+    if (def && (!(d->aux_dx > 0.f) || !d->set[0].field.gq || !d->set[1].field.gq))
+        return VPMC_ERR_ARG;   // the fields' search radius covers the deformation
     if (n_total == 0) return VPMC_OK;
     if (!pos_xy || !w || !omega0 || !sub_pos || !sub_neg) return VPMC_ERR_ARG;
-    hipStream_t s = as_stream(stream);
     McD m;
     VPMC_TRY(make_mcd(d, &m));
     BinGeom g;
@@ -987,6 +1401,16 @@ int vpmc_step_f64(vpmc_mc_desc *d, int n_total, double *pos_xy, const double *w,
     const size_t cells = (size_t)fnx * (size_t)fny;
     int *cstart = cv.take<int>(2 * cells + 2);
     double2 *vel = cv.take<double2>(2 * (size_t)n_total);
+    double4 *G64[2] = {nullptr, nullptr}, *sj64[2] = {nullptr, nullptr};
+    double2 *vaux = nullptr;
+    if (def) {
+        for (int k = 0; k < 2; ++k) {
+            G64[k] = cv.take<double4>(nm);
+            sj64[k] = cv.take<double4>(nm);
+        }
+        vaux = cv.take<double2>(8 * (size_t)n_total);
+        if (!G64[0] || !G64[1] || !sj64[0] || !sj64[1] || !vaux) return VPMC_ERR_WORKSPACE;
+    }
     const size_t fb = vpmc_field_workspace_bytes(mx, fnx, fny);
     const size_t rb = radix_workspace_bytes((int)S, 4);
     void *fws = cv.take<char>(fb > rb ? fb : rb);   // the field builds, then the sort
@@ -1010,7 +1434,8 @@ int vpmc_step_f64(vpmc_mc_desc *d, int n_total, double *pos_xy, const double *w,
         if (S.n > 0) {
             hipLaunchKernelGGL(k_f64_recs, dim3((unsigned)((S.n + 255) / 256)), dim3(256), 0, s,
                                S.n, S.count_dev, S.field.order, subs[k], (const double2 *)pos_xy,
-                               w, omega0, k ? -1.0 : 1.0, rec[k], set64[k]);
+                               w, omega0, k ? -1.0 : 1.0, rec[k], set64[k],
+                               (const double4 *)jac64, G64[k], sj64[k]);
             VPMC_CHECK_LAUNCH();
         }
     }
@@ -1020,7 +1445,14 @@ int vpmc_step_f64(vpmc_mc_desc *d, int n_total, double *pos_xy, const double *w,
         a.rec[k] = rec[k];
         a.set64[k] = set64[k];
         a.cell_start[k] = d->set[k].field.cell_start;
+        a.G64[k] = G64[k];
+        a.setjac64[k] = sj64[k];
     }
+    // the sample generation reads F through the FP64 copies only
+    m.set[0].set_jac = m.set[1].set_jac = nullptr;
+    a.om2 = def ? contrib : nullptr;   // (om, om0) in place of the contributions
+    a.vel_aux = vaux;
+    a.aux_dx = (double)d->aux_dx;
     a.geom = m.set[0].field.g;
     a.h_w = (double)d->h_w;
     a.inv_hw = 1.0 / a.h_w;
@@ -1045,12 +1477,39 @@ int vpmc_step_f64(vpmc_mc_desc *d, int n_total, double *pos_xy, const double *w,
     hipLaunchKernelGGL(k_f64_eval, dim3(gs), dim3(ADJ_THREADS), 0, s, m, g, a,
                        (const uint32_t *)order, (const uint32_t *)key_sorted);
     VPMC_CHECK_LAUNCH();
-    hipLaunchKernelGGL(k_f64_reduce, dim3((unsigned)((2 * (size_t)n_total + 255) / 256)),
-                       dim3(256), 0, s, n_total, g.NS, a.inv_tot, (const double2 *)contrib, vel);
+    const unsigned gp = (unsigned)((2 * (size_t)n_total + 255) / 256);
+    if (def) {
+        hipLaunchKernelGGL(k_f64_reduce5, dim3(gp), dim3(256), 0, s, m, g, a, vel);
+        hipLaunchKernelGGL(k_f64_jac, dim3(gn), dim3(256), 0, s, n_total,
+                           (const double2 *)vaux, 0.5 * dt / (double)d->aux_dx,
+                           (double4 *)jac64);
+    } else {
+        hipLaunchKernelGGL(k_f64_reduce, dim3(gp), dim3(256), 0, s, n_total, g.NS, a.inv_tot,
+                           (const double2 *)contrib, vel);
+    }
     hipLaunchKernelGGL(k_f64_advect, dim3(gn), dim3(256), 0, s, n_total, (double2 *)pos_xy,
                        (const double2 *)vel, dt, a.L, (float *)nullptr);
     VPMC_CHECK_LAUNCH();
     return VPMC_OK;
+}
+
+int vpmc_step_f64(vpmc_mc_desc *d, int n_total, double *pos_xy, const double *w,
+                  const float *omega0, const int *sub_pos, const int *sub_neg, double dt,
+                  void *workspace, size_t workspace_bytes, void *stream)
+{
+    if (d && d->aux) return VPMC_ERR_ARG;   // the deformed step: vpmc_step_f64_deformed
+    return step_f64_impl(d, n_total, pos_xy, w, omega0, sub_pos, sub_neg, dt, nullptr,
+                         workspace, workspace_bytes, as_stream(stream));
+}
+
+int vpmc_step_f64_deformed(vpmc_mc_desc *d, int n_total, double *pos_xy, const double *w,
+                           const float *omega0, const int *sub_pos, const int *sub_neg,
+                           double dt, double *jac64, void *workspace, size_t workspace_bytes,
+                           void *stream)
+{
+    if (!jac64) return VPMC_ERR_ARG;
+    return step_f64_impl(d, n_total, pos_xy, w, omega0, sub_pos, sub_neg, dt, jac64, workspace,
+                         workspace_bytes, as_stream(stream));
 }
 
 size_t vpmc_grid_vorticity_workspace_bytes(int n, int nx, int ny)

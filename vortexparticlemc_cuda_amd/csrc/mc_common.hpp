@@ -157,12 +157,16 @@ __device__ __forceinline__ void kper_smooth(const McD &d, float rx, float ry, fl
 // (qx, qy): the query point inside the box; (rx, ry): the displacement the
 // MIS contribution uses (reference: dst - sp unwrapped; consistent: min image).
 // src (optional) receives the set-local index of a branch-1 sample's
-// particle, -1 for a branch-2 sample.
+// particle, -1 for a branch-2 sample; jit (optional) the consistent branch-1
+// jitter before the deformation F is applied (the frozen offset the adjoint
+// differentiates F through), (0, 0) otherwise.
 template <int MODE, bool PER>
 __device__ __forceinline__ bool gen_sample(const McD &d, const SetD &S, int n_s, int owner,
                                            int si, int k, float2 dst, float &qx, float &qy,
-                                           float &rx, float &ry, int *src = nullptr)
+                                           float &rx, float &ry, int *src = nullptr,
+                                           float2 *jit = nullptr)
 {
+    if (jit) *jit = make_float2(0.f, 0.f);
     const bool b1 = k < d.N1;
     int t = 0, kb = b1 ? k : k - d.N1;   // t: target of a disk sample (0 = owner)
     if (!b1 && kb >= d.N2) {
@@ -208,6 +212,7 @@ __device__ __forceinline__ bool gen_sample(const McD &d, const SetD &S, int n_s,
             float s, c;
             sincos_rev(u[2], s, c);
             float jx = rho * c, jy = rho * s;
+            if (jit) *jit = make_float2(jx, jy);
             if (S.set_jac) {   // the deformed blob: F_j u (src/cuda/monte_carlo_bs.cu:210-212)
                 const float4 F = S.set_jac[j];
                 const float ex = fmaf(F.x, jx, F.y * jy), ey = fmaf(F.z, jx, F.w * jy);
