@@ -208,15 +208,19 @@ public:
     // numbers, vpmc_step_adjoint): the current positions are the state the
     // forward step `step` started from; lambda (device float2[N_cur]) holds
     // dL/dx after that step and becomes dL/dx before it; grad_w (device
-    // float[N_cur]) accumulates dL/dw.  Consistent periodic estimator,
-    // no kernel deformation.
-    void step_adjoint(uint32_t step, float2 *lambda, float *grad_w);
+    // float[N_cur]) accumulates dL/dw.  Consistent periodic estimator.  With
+    // kernel_deformation (vpmc_step_adjoint_deformed) particles.F is the F
+    // the step started from and jac_bar (device float4[N_cur]) holds dL/dF
+    // after the step and becomes dL/dF before it.
+    void step_adjoint(uint32_t step, float2 *lambda, float *grad_w, float4 *jac_bar = nullptr);
     // The FP64 forward of the same frozen-sample step (vpmc_step_f64): pos
     // (device double2[N_cur]) advances in place by the velocity at pos with
     // strengths w (device double[N_cur]); the draws, the sampling tables and
-    // the MIS density are those of the current strengths.  No counterpart in
-    // the reference: the accurate trajectory for step_adjoint's gradients.
-    void step_f64(uint32_t step, double2 *pos, const double *w);
+    // the MIS density are those of the current strengths; with
+    // kernel_deformation, jac64 (device double4[N_cur]) is the FP64 F,
+    // advanced in place (vpmc_step_f64_deformed).  No counterpart in the
+    // reference: the accurate trajectory for step_adjoint's gradients.
+    void step_f64(uint32_t step, double2 *pos, const double *w, double4 *jac64 = nullptr);
     int subset_count(int sign) const { return sign > 0 ? cnt_pos_ : cnt_neg_; }
 
     struct Impl;

@@ -62,15 +62,17 @@ def test_phash_simulation_steps_like_the_mirror(device):
     assert not np.array_equal(c.positions(), s.positions())
 
 
-@pytest.mark.parametrize("traj", ["f64", "f32"])
-def test_phash_rollout_gradient_matches_python_autograd(device, traj):
+@pytest.mark.parametrize("traj,deform", [("f64", False), ("f32", False), ("f64", True),
+                                         ("f32", True)])
+def test_phash_rollout_gradient_matches_python_autograd(device, traj, deform):
     import torch
 
     import vortexparticlemc_cuda_amd as v
     from vortexparticlemc_cuda_amd import adjoint as A
     from vortexparticlemc_cuda_amd._capi import MODE_CONSISTENT
     from vortexparticlemc_cuda_amd.sim import SimParam, Simulation
-    _say("phash.rollout backward == adjoint.rollout backward, %s trajectory" % traj)
+    _say("phash.rollout backward == adjoint.rollout backward, %s trajectory, deform=%s"
+         % (traj, deform))
     phash = v.phash()
     pos, w = _problem()
     T, K = 5, 2
@@ -78,10 +80,12 @@ def test_phash_rollout_gradient_matches_python_autograd(device, traj):
     P.N1 = P.N2 = 16
     s = phash.Simulation(P)
     s.seed = 3
+    s.kernel_deformation = deform   # the state is (positions, F), F = I at the start
     s.set_particles(pos, w)
     Pm = SimParam(dt=2e-3, resX=32, hwr=4)
     Pm.N1 = Pm.N2 = 16
-    m = Simulation(Pm, device=device, cells_per_hw=8, mode=MODE_CONSISTENT, seed=3)
+    m = Simulation(Pm, device=device, cells_per_hw=8, mode=MODE_CONSISTENT, seed=3,
+                   deform=deform)
     m.set_particles(pos, w)
     rng = np.random.default_rng(1)
     target = torch.as_tensor(rng.standard_normal((pos.shape[0], 2)).astype(np.float32),

@@ -109,15 +109,38 @@ void store_positions(Simulation &sim, const Tensor &p)
            "rollout");
 }
 
-// steps t0 .. t1-1 of the FP64 trajectory from X (device (N, 2) double, in
-// place); states receives the positions before each step
-void run_f64(Simulation &sim, Tensor &X, const Tensor &w64, int64_t t0, int64_t t1,
-             std::vector<Tensor> *states)
+// particles.F as a (N, 4) tensor (kernel deformation: F is part of the state)
+Tensor jac_tensor(Simulation &sim, const Tensor &like)
 {
+    Tensor out = torch::empty({sim.particles_.N_cur, 4}, like.options().dtype(torch::kFloat32));
+    hip_ok(hipMemcpy(out.data_ptr<float>(), sim.particles_.d_F(),
+                     sim.particles_.N_cur * sizeof(float4), hipMemcpyDeviceToDevice),
+           "rollout");
+    return out;
+}
+
+void store_jac(Simulation &sim, const Tensor &F)
+{
+    hip_ok(hipMemcpy(sim.particles_.d_F(), F.data_ptr<float>(),
+                     sim.particles_.N_cur * sizeof(float4), hipMemcpyDeviceToDevice),
+           "rollout");
+}
+
+// steps t0 .. t1-1 of the FP64 trajectory from X (device (N, 2) double, in
+// place) and, with kernel deformation, F64 (device (N, 4) double, in place);
+// states receives the state before each step (positions, then F)
+void run_f64(Simulation &sim, Tensor &X, Tensor &F64, const Tensor &w64, int64_t t0,
+             int64_t t1, std::vector<Tensor> *states)
+{
+    const bool def = sim.kernel_deformation;
     for (int64_t t = t0; t < t1; ++t) {
-        if (states) states->push_back(X.clone());
+        if (states) {
+            states->push_back(X.clone());
+            if (def) states->push_back(F64.clone());
+        }
         sim.step_f64((uint32_t)t, reinterpret_cast<double2 *>(X.data_ptr<double>()),
-                     w64.data_ptr<double>());
+                     w64.data_ptr<double>(),
+                     def ? reinterpret_cast<double4 *>(F64.data_ptr<double>()) : nullptr);
     }
 }
 
@@ -125,9 +148,12 @@ void run_f64(Simulation &sim, Tensor &X, const Tensor &w64, int64_t t0, int64_t 
 // 0 .. steps-1), differentiable in pos0 and w under common random numbers.
 // traj64: the FP64 forward of the frozen-sample step (Simulation::step_f64),
 // positions returned in FP64, the step adjoints linearised at those states
-// rounded to FP32; otherwise step_cuda(true).  The forward keeps the
-// positions every `checkpoint` steps; the backward recomputes each segment
-// from its checkpoint and runs Simulation::step_adjoint backwards through it.
+// rounded to FP32; otherwise step_cuda(true).  The forward keeps the state
+// every `checkpoint` steps; the backward recomputes each segment from its
+// checkpoint and runs Simulation::step_adjoint backwards through it.  With
+// kernel deformation the state is (positions, F): the rollout starts from
+// the simulation's current F and its checkpoints keep F beside the
+// positions (one or two tensors per checkpoint).
 struct RolloutFunction : public torch::autograd::Function<RolloutFunction> {
     static Tensor forward(torch::autograd::AutogradContext *ctx, const Tensor &pos0,
                           const Tensor &w, Simulation *sim, int64_t steps, int64_t checkpoint,
@@ -135,21 +161,26 @@ struct RolloutFunction : public torch::autograd::Function<RolloutFunction> {
     {
         const Tensor p = device_f32(pos0, "rollout"), ww = device_f32(w, "rollout");
         load_state(*sim, p, ww);
+        const bool def = sim->kernel_deformation;
         const int64_t K = checkpoint > 0 ? checkpoint : 1;
         std::vector<Tensor> ckpt;
         Tensor out;
         if (traj64) {
             Tensor X = pos0.detach().to(torch::kFloat64).contiguous().clone();
             const Tensor w64 = w.detach().to(torch::kFloat64).contiguous();
+            Tensor F64 = def ? jac_tensor(*sim, p).to(torch::kFloat64).contiguous() : Tensor();
             for (int64_t t0 = 0; t0 < steps; t0 += K) {
                 ckpt.push_back(X.clone());
-                run_f64(*sim, X, w64, t0, std::min(steps, t0 + K), nullptr);
+                if (def) ckpt.push_back(F64.clone());
+                run_f64(*sim, X, F64, w64, t0, std::min(steps, t0 + K), nullptr);
             }
+            if (def) store_jac(*sim, F64.to(torch::kFloat32).contiguous());
             ckpt.push_back(w64);
             out = X;
         } else {
             for (int64_t t0 = 0; t0 < steps; t0 += K) {
                 ckpt.push_back(positions_tensor(*sim, p));
+                if (def) ckpt.push_back(jac_tensor(*sim, p));
                 for (int64_t t = t0; t < std::min(steps, t0 + K); ++t) {
                     sim->step_index = (uint32_t)t;
                     sim->step_cuda(true);
@@ -163,6 +194,7 @@ struct RolloutFunction : public torch::autograd::Function<RolloutFunction> {
         ctx->saved_data["steps"] = steps;
         ctx->saved_data["K"] = K;
         ctx->saved_data["traj64"] = traj64;
+        ctx->saved_data["def"] = def;
         ctx->saved_data["dtype"] = pos0.scalar_type();
         return traj64 ? out : positions_tensor(*sim, p).to(pos0.scalar_type());
     }
@@ -173,35 +205,47 @@ struct RolloutFunction : public torch::autograd::Function<RolloutFunction> {
         Simulation *sim = reinterpret_cast<Simulation *>((intptr_t)ctx->saved_data["sim"].toInt());
         const int64_t steps = ctx->saved_data["steps"].toInt(), K = ctx->saved_data["K"].toInt();
         const bool traj64 = ctx->saved_data["traj64"].toBool();
+        const bool def = ctx->saved_data["def"].toBool();
+        if (def != sim->kernel_deformation)
+            throw std::invalid_argument("rollout backward: kernel_deformation changed");
         std::vector<Tensor> ckpt = ctx->get_saved_variables();
         Tensor w64;
         if (traj64) {   // the strengths ride behind the checkpoints
             w64 = ckpt.back();
             ckpt.pop_back();
         }
+        const int per = def ? 2 : 1;   // tensors per checkpoint
         const int N = sim->particles_.N_cur;
         Tensor lam = grad[0].detach().to(torch::kFloat32).contiguous().clone();
         Tensor gw = torch::zeros({N}, lam.options());
+        Tensor jbar = def ? torch::zeros({N, 4}, lam.options()) : Tensor();   // dL/dF_T = 0
         hip_ok(hipDeviceSynchronize(), "rollout backward");
-        for (int64_t seg = (int64_t)ckpt.size() - 1; seg >= 0; --seg) {
+        const int64_t nseg = (int64_t)ckpt.size() / per;
+        for (int64_t seg = nseg - 1; seg >= 0; --seg) {
             const int64_t t0 = seg * K, t1 = std::min(steps, t0 + K);
-            std::vector<Tensor> states;
+            std::vector<Tensor> states;   // per step: positions (then F)
             if (traj64) {   // recompute in FP64, linearise at the rounded states
-                Tensor X = ckpt[seg].clone();
-                run_f64(*sim, X, w64, t0, t1, &states);
+                Tensor X = ckpt[per * seg].clone();
+                Tensor F64 = def ? ckpt[per * seg + 1].clone() : Tensor();
+                run_f64(*sim, X, F64, w64, t0, t1, &states);
                 for (Tensor &st : states) st = st.to(torch::kFloat32).contiguous();
             } else {
-                store_positions(*sim, ckpt[seg]);
+                store_positions(*sim, ckpt[per * seg]);
+                if (def) store_jac(*sim, ckpt[per * seg + 1]);
                 for (int64_t t = t0; t < t1; ++t) {   // recompute the segment
                     states.push_back(positions_tensor(*sim, lam));
+                    if (def) states.push_back(jac_tensor(*sim, lam));
                     sim->step_index = (uint32_t)t;
                     sim->step_cuda(true);
                 }
             }
             for (int64_t t = t1 - 1; t >= t0; --t) {
-                store_positions(*sim, states[t - t0]);
+                store_positions(*sim, states[per * (t - t0)]);
+                if (def) store_jac(*sim, states[per * (t - t0) + 1]);
                 sim->step_adjoint((uint32_t)t, reinterpret_cast<float2 *>(lam.data_ptr<float>()),
-                                  gw.data_ptr<float>());
+                                  gw.data_ptr<float>(),
+                                  def ? reinterpret_cast<float4 *>(jbar.data_ptr<float>())
+                                      : nullptr);
             }
         }
         sim->step_index = (uint32_t)steps;
@@ -305,8 +349,6 @@ PYBIND11_MODULE(phash, m)
         "rollout",
         [](Simulation &sim, const Tensor &pos0, const Tensor &w, int64_t steps,
            int64_t checkpoint, const std::string &trajectory) {
-            if (sim.kernel_deformation)
-                throw std::invalid_argument("rollout: kernel deformation is not differentiated");
             if (trajectory != "f64" && trajectory != "f32")
                 throw std::invalid_argument("rollout: trajectory must be 'f64' or 'f32'");
             return RolloutFunction::apply(pos0, w, &sim, steps, checkpoint, trajectory == "f64");
@@ -319,5 +361,6 @@ PYBIND11_MODULE(phash, m)
         "the FP64 forward of the frozen-sample step, positions returned in FP64, gradients "
         "to ~1e-5 of the exact derivative; 'f32': step_cuda(true), ~3x cheaper, the FP32 "
         "rounding of the positions limits the gradient to ~1e-4.  The simulation must "
-        "outlive the backward pass.");
+        "outlive the backward pass.  With kernel_deformation the state is (positions, F): "
+        "the rollout starts from the simulation's current F (vpmc_step_adjoint_deformed).");
 }

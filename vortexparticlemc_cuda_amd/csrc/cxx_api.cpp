@@ -832,12 +832,11 @@ void Simulation::step_cuda(bool periodic)
     ++step_index;
 }
 
-void Simulation::step_adjoint(uint32_t step, float2 *lambda, float *grad_w)
+void Simulation::step_adjoint(uint32_t step, float2 *lambda, float *grad_w, float4 *jac_bar)
 {
     // the adjoint of step_cuda(true) at Philox step `step`, from the state the
-    // forward step started from (the current positions)
-    if (kernel_deformation)
-        throw std::invalid_argument("step_adjoint: kernel deformation is not differentiated");
+    // forward step started from (the current positions, and particles.F with
+    // kernel deformation)
     Impl &I = *impl_;
     const int N = particles_.N_cur;
     if (I.nx == 0) do_spatial_hashing();
@@ -847,6 +846,26 @@ void Simulation::step_adjoint(uint32_t step, float2 *lambda, float *grad_w)
                                raw_ptr(cdf_neg_), I.pts, I.cell_start, I.order, I.set_xy, I.nx,
                                I.ny, raw_ptr(I.vel_sign), true);
     d.step = step;
+    if (kernel_deformation) {   // F is state: dL/dF' in jac_bar becomes dL/dF
+        if (!jac_bar)
+            throw std::invalid_argument("step_adjoint: a deformed step needs jac_bar (dL/dF)");
+        attach_deformation(d, I, deformation_bound);
+        I.vel_aux.resize(8 * (size_t)N);
+        d.aux = 1;
+        d.aux_dx = P_.dx;
+        d.vel_aux = (float *)raw_ptr(I.vel_aux);
+        d.jac = (float *)raw_ptr(particles_.F);
+        const size_t b = vpmc_step_adjoint_deformed_workspace_bytes(&d);
+        if (b == 0)
+            throw std::invalid_argument(
+                "step_adjoint: the adjoint differentiates the consistent periodic estimator");
+        vcheck(vpmc_step_adjoint_deformed(&d, N, (const float *)particles_.d_p(),
+                                          particles_.d_omega(), raw_ptr(sub_pos_),
+                                          raw_ptr(sub_neg_), P_.dt, (float *)lambda,
+                                          (float *)jac_bar, grad_w, I.ws.get(b), b, stream),
+               "step_adjoint");
+        return;
+    }
     const size_t b = vpmc_step_adjoint_workspace_bytes(&d);
     if (b == 0)
         throw std::invalid_argument(
@@ -857,10 +876,8 @@ void Simulation::step_adjoint(uint32_t step, float2 *lambda, float *grad_w)
            "step_adjoint");
 }
 
-void Simulation::step_f64(uint32_t step, double2 *pos, const double *w)
+void Simulation::step_f64(uint32_t step, double2 *pos, const double *w, double4 *jac64)
 {
-    if (kernel_deformation)
-        throw std::invalid_argument("step_f64: kernel deformation is not supported");
     Impl &I = *impl_;
     const int N = particles_.N_cur;
     if (I.nx == 0) do_spatial_hashing();
@@ -870,12 +887,27 @@ void Simulation::step_f64(uint32_t step, double2 *pos, const double *w)
                                raw_ptr(cdf_neg_), I.pts, I.cell_start, I.order, I.set_xy, I.nx,
                                I.ny, raw_ptr(I.vel_sign), true);
     d.step = step;
+    if (kernel_deformation) {   // the FP64 F advances in jac64
+        if (!jac64)
+            throw std::invalid_argument("step_f64: a deformed step needs jac64 (F in FP64)");
+        attach_deformation(d, I, deformation_bound);
+        I.vel_aux.resize(8 * (size_t)N);   // the descriptor's aux buffer (FP64 sums stay apart)
+        d.aux = 1;
+        d.aux_dx = P_.dx;
+        d.vel_aux = (float *)raw_ptr(I.vel_aux);
+    }
     const size_t b = vpmc_step_f64_workspace_bytes(&d);
     if (b == 0)
         throw std::invalid_argument("step_f64: the consistent periodic estimator only");
-    vcheck(vpmc_step_f64(&d, N, (double *)pos, w, particles_.d_omega(), raw_ptr(sub_pos_),
-                         raw_ptr(sub_neg_), (double)P_.dt, I.ws.get(b), b, stream),
-           "step_f64");
+    if (kernel_deformation)
+        vcheck(vpmc_step_f64_deformed(&d, N, (double *)pos, w, particles_.d_omega(),
+                                      raw_ptr(sub_pos_), raw_ptr(sub_neg_), (double)P_.dt,
+                                      (double *)jac64, I.ws.get(b), b, stream),
+               "step_f64");
+    else
+        vcheck(vpmc_step_f64(&d, N, (double *)pos, w, particles_.d_omega(), raw_ptr(sub_pos_),
+                             raw_ptr(sub_neg_), (double)P_.dt, I.ws.get(b), b, stream),
+               "step_f64");
 }
 
 void Simulation::compute_velocity_naive(bool periodic, float eps_core)
