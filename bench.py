@@ -876,7 +876,93 @@ def adjoint_bench(args, device):
         out["fd_seconds"] = time.perf_counter() - t0
     del sim
     torch.cuda.empty_cache()
+    if args.adj_deform_fd > 0:
+        out["deformed"] = adjoint_deformed_bench(args, device, pos, pos_t, w_tg, w0, rng)
     return out
+
+
+def adjoint_deformed_bench(args, device, pos, pos_t, w_tg, w0, rng):
+    """The adjoint workload with the reference's deformed CUDA step (aux
+    targets and F every step, F = I at the start; the reference run deformed
+    too): forward and backward along the FP64 trajectory, and dL/dx0, dL/dw
+    against central differences (eps = 1e-6) of the FP64 deformed rollout on
+    `adj_deform_fd` coordinates."""
+    import torch
+
+    from vortexparticlemc_cuda_amd import adjoint as A
+    from vortexparticlemc_cuda_amd.sim import SimParam, Simulation
+
+    Nx, hwr, N, T, K, G = args.adj_Nx, 16, 64, args.adj_steps, 10, 64
+    P = SimParam(dt=1e-3, resX=Nx, hwr=hwr)
+    P.N1 = P.N2 = N
+    h = float(P.h_w)
+    sim = Simulation(P, device=device, cells_per_hw=8, seed=0, deform=True)
+    sim.set_particles(pos, w_tg.cpu().numpy())
+    n = pos.shape[0]
+    I = torch.tensor([1.0, 0.0, 0.0, 1.0], device=device).repeat(n, 1)
+    with torch.no_grad():
+        X_ref = A.rollout_f64(sim, pos_t.double(), w_tg.double(), T, F0=I)
+        ref = A.grid_vorticity_f64(X_ref, w_tg.double(), h, G, G)
+    sim.set_particles(pos, w_tg.cpu().numpy())   # F = I again
+    p = pos_t.clone().requires_grad_(True)
+    wt = w0.clone().requires_grad_(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    XT = A.rollout(sim, p, wt, T, checkpoint=K, trajectory="f64")
+    loss = ((A.grid_vorticity(XT, wt, h, G, G, f64=True) - ref) ** 2).mean()
+    torch.cuda.synchronize()
+    t_fwd = time.perf_counter() - t0
+    max_det = float((sim.jac[:n, 0] * sim.jac[:n, 3] - sim.jac[:n, 1] * sim.jac[:n, 2] - 1)
+                    .abs().max())
+    max_FmI = float((sim.jac[:n] - I).abs().max())
+    t0 = time.perf_counter()
+    loss.backward()
+    torch.cuda.synchronize()
+    t_bwd = time.perf_counter() - t0
+    gp = p.grad.double().cpu().numpy()
+    gw = wt.grad.double().cpu().numpy()
+    log("adjoint, deformed step: loss %.9g, forward %.3f s, backward %.3f s, max |F_T - I| %.3g"
+        % (float(loss.detach()), t_fwd, t_bwd, max_FmI))
+    p64, w64 = pos_t.double(), w0.double()
+
+    def L(pp, ww):
+        X = A.rollout_f64(sim, pp, ww, T, F0=I)
+        o = A.grid_vorticity_f64(X, ww, h, G, G)
+        return float(((o - ref) ** 2).mean())
+
+    t0 = time.perf_counter()
+    wn = w0.abs().cpu().numpy()
+    cand = np.nonzero(wn > 0.2 * wn.max())[0]
+    pick = rng.choice(cand, size=args.adj_deform_fd, replace=False)
+    axis = rng.integers(2, size=args.adj_deform_fd)
+    eps_x, eps_w = 1e-6, 1e-6 * float(w0.abs().max())
+    ex, ew = [], []
+    for c, i in enumerate(pick):
+        if c % 2 == 0:
+            k = int(axis[c])
+            a, b = p64.clone(), p64.clone()
+            a[i, k] += eps_x
+            b[i, k] -= eps_x
+            fd = (L(a, w64) - L(b, w64)) / (2 * eps_x)
+            ex.append(abs(fd - gp[i, k]) / np.abs(gp).max())
+        else:
+            a, b = w64.clone(), w64.clone()
+            a[i] += eps_w
+            b[i] -= eps_w
+            fd = (L(p64, a) - L(p64, b)) / (2 * eps_w)
+            ew.append(abs(fd - gw[i]) / np.abs(gw).max())
+    t_fd = time.perf_counter() - t0
+    log("adjoint, deformed step, vs FP64 central differences (eps 1e-6): max err/max|g| "
+        "positions %.3g, strengths %.3g (%d coordinates, %.1f s)"
+        % (max(ex), max(ew), len(pick), t_fd))
+    del sim
+    torch.cuda.empty_cache()
+    return {"semantics": "aux targets at dst -/+ dx and kernel deformation F every step "
+                         "(F = I at the start), FP64 trajectory",
+            "loss": float(loss.detach()), "forward_s": t_fwd, "backward_s": t_bwd,
+            "max_abs_F_minus_I": max_FmI, "max_abs_det_minus_1": max_det,
+            "fd_coordinates": int(len(pick)), "fd_max_err_pos_eps1e-6": max(ex),
+            "fd_max_err_w_eps1e-6": max(ew), "fd_seconds": t_fd}
 
 
 def read_traffic(kernel, workload, want):
@@ -1156,6 +1242,8 @@ def main():
     ap.add_argument("--adj-Nx", type=int, default=128,
                     help="lattice side of the adjoint workload (0: skip it)")
     ap.add_argument("--adj-steps", type=int, default=50)
+    ap.add_argument("--adj-deform-fd", type=int, default=6,
+                    help="coordinates of the deformed adjoint's FD check (0: no deformed run)")
     ap.add_argument("--adj-fd", type=int, default=16,
                     help="coordinates checked by FP64 central differences")
     args = ap.parse_args()
