@@ -41,6 +41,21 @@
 //   sources  stable radix sort of the source gradients by source particle,
 //            then one thread per particle adds its run in sample order
 // Every sum has a fixed order, so the gradients are deterministic.
+//
+// The deformed step (src/cuda/simulation_step.cu:51-73: four auxiliary
+// targets at x_i -/+ dx e_x, x_i +/- dx e_y and F per particle) has the
+// state (x, F): particle m's blob is W(|F_m^-1 (y - x_m)|) through its
+// quadratic form G_m = F_m^-T F_m^-1, a branch-1 sample is y = x_j + F_j u
+// with u frozen, and the step returns F' = renorm(I + dt/(2 dx)
+// [u_R - u_L | u_T - u_B]) beside x'.  Its adjoint (vpmc_step_adjoint_deformed)
+// runs the step's forward estimate, turns dL/dF' into the four aux targets'
+// adjoint weights (k_jac_adj), and then the passes above with: every
+// branch-1 sample weighted at the five targets, each disk sample at its own
+// (k_adj_final); om and grad om = sum a_m W'(q)/(q h_w^2) G_m (y - x_m) per
+// sample (k_adj_eval_def); dL/dG_m = a_m (beta + beta0) W'(q)/(2 q h_w^2)
+// (dx^2, dx dy, dy^2) for (G11, 2 G12, G22), mapped to dL/dF_m
+// (k_adj_field<true>, quad_form_vjp); and dL/dF_j += (dL/dy) u^T from the
+// source records (k_adj_sources).
 #include <math.h>
 
 #include <mutex>
