@@ -130,7 +130,7 @@ typedef struct vpmc_field {
     const int   *order;               /* sorted position -> local index        */
     /* deformed kernels (F in the query, include/vorticity_query.cuh:52-66):
        gq (optional) holds per record the quadratic form G = F^-T F^-1 of
-       its particle's deformation F as float4 (G11, 2 G12, G22, 0), so that
+       its particle's deformation F as float4 (G11, 2 G12, G22, sigma_max(F)^2), so that
        the kernel is W(sqrt(d^T G d)) = W(|F^-1 d|); queries then search
        gq_bound * h_w (0: 2, the reference's 2 h_w cutoff), the largest
        stretch sigma_max(F) the records may carry. */

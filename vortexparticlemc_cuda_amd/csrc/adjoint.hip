@@ -354,6 +354,9 @@ __global__ void __launch_bounds__(ADJ_THREADS)
     float acc = 0.f, gx = 0.f, gy = 0.f;
     auto fn = [&](const float4 &c, float dx, float dy, int k) {   // (dx, dy) = x - y
         const float4 Q = f.gq[k];   // (G11, 2 G12, G22, sigma_max^2)
+        // the search covers the deformation bound; this record reaches only
+        // sigma_max h_w (beyond it its W and W' are exactly zero)
+        if (fmaf(dx, dx, dy * dy) > Q.w * g.rs2_1) return;
         const float r2 = fmaxf(fmaf(dx, fmaf(Q.y, dy, Q.x * dx), Q.z * dy * dy), 0.f);
         float w, gr;
         spline_wg(r2, m.inv_hw, w, gr);
@@ -581,6 +584,9 @@ __global__ void __launch_bounds__(ADJ_THREADS)
                 float w, gr;
                 if constexpr (DEF) {
                     const float4 G = Q[j];
+                    // the stencil covers the deformation bound, the particle
+                    // only its own reach sigma_max h_w (zero terms beyond)
+                    if (fmaf(dx, dx, dy * dy) > G.w * g.rs2_1) continue;
                     const float r2 =
                         fmaxf(fmaf(dx, fmaf(G.y, dy, G.x * dx), G.z * dy * dy), 0.f);
                     spline_wg(r2, d.inv_hw, w, gr);
@@ -874,7 +880,7 @@ struct F64Args {
     double2 *y, *r;             // per sample: query point, owner - point
     Kper64 kc;
     // deformed step (all NULL / 0 otherwise): per set, cell order, the
-    // records' quadratic forms (G11, 2 G12, G22, 0) of F = jac64; per set,
+    // records' quadratic forms (G11, 2 G12, G22, sigma_max^2) of F = jac64; per set,
     // local order, F; per sample (om, om0); the aux estimates
     // [sign][L, R, T, B][owner]
     const double4 *G64[2];
@@ -884,14 +890,17 @@ struct F64Args {
     double aux_dx;
 };
 
-// G = F^-T F^-1 of F = (a b; c d) as (G11, 2 G12, G22, 0), FP64
+// G = F^-T F^-1 of F = (a b; c d) as (G11, 2 G12, G22, sigma_max(F)^2), FP64
+// (sigma_max^2 the larger eigenvalue of F F^T, with a relative margin far
+// above FP64 rounding: |F^-1 d| >= |d| / sigma_max)
 __device__ __forceinline__ double4 quad_form_f64(double4 F)
 {
     const double det = F.x * F.w - F.y * F.z;
     const double i2 = 1.0 / (det * det);
     const double p = F.x * F.x + F.y * F.y, q = F.z * F.z + F.w * F.w;
     const double r = F.x * F.z + F.y * F.w;
-    return make_double4(q * i2, -2.0 * r * i2, p * i2, 0.0);
+    const double smax2 = 0.5 * ((p + q) + sqrt((p - q) * (p - q) + 4.0 * r * r));
+    return make_double4(q * i2, -2.0 * r * i2, p * i2, smax2 * (1.0 + 1e-12));
 }
 
 __global__ void k_f64_recs(int n_s, const int *__restrict__ count_dev,
@@ -1012,8 +1021,9 @@ __global__ void __launch_bounds__(ADJ_THREADS) k_f64_eval(McD d, BinGeom g, F64A
         auto fn = [&](const double4 &c, double dx, double dy, int k) {
             double w, gr;
             double r2 = dx * dx + dy * dy;
-            if (G64) {   // deformed record: |F^-1 d|^2
+            if (G64) {   // deformed record: |F^-1 d|^2, zero beyond sigma_max h_w
                 const double4 G = G64[k];
+                if (r2 > G.w * a.h_w * a.h_w) return;
                 r2 = fmax(G.x * dx * dx + G.y * dx * dy + G.z * dy * dy, 0.0);
             }
             spline_wg64(r2, a.inv_hw, w, gr);

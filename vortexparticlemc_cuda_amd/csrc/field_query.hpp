@@ -19,7 +19,7 @@ struct FieldD {
     const float4 *pts;
     const int *cell_start;
     FieldGeom g;
-    const float4 *gq;   // deformed records: quadratic forms (G11, 2 G12, G22, 0), or NULL
+    const float4 *gq;   // deformed records: quadratic forms (G11, 2 G12, G22, sigma_max^2), or NULL
 };
 
 struct EvalCount {
