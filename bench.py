@@ -579,6 +579,10 @@ def ensemble_headline(args, device):
                                      "the launch structure of monte_carlo_bs_kernel_safe",
             "target_batched_over_plain": 5.0,
             "cells_per_hw": cph, "spl": spl, "eval_threads": th, "autotune_ms": tune_ms,
+            # kernel evaluations within h_w per second: over the evaluation
+            # kernel's own time, and over the whole batched step
+            "kernel_evals_per_s": n_in / eval_s,
+            "kernel_evals_per_s_step": n_in / (t_graph / K),
             "deformed": {"ms_per_step": t_def * 1e3, "particle_steps_per_s": N / t_def,
                          "samples_per_pair": N1 + 5 * N1},
             "roofline": {"kernel": "k_ens_eval", "bound": "valu",
