@@ -398,13 +398,20 @@ def ensemble_error(ens, pos, w, hw, probes, device):
     return (num / den) ** 0.5
 
 
-def plain_ensemble_loop(Ps, pos, w, shape, device, seed=0):
-    """one plain Simulation per member (the per-simulation GPU path)"""
+def plain_ensemble_loop(Ps, pos, w, shape, device, seed=0, N1=None):
+    """one plain Simulation per member (the per-simulation GPU path); N1:
+    N1 = N2 other than the members' own"""
+    import copy
+
     from vortexparticlemc_cuda_amd.sim import Simulation
     cph, spl, th = shape
     sims = []
     for s in range(len(pos)):
-        sim = Simulation(Ps[s], device=device, cells_per_hw=cph, seed=seed, sim=s, spl=spl,
+        P = Ps[s]
+        if N1 is not None:
+            P = copy.copy(P)
+            P.N1 = P.N2 = int(N1)
+        sim = Simulation(P, device=device, cells_per_hw=cph, seed=seed, sim=s, spl=spl,
                          eval_threads=th)
         sim.set_particles(pos[s], w[s])
         sims.append(sim)
@@ -557,6 +564,11 @@ def ensemble_headline(args, device):
     per_launch = N * Kd / t_direct
     del sims
     torch.cuda.empty_cache()
+    trend = [{"N1_eq_N2": N1, "rms_err_vs_direct": err, "batched_ms_per_step": t_graph / K * 1e3,
+              "plain_ms_per_step": t_plain / Kp * 1e3, "batched_over_plain": batched / plain}]
+    for n1 in args.ens_N_trend:   # lower errors: more samples, same launch shapes
+        trend.append(ensemble_trend_point(Ps, pos, w, hw, n1, dt, (cph, spl, th), pshape,
+                                          args.ens_probes, device))
     flop = FLOP_PER_EVAL * n_in
     return {"workload": "64 periodic simulations on [0,1]^2, Nx from {64,96,128,192,256} "
                         "(seed 20261203), 8 Gaussian vortices each, hwr=8, N1=N2=%d, dt=5e-4, "
@@ -578,6 +590,9 @@ def ensemble_headline(args, device):
             "reference_launch_note": "per-simulation loop with one workgroup per (owner, sign), "
                                      "the launch structure of monte_carlo_bs_kernel_safe",
             "target_batched_over_plain": 5.0,
+            # batched over plain as the error falls (N1 = N2 grows; the two
+            # paths compute the same estimates, so their errors are equal)
+            "error_trend": trend,
             "cells_per_hw": cph, "spl": spl, "eval_threads": th, "autotune_ms": tune_ms,
             # kernel evaluations within h_w per second: over the evaluation
             # kernel's own time, and over the whole batched step
@@ -595,6 +610,40 @@ def ensemble_headline(args, device):
                          "step_frac_of_fp32_peak": (flop + FLOP_PER_SAMPLE * N * 2 * 2 * N1) /
                                                    (t_graph / K) / PEAK_FP32},
             "rbf_init_s": t_cg, "_data": (Ps, pos, w)}
+
+
+def ensemble_trend_point(Ps, pos, w, hw, n1, dt, shape, pshape, probes, device, K=3):
+    """the ensemble at N1 = N2 = n1: its RMS error (step 0), the batched step
+    (eager, launch shape `shape`) and the plain per-simulation loop (launch
+    shape `pshape`), each timed over K steps"""
+    import torch
+
+    from vortexparticlemc_cuda_amd.ensemble import Ensemble
+    cph, spl, th = shape
+    ens = Ensemble(pos, w, hw, N1=n1, N2=n1, dt=dt, seed=0, cells_per_hw=cph, spl=spl,
+                   eval_threads=th, device=device)
+    ens.step(1)
+    err = ensemble_error(ens, pos, w, hw, probes, device)
+    ens.step(1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ens.step(K)
+    torch.cuda.synchronize()
+    tb = (time.perf_counter() - t0) / K
+    ens.close()
+    del ens
+    torch.cuda.empty_cache()
+    sims = plain_ensemble_loop(Ps, pos, w, pshape, device, N1=n1)
+    for sim in sims:
+        sim.step()
+    tp = time_plain_loop(sims, K) / K
+    del sims
+    torch.cuda.empty_cache()
+    log("ensemble at N1=N2=%d: RMS error %.4g, batched %.1f ms/step, plain %.1f ms/step (%.2fx)"
+        % (n1, err, tb * 1e3, tp * 1e3, tp / tb))
+    return {"N1_eq_N2": n1, "rms_err_vs_direct": err, "batched_ms_per_step": tb * 1e3,
+            "plain_ms_per_step": tp * 1e3, "batched_over_plain": tp / tb,
+            "steps_timed": K, "timing": "eager steps (no graph) for both paths"}
 
 
 PLAIN_ENS_TUNE = ((8, 1, 256), (6, 1, 256), (5, 1, 256), (4, 1, 256), (8, 2, 256), (12, 1, 256))
@@ -1205,6 +1254,8 @@ def main():
     ap.add_argument("--ens-shape", default="",
                     help="cells_per_hw,spl,eval_threads of the ensemble instead of the tuner's")
     ap.add_argument("--ens-plain-steps", type=int, default=10)
+    ap.add_argument("--ens-N-trend", type=int, nargs="*", default=[256],
+                    help="N1 = N2 values of the batched-over-plain error trend (none: skip)")
     ap.add_argument("--ens-direct-steps", type=int, default=2)
     ap.add_argument("--ens-deform-steps", type=int, default=5)
     ap.add_argument("--Nx", type=int, default=200)
