@@ -45,7 +45,50 @@ struct BinBufs {
     int *start;               // ntile + 1 tile offsets into srec
     void *sort_ws;
     size_t sort_bytes;
+    int *coff;                // tile chunks: ntile + 1 offsets
+    int4 *ctab;               // chunk -> (tile, first sample, end, 0)
+    void *scan_ws;
+    size_t scan_bytes;
 };
+
+// samples per lane and workgroup size of the evaluation launch (spl 0: 2,
+// threads 0: 256, as launch_eval picks them)
+inline int eval_spl(int spl) { return spl == 1 ? 1 : (spl == 4 ? 4 : 2); }
+inline int eval_threads_of(int threads) { return threads == 0 ? 256 : threads; }
+// the smallest chunk any launch shape uses (64-thread tiles, 1 sample per
+// lane): sizes the chunk table for every shape
+constexpr int TILE_CHUNK_MIN = TILE_CHUNK_BATCHES * WAVE;
+
+// chunks of tile t: its samples over `chunk` (0 for an empty tile);
+// nch[ntile] = 0 (the scan's total slot)
+__global__ void k_tile_nch(int ntile, const int *__restrict__ start, int chunk,
+                           int *__restrict__ nch)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t > ntile) return;
+    nch[t] = t < ntile ? (start[t + 1] - start[t] + chunk - 1) / chunk : 0;
+}
+
+// the chunk table: tile t writes its chunks' entries (most tiles have one)
+__global__ void k_chunk_fill(int ntile, const int *__restrict__ coff,
+                             const int *__restrict__ start, int chunk, int4 *__restrict__ ctab)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ntile) return;
+    const int b = start[t], e = start[t + 1];
+    for (int c = coff[t], p = b; c < coff[t + 1]; ++c, p += chunk)
+        ctab[c] = make_int4(t, p, min(p + chunk, e), 0);
+}
+
+int tile_chunks_build(int *coff, int ntile, const int *start, int chunk, int4 *ctab,
+                      void *scan_ws, size_t scan_bytes, hipStream_t s)
+{
+    VPMC_TRY(scan_i32(coff, coff, ntile + 1, false, nullptr, scan_ws, scan_bytes, s));
+    hipLaunchKernelGGL(k_chunk_fill, dim3((unsigned)((ntile + 255) / 256)), dim3(256), 0, s, ntile,
+                       (const int *)coff, start, chunk, ctab);
+    VPMC_CHECK_LAUNCH();
+    return VPMC_OK;
+}
 
 template <int MODE, bool PER>
 __global__ void __launch_bounds__(BIN_THREADS)
@@ -59,15 +102,15 @@ __global__ void __launch_bounds__(BIN_THREADS)
 
 template <int THREADS, int SPL, bool COUNT, bool DEF = false>
 __global__ void __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(EVAL_WAVES_PER_EU<THREADS, SPL, DEF>)))
-    k_bin_eval(McD m, BinGeom g, const int *__restrict__ start, const SRec *__restrict__ srec,
-               float *__restrict__ om, unsigned long long *__restrict__ eval_count)
+    k_bin_eval(McD m, BinGeom g, const int *__restrict__ start, TileChunks tc,
+               const SRec *__restrict__ srec, float *__restrict__ om,
+               unsigned long long *__restrict__ eval_count)
 {
-    // workgroups are dealt round-robin to the 8 XCDs: give each XCD a
-    // contiguous range of tiles so that neighbouring cells share its L2
-    const int tile = (int)(blockIdx.x & 7) * g.per_xcd + (int)(blockIdx.x >> 3);
-    if (tile >= g.ntile) return;
-    const int beg = start[tile], end = start[tile + 1];
-    if (beg >= end) return;   // uniform over the workgroup
+    // one workgroup per tile chunk; workgroups are dealt round-robin to the
+    // 8 XCDs, and each XCD takes a contiguous range of chunks so that
+    // neighbouring cells share its L2
+    int tile, beg, end;
+    if (!chunk_range(tc, g.ntile, tile, beg, end)) return;   // uniform
     __shared__ TileLds<THREADS, DEF> L;
     EvalCount ec;
     bin_eval_tile<THREADS, SPL, COUNT, DEF>(m, g, tile, beg, end, srec, om, L, ec);
@@ -94,40 +137,59 @@ __global__ void __launch_bounds__(BIN_THREADS)
 // eval launch for runtime workgroup-size and samples-per-lane choices
 template <int THREADS, bool COUNT, bool DEF>
 static void launch_eval_t(int spl, const McD &m, const BinGeom &g, const int *start,
-                          const SRec *srec, float *om, unsigned long long *cnt, hipStream_t s)
+                          const TileChunks &tc, const SRec *srec, float *om,
+                          unsigned long long *cnt, hipStream_t s)
 {
-    const dim3 grid((unsigned)(8 * g.per_xcd)), block(THREADS);
+    const size_t cmax = tile_chunks_max(g.ntile, g.S, tc.chunk);
+    const dim3 grid((unsigned)(8 * ((cmax + 7) / 8))), block(THREADS);
     if (spl == 1)
         hipLaunchKernelGGL((k_bin_eval<THREADS, 1, COUNT, DEF>), grid, block, 0, s, m, g, start,
-                           srec, om, cnt);
+                           tc, srec, om, cnt);
     else if (spl == 4)
         hipLaunchKernelGGL((k_bin_eval<THREADS, 4, COUNT, DEF>), grid, block, 0, s, m, g, start,
-                           srec, om, cnt);
+                           tc, srec, om, cnt);
     else
         hipLaunchKernelGGL((k_bin_eval<THREADS, 2, COUNT, DEF>), grid, block, 0, s, m, g, start,
-                           srec, om, cnt);
+                           tc, srec, om, cnt);
 }
 
 template <bool COUNT, bool DEF>
 static void launch_eval_d(int threads, int spl, const McD &m, const BinGeom &g,
-                          const int *start, const SRec *srec, float *om,
+                          const int *start, const TileChunks &tc, const SRec *srec, float *om,
                           unsigned long long *cnt, hipStream_t s)
 {
-    if (threads == 64) launch_eval_t<64, COUNT, DEF>(spl, m, g, start, srec, om, cnt, s);
-    else if (threads == 128) launch_eval_t<128, COUNT, DEF>(spl, m, g, start, srec, om, cnt, s);
-    else launch_eval_t<256, COUNT, DEF>(spl, m, g, start, srec, om, cnt, s);
+    if (threads == 64) launch_eval_t<64, COUNT, DEF>(spl, m, g, start, tc, srec, om, cnt, s);
+    else if (threads == 128)
+        launch_eval_t<128, COUNT, DEF>(spl, m, g, start, tc, srec, om, cnt, s);
+    else launch_eval_t<256, COUNT, DEF>(spl, m, g, start, tc, srec, om, cnt, s);
+}
+
+// the tiles' chunks (TILE_CHUNK_BATCHES batches of the launch shape each,
+// TILE_CHUNK_DEF times that for deformed tiles) from the sort's tile offsets
+static int build_chunks(int threads, int spl, bool def, const BinGeom &g, const BinBufs &B,
+                        TileChunks *tc, hipStream_t s)
+{
+    threads = eval_threads_of(threads);
+    if (threads != 64 && threads != 128 && threads != 256) return VPMC_ERR_ARG;
+    *tc = TileChunks{B.coff, B.ctab,
+                     TILE_CHUNK_BATCHES * threads * eval_spl(spl) * (def ? TILE_CHUNK_DEF : 1)};
+    hipLaunchKernelGGL(k_tile_nch, dim3((unsigned)((g.ntile + 1 + 255) / 256)), dim3(256), 0, s,
+                       g.ntile, (const int *)B.start, tc->chunk, B.coff);
+    return tile_chunks_build(B.coff, g.ntile, B.start, tc->chunk, B.ctab, B.scan_ws,
+                             B.scan_bytes, s);
 }
 
 // deformed sets (field.gq != NULL) take the quadratic-form records
 template <bool COUNT>
-static int launch_eval(int threads, int spl, const McD &m, const BinGeom &g, const int *start,
-                       const SRec *srec, float *om, unsigned long long *cnt, hipStream_t s)
+static int launch_eval(int threads, int spl, const McD &m, const BinGeom &g, const BinBufs &B,
+                       const TileChunks &tc, unsigned long long *cnt, hipStream_t s)
 {
-    if (threads == 0) threads = 256;
+    threads = eval_threads_of(threads);
     if (threads != 64 && threads != 128 && threads != 256) return VPMC_ERR_ARG;
     if (!tile_threads_ok(g, threads)) return VPMC_ERR_GRID;
-    if (m.set[0].field.gq) launch_eval_d<COUNT, true>(threads, spl, m, g, start, srec, om, cnt, s);
-    else launch_eval_d<COUNT, false>(threads, spl, m, g, start, srec, om, cnt, s);
+    if (m.set[0].field.gq)
+        launch_eval_d<COUNT, true>(threads, spl, m, g, B.start, tc, B.srec, B.om, cnt, s);
+    else launch_eval_d<COUNT, false>(threads, spl, m, g, B.start, tc, B.srec, B.om, cnt, s);
     return VPMC_OK;
 }
 
@@ -192,7 +254,9 @@ static size_t bin_ws_bytes(const BinGeom &g)
     size_t S = g.S > 0 ? g.S : 1, T = (size_t)g.ntile;
     return align256(S * 4) + align256(S * sizeof(SRec)) + align256(S * 8) + align256(S * 4) +
            align256(S * sizeof(SRec)) +
-           align256(S * 4) + align256((T + 1) * 4) + radix_workspace_bytes((int)S, sizeof(SRec)) + 4096;
+           align256(S * 4) + align256((T + 1) * 4) + radix_workspace_bytes((int)S, sizeof(SRec)) +
+           align256((T + 1) * 4) + align256(tile_chunks_max(g.ntile, S, TILE_CHUNK_MIN) * 16) +
+           scan_workspace_bytes(g.ntile + 1, 4) + 4096;
 }
 
 static int carve_bin(const BinGeom &g, void *ws, size_t bytes, BinBufs *B)
@@ -208,8 +272,12 @@ static int carve_bin(const BinGeom &g, void *ws, size_t bytes, BinBufs *B)
     B->start = cv.take<int>(T + 1);
     B->sort_bytes = radix_workspace_bytes((int)S, sizeof(SRec));
     B->sort_ws = cv.take<char>(B->sort_bytes);
+    B->coff = cv.take<int>(T + 1);
+    B->ctab = cv.take<int4>(tile_chunks_max(g.ntile, S, TILE_CHUNK_MIN));
+    B->scan_bytes = scan_workspace_bytes(g.ntile + 1, 4);
+    B->scan_ws = cv.take<char>(B->scan_bytes);
     if (!B->key || !B->rec0 || !B->r || !B->om || !B->srec || !B->keys_sorted || !B->start ||
-        !B->sort_ws)
+        !B->sort_ws || !B->coff || !B->ctab || !B->scan_ws)
         return VPMC_ERR_WORKSPACE;
     return VPMC_OK;
 }
@@ -289,13 +357,14 @@ int mc_binned_impl(const vpmc_mc_desc *d, void *ws, size_t ws_bytes, hipStream_t
         VPMC_TRY(radix_sort_samples(B.key, (int)g.S, key_bits_for((uint32_t)g.ntile),
                                     B.rec0, B.srec, B.keys_sorted, g.ntile, B.start,
                                     B.sort_ws, B.sort_bytes, s));
+        TileChunks tc;
+        VPMC_TRY(build_chunks(d->eval_threads, d->spl, mc.set[0].field.gq != nullptr, g, B, &tc,
+                              s));
         if (o0 == 0) VPMC_TRY(record_event(d->eval_events[0], s));
         if (d->eval_count)
-            VPMC_TRY(launch_eval<true>(d->eval_threads, d->spl, mc, g, B.start, B.srec, B.om,
-                                       d->eval_count, s));
+            VPMC_TRY(launch_eval<true>(d->eval_threads, d->spl, mc, g, B, tc, d->eval_count, s));
         else
-            VPMC_TRY(launch_eval<false>(d->eval_threads, d->spl, mc, g, B.start, B.srec, B.om,
-                                        nullptr, s));
+            VPMC_TRY(launch_eval<false>(d->eval_threads, d->spl, mc, g, B, tc, nullptr, s));
         VPMC_CHECK_LAUNCH();
         if (o0 + cnt >= m.n_owner) VPMC_TRY(record_event(d->eval_events[1], s));
         launch_mode(d->mode, d->periodic, mc, g, B, true, s);

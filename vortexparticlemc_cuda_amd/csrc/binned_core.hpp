@@ -50,6 +50,60 @@ inline void set_ns_division(BinGeom *g)
 // grid of the tiles and stencil for descriptor geometry (host)
 int bin_geom(const McD &m, int N1, int N2, int n_owner, BinGeom *g);
 
+// Tile chunks.  Branch-1 samples follow |w|, so the tiles at a vortex core
+// hold tens of times the average sample count; one workgroup per tile would
+// make such a tile the critical path of the launch.  A tile's samples are
+// evaluated in chunks of TILE_CHUNK_BATCHES batches of THREADS * SPL samples,
+// one workgroup per chunk (each stages the tile's candidates): a sample's
+// terms and their order do not depend on which workgroup evaluates it, and a
+// chunk boundary falls on a batch boundary, so the tile's last (partly
+// filled) wave is the same with and without chunks.
+constexpr int TILE_CHUNK_BATCHES = 8;
+// deformed tiles: three times the samples (the aux targets' disk samples)
+// and a costlier staging of the 2 h_w stencil, so longer chunks
+constexpr int TILE_CHUNK_DEF = 3;
+
+struct TileChunks {
+    const int *coff;    // ntile + 1 chunk offsets; coff[ntile] = chunks
+    const int4 *ctab;   // chunk -> (tile, first sample, end, 0)
+    int chunk;          // samples per chunk
+};
+
+// chunks of a launch over ntile tiles and S samples: an upper bound the host
+// sizes the grid and the buffers by (each chunk past a tile's first holds
+// `chunk` samples)
+inline size_t tile_chunks_max(int ntile, size_t S, int chunk)
+{
+    return (size_t)ntile + S / (size_t)chunk;
+}
+
+// workgroup blockIdx.x of a grid of 8 * ceil(max / 8) -> the chunk's tile and
+// sample range (the chunk count is a scalar load every workgroup of a CU
+// shares, the chunk's entry one load); chunk c is taken by XCD
+// c / ceil(C / 8), so each XCD runs a contiguous range of tiles (their
+// stencils share its L2).  false (uniform over the workgroup): no chunk.
+__device__ __forceinline__ bool chunk_range(const TileChunks &tc, int ntile, int &tile, int &beg,
+                                            int &end)
+{
+    const int C = tc.coff[ntile];
+    const int per = (C + 7) >> 3;
+    const int i = (int)(blockIdx.x >> 3);
+    if (i >= per) return false;
+    const int c = (int)(blockIdx.x & 7) * per + i;
+    if (c >= C) return false;
+    const int4 e = tc.ctab[c];
+    tile = e.x;
+    beg = e.y;
+    end = e.z;
+    return true;
+}
+
+// host: chunk offsets and the chunk table from the per-tile chunk counts in
+// coff[0 .. ntile] (coff[ntile] = 0 on entry; scanned in place) and the tile
+// offsets; scan workspace: scan_workspace_bytes(ntile + 1, 4)
+int tile_chunks_build(int *coff, int ntile, const int *start, int chunk, int4 *ctab,
+                      void *scan_ws, size_t scan_bytes, hipStream_t s);
+
 // One sample: draws, query point, MIS displacement and tile key.  Writes
 // key[out] (tile_base + local tile, or dead_key without a query),
 // rec0[out] = (the point's offsets in its cell, out) (the record the sort

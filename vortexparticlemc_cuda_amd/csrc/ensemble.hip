@@ -136,15 +136,28 @@ __global__ void __launch_bounds__(BIN_THREADS)
     bin_gen_one<MODE, true>(d, g, sid, out, (uint32_t)t.toff[s], dk, key, rec0, r);
 }
 
+// chunks of tile k of the ensemble's tiles (0 for an empty tile and for a
+// simulation's dead tile, whose samples have no query); nch[ntile] = 0
+__global__ void k_ens_tile_nch(EnsTabs t, int n_sim, int ntile, const int *__restrict__ start,
+                               int chunk, int *__restrict__ nch)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k > ntile) return;
+    int c = 0;
+    if (k < ntile) {
+        const int s = seg_of(t.toff, n_sim, k);
+        if (k - t.toff[s] < t.geom[s].ntile) c = (start[k + 1] - start[k] + chunk - 1) / chunk;
+    }
+    nch[k] = c;
+}
+
 template <int THREADS, int SPL, bool DEF>
 __global__ void __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(EVAL_WAVES_PER_EU<THREADS, SPL, DEF>)))
-    k_ens_eval(EnsTabs t, int n_sim, int ntile, int per_xcd, const int *__restrict__ start,
+    k_ens_eval(EnsTabs t, int n_sim, int ntile, const int *__restrict__ start, TileChunks tc,
                const SRec *__restrict__ srec, float *__restrict__ om)
 {
-    const int tile = (int)(blockIdx.x & 7) * per_xcd + (int)(blockIdx.x >> 3);
-    if (tile >= ntile) return;
-    const int beg = start[tile], end = start[tile + 1];
-    if (beg >= end) return;   // uniform over the workgroup
+    int tile, beg, end;   // one workgroup per tile chunk (chunk_range)
+    if (!chunk_range(tc, ntile, tile, beg, end)) return;   // uniform
     const int s = seg_of(t.toff, n_sim, tile);
     if (tile - t.toff[s] >= t.geom[s].ntile) return;   // the dead tile: no query
     __shared__ TileLds<THREADS, DEF> L;
@@ -215,6 +228,11 @@ struct EnsLayout {
     float *om = nullptr;
     SRec *srec = nullptr;
     int *tstart = nullptr;
+    int *tcoff = nullptr;                     // tile chunks: offsets, table
+    int4 *ctab = nullptr;
+    void *cscan_ws = nullptr;
+    size_t cscan_bytes = 0;
+    int chunk = 0;                            // samples per tile chunk
     int4 *sblk = nullptr;
     float4 *fgq = nullptr, *set_jac = nullptr;   // deformed fields / sets
     float2 *vel_aux = nullptr;
@@ -274,6 +292,11 @@ static int ens_plan(const vpmc_ensemble_desc *e, EnsLayout *L)
     L->E = L->eoff[2 * ns];
     L->C = L->coff[2 * ns];
     L->per_xcd = (L->T + 7) / 8;
+    {   // the evaluation's launch shape (as ens_estimate picks it) sets the chunk
+        const int th = e->eval_threads ? e->eval_threads : 256;
+        const int spl = e->spl == 1 ? 1 : (e->spl == 4 ? 4 : 2);
+        L->chunk = TILE_CHUNK_BATCHES * th * spl * (L->def ? TILE_CHUNK_DEF : 1);
+    }
     std::vector<uint32_t> base(ns);
     for (int s = 0; s < ns; ++s) base[s] = (uint32_t)L->toff[s];
     VPMC_TRY(radix_seg_blocks(L->soff, base, &L->blk, &L->blk_base));
@@ -326,6 +349,10 @@ static size_t ens_carve(EnsLayout &L, void *ws, size_t bytes)
     L.srec = cv.take<SRec>(S);
     L.skeys_sorted = cv.take<uint32_t>(S);
     L.tstart = cv.take<int>((size_t)L.T + 1);
+    L.tcoff = cv.take<int>((size_t)L.T + 1);
+    L.ctab = cv.take<int4>(tile_chunks_max(L.T, S, L.chunk));
+    L.cscan_bytes = scan_workspace_bytes(L.T + 1, 4);
+    L.cscan_ws = cv.take<char>(L.cscan_bytes);
     L.sblk = cv.take<int4>((size_t)(L.nblk > 0 ? L.nblk : 1));
     L.sblk_base = cv.take<uint32_t>((size_t)(L.nblk > 0 ? L.nblk : 1));
     L.ssort_bytes = radix_seg_workspace_bytes((int)S, L.nblk);
@@ -430,11 +457,18 @@ static int ens_estimate(const vpmc_ensemble_desc *e, EnsLayout &L, hipStream_t s
                                   L.skeys_sorted, L.T, L.tstart, L.sblk, L.sblk_base, L.nblk,
                                   L.ssort_ws, L.ssort_bytes, st));
     const int th = e->eval_threads ? e->eval_threads : 256;
+    // the tiles' chunks from the tile offsets
+    hipLaunchKernelGGL(k_ens_tile_nch, dim3((unsigned)((L.T + 1 + 255) / 256)), dim3(256), 0, st,
+                       L.t, ns, L.T, (const int *)L.tstart, L.chunk, L.tcoff);
+    VPMC_TRY(tile_chunks_build(L.tcoff, L.T, L.tstart, L.chunk, L.ctab, L.cscan_ws,
+                               L.cscan_bytes, st));
+    const TileChunks tc{L.tcoff, L.ctab, L.chunk};
+    const size_t cmax = tile_chunks_max(L.T, L.S, L.chunk);
+    const unsigned egrid = (unsigned)(8 * ((cmax + 7) / 8));
     VPMC_TRY(record_event(e->eval_events[0], st));
 #define VPMC_ENS_EVAL(TH, SP, DF)                                                              \
-    hipLaunchKernelGGL((k_ens_eval<TH, SP, DF>), dim3((unsigned)(8 * L.per_xcd)), dim3(TH), 0,  \
-                       st, L.t, ns, L.T, L.per_xcd, (const int *)L.tstart,                    \
-                       (const SRec *)L.srec, L.om)
+    hipLaunchKernelGGL((k_ens_eval<TH, SP, DF>), dim3(egrid), dim3(TH), 0, st, L.t, ns, L.T,   \
+                       (const int *)L.tstart, tc, (const SRec *)L.srec, L.om)
 #define VPMC_ENS_EVAL_SPL(TH, DF)                        \
     do {                                                 \
         if (e->spl == 1) VPMC_ENS_EVAL(TH, 1, DF);       \
