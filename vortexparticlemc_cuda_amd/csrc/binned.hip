@@ -102,8 +102,8 @@ __global__ void __launch_bounds__(BIN_THREADS)
 
 template <int THREADS, int SPL, bool COUNT, bool DEF = false>
 __global__ void __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(EVAL_WAVES_PER_EU<THREADS, SPL, DEF>)))
-    k_bin_eval(McD m, BinGeom g, const int *__restrict__ start, TileChunks tc,
-               const SRec *__restrict__ srec, float *__restrict__ om,
+    k_bin_eval(McD m, BinGeom g, TileChunks tc, const SRec *__restrict__ srec,
+               float *__restrict__ om,
                unsigned long long *__restrict__ eval_count)
 {
     // one workgroup per tile chunk; workgroups are dealt round-robin to the
@@ -136,32 +136,30 @@ __global__ void __launch_bounds__(BIN_THREADS)
 
 // eval launch for runtime workgroup-size and samples-per-lane choices
 template <int THREADS, bool COUNT, bool DEF>
-static void launch_eval_t(int spl, const McD &m, const BinGeom &g, const int *start,
-                          const TileChunks &tc, const SRec *srec, float *om,
-                          unsigned long long *cnt, hipStream_t s)
+static void launch_eval_t(int spl, const McD &m, const BinGeom &g, const TileChunks &tc,
+                          const SRec *srec, float *om, unsigned long long *cnt, hipStream_t s)
 {
     const size_t cmax = tile_chunks_max(g.ntile, g.S, tc.chunk);
     const dim3 grid((unsigned)(8 * ((cmax + 7) / 8))), block(THREADS);
     if (spl == 1)
-        hipLaunchKernelGGL((k_bin_eval<THREADS, 1, COUNT, DEF>), grid, block, 0, s, m, g, start,
-                           tc, srec, om, cnt);
+        hipLaunchKernelGGL((k_bin_eval<THREADS, 1, COUNT, DEF>), grid, block, 0, s, m, g, tc,
+                           srec, om, cnt);
     else if (spl == 4)
-        hipLaunchKernelGGL((k_bin_eval<THREADS, 4, COUNT, DEF>), grid, block, 0, s, m, g, start,
-                           tc, srec, om, cnt);
+        hipLaunchKernelGGL((k_bin_eval<THREADS, 4, COUNT, DEF>), grid, block, 0, s, m, g, tc,
+                           srec, om, cnt);
     else
-        hipLaunchKernelGGL((k_bin_eval<THREADS, 2, COUNT, DEF>), grid, block, 0, s, m, g, start,
-                           tc, srec, om, cnt);
+        hipLaunchKernelGGL((k_bin_eval<THREADS, 2, COUNT, DEF>), grid, block, 0, s, m, g, tc,
+                           srec, om, cnt);
 }
 
 template <bool COUNT, bool DEF>
 static void launch_eval_d(int threads, int spl, const McD &m, const BinGeom &g,
-                          const int *start, const TileChunks &tc, const SRec *srec, float *om,
+                          const TileChunks &tc, const SRec *srec, float *om,
                           unsigned long long *cnt, hipStream_t s)
 {
-    if (threads == 64) launch_eval_t<64, COUNT, DEF>(spl, m, g, start, tc, srec, om, cnt, s);
-    else if (threads == 128)
-        launch_eval_t<128, COUNT, DEF>(spl, m, g, start, tc, srec, om, cnt, s);
-    else launch_eval_t<256, COUNT, DEF>(spl, m, g, start, tc, srec, om, cnt, s);
+    if (threads == 64) launch_eval_t<64, COUNT, DEF>(spl, m, g, tc, srec, om, cnt, s);
+    else if (threads == 128) launch_eval_t<128, COUNT, DEF>(spl, m, g, tc, srec, om, cnt, s);
+    else launch_eval_t<256, COUNT, DEF>(spl, m, g, tc, srec, om, cnt, s);
 }
 
 // the tiles' chunks (TILE_CHUNK_BATCHES batches of the launch shape each,
@@ -188,8 +186,8 @@ static int launch_eval(int threads, int spl, const McD &m, const BinGeom &g, con
     if (threads != 64 && threads != 128 && threads != 256) return VPMC_ERR_ARG;
     if (!tile_threads_ok(g, threads)) return VPMC_ERR_GRID;
     if (m.set[0].field.gq)
-        launch_eval_d<COUNT, true>(threads, spl, m, g, B.start, tc, B.srec, B.om, cnt, s);
-    else launch_eval_d<COUNT, false>(threads, spl, m, g, B.start, tc, B.srec, B.om, cnt, s);
+        launch_eval_d<COUNT, true>(threads, spl, m, g, tc, B.srec, B.om, cnt, s);
+    else launch_eval_d<COUNT, false>(threads, spl, m, g, tc, B.srec, B.om, cnt, s);
     return VPMC_OK;
 }
 

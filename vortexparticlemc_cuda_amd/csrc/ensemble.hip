@@ -153,8 +153,8 @@ __global__ void k_ens_tile_nch(EnsTabs t, int n_sim, int ntile, const int *__res
 
 template <int THREADS, int SPL, bool DEF>
 __global__ void __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(EVAL_WAVES_PER_EU<THREADS, SPL, DEF>)))
-    k_ens_eval(EnsTabs t, int n_sim, int ntile, const int *__restrict__ start, TileChunks tc,
-               const SRec *__restrict__ srec, float *__restrict__ om)
+    k_ens_eval(EnsTabs t, int n_sim, int ntile, TileChunks tc, const SRec *__restrict__ srec,
+               float *__restrict__ om)
 {
     int tile, beg, end;   // one workgroup per tile chunk (chunk_range)
     if (!chunk_range(tc, ntile, tile, beg, end)) return;   // uniform
@@ -468,7 +468,7 @@ static int ens_estimate(const vpmc_ensemble_desc *e, EnsLayout &L, hipStream_t s
     VPMC_TRY(record_event(e->eval_events[0], st));
 #define VPMC_ENS_EVAL(TH, SP, DF)                                                              \
     hipLaunchKernelGGL((k_ens_eval<TH, SP, DF>), dim3(egrid), dim3(TH), 0, st, L.t, ns, L.T,   \
-                       (const int *)L.tstart, tc, (const SRec *)L.srec, L.om)
+                       tc, (const SRec *)L.srec, L.om)
 #define VPMC_ENS_EVAL_SPL(TH, DF)                        \
     do {                                                 \
         if (e->spl == 1) VPMC_ENS_EVAL(TH, 1, DF);       \
