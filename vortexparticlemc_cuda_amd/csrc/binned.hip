@@ -59,25 +59,25 @@ inline int eval_threads_of(int threads) { return threads == 0 ? 256 : threads; }
 // lane): sizes the chunk table for every shape
 constexpr int TILE_CHUNK_MIN = TILE_CHUNK_BATCHES * WAVE;
 
-// chunks of tile t: its samples over `chunk` (0 for an empty tile);
-// nch[ntile] = 0 (the scan's total slot)
+// chunks of tile t (tile_nch); nch[ntile] = 0 (the scan's total slot)
 __global__ void k_tile_nch(int ntile, const int *__restrict__ start, int chunk,
                            int *__restrict__ nch)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t > ntile) return;
-    nch[t] = t < ntile ? (start[t + 1] - start[t] + chunk - 1) / chunk : 0;
+    nch[t] = t < ntile ? tile_nch(start[t + 1] - start[t], chunk) : 0;
 }
 
-// the chunk table: tile t writes its chunks' entries (most tiles have one)
+// the chunk table: tile t writes its chunks' entries (most tiles have one);
+// chunks of `chunk` samples, the last one to the tile's end
 __global__ void k_chunk_fill(int ntile, const int *__restrict__ coff,
                              const int *__restrict__ start, int chunk, int4 *__restrict__ ctab)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= ntile) return;
-    const int b = start[t], e = start[t + 1];
-    for (int c = coff[t], p = b; c < coff[t + 1]; ++c, p += chunk)
-        ctab[c] = make_int4(t, p, min(p + chunk, e), 0);
+    const int b = start[t], e = start[t + 1], c0 = coff[t], c1 = coff[t + 1];
+    for (int c = c0, p = b; c < c1; ++c, p += chunk)
+        ctab[c] = make_int4(t, p, c + 1 < c1 ? p + chunk : e, 0);
 }
 
 int tile_chunks_build(int *coff, int ntile, const int *start, int chunk, int4 *ctab,

@@ -53,8 +53,9 @@ int bin_geom(const McD &m, int N1, int N2, int n_owner, BinGeom *g);
 // Tile chunks.  Branch-1 samples follow |w|, so the tiles at a vortex core
 // hold tens of times the average sample count; one workgroup per tile would
 // make such a tile the critical path of the launch.  A tile's samples are
-// evaluated in chunks of TILE_CHUNK_BATCHES batches of THREADS * SPL samples,
-// one workgroup per chunk (each stages the tile's candidates): a sample's
+// evaluated in chunks of TILE_CHUNK_BATCHES batches of THREADS * SPL samples
+// (the last chunk up to twice that), one workgroup per chunk (each stages
+// the tile's candidates): a sample's
 // terms and their order do not depend on which workgroup evaluates it, and a
 // chunk boundary falls on a batch boundary, so the tile's last (partly
 // filled) wave is the same with and without chunks.
@@ -69,9 +70,18 @@ struct TileChunks {
     int chunk;          // samples per chunk
 };
 
+// chunks of a tile of cnt samples: cnt / chunk (at least one for a tile with
+// samples), the last taking the remainder, so that a chunk holds `chunk` to
+// 2 chunk - 1 samples and a tile just past `chunk` is not split off into a
+// nearly empty chunk that stages the whole stencil again
+__device__ __forceinline__ int tile_nch(int cnt, int chunk)
+{
+    return cnt > 0 ? max(1, cnt / chunk) : 0;
+}
+
 // chunks of a launch over ntile tiles and S samples: an upper bound the host
-// sizes the grid and the buffers by (each chunk past a tile's first holds
-// `chunk` samples)
+// sizes the grid and the buffers by (each chunk past a tile's first holds at
+// least `chunk` samples)
 inline size_t tile_chunks_max(int ntile, size_t S, int chunk)
 {
     return (size_t)ntile + S / (size_t)chunk;

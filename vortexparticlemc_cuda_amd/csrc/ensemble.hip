@@ -146,7 +146,7 @@ __global__ void k_ens_tile_nch(EnsTabs t, int n_sim, int ntile, const int *__res
     int c = 0;
     if (k < ntile) {
         const int s = seg_of(t.toff, n_sim, k);
-        if (k - t.toff[s] < t.geom[s].ntile) c = (start[k + 1] - start[k] + chunk - 1) / chunk;
+        if (k - t.toff[s] < t.geom[s].ntile) c = tile_nch(start[k + 1] - start[k], chunk);
     }
     nch[k] = c;
 }
