@@ -148,6 +148,38 @@ def test_split_tail_waves_change_no_bit(device, threads, hwr):
     sim.chunk_samples = 0
 
 
+def test_tile_chunks_change_no_bit(device):
+    """A tile's samples are evaluated in chunks of 8 batches (512 samples
+    with 64-thread tiles, 2048 with 256-thread ones), one workgroup each.  On
+    a Gaussian-vortex member of the ensemble workload (hwr 8, N1 = N2 = 64)
+    branch-1 samples crowd the vortex cores, so over all owners the heaviest
+    tiles split into several chunks; with the owners in chunks of 397 a
+    tile holds a few hundredth of those samples and rarely splits.  For each
+    workgroup size the velocities must agree bit for bit (a sample's terms
+    and their order do not depend on the workgroup evaluating it)."""
+    from vortexparticlemc_cuda_amd.sim import SimParam, Simulation
+    from vortexparticlemc_cuda_amd.workloads import gaussian_ensemble
+    _say("tile chunks, Gaussian-vortex member")
+    nx, pos, om, _ = gaussian_ensemble(4)[0]
+    P = SimParam(dt=5e-4, resX=nx, hwr=8)
+    sim = Simulation(P, device=device, cells_per_hw=5, mode=1, seed=3)
+    h2 = (1.0 / nx) ** 2
+    sim.set_particles(pos, (om * h2).astype(np.float32))
+    sim.spl = 1
+    for threads in (64, 256):
+        out = []
+        for chunk_samples in (0, 2 * 128 * 397):
+            sim.eval_threads = threads
+            sim.chunk_samples = chunk_samples
+            sim.rebuild_fields()
+            sim.vel_sign.zero_()
+            sim.launch_estimate(sim.desc(64, 64))
+            out.append(sim.vel_sign.cpu().numpy().copy())
+        assert np.abs(out[0]).max() > 0
+        assert np.array_equal(out[0], out[1]), threads
+    sim.chunk_samples = 0
+
+
 def test_reference_step_at_large_size_runs_in_chunks(device):
     """the CUDA step's semantics (aux targets + F) on the large run: double
     shear layer, 1,048,576 particles, hwr = 8, N1 = N2 = 256: 3.2e9 sample
