@@ -95,9 +95,14 @@ __global__ void __launch_bounds__(BIN_THREADS)
     k_bin_gen(McD d, BinGeom g, uint32_t *__restrict__ key, SRec *__restrict__ rec0,
               float2 *__restrict__ r)
 {
-    size_t sid = (size_t)blockIdx.x * BIN_THREADS + threadIdx.x;
-    if (sid >= g.S) return;
-    bin_gen_one<MODE, PER>(d, g, sid, sid, 0u, (uint32_t)g.ntile, key, rec0, r);
+    // two samples a lane, 64 apart (each wave's stores stay 64 consecutive
+    // samples wide), their loads overlapped (bin_gen_pair)
+    const size_t a = (size_t)blockIdx.x * (2 * BIN_THREADS) + (threadIdx.x / WAVE) * (2 * WAVE) +
+                     (threadIdx.x & (WAVE - 1));
+    if (a >= g.S) return;
+    const size_t ids[2] = {a, a + WAVE};
+    bin_gen_pair<MODE, PER>(d, g, ids, ids, ids[1] < g.S ? 2 : 1, 0u, (uint32_t)g.ntile, key,
+                            rec0, r);
 }
 
 template <int THREADS, int SPL, bool COUNT, bool DEF = false>
@@ -285,7 +290,7 @@ static void launch_gen_final(const McD &m, const BinGeom &g, const BinBufs &B, b
                              hipStream_t s)
 {
     if (!final) {
-        unsigned gb = (unsigned)((g.S + BIN_THREADS - 1) / BIN_THREADS);
+        unsigned gb = (unsigned)((g.S + 2 * BIN_THREADS - 1) / (2 * BIN_THREADS));
         hipLaunchKernelGGL((k_bin_gen<MODE, PER>), dim3(gb), dim3(BIN_THREADS), 0, s, m, g, B.key,
                            B.rec0, B.r);
     } else {

@@ -147,6 +147,57 @@ __device__ __forceinline__ void bin_gen_one(const McD &d, const BinGeom &g, size
     key[out] = tile_base + (uint32_t)(si * g.ncell + cx * g.ny + cy);
 }
 
+// Samples sid[0], sid[1] of one thread (count 2), or sid[0] alone (count 1):
+// bin_gen_one's outputs for each, their loads overlapped by gen_two; the
+// query point, displacement and record of every sample come from one copy of
+// the code (a loop that is not unrolled), so a sample's bits do not depend
+// on its place in the pair.
+template <int MODE, bool PER>
+__device__ __forceinline__ void bin_gen_pair(const McD &d, const BinGeom &g, const size_t *sid,
+                                             const size_t *out, int count, uint32_t tile_base,
+                                             uint32_t dead_key, uint32_t *__restrict__ key,
+                                             SRec *__restrict__ rec0, float2 *__restrict__ r)
+{
+    int owner[2], si[2], k[2], n[2];
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+        const size_t sq = sid[count > 1 ? q : 0];
+        const uint32_t pair = div_ns(g, (uint32_t)sq);   // sid < 2^31 (bin_geom)
+        k[q] = (int)((uint32_t)sq - pair * g.NS);
+        owner[q] = (int)(pair >> 1);
+        si[q] = (int)(pair & 1);
+        const SetD &S = d.set[si[q]];
+        n[q] = S.count_dev ? *S.count_dev : S.n;
+    }
+    GenTwo g2;
+    gen_two<MODE, PER>(d, d.set[si[0]], d.set[si[1]], n[0], n[1], owner[0], owner[1], si[0],
+                       si[1], k[0], k[1], d.owner[owner[0]], d.owner[owner[1]], g2);
+#pragma unroll 1
+    for (int q = 0; q < count; ++q) {
+        const bool o = q != 0;
+        GenState st = g2.st[0];
+        if (o) st = g2.st[1];
+        const int jq = o ? g2.j[1] : g2.j[0], sq = o ? si[1] : si[0], nq = o ? n[1] : n[0];
+        const float2 xq = o ? g2.xj[1] : g2.xj[0];
+        const float4 Fq = o ? g2.F[1] : g2.F[0];
+        const size_t oq = o ? out[1] : out[0];
+        float qx = 0.f, qy = 0.f, rx = 0.f, ry = 0.f;
+        const bool live = nq > 0 && gen_finish<MODE, PER>(d, st, jq, xq, Fq,
+                                                          d.set[sq].set_jac != nullptr, qx, qy,
+                                                          rx, ry, nullptr, nullptr);
+        r[oq] = live ? make_float2(rx, ry) : make_float2(__builtin_nanf(""), __builtin_nanf(""));
+        if (!live) {
+            rec0[oq] = SRec{0u, (uint32_t)oq};
+            key[oq] = dead_key;
+            continue;
+        }
+        const int cx = min(max((int)(qx * g.inv_cx), 0), g.nx - 1);
+        const int cy = min(max((int)(qy * g.inv_cy), 0), g.ny - 1);
+        rec0[oq] = SRec{srec_pack(qx, qy, cx, cy, g.cw, g.ch), (uint32_t)oq};
+        key[oq] = tile_base + (uint32_t)(sq * g.ncell + cx * g.ny + cy);
+    }
+}
+
 // The pair loop evaluates the spline in units where r'^2 = 3 q^2 (q = r / h_w,
 // coordinates scaled by sqrt(3) / h_w): with t = max(1 - q, 0),
 //   W(q) / (2 sig2) = t^3 - 4 max(t - 1/2, 0)^3 = min(t^3, 1/2 - 3 q^2 t),

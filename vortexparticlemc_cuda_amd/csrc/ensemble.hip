@@ -116,24 +116,31 @@ __global__ void k_ens_update_jac(EnsTabs t, int n_sim, int N, float dt, float4 *
 }
 
 template <int MODE>
-__global__ void __launch_bounds__(BIN_THREADS)
+__global__ void __launch_bounds__(BIN_THREADS / 2)
     k_ens_gen(EnsTabs t, int n_sim, uint32_t *__restrict__ key,
               SRec *__restrict__ rec0, float2 *__restrict__ r)
 {
+    // BIN_THREADS samples a block of BIN_THREADS / 2 lanes: two a lane, 64
+    // apart, their loads overlapped (bin_gen_pair); a simulation's sample range
+    // is padded to BIN_THREADS, so the block stays in one simulation
     const int first = blockIdx.x * BIN_THREADS;
     const int s = seg_of(t.soff, n_sim, first);   // block-uniform
-    const size_t out = (size_t)first + threadIdx.x;
     const McD &d = t.sim[s];
     const BinGeom &g = t.geom[s];
-    const size_t sid = out - (size_t)t.soff[s];
     const uint32_t dk = (uint32_t)(t.toff[s] + g.ntile);   // the simulation's dead tile
-    if (sid >= g.S) {   // padding of the simulation's sample range
-        key[out] = dk;
-        rec0[out] = SRec{0u, (uint32_t)out};
-        r[out] = make_float2(__builtin_nanf(""), __builtin_nanf(""));   // no query
-        return;
+    const size_t out0 = (size_t)first + (threadIdx.x / WAVE) * (2 * WAVE) +
+                        (threadIdx.x & (WAVE - 1));
+    const size_t out1 = out0 + WAVE;
+    const size_t sid0 = out0 - (size_t)t.soff[s], sid1 = out1 - (size_t)t.soff[s];
+    const size_t outs[2] = {out0, out1}, sids[2] = {sid0, sid1};
+    const int count = sid1 < g.S ? 2 : (sid0 < g.S ? 1 : 0);
+    if (count > 0)
+        bin_gen_pair<MODE, true>(d, g, sids, outs, count, (uint32_t)t.toff[s], dk, key, rec0, r);
+    for (int q = count; q < 2; ++q) {   // padding of the simulation's sample range
+        key[outs[q]] = dk;
+        rec0[outs[q]] = SRec{0u, (uint32_t)outs[q]};
+        r[outs[q]] = make_float2(__builtin_nanf(""), __builtin_nanf(""));   // no query
     }
-    bin_gen_one<MODE, true>(d, g, sid, out, (uint32_t)t.toff[s], dk, key, rec0, r);
 }
 
 // chunks of tile k of the ensemble's tiles (0 for an empty tile and for a
@@ -449,7 +456,8 @@ template <int MODE>
 static int ens_estimate(const vpmc_ensemble_desc *e, EnsLayout &L, hipStream_t st)
 {
     const int ns = L.n_sim;
-    hipLaunchKernelGGL((k_ens_gen<MODE>), dim3((unsigned)(L.S / BIN_THREADS)), dim3(BIN_THREADS), 0,
+    hipLaunchKernelGGL((k_ens_gen<MODE>), dim3((unsigned)(L.S / BIN_THREADS)), dim3(BIN_THREADS / 2),
+                       0,
                        st, L.t, ns, L.skey, L.rec0, L.r);
     VPMC_CHECK_LAUNCH();
     // segmented by simulation: samples are grouped by simulation already

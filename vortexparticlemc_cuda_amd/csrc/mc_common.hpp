@@ -151,39 +151,77 @@ __device__ __forceinline__ void kper_smooth(const McD &d, float rx, float ry, fl
     ky = -Ar * d.inv_2piL;
 }
 
-// One sample of owner `owner`, sign set si, index k in [0, N1+N2):
-// returns false for a sample that contributes nothing without a query (a
-// branch-2 point outside the min-image cell, or a non-finite point).
-// (qx, qy): the query point inside the box; (rx, ry): the displacement the
-// MIS contribution uses (reference: dst - sp unwrapped; consistent: min image).
-// src (optional) receives the set-local index of a branch-1 sample's
-// particle, -1 for a branch-2 sample; jit (optional) the consistent branch-1
-// jitter before the deformation F is applied (the frozen offset the adjoint
-// differentiates F through), (0, 0) otherwise.
+// One sample's generation in three phases, so that a thread generating two
+// samples can issue both samples' dependent loads together (gen_two):
+//   gen_begin   draws (Philox or the replay stream), the target, and for a
+//               branch-1 sample the CDF search bracket from the guide table;
+//   the search  std::upper_bound inside the bracket (gen_search);
+//   gen_finish  the query point and the MIS displacement from the drawn
+//               particle's position (and F).
+struct GenState {
+    float u[3];
+    float2 dst;   // the target: the owner or an auxiliary target
+    bool b1;      // branch 1 (a particle drawn from |w|)
+    int lo, hi;   // branch 1: the search bracket, then lo = the drawn index
+};
+
 template <int MODE, bool PER>
-__device__ __forceinline__ bool gen_sample(const McD &d, const SetD &S, int n_s, int owner,
-                                           int si, int k, float2 dst, float &qx, float &qy,
-                                           float &rx, float &ry, int *src = nullptr,
-                                           float2 *jit = nullptr)
+__device__ __forceinline__ void gen_begin(const McD &d, const SetD &S, int owner, int si, int k,
+                                          float2 dst, GenState &st)
 {
-    if (jit) *jit = make_float2(0.f, 0.f);
     const bool b1 = k < d.N1;
     int t = 0, kb = b1 ? k : k - d.N1;   // t: target of a disk sample (0 = owner)
     if (!b1 && kb >= d.N2) {
         t = 1 + (kb - d.N2) / d.N2;
         kb = (kb - d.N2) - (t - 1) * d.N2;
     }
-    float u[3];
-    draws(d, owner, si, b1 ? 1 : 2 + t, kb, u);
+    draws(d, owner, si, b1 ? 1 : 2 + t, kb, st.u);
     if (t) {   // the auxiliary target, as the reference forms it (dst.x - dx, ...)
         const float2 o = aux_offset(t, d.aux_dx);
         dst = make_float2(__fadd_rn(dst.x, o.x), __fadd_rn(dst.y, o.y));
     }
+    st.dst = dst;
+    st.b1 = b1;
+    st.lo = st.hi = 0;
+    if (b1) {
+        const int g = guide_bucket(st.u[0], S.glog2);
+        st.lo = S.guide[g];
+        st.hi = S.guide[g + 1];
+    }
+}
+
+// upper_bound of the branch-1 draw inside its bracket (guided_upper_bound)
+__device__ __forceinline__ int gen_search(const SetD &S, int n_s, const GenState &st)
+{
+    int lo = st.lo, hi = st.hi;
+    const double ud = (double)st.u[0];
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (ud < S.cdf[mid]) hi = mid; else lo = mid + 1;
+    }
+    return lo >= n_s ? n_s - 1 : lo;
+}
+
+// returns false for a sample that contributes nothing without a query (a
+// branch-2 point outside the min-image cell, or a non-finite point).
+// (qx, qy): the query point inside the box; (rx, ry): the displacement the
+// MIS contribution uses (reference: dst - sp unwrapped; consistent: min image).
+// j, xj, F: a branch-1 sample's drawn set-local index, its position and
+// (deformed sets) its F.  src (optional) receives j, -1 for a branch-2
+// sample; jit (optional) the consistent branch-1 jitter before F is applied
+// (the frozen offset the adjoint differentiates F through), (0, 0) otherwise.
+template <int MODE, bool PER>
+__device__ __forceinline__ bool gen_finish(const McD &d, const GenState &st, int j, float2 xj,
+                                           float4 F, bool def, float &qx, float &qy, float &rx,
+                                           float &ry, int *src, float2 *jit)
+{
+    if (jit) *jit = make_float2(0.f, 0.f);
+    const float *u = st.u;
+    const float2 dst = st.dst;
+    const bool b1 = st.b1;
     float spx, spy;
     if (MODE == VPMC_MODE_REFERENCE) {
         if (b1) {   // src/cpu/monte_carlo_bs.cpp:44-58
-            int j = guided_upper_bound(u[0], S.cdf, S.guide, S.glog2, n_s);
-            float2 xj = S.set_xy[j];
             spx = xj.x;
             spy = xj.y;
             if (PER) {   // include/monte_carlo_bs.hpp:92-102
@@ -205,16 +243,13 @@ __device__ __forceinline__ bool gen_sample(const McD &d, const SetD &S, int n_s,
         ry = dst.y - spy;
     } else {
         if (b1) {   // particle from |w|/Omega, jitter from W (exact radial inverse)
-            int j = guided_upper_bound(u[0], S.cdf, S.guide, S.glog2, n_s);
             if (src) *src = j;
-            float2 xj = S.set_xy[j];
             float rho = mass_inverse(u[1]) * d.h_w;
             float s, c;
             sincos_rev(u[2], s, c);
             float jx = rho * c, jy = rho * s;
             if (jit) *jit = make_float2(jx, jy);
-            if (S.set_jac) {   // the deformed blob: F_j u (src/cuda/monte_carlo_bs.cu:210-212)
-                const float4 F = S.set_jac[j];
+            if (def) {   // the deformed blob: F_j u (src/cuda/monte_carlo_bs.cu:210-212)
                 const float ex = fmaf(F.x, jx, F.y * jy), ey = fmaf(F.z, jx, F.w * jy);
                 jx = ex;
                 jy = ey;
@@ -255,6 +290,78 @@ __device__ __forceinline__ bool gen_sample(const McD &d, const SetD &S, int n_s,
         qy = spy;
     }
     return true;
+}
+
+// One sample of owner `owner`, sign set si, index k in [0, N1+N2) (the three
+// phases above in turn; outputs as gen_finish's)
+template <int MODE, bool PER>
+__device__ __forceinline__ bool gen_sample(const McD &d, const SetD &S, int n_s, int owner,
+                                           int si, int k, float2 dst, float &qx, float &qy,
+                                           float &rx, float &ry, int *src = nullptr,
+                                           float2 *jit = nullptr)
+{
+    GenState st;
+    gen_begin<MODE, PER>(d, S, owner, si, k, dst, st);
+    int j = 0;
+    float2 xj = make_float2(0.f, 0.f);
+    float4 F = make_float4(1.f, 0.f, 0.f, 1.f);
+    if (st.b1) {
+        j = gen_search(S, n_s, st);
+        xj = S.set_xy[j];
+        if (S.set_jac) F = S.set_jac[j];
+    }
+    return gen_finish<MODE, PER>(d, st, j, xj, F, S.set_jac != nullptr, qx, qy, rx, ry, src,
+                                 jit);
+}
+
+// Two samples of one thread: the draws and every dependent load of both
+// (guide brackets, the interleaved upper_bound searches, positions and F)
+// issued together, so that a wave keeps two dependent-load chains in flight
+// where one sample would keep one.  gen_finish then runs once per sample
+// from the outputs (the caller loops over them without unrolling, so that
+// both samples go through one copy of its arithmetic).
+struct GenTwo {
+    GenState st[2];
+    int j[2];
+    float2 xj[2];
+    float4 F[2];
+};
+
+template <int MODE, bool PER>
+__device__ __forceinline__ void gen_two(const McD &d, const SetD &Sa, const SetD &Sb, int n_a,
+                                        int n_b, int oa, int ob, int sa, int sb, int ka, int kb,
+                                        float2 da, float2 db, GenTwo &g2)
+{
+    gen_begin<MODE, PER>(d, Sa, oa, sa, ka, da, g2.st[0]);
+    gen_begin<MODE, PER>(d, Sb, ob, sb, kb, db, g2.st[1]);
+    int lo[2] = {g2.st[0].lo, g2.st[1].lo}, hi[2] = {g2.st[0].hi, g2.st[1].hi};
+    const double ud[2] = {(double)g2.st[0].u[0], (double)g2.st[1].u[0]};
+    const SetD *S[2] = {&Sa, &Sb};
+    while (lo[0] < hi[0] || lo[1] < hi[1]) {   // both upper_bound searches, interleaved
+        double c[2];
+        int mid[2];
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            mid[q] = (lo[q] + hi[q]) >> 1;
+            c[q] = lo[q] < hi[q] ? S[q]->cdf[mid[q]] : 0.0;
+        }
+#pragma unroll
+        for (int q = 0; q < 2; ++q)
+            if (lo[q] < hi[q]) {
+                if (ud[q] < c[q]) hi[q] = mid[q]; else lo[q] = mid[q] + 1;
+            }
+    }
+    const int n[2] = {n_a, n_b};
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+        g2.j[q] = lo[q] >= n[q] ? n[q] - 1 : lo[q];
+        g2.xj[q] = make_float2(0.f, 0.f);
+        g2.F[q] = make_float4(1.f, 0.f, 0.f, 1.f);
+        if (g2.st[q].b1 && n[q] > 0) {
+            g2.xj[q] = S[q]->set_xy[g2.j[q]];
+            if (S[q]->set_jac) g2.F[q] = S[q]->set_jac[g2.j[q]];
+        }
+    }
 }
 
 // MIS contribution of one sample with vorticity om at displacement r

@@ -208,13 +208,19 @@ __device__ __forceinline__ double spline_d(double r, double h_w)
 // injected draws may equal 1 (std::uniform_real_distribution<float> can return
 // 1.0f) and the bucket index is clamped to the table, so u >= 1 gives n-1 and
 // u < 0 gives 0 exactly as upper_bound does.
+// the guide bucket of draw u (0 or M - 1 outside [0, 1), NaN included: no
+// cdf value exceeds such a draw)
+__device__ __forceinline__ int guide_bucket(float u, int glog2)
+{
+    const int M = 1 << glog2;
+    if (u >= 0.f && u < 1.f) return min((int)(u * (float)M), M - 1);
+    return (u < 0.f) ? 0 : M - 1;
+}
+
 __device__ __forceinline__ int guided_upper_bound(float u, const double *__restrict__ cdf,
                                                   const int *__restrict__ guide, int glog2, int n)
 {
-    const int M = 1 << glog2;
-    int kb;
-    if (u >= 0.f && u < 1.f) kb = min((int)(u * (float)M), M - 1);
-    else kb = (u < 0.f) ? 0 : M - 1;   // u >= 1 or NaN: no cdf value exceeds it
+    const int kb = guide_bucket(u, glog2);
     int lo = guide[kb], hi = guide[kb + 1];
     double ud = (double)u;
     while (lo < hi) {
