@@ -114,44 +114,16 @@ __device__ __forceinline__ bool chunk_range(const TileChunks &tc, int ntile, int
 int tile_chunks_build(int *coff, int ntile, const int *start, int chunk, int4 *ctab,
                       void *scan_ws, size_t scan_bytes, hipStream_t s);
 
-// One sample: draws, query point, MIS displacement and tile key.  Writes
+// Samples sid[0], sid[1] of one thread (count 2), or sid[0] alone (count 1):
+// draws, query point, MIS displacement and tile key of each.  Writes
 // key[out] (tile_base + local tile, or dead_key without a query),
 // rec0[out] = (the point's offsets in its cell, out) (the record the sort
-// moves) and r[out],
-// NaN for a sample without a query: the final pass reads no vorticity for
-// it, so om needs no clearing (om[out] is written by the evaluation for
-// every live sample).
-template <int MODE, bool PER>
-__device__ __forceinline__ void bin_gen_one(const McD &d, const BinGeom &g, size_t sid,
-                                            size_t out, uint32_t tile_base, uint32_t dead_key,
-                                            uint32_t *__restrict__ key, SRec *__restrict__ rec0,
-                                            float2 *__restrict__ r)
-{
-    const uint32_t pair = div_ns(g, (uint32_t)sid);   // sid < 2^31 (bin_geom)
-    const int k = (int)((uint32_t)sid - pair * g.NS);
-    const int owner = (int)(pair >> 1), si = (int)(pair & 1);
-    const SetD &S = d.set[si];
-    const int n_s = S.count_dev ? *S.count_dev : S.n;
-    float qx = 0.f, qy = 0.f, rx = 0.f, ry = 0.f;
-    bool live = n_s > 0 &&
-                gen_sample<MODE, PER>(d, S, n_s, owner, si, k, d.owner[owner], qx, qy, rx, ry);
-    r[out] = live ? make_float2(rx, ry) : make_float2(__builtin_nanf(""), __builtin_nanf(""));
-    if (!live) {
-        rec0[out] = SRec{0u, (uint32_t)out};
-        key[out] = dead_key;
-        return;
-    }
-    int cx = min(max((int)(qx * g.inv_cx), 0), g.nx - 1);
-    int cy = min(max((int)(qy * g.inv_cy), 0), g.ny - 1);
-    rec0[out] = SRec{srec_pack(qx, qy, cx, cy, g.cw, g.ch), (uint32_t)out};
-    key[out] = tile_base + (uint32_t)(si * g.ncell + cx * g.ny + cy);
-}
-
-// Samples sid[0], sid[1] of one thread (count 2), or sid[0] alone (count 1):
-// bin_gen_one's outputs for each, their loads overlapped by gen_two; the
-// query point, displacement and record of every sample come from one copy of
-// the code (a loop that is not unrolled), so a sample's bits do not depend
-// on its place in the pair.
+// moves) and r[out], NaN for a sample without a query: the final pass reads
+// no vorticity for it, so om needs no clearing (om[out] is written by the
+// evaluation for every live sample).  The two samples' loads overlap
+// (gen_two); the query point, displacement and record of every sample come
+// from one copy of the code (a loop that is not unrolled), so a sample's
+// bits do not depend on its place in the pair.
 template <int MODE, bool PER>
 __device__ __forceinline__ void bin_gen_pair(const McD &d, const BinGeom &g, const size_t *sid,
                                              const size_t *out, int count, uint32_t tile_base,
