@@ -91,7 +91,7 @@ int tile_chunks_build(int *coff, int ntile, const int *start, int chunk, int4 *c
 }
 
 template <int MODE, bool PER>
-__global__ void __launch_bounds__(BIN_THREADS)
+__global__ void __launch_bounds__(BIN_THREADS) __attribute__((amdgpu_waves_per_eu(8)))
     k_bin_gen(McD d, BinGeom g, uint32_t *__restrict__ key, SRec *__restrict__ rec0,
               float2 *__restrict__ r)
 {

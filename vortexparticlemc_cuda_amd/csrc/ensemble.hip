@@ -116,7 +116,7 @@ __global__ void k_ens_update_jac(EnsTabs t, int n_sim, int N, float dt, float4 *
 }
 
 template <int MODE>
-__global__ void __launch_bounds__(BIN_THREADS / 2)
+__global__ void __launch_bounds__(BIN_THREADS / 2) __attribute__((amdgpu_waves_per_eu(8)))
     k_ens_gen(EnsTabs t, int n_sim, uint32_t *__restrict__ key,
               SRec *__restrict__ rec0, float2 *__restrict__ r)
 {
